@@ -1,0 +1,116 @@
+// Device-side layout of a render batch. Built by the host runtime from the
+// ABI plans (include/clay_hip.h); shared by the kernels and the runtime.
+//
+// Layout choices for MI355X:
+//  - the sample bank is int16 (half the bytes of the reference float bank);
+//  - the sinc table is phase-major ([phase][tap]) so one lane reads its taps
+//    as contiguous 16-byte vectors;
+//  - spans are split into a hot record (mix/source kernels) and a cold record
+//    (filter and pitch recurrences);
+//  - control tracks are absolute point ranges in one concatenated array.
+#pragma once
+
+#include <stdint.h>
+
+#include "../../../include/clay_hip.h"
+
+namespace clay_dev {
+
+constexpr int kPhases = CLAY_SINC_PHASES;
+constexpr int kTailBlock = 1024;   // natural end / finalize block (src/cuda/render_types.h:92)
+constexpr int kTileFrames = 256;   // output frames per mix tile = threads per mix block
+
+// Hot span record: everything the per-frame voice audio needs.
+struct Span {
+    uint64_t sample;                // bank offset
+    double position, step, pitchSlope;
+    uint64_t filtered;              // scratch offset (floats) of this span's first frame in the segment
+    uint64_t phase;                 // phase scratch offset (doubles)
+    int32_t begin, end, sourceEnd, loopFirst, loopEnd, loopCrossfade;
+    int32_t channels, generator, stereoMatrix, variablePitch;
+    uint32_t noiseSeed;
+    int32_t noiseStart;
+    int32_t firstMod, modCount;
+    int32_t filterTrack;            // -1 = dry voice
+    int32_t job;                    // batch job index
+    int32_t next;                   // next span of the same voice chain, -1 at the end
+    int32_t attackStart, holdStart, decayStart, releaseStart, fadeStart;
+    float start, attackStep, decayRate, sustain, releaseValue, releaseRate, fadeValue, fadeStep;
+    float gain, pan, left, right;
+    float widthDirect, widthCross, positionLeft, positionRight, width, stereoPosition;
+};
+
+// Cold span record: filter slots and the filter/pitch envelopes.
+struct SpanCold {
+    clay_voice_filter filters[CLAY_FILTER_SLOTS];
+    clay_envelope_shape extra[2];
+    int32_t active[2];
+    int32_t sampleRate;
+    int32_t pad;
+};
+
+// Modulation binding with absolute control point ranges (first < 0: constant).
+struct Mod {
+    int32_t first, last;
+    int32_t target;
+    float depth;
+    int32_t shapeFirst, shapeLast;
+};
+
+struct Job {
+    uint64_t mix;                   // frame offset of the job's mix in the batch mix buffer
+    int32_t frames;
+    int32_t controls[4];            // point offsets: CC7 [c0,c1), CC11 [c1,c2), CC10 [c2,c3)
+    int32_t musicEnd, limit, fade;
+    int32_t blockFirst;             // first 1024-frame block of the job
+    int32_t windowFirst;            // first 50 ms window of the job
+    int32_t bodyFrames, bodyFirst, onsetFirst, onsetCount;
+    int32_t pad;
+};
+
+struct Tile {
+    int32_t job, frame, frames, first, count;
+};
+
+struct FilterTile {
+    int32_t span, frame;
+};
+
+// Filter and pitch state carried across segments (src/cuda/render_types.h:22-28).
+struct VoiceState {
+    double position, step;
+    int32_t pitchSpan, previousLoopEnd, previousLoopStart;
+    int32_t pitchInitialized, filterInitialized;
+    int32_t pad;
+    float z1[CLAY_FILTER_SLOTS][3][2], z2[CLAY_FILTER_SLOTS][3][2];
+    float coefficients[CLAY_FILTER_SLOTS][5];
+};
+
+struct StereoStats {
+    double power;
+    float peak;
+    int32_t pad;
+};
+
+struct MonoStats {
+    double power;
+    int32_t peak;
+    int32_t pad;
+};
+
+struct PcmJob {
+    uint64_t sourceOffset;
+    int32_t blockFirst;
+    int32_t pad;
+};
+
+struct TailConfig {
+    int32_t window, quietFrames, padding, fade;
+    double relative, floor, ceiling;
+};
+
+struct AudioJobDesc {
+    int32_t bodyFrames, bodyFirst, onsetFirst, onsetCount;
+};
+
+} // namespace clay_dev

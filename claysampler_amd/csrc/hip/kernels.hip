@@ -1,0 +1,1145 @@
+// HIP kernels of the ClaySampler render core for gfx950 (MI355X).
+//
+// Render (per segment):  pitchChains -> filterSources -> filterChains -> mixTiles
+// Natural finishing:     scanStereo -> selectNative -> quantizeMono -> selectTrim
+// Every policy:          layoutPcm -> finalizePcm
+// Quality / groups:      measureBody -> measureOnsets -> summarizeAudio
+//                        -> matchVolume -> layoutAccepted -> packAccepted
+//
+// Reference behaviour: src/cuda/render.cu, tail.cu, quality.cu. The kernels
+// here are written for wave64: reductions use 64-lane shuffles, block
+// reductions combine at most 16 waves through LDS, and per-voice parameters
+// are read through wave-uniform (scalar) loads.
+#include <hip/hip_runtime.h>
+
+#include "device_types.h"
+
+using namespace clay_dev;
+
+namespace {
+
+constexpr float kQuarterPi = .7853981633974483f;
+
+// ---------------------------------------------------------------- envelopes
+// src/cuda/render.cu:6-19
+__device__ __forceinline__ float envelopeSpan(const Span& v, int frame)
+{
+    if (v.releaseStart >= 0 && frame >= v.releaseStart) {
+        if (frame >= v.fadeStart)
+            return fmaxf(0.f, v.fadeValue - float(frame - v.fadeStart + 1) * v.fadeStep);
+        return v.releaseValue * expf(float(frame - v.releaseStart + 1) * v.releaseRate);
+    }
+    if (frame < v.attackStart)
+        return v.start;
+    if (frame < v.holdStart)
+        return fminf(1.f, v.start + float(frame - v.attackStart + 1) * v.attackStep);
+    if (frame < v.decayStart)
+        return 1.f;
+    if (v.decayRate == 0)
+        return v.sustain;
+    return fmaxf(v.sustain, expf(float(frame - v.decayStart + 1) * v.decayRate));
+}
+
+__device__ __forceinline__ float envelopeShape(const clay_envelope_shape& v, int frame)
+{
+    if (v.release_start >= 0 && frame >= v.release_start) {
+        if (frame >= v.fade_start)
+            return fmaxf(0.f, v.fade_value - float(frame - v.fade_start + 1) * v.fade_step);
+        return v.release_value * expf(float(frame - v.release_start + 1) * v.release_rate);
+    }
+    if (frame < v.attack_start)
+        return v.start;
+    if (frame < v.hold_start)
+        return fminf(1.f, v.start + float(frame - v.attack_start + 1) * v.attack_step);
+    if (frame < v.decay_start)
+        return 1.f;
+    if (v.decay_rate == 0)
+        return v.sustain;
+    return fmaxf(v.sustain, expf(float(frame - v.decay_start + 1) * v.decay_rate));
+}
+
+// ---------------------------------------------------------------- controls
+// Last point at or before `frame` (the first point when none), then linear
+// (src/cuda/render.cu:107-118).
+__device__ __forceinline__ float controlAt(const clay_control_point* __restrict__ points, int lo, int hi, int frame)
+{
+    while (lo + 1 < hi) {
+        const int mid = lo + (hi - lo) / 2;
+        if (points[mid].frame <= frame)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    const clay_control_point p = points[lo];
+    return fmaf(p.step, float(frame - p.frame), p.value);
+}
+
+__device__ __forceinline__ float modulationAt(const Mod& m, const clay_control_point* __restrict__ points, int frame)
+{
+    float v = m.first < 0 ? m.depth : m.depth * controlAt(points, m.first, m.last, frame);
+    if (m.shapeFirst >= 0)
+        v *= controlAt(points, m.shapeFirst, m.shapeLast, frame);
+    return v;
+}
+
+__device__ __forceinline__ float jobGain(const clay_control_point* __restrict__ points, const Job& j, int frame)
+{
+    return controlAt(points, j.controls[0], j.controls[1], frame) * controlAt(points, j.controls[1], j.controls[2], frame);
+}
+
+// ---------------------------------------------------------------- resampling
+// Source frame of a span, zero outside [0, sourceEnd) (src/cuda/render.cu:21-28).
+// Values are raw int16 (scaled by 2^-15 by the caller).
+__device__ __forceinline__ float2 fetchRaw(const int16_t* __restrict__ bank, const Span& v, int frame)
+{
+    if (frame < 0 || frame >= v.sourceEnd)
+        return make_float2(0.f, 0.f);
+    const uint64_t i = v.sample + uint64_t(frame) * uint64_t(v.channels);
+    const float left = float(bank[i]);
+    return make_float2(left, v.channels == 1 ? left : float(bank[i + 1]));
+}
+
+// Windowed-sinc tap sum at the nearest of 2048 phases (src/cuda/render.cu:30-61).
+// Accumulating the raw int16 values and scaling once by 2^-15 gives the same
+// float result as weighting the pre-scaled float bank of the reference.
+template<int T>
+__device__ __forceinline__ float2 interpolateAt(const int16_t* __restrict__ bank, const float* __restrict__ sinc,
+                                                const Span& v, double pos)
+{
+    constexpr float scale = 1.f / 32768.f;
+    const double floorPos = floor(pos);
+    const int centre = int(floorPos);
+    if (pos == floorPos) {
+        const float2 s = fetchRaw(bank, v, centre);
+        return make_float2(s.x * scale, s.y * scale);
+    }
+    const int phase = __double2int_rn((pos - floorPos) * kPhases);
+    const float4* __restrict__ w4 = reinterpret_cast<const float4*>(sinc + size_t(phase) * T);
+    constexpr int first = T / 2 - 1;
+    float a0 = 0.f, a1 = 0.f, b0 = 0.f, b1 = 0.f;
+    if (centre >= first && centre < v.sourceEnd - T / 2) {
+        const int16_t* __restrict__ src = bank + v.sample + uint64_t(centre - first) * uint64_t(v.channels);
+        if (v.channels == 1) {
+#pragma unroll
+            for (int q = 0; q < T / 4; ++q) {
+                const float4 w = w4[q];
+                a0 = fmaf(w.x, float(src[4 * q + 0]), a0);
+                a1 = fmaf(w.y, float(src[4 * q + 1]), a1);
+                a0 = fmaf(w.z, float(src[4 * q + 2]), a0);
+                a1 = fmaf(w.w, float(src[4 * q + 3]), a1);
+            }
+            const float m = (a0 + a1) * scale;
+            return make_float2(m, m);
+        }
+#pragma unroll
+        for (int q = 0; q < T / 4; ++q) {
+            const float4 w = w4[q];
+            a0 = fmaf(w.x, float(src[8 * q + 0]), a0);
+            b0 = fmaf(w.x, float(src[8 * q + 1]), b0);
+            a1 = fmaf(w.y, float(src[8 * q + 2]), a1);
+            b1 = fmaf(w.y, float(src[8 * q + 3]), b1);
+            a0 = fmaf(w.z, float(src[8 * q + 4]), a0);
+            b0 = fmaf(w.z, float(src[8 * q + 5]), b0);
+            a1 = fmaf(w.w, float(src[8 * q + 6]), a1);
+            b1 = fmaf(w.w, float(src[8 * q + 7]), b1);
+        }
+        return make_float2((a0 + a1) * scale, (b0 + b1) * scale);
+    }
+    for (int q = 0; q < T / 4; ++q) {
+        const float4 w = w4[q];
+        const float ws[4] = {w.x, w.y, w.z, w.w};
+        for (int k = 0; k < 4; ++k) {
+            const float2 s = fetchRaw(bank, v, centre - first + 4 * q + k);
+            a0 = fmaf(ws[k], s.x, a0);
+            b0 = fmaf(ws[k], s.y, b0);
+        }
+    }
+    return make_float2(a0 * scale, b0 * scale);
+}
+
+// Source position, generators, loop wrap and loop crossfade (src/cuda/render.cu:63-100).
+// The position arithmetic uses explicitly rounded double operations so that
+// the phase index equals the one of a plain IEEE double evaluation.
+template<int T>
+__device__ float2 interpolate(const int16_t* __restrict__ bank, const float* __restrict__ sinc, const Span& v,
+                              int frame, double pos)
+{
+    if (pos < 0) {
+        const int elapsed = frame - v.begin;
+        double distance = elapsed;
+        if (v.pitchSlope != 0)
+            distance = __ddiv_rn(expm1(__dmul_rn(v.pitchSlope, double(elapsed))), expm1(v.pitchSlope));
+        pos = __dadd_rn(v.position, __dmul_rn(distance, v.step));
+    }
+    if (v.generator != CLAY_GEN_NONE) {
+        if (v.generator == CLAY_GEN_NOISE) {
+            unsigned bits = v.noiseSeed + unsigned(frame - v.noiseStart);
+            bits = (bits ^ (bits >> 16)) * 0x7feb352du;
+            bits = (bits ^ (bits >> 15)) * 0x846ca68bu;
+            bits ^= bits >> 16;
+            const float x = float(bits >> 8) * (2.f / 16777216.f) - 1.f;
+            return make_float2(x, x);
+        }
+        const float x = v.generator == CLAY_GEN_SINE ? -sinf(6.28318530717958647692f * float(pos - floor(pos))) : 0.f;
+        return make_float2(x, x);
+    }
+    const int period = v.loopEnd - v.loopFirst;
+    if (v.loopEnd > 0 && pos >= v.loopEnd)
+        pos = __dadd_rn(double(v.loopFirst), fmod(__dsub_rn(pos, double(v.loopFirst)), double(period)));
+    if (pos >= v.sourceEnd)
+        return make_float2(0.f, 0.f);
+    float2 s = interpolateAt<T>(bank, sinc, v, pos);
+    const int fadeStart = v.loopEnd - v.loopCrossfade;
+    if (v.loopEnd > 0 && v.loopCrossfade > 0 && pos >= fadeStart) {
+        const float2 entry = interpolateAt<T>(bank, sinc, v, __dsub_rn(pos, double(period)));
+        const float phase = float((pos - fadeStart) / v.loopCrossfade);
+        const float blend = .5f - .5f * cosf(3.14159265358979323846f * phase);
+        s.x = fmaf(blend, entry.x - s.x, s.x);
+        s.y = fmaf(blend, entry.y - s.y, s.y);
+    }
+    return s;
+}
+
+// src/cuda/render.cu:130-136
+__device__ __forceinline__ float2 panGains(const Span& v, float offset)
+{
+    if (offset == 0)
+        return make_float2(v.left, v.right);
+    const float p = fminf(1.f, fmaxf(-1.f, v.pan + offset));
+    return make_float2(v.gain * cosf((1 + p) * kQuarterPi), v.gain * cosf((1 - p) * kQuarterPi));
+}
+
+// Envelope, modulation, resampling, pan and stereo matrix of one voice frame
+// (src/cuda/render.cu:138-187).
+template<int T>
+__device__ float2 voiceAudio(const int16_t* __restrict__ bank, const float* __restrict__ sinc, const Span& v,
+                             const clay_control_point* __restrict__ points, const Mod* __restrict__ mods, float pan,
+                             int frame, bool filterInput, double phase)
+{
+    float level = envelopeSpan(v, frame);
+    if (level <= 0)
+        return make_float2(0.f, 0.f);
+    float db = 0, widthOffset = 0, positionOffset = 0;
+    for (int j = v.firstMod; j < v.firstMod + v.modCount; ++j) {
+        const Mod m = mods[j];
+        if (m.target > CLAY_MOD_POSITION)
+            continue;
+        const float x = modulationAt(m, points, frame);
+        switch (m.target) {
+        case CLAY_MOD_AMPLITUDE: level *= x; break;
+        case CLAY_MOD_VOLUME: db += x; break;
+        case CLAY_MOD_PAN: pan += x; break;
+        case CLAY_MOD_WIDTH: widthOffset += x; break;
+        case CLAY_MOD_POSITION: positionOffset += x; break;
+        default: break;
+        }
+    }
+    if (db != 0)
+        level *= exp2f(db * .166096404744368f);
+    const float2 s = interpolate<T>(bank, sinc, v, frame, phase);
+    if (filterInput && v.channels == 1)
+        return make_float2(s.x * level, s.x * level);
+    const float2 g = panGains(v, pan);
+    const float l = s.x * g.x, r = s.y * g.y;
+    if (v.stereoMatrix) {
+        float direct = v.widthDirect, cross = v.widthCross;
+        if (widthOffset != 0) {
+            const float w = fminf(1.f, fmaxf(-1.f, v.width + widthOffset));
+            direct = cosf((1 - w) * kQuarterPi);
+            cross = cosf((1 + w) * kQuarterPi);
+        }
+        float pl = v.positionLeft, pr = v.positionRight;
+        if (positionOffset != 0) {
+            const float p = fminf(1.f, fmaxf(-1.f, v.stereoPosition + positionOffset));
+            pl = cosf((1 + p) * kQuarterPi);
+            pr = cosf((1 - p) * kQuarterPi);
+        }
+        return make_float2(fmaf(r, cross, l * direct) * (level * pl), fmaf(l, cross, r * direct) * (level * pr));
+    }
+    return make_float2(l * level, r * level);
+}
+
+// ---------------------------------------------------------------- filters
+struct Coeff {
+    float b0, b1, b2, a1, a2;
+};
+
+// src/cuda/render.cu:278-310
+__device__ Coeff coefficients(int type, float cutoff, float resonance, int rate, float bandwidth, float gain)
+{
+    const float w = 6.28318530717958647692f * fminf(fminf(20000.f, float(rate) * .499f), fmaxf(1.f, cutoff)) / float(rate);
+    if (type == CLAY_FILTER_LOW1 || type == CLAY_FILTER_HIGH1) {
+        const float p = expf(-w);
+        if (type == CLAY_FILTER_LOW1)
+            return {1 - p, 0, 0, -p, 0};
+        const float h = (1 + p) * .5f;
+        return {h, -h, 0, -p, 0};
+    }
+    float s, c;
+    sincosf(w, &s, &c);
+    if (type == CLAY_FILTER_PEAK) {
+        const float a = exp2f(fminf(60.f, fmaxf(-96.f, gain)) * .083048202372184f);
+        const float width = fminf(12.f, fmaxf(.01f, bandwidth));
+        const float alpha = s * sinhf(fminf(20.f, .346573590279973f * width * w / s));
+        const float inv = 1 / (1 + alpha / a);
+        const float mid = -2 * c * inv;
+        return {(1 + alpha * a) * inv, mid, (1 - alpha * a) * inv, mid, (1 - alpha / a) * inv};
+    }
+    const float alpha = .5f * s * exp2f(-.166096404744368f * fminf(60.f, fmaxf(-60.f, resonance)));
+    const float inv = 1.f / (1 + alpha);
+    const float a1 = -2 * c * inv, a2 = (1 - alpha) * inv;
+    if (type == CLAY_FILTER_BAND2 || type == CLAY_FILTER_BAND4 || type == CLAY_FILTER_BAND6)
+        return {alpha * inv, 0, -alpha * inv, a1, a2};
+    if (type == CLAY_FILTER_NOTCH2)
+        return {inv, a1, inv, a1, a2};
+    const bool high = type == CLAY_FILTER_HIGH2 || type == CLAY_FILTER_HIGH4 || type == CLAY_FILTER_HIGH6;
+    const float b = (1 + (high ? c : -c)) * .5f * inv;
+    return {b, (high ? -2 : 2) * b, b, a1, a2};
+}
+
+__device__ __forceinline__ int sections(int type)
+{
+    if (type == CLAY_FILTER_NONE) return 0;
+    if (type == CLAY_FILTER_PEAK) return 1;
+    if (type >= CLAY_FILTER_LOW6) return 3;
+    if (type >= CLAY_FILTER_LOW4) return 2;
+    return 1;
+}
+
+__device__ __forceinline__ float filterSample(float x, const Coeff& c, float& z1, float& z2)
+{
+    const float y = fmaf(c.b0, x, z1);
+    z1 = fmaf(-c.a1, y, fmaf(c.b1, x, z2));
+    z2 = fmaf(-c.a2, y, c.b2 * x);
+    return y;
+}
+
+// ---------------------------------------------------------------- reductions
+template<class T, class Op>
+__device__ __forceinline__ T waveReduce(T v, Op op)
+{
+#pragma unroll
+    for (int d = 32; d; d >>= 1)
+        v = op(v, __shfl_xor(v, d, 64));
+    return v;
+}
+
+// Block reduction for up to 1024 threads; every thread gets the result.
+template<class T, class Op>
+__device__ T blockReduce(T v, T identity, Op op)
+{
+    __shared__ T partial[16];
+    v = waveReduce(v, op);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 0)
+        partial[wave] = v;
+    __syncthreads();
+    const int waves = (blockDim.x + 63) >> 6;
+    T r = identity;
+    for (int i = 0; i < waves; ++i)
+        r = op(r, partial[i]);
+    __syncthreads();
+    return r;
+}
+
+struct Add {
+    template<class T> __device__ T operator()(T a, T b) const { return a + b; }
+};
+struct Max {
+    template<class T> __device__ T operator()(T a, T b) const { return a > b ? a : b; }
+};
+struct Min {
+    template<class T> __device__ T operator()(T a, T b) const { return a < b ? a : b; }
+};
+
+// Index of the last entry whose `first` is <= block (src/cuda/tail.cu:32-44).
+template<class F>
+__device__ __forceinline__ int ownerOf(int count, int block, F firstOf)
+{
+    int lo = 0, hi = count;
+    while (lo + 1 < hi) {
+        const int mid = (lo + hi) / 2;
+        if (firstOf(mid) <= block)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ int uniform(int x) { return __builtin_amdgcn_readfirstlane(x); }
+
+} // namespace
+
+// ====================================================================== render
+
+// Variable-pitch recurrence per voice chain: per-frame source phase
+// (src/cuda/render.cu:222-266). One lane per chain.
+extern "C" __global__ void clayPitchChains(const Span* __restrict__ spans, const SpanCold* __restrict__ cold,
+                                           const int* __restrict__ heads, const clay_control_point* __restrict__ points,
+                                           const Mod* __restrict__ mods, double* __restrict__ phase,
+                                           VoiceState* __restrict__ states, int segFirst, int segEnd, int count)
+{
+    const int track = blockIdx.x * blockDim.x + threadIdx.x;
+    if (track >= count)
+        return;
+    const int head = heads[track];
+    if (!spans[head].variablePitch)
+        return;
+    VoiceState& st = states[spans[head].filterTrack];  // one state per voice chain
+    double position = st.pitchInitialized ? st.position : spans[head].position;
+    int prevLoopEnd = st.previousLoopEnd, prevLoopStart = st.previousLoopStart;
+    for (int s = head; s >= 0; s = spans[s].next) {
+        const Span& v = spans[s];
+        if (v.end <= segFirst || v.begin >= segEnd)
+            continue;
+        const bool resumed = st.pitchInitialized && st.pitchSpan == s;
+        if (!resumed && !v.loopEnd && prevLoopEnd > 0 && position >= prevLoopEnd)
+            position = prevLoopStart + fmod(position - prevLoopStart, double(prevLoopEnd - prevLoopStart));
+        prevLoopEnd = v.loopEnd;
+        prevLoopStart = v.loopFirst;
+        const double slope = exp(v.pitchSlope);
+        double step = resumed ? st.step : v.step;
+        const clay_envelope_shape& pitchEnv = cold[s].extra[1];
+        const bool envActive = cold[s].active[1] != 0;
+        const int first = max(v.begin, segFirst), stop = min(v.end, segEnd);
+        for (int f = first; f < stop; ++f) {
+            phase[v.phase + uint64_t(f - first)] = position;
+            float cents = 0, depth = pitchEnv.depth;
+            for (int j = v.firstMod; j < v.firstMod + v.modCount; ++j) {
+                const Mod m = mods[j];
+                if (m.target == CLAY_MOD_PITCH)
+                    cents += modulationAt(m, points, f);
+                if (m.target == CLAY_MOD_PITCHEG_DEPTH)
+                    depth += modulationAt(m, points, f);
+            }
+            if (envActive)
+                cents += depth * envelopeShape(pitchEnv, f);
+            position += step * exp2f(fminf(24000.f, fmaxf(-24000.f, cents)) / 1200);
+            step *= slope;
+        }
+        st.pitchInitialized = 1;
+        st.pitchSpan = s;
+        st.position = position;
+        st.step = step;
+        st.previousLoopEnd = prevLoopEnd;
+        st.previousLoopStart = prevLoopStart;
+    }
+}
+
+// Voice audio of filtered / variable-pitch spans into scratch, times the job
+// gain, before pan for mono sources (src/cuda/render.cu:195-220).
+template<int T>
+__global__ void __launch_bounds__(kTileFrames) clayFilterSources(
+    const int16_t* __restrict__ bank, const float* __restrict__ sinc, const Span* __restrict__ spans,
+    const Job* __restrict__ jobs, const FilterTile* __restrict__ tiles, const clay_control_point* __restrict__ points,
+    const Mod* __restrict__ mods, const double* __restrict__ phase, float* __restrict__ filtered, int segFirst, int segEnd,
+    int firstTile)
+{
+    const FilterTile tile = tiles[firstTile + blockIdx.x];
+    const Span& v = spans[uniform(tile.span)];
+    const int frame = tile.frame + int(threadIdx.x);
+    if (frame >= v.end || frame < segFirst || frame >= segEnd)
+        return;
+    const Job& j = jobs[v.job];
+    const int local = frame - max(v.begin, segFirst);
+    const double position = v.variablePitch ? phase[v.phase + uint64_t(local)] : -1.0;
+    const float pan = controlAt(points, j.controls[2], j.controls[3], frame);
+    const float2 s = voiceAudio<T>(bank, sinc, v, points, mods, pan, frame, true, position);
+    const float g = jobGain(points, j, frame);
+    float* dst = filtered + v.filtered + uint64_t(local) * uint64_t(v.channels);
+    dst[0] = s.x * g;
+    if (v.channels == 2)
+        dst[1] = s.y * g;
+}
+
+// Serial filter recurrence per voice chain, in place on scratch
+// (src/cuda/render.cu:338-448). Coefficients are recomputed only when a
+// modulation input changes; equal inputs give equal coefficients, so this is
+// exact. State persists across segments in VoiceState.
+extern "C" __global__ void clayFilterChains(const Span* __restrict__ spans, const SpanCold* __restrict__ cold,
+                                            const int* __restrict__ heads, const clay_control_point* __restrict__ points,
+                                            const Mod* __restrict__ mods, float* __restrict__ filtered,
+                                            VoiceState* __restrict__ states, int segFirst, int segEnd, int count)
+{
+    const int track = blockIdx.x * blockDim.x + threadIdx.x;
+    if (track >= count)
+        return;
+    const int head = heads[track];
+    bool any = false;
+    for (int f = 0; f < CLAY_FILTER_SLOTS; ++f)
+        any |= cold[head].filters[f].type != CLAY_FILTER_NONE;
+    if (!any)
+        return;
+    VoiceState& st = states[spans[head].filterTrack];  // one state per voice chain
+    float z1[CLAY_FILTER_SLOTS][3][2], z2[CLAY_FILTER_SLOTS][3][2];
+    Coeff current[CLAY_FILTER_SLOTS];
+    bool initialized = st.filterInitialized != 0;
+    for (int f = 0; f < CLAY_FILTER_SLOTS; ++f) {
+        current[f] = {st.coefficients[f][0], st.coefficients[f][1], st.coefficients[f][2], st.coefficients[f][3],
+                      st.coefficients[f][4]};
+        for (int k = 0; k < 3; ++k)
+            for (int c = 0; c < 2; ++c) {
+                z1[f][k][c] = st.z1[f][k][c];
+                z2[f][k][c] = st.z2[f][k][c];
+            }
+    }
+    for (int s = head; s >= 0; s = spans[s].next) {
+        const Span& v = spans[s];
+        if (v.end <= segFirst || v.begin >= segEnd)
+            continue;
+        const SpanCold& cv = cold[s];
+        const int rate = cv.sampleRate;
+        const float amount = 1 - expf(-1000.f / float(rate));
+        int count[CLAY_FILTER_SLOTS];
+        for (int f = 0; f < CLAY_FILTER_SLOTS; ++f)
+            count[f] = sections(cv.filters[f].type);
+        bool moving = cv.active[0] != 0;
+        for (int j = v.firstMod; j < v.firstMod + v.modCount; ++j) {
+            const int t = mods[j].target;
+            moving |= (t >= CLAY_MOD_CUTOFF && t <= CLAY_MOD_RESONANCE2) || t == CLAY_MOD_FILEG_DEPTH ||
+                      t >= CLAY_MOD_EQ1_FREQUENCY;
+        }
+        // Inputs of the last target computation; targets are reused while they repeat.
+        float lastIn[CLAY_FILTER_SLOTS][3];
+        Coeff target[CLAY_FILTER_SLOTS];
+        bool haveTarget = false;
+        const int first = max(v.begin, segFirst), stop = min(v.end, segEnd);
+        float* x = filtered + v.filtered;
+        for (int fr = first; fr < stop; ++fr, x += v.channels) {
+            if (moving || !initialized) {
+                float cents[2] = {0, 0}, db[2] = {0, 0}, eq[3][3] = {};
+                float depth = cv.extra[0].depth;
+                for (int j = v.firstMod; j < v.firstMod + v.modCount; ++j) {
+                    const Mod m = mods[j];
+                    const int t = m.target;
+                    if (t == CLAY_MOD_CUTOFF) cents[0] += modulationAt(m, points, fr);
+                    if (t == CLAY_MOD_CUTOFF2) cents[1] += modulationAt(m, points, fr);
+                    if (t == CLAY_MOD_RESONANCE) db[0] += modulationAt(m, points, fr);
+                    if (t == CLAY_MOD_RESONANCE2) db[1] += modulationAt(m, points, fr);
+                    if (t == CLAY_MOD_FILEG_DEPTH) depth += modulationAt(m, points, fr);
+                    if (t >= CLAY_MOD_EQ1_FREQUENCY) {
+                        const int id = t - CLAY_MOD_EQ1_FREQUENCY;
+                        eq[id / 3][id % 3] += modulationAt(m, points, fr);
+                    }
+                }
+                if (cv.active[0])
+                    cents[0] += depth * envelopeShape(cv.extra[0], fr);
+                for (int f = 0; f < CLAY_FILTER_SLOTS; ++f) {
+                    if (!count[f])
+                        continue;
+                    const clay_voice_filter& flt = cv.filters[f];
+                    float in[3];
+                    if (f < 2) {
+                        in[0] = flt.cutoff * exp2f(cents[f] / 1200);
+                        in[1] = flt.resonance + db[f];
+                        in[2] = 0;
+                    } else {
+                        in[0] = flt.cutoff + eq[f - 2][0];
+                        in[1] = flt.bandwidth + eq[f - 2][1];
+                        in[2] = flt.gain + eq[f - 2][2];
+                    }
+                    if (!haveTarget || in[0] != lastIn[f][0] || in[1] != lastIn[f][1] || in[2] != lastIn[f][2]) {
+                        target[f] = f < 2 ? coefficients(flt.type, in[0], in[1], rate, 0, 0)
+                                          : coefficients(flt.type, in[0], 0, rate, in[1], in[2]);
+                        lastIn[f][0] = in[0];
+                        lastIn[f][1] = in[1];
+                        lastIn[f][2] = in[2];
+                    }
+                    if (initialized) {
+                        Coeff& c = current[f];
+                        c.b0 = fmaf(amount, target[f].b0 - c.b0, c.b0);
+                        c.b1 = fmaf(amount, target[f].b1 - c.b1, c.b1);
+                        c.b2 = fmaf(amount, target[f].b2 - c.b2, c.b2);
+                        c.a1 = fmaf(amount, target[f].a1 - c.a1, c.a1);
+                        c.a2 = fmaf(amount, target[f].a2 - c.a2, c.a2);
+                    } else {
+                        current[f] = target[f];
+                    }
+                }
+                haveTarget = true;
+                initialized = true;
+            }
+            float l = x[0], r = v.channels == 2 ? x[1] : 0.f;
+            for (int f = 0; f < CLAY_FILTER_SLOTS; ++f)
+                for (int k = 0; k < count[f]; ++k) {
+                    l = filterSample(l, current[f], z1[f][k][0], z2[f][k][0]);
+                    if (v.channels == 2)
+                        r = filterSample(r, current[f], z1[f][k][1], z2[f][k][1]);
+                }
+            x[0] = l;
+            if (v.channels == 2)
+                x[1] = r;
+        }
+    }
+    st.filterInitialized = initialized;
+    for (int f = 0; f < CLAY_FILTER_SLOTS; ++f) {
+        st.coefficients[f][0] = current[f].b0;
+        st.coefficients[f][1] = current[f].b1;
+        st.coefficients[f][2] = current[f].b2;
+        st.coefficients[f][3] = current[f].a1;
+        st.coefficients[f][4] = current[f].a2;
+        for (int k = 0; k < 3; ++k)
+            for (int c = 0; c < 2; ++c) {
+                st.z1[f][k][c] = z1[f][k][c];
+                st.z2[f][k][c] = z2[f][k][c];
+            }
+    }
+}
+
+// Per output frame: the dry voices of the tile rendered inline, times the job
+// gain, plus the filtered voices from scratch (pan applied for mono). Voices
+// are summed in span order, so every execution form yields the same bits
+// (src/cuda/render.cu:458-523). Natural writes float2, orig/custom short2.
+template<int T, bool FloatOut>
+__global__ void __launch_bounds__(kTileFrames) clayMixTiles(
+    const int16_t* __restrict__ bank, const float* __restrict__ sinc, const Span* __restrict__ spans,
+    const Job* __restrict__ jobs, const Tile* __restrict__ tiles, const int* __restrict__ voices,
+    const clay_control_point* __restrict__ points, const Mod* __restrict__ mods, const float* __restrict__ filtered,
+    void* __restrict__ out, int segFirst, int segEnd, int firstTile)
+{
+    const Tile tile = tiles[firstTile + blockIdx.x];
+    const int lane = int(threadIdx.x);
+    const int frame = tile.frame + lane;
+    if (lane >= tile.frames || frame < segFirst || frame >= segEnd)
+        return;
+    const Job& j = jobs[uniform(tile.job)];
+    const float pan = controlAt(points, j.controls[2], j.controls[3], frame);
+    float left = 0, right = 0, wetLeft = 0, wetRight = 0;
+    for (int i = tile.first; i < tile.first + tile.count; ++i) {
+        const Span& v = spans[uniform(voices[i])];
+        if (frame < v.begin || frame >= v.end)
+            continue;
+        if (v.filterTrack >= 0) {
+            const float* src = filtered + v.filtered + uint64_t(frame - max(v.begin, segFirst)) * uint64_t(v.channels);
+            float2 s = make_float2(src[0], v.channels == 1 ? src[0] : src[1]);
+            if (v.channels == 1) {
+                float offset = pan;
+                for (int k = v.firstMod; k < v.firstMod + v.modCount; ++k) {
+                    const Mod m = mods[k];
+                    if (m.target == CLAY_MOD_PAN)
+                        offset += modulationAt(m, points, frame);
+                }
+                const float2 g = panGains(v, offset);
+                s.x *= g.x;
+                s.y *= g.y;
+            }
+            wetLeft += s.x;
+            wetRight += s.y;
+        } else {
+            const float2 s = voiceAudio<T>(bank, sinc, v, points, mods, pan, frame, false, -1.0);
+            left += s.x;
+            right += s.y;
+        }
+    }
+    const float amp = jobGain(points, j, frame);
+    const float2 mixed = make_float2(fmaf(left, amp, wetLeft), fmaf(right, amp, wetRight));
+    const uint64_t at = j.mix + uint64_t(frame);
+    if (FloatOut) {
+        static_cast<float2*>(out)[at] = mixed;
+    } else {
+        const short l = short(__float2int_rn(fminf(32767.f, fmaxf(-32768.f, mixed.x * 32768.f))));
+        const short r = short(__float2int_rn(fminf(32767.f, fmaxf(-32768.f, mixed.y * 32768.f))));
+        static_cast<short2*>(out)[at] = make_short2(l, r);
+    }
+}
+
+#define CLAY_INSTANTIATE_TAPS(T)                                                                                       \
+    template __global__ void clayFilterSources<T>(const int16_t*, const float*, const Span*, const Job*,                \
+                                                  const FilterTile*, const clay_control_point*, const Mod*,              \
+                                                  const double*, float*, int, int, int);                                \
+    template __global__ void clayMixTiles<T, true>(const int16_t*, const float*, const Span*, const Job*, const Tile*, \
+                                                   const int*, const clay_control_point*, const Mod*, const float*,     \
+                                                   void*, int, int, int);                                               \
+    template __global__ void clayMixTiles<T, false>(const int16_t*, const float*, const Span*, const Job*,             \
+                                                    const Tile*, const int*, const clay_control_point*, const Mod*,     \
+                                                    const float*, void*, int, int, int);
+CLAY_INSTANTIATE_TAPS(16)
+CLAY_INSTANTIATE_TAPS(32)
+CLAY_INSTANTIATE_TAPS(48)
+CLAY_INSTANTIATE_TAPS(64)
+
+// ====================================================================== tail
+
+// Power and peak of each 1024-frame block of the float mix (src/cuda/tail.cu:54-81).
+extern "C" __global__ void clayScanStereo(const Job* __restrict__ jobs, int count, const float2* __restrict__ mix,
+                                          StereoStats* __restrict__ stats, int firstBlock)
+{
+    const int block = firstBlock + blockIdx.x;
+    const Job j = jobs[ownerOf(count, block, [&](int i) { return jobs[i].blockFirst; })];
+    const int first = (block - j.blockFirst) * kTailBlock;
+    const int end = min(first + kTailBlock, j.frames);
+    double power = 0;
+    float peak = 0;
+    for (int f = first + int(threadIdx.x); f < end; f += blockDim.x) {
+        const float2 x = mix[j.mix + uint64_t(f)];
+        if (!isfinite(x.x) || !isfinite(x.y)) {
+            peak = __int_as_float(0x7f800000);
+            continue;
+        }
+        peak = fmaxf(peak, fmaxf(fabsf(x.x), fabsf(x.y)));
+        power += double(__fmul_rn(x.x, x.x)) + double(__fmul_rn(x.y, x.y));
+    }
+    power = blockReduce(power, 0.0, Add{});
+    peak = blockReduce(peak, 0.f, Max{});
+    if (threadIdx.x == 0)
+        stats[block] = {power, peak, 0};
+}
+
+// Natural endpoint, silence-beyond-plan rule and peak repair, one lane per job
+// (src/cuda/tail.cu:83-145).
+extern "C" __global__ void claySelectNative(const Job* __restrict__ jobs, int count, const float2* __restrict__ mix,
+                                            const StereoStats* __restrict__ stats, clay_tail_result* __restrict__ results)
+{
+    const int index = blockIdx.x * blockDim.x + threadIdx.x;
+    if (index >= count)
+        return;
+    const Job j = jobs[index];
+    clay_tail_result r{};
+    float peak = 0;
+    int end = 0;
+    for (int first = 0; first < j.frames; first += kTailBlock) {
+        const StereoStats st = stats[j.blockFirst + first / kTailBlock];
+        if (!isfinite(st.peak)) {
+            r.status = CLAY_TAIL_NONFINITE;
+            results[index] = r;
+            return;
+        }
+        peak = fmaxf(peak, st.peak);
+        const int blockEnd = min(j.limit, first + kTailBlock);
+        if (blockEnd < j.musicEnd)
+            continue;
+        const double threshold = 1e-12 * (blockEnd - first) * 2;
+        double power = st.power;
+        // A serial re-sum decides near-ties in a fixed order.
+        if (fabs(power - threshold) <= threshold * 1e-12) {
+            power = 0;
+            for (int f = first; f < min(blockEnd, j.frames); ++f) {
+                const float2 x = mix[j.mix + uint64_t(f)];
+                power += __fmul_rn(x.x, x.x);
+                power += __fmul_rn(x.y, x.y);
+            }
+        }
+        if (power <= threshold) {
+            end = blockEnd;
+            break;
+        }
+    }
+    if (!end) {
+        const int silent = (j.frames + kTailBlock - 1) / kTailBlock * kTailBlock;
+        if (silent < j.limit)
+            end = min(j.limit, max(silent + kTailBlock, (j.musicEnd + kTailBlock - 1) / kTailBlock * kTailBlock));
+    }
+    if (!end) {
+        r.status = CLAY_TAIL_DURATION;
+    } else {
+        const double attenuation = peak >= 1 ? 20 * log10(double(peak)) + 1 : 0;
+        if (attenuation > 5 + 1e-6) {
+            r.status = CLAY_TAIL_PEAK;
+        } else {
+            r.frames = end;
+            r.data_frames = min(end, j.frames);
+            r.scale = float(32768 * pow(10., -attenuation / 20));
+        }
+    }
+    results[index] = r;
+}
+
+// Mono PCM16 (channel rounding then ties-to-even average) and 50 ms window
+// statistics (src/cuda/tail.cu:147-181).
+extern "C" __global__ void clayQuantizeMono(const Job* __restrict__ jobs, int count, const float2* __restrict__ mix,
+                                            const clay_tail_result* __restrict__ results, int16_t* __restrict__ mono,
+                                            MonoStats* __restrict__ stats, int window, int firstBlock)
+{
+    const int block = firstBlock + blockIdx.x;
+    const int index = ownerOf(count, block, [&](int i) { return jobs[i].windowFirst; });
+    const Job j = jobs[index];
+    const clay_tail_result r = results[index];
+    const int first = (block - j.windowFirst) * window;
+    const int end = min(first + window, r.data_frames);
+    if (first >= end)
+        return;
+    double power = 0;
+    int peak = 0;
+    for (int f = first + int(threadIdx.x); f < end; f += blockDim.x) {
+        const float2 x = mix[j.mix + uint64_t(f)];
+        const int l = __float2int_rn(fminf(32767.f, fmaxf(-32768.f, x.x * r.scale)));
+        const int rr = __float2int_rn(fminf(32767.f, fmaxf(-32768.f, x.y * r.scale)));
+        const int sum = l + rr;
+        const int v = (sum >> 1) + ((sum & 1) & ((sum >> 1) & 1));
+        mono[j.mix + uint64_t(f)] = int16_t(v);
+        power += double(v) * v;
+        peak = max(peak, abs(v));
+    }
+    power = blockReduce(power, 0.0, Add{});
+    peak = blockReduce(peak, 0, Max{});
+    if (threadIdx.x == 0)
+        stats[block] = {power, peak, 0};
+}
+
+// Weak-tail trim, one lane per job (src/cuda/tail.cu:183-213).
+extern "C" __global__ void claySelectTrim(const Job* __restrict__ jobs, int count, const MonoStats* __restrict__ stats,
+                                          clay_tail_result* __restrict__ results, TailConfig cfg)
+{
+    const int index = blockIdx.x * blockDim.x + threadIdx.x;
+    if (index >= count || results[index].status != CLAY_TAIL_OK)
+        return;
+    clay_tail_result& r = results[index];
+    const MonoStats* w = stats + jobs[index].windowFirst;
+    const int windows = (r.data_frames + cfg.window - 1) / cfg.window;
+    double maxPower = 0;
+    for (int i = 0; i < windows; ++i)
+        maxPower = fmax(maxPower, w[i].power);
+    const double threshold = fmin(cfg.ceiling, fmax(cfg.floor, sqrt(maxPower / cfg.window) * cfg.relative));
+    for (int i = windows - 1; i >= 0; --i) {
+        if (sqrt(w[i].power / cfg.window) > threshold || w[i].peak > threshold * 4) {
+            const int lastSound = min(r.frames, (i + 1) * cfg.window);
+            if (r.frames - lastSound >= cfg.quietFrames) {
+                r.frames = lastSound + cfg.padding;
+                r.data_frames = min(r.frames, r.data_frames);
+                r.fade = min(r.frames, cfg.fade);
+            }
+            break;
+        }
+    }
+}
+
+// Output offsets: exclusive prefix of dataFrames (src/cuda/tail.cu:216-238).
+// orig/custom results are initialised from the plan here.
+extern "C" __global__ void clayLayoutPcm(const Job* __restrict__ jobs, int count, clay_tail_result* __restrict__ results,
+                                         int fromPlan)
+{
+    if (threadIdx.x || blockIdx.x)
+        return;
+    uint64_t offset = 0;
+    for (int i = 0; i < count; ++i) {
+        clay_tail_result& r = results[i];
+        if (fromPlan) {
+            r = clay_tail_result{};
+            r.frames = jobs[i].frames;
+            r.data_frames = jobs[i].frames;
+            r.fade = jobs[i].fade;
+        }
+        r.offset = offset;
+        r.peak = 0;
+        r.first = r.frames;
+        r.last = -1;
+        offset += uint64_t(r.data_frames);
+    }
+}
+
+// Fade, pack, peak, audible extent and clip count per 1024-frame block
+// (src/cuda/tail.cu:240-297). `channels` is 1 (natural) or 2.
+extern "C" __global__ void clayFinalizePcm(const Job* __restrict__ jobs, int count, const int16_t* __restrict__ input,
+                                           int16_t* __restrict__ output, clay_tail_result* __restrict__ results,
+                                           int channels, int countClipped, int firstBlock)
+{
+    const int block = firstBlock + blockIdx.x;
+    const int index = ownerOf(count, block, [&](int i) { return jobs[i].blockFirst; });
+    const Job j = jobs[index];
+    const uint64_t outAt = results[index].offset;
+    const int frames = results[index].frames, fade = results[index].fade;
+    const int first = (block - j.blockFirst) * kTailBlock;
+    const int end = min(first + kTailBlock, results[index].data_frames);
+    if (first >= end)
+        return;
+    int peak = 0, audibleFirst = frames, audibleLast = -1, clipped = 0;
+    const int fadeStart = frames - fade;
+    constexpr double pi = 3.14159265358979323846;
+    for (int f = first + int(threadIdx.x); f < end; f += blockDim.x) {
+        int framePeak = 0;
+        for (int c = 0; c < channels; ++c) {
+            int v = input[(j.mix + uint64_t(f)) * uint64_t(channels) + uint64_t(c)];
+            if (f >= fadeStart) {
+                const int i = f - fadeStart;
+                const double gain = fade == 1 ? 0 : .5 * (1 + cos(pi * i / (fade - 1)));
+                v = int16_t(__double2int_rn(v * gain));
+            }
+            output[(outAt + uint64_t(f)) * uint64_t(channels) + uint64_t(c)] = int16_t(v);
+            framePeak = max(framePeak, abs(v));
+            clipped += abs(v) >= 32767;
+        }
+        peak = max(peak, framePeak);
+        if (framePeak > 3) {
+            audibleFirst = min(audibleFirst, f);
+            audibleLast = f;
+        }
+    }
+    peak = blockReduce(peak, 0, Max{});
+    audibleFirst = blockReduce(audibleFirst, frames, Min{});
+    audibleLast = blockReduce(audibleLast, -1, Max{});
+    if (countClipped) {
+        clipped = blockReduce(clipped, 0, Add{});
+        if (threadIdx.x == 0 && clipped)
+            atomicAdd(&results[index].audio.clipped, clipped);
+    }
+    if (threadIdx.x == 0 && audibleLast >= 0) {
+        atomicMax(&results[index].peak, peak);
+        atomicMin(&results[index].first, audibleFirst);
+        atomicMax(&results[index].last, audibleLast);
+    }
+}
+
+// ====================================================================== quality
+
+namespace {
+
+__device__ __forceinline__ unsigned long long monoPower(const int16_t* pcm, int channels, int frame)
+{
+    const int v = channels == 1 ? 2 * int(pcm[frame]) : int(pcm[2 * frame]) + int(pcm[2 * frame + 1]);
+    return (unsigned long long)((long long)v * v);
+}
+
+__device__ __forceinline__ int bodyBlockStart(int block, const clay_audio_config& a)
+{
+    const int chunk = 2 * a.rate, span = 5 * a.window;
+    const int windows = (chunk + span - 1) / span;
+    return (block / windows) * chunk + (block % windows) * span;
+}
+
+__device__ double levelOf(unsigned long long power, int window)
+{
+    return power ? fmax(-120., 10 * log10(double(power) / (window * 4294967296.))) : -120.;
+}
+
+} // namespace
+
+// Sum of (L+R)^2 over 250 ms body windows (src/cuda/quality.cu:21-45).
+extern "C" __global__ void clayMeasureBody(const AudioJobDesc* __restrict__ jobs, int count, const int16_t* __restrict__ pcm,
+                                           const clay_tail_result* __restrict__ results,
+                                           unsigned long long* __restrict__ powers, clay_audio_config cfg, int channels,
+                                           int firstBlock)
+{
+    const int block = firstBlock + blockIdx.x;
+    const int index = ownerOf(count, block, [&](int i) { return jobs[i].bodyFirst; });
+    const clay_tail_result& r = results[index];
+    if (r.status != CLAY_TAIL_OK)
+        return;
+    const AudioJobDesc j = jobs[index];
+    const int w = block - j.bodyFirst;
+    const int first = bodyBlockStart(w, cfg);
+    const int end = min(min(j.bodyFrames, r.data_frames), bodyBlockStart(w + 1, cfg));
+    const int16_t* audio = pcm + r.offset * uint64_t(channels);
+    unsigned long long power = 0;
+    for (int f = first + int(threadIdx.x); f < end; f += blockDim.x)
+        power += monoPower(audio, channels, f);
+    power = blockReduce(power, 0ull, Add{});
+    if (threadIdx.x == 0)
+        powers[block] = power;
+}
+
+// Loudest of five 50 ms windows after each onset; one wave per window
+// (src/cuda/quality.cu:47-77). Launched with 5 waves (320 threads).
+extern "C" __global__ void clayMeasureOnsets(const AudioJobDesc* __restrict__ jobs, int count, const int* __restrict__ onsets,
+                                             const int16_t* __restrict__ pcm, const clay_tail_result* __restrict__ results,
+                                             unsigned long long* __restrict__ powers, int window, int channels,
+                                             int firstBlock)
+{
+    const int block = firstBlock + blockIdx.x;
+    const int index = ownerOf(count, block, [&](int i) { return jobs[i].onsetFirst; });
+    const clay_tail_result& r = results[index];
+    if (r.status != CLAY_TAIL_OK)
+        return;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int first = onsets[block] + wave * window;
+    const int end = min(first + window, r.data_frames);
+    const int16_t* audio = pcm + r.offset * uint64_t(channels);
+    unsigned long long power = 0;
+    for (int f = first + lane; f < end; f += 64)
+        power += monoPower(audio, channels, f);
+    power = waveReduce(power, Add{});
+    __shared__ unsigned long long maxima[5];
+    if (lane == 0)
+        maxima[wave] = power;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long best = maxima[0];
+        for (int k = 1; k < 5; ++k)
+            best = best > maxima[k] ? best : maxima[k];
+        powers[block] = best;
+    }
+}
+
+// Body RMS, silence, onset percentiles and the verdict, one lane per job
+// (src/cuda/quality.cu:79-170). Onset powers are sorted in place.
+extern "C" __global__ void claySummarizeAudio(const AudioJobDesc* __restrict__ jobs, int count,
+                                              const unsigned long long* __restrict__ body,
+                                              unsigned long long* __restrict__ onsets,
+                                              clay_tail_result* __restrict__ results, clay_audio_config cfg, int channels)
+{
+    const int index = blockIdx.x * blockDim.x + threadIdx.x;
+    if (index >= count || results[index].status != CLAY_TAIL_OK)
+        return;
+    const AudioJobDesc j = jobs[index];
+    clay_tail_result& r = results[index];
+    clay_audio_metrics& st = r.audio;
+    const int end = min(r.frames, j.bodyFrames);
+    unsigned long long total = 0;
+    int silent = 0, run = 0;
+    for (int w = 0, first = 0; first < end; ++w) {
+        const int next = min(end, bodyBlockStart(w + 1, cfg));
+        const unsigned long long wp = body[j.bodyFirst + w];
+        total += wp;
+        if (cfg.enabled) {
+            if (double(wp) < (next - first) * cfg.silence_power) {
+                silent += next - first;
+                run += next - first;
+                st.longest_silence = max(run, st.longest_silence);
+            } else {
+                run = 0;
+            }
+        }
+        first = next;
+    }
+    st.body_rms = j.bodyFrames ? sqrt(double(total) / j.bodyFrames) * .5 : 0;
+    if (!cfg.enabled)
+        return;
+    st.measured = 1;
+    st.median = st.quartile = -120;
+    st.silent_fraction = double(silent) / max(1, end);
+    st.clipped_fraction = double(st.clipped) / (double(max(1, r.frames)) * channels);
+    if (j.onsetCount) {
+        unsigned long long* v = onsets + j.onsetFirst;
+        const int n = j.onsetCount;
+        // Heap sort, ascending.
+        const auto sift = [&](int root, int size) {
+            const unsigned long long value = v[root];
+            for (int child = 2 * root + 1; child < size; child = 2 * root + 1) {
+                if (child + 1 < size && v[child] < v[child + 1])
+                    ++child;
+                if (value >= v[child])
+                    break;
+                v[root] = v[child];
+                root = child;
+            }
+            v[root] = value;
+        };
+        for (int root = n / 2; root-- > 0;)
+            sift(root, n);
+        for (int last = n - 1; last > 0; --last) {
+            const unsigned long long top = v[0];
+            v[0] = v[last];
+            v[last] = top;
+            sift(0, last);
+        }
+        const auto percentile = [&](double fraction) {
+            const double position = (n - 1) * fraction;
+            const int lo = int(position);
+            const double lower = levelOf(v[lo], cfg.window);
+            return lo + 1 < n ? lower + (levelOf(v[lo + 1], cfg.window) - lower) * (position - lo) : lower;
+        };
+        st.median = percentile(.5);
+        st.quartile = percentile(.25);
+    }
+    if (r.frames > cfg.max_frames) r.status = CLAY_TAIL_AUDIO_DURATION;
+    else if (!j.onsetCount || st.median < cfg.min_median || st.quartile < cfg.min_quartile) r.status = CLAY_TAIL_QUIET_PERFORMANCE;
+    else if (st.silent_fraction > cfg.max_silent_fraction || st.longest_silence > cfg.max_silent_frames) r.status = CLAY_TAIL_EXCESSIVE_SILENCE;
+    else if (r.peak < cfg.min_peak) r.status = CLAY_TAIL_SILENT_AUDIO;
+    else if (st.clipped_fraction > cfg.max_clipped_fraction) r.status = CLAY_TAIL_CLIPPING;
+    st.verdict = r.status;
+}
+
+// Shared gain per volume group (src/cuda/quality.cu:172-209). `groups` holds
+// [first, end) batch job ranges.
+extern "C" __global__ void clayMatchVolume(const int2* __restrict__ groups, int count, int minimum,
+                                           clay_tail_result* __restrict__ results)
+{
+    const int index = blockIdx.x * blockDim.x + threadIdx.x;
+    if (index >= count)
+        return;
+    const int2 g = groups[index];
+    double desired = 0, safe = 1e30;
+    int accepted = 0;
+    for (int i = g.x; i < g.y; ++i) {
+        clay_tail_result& r = results[i];
+        if (r.status != CLAY_TAIL_OK)
+            continue;
+        const double rms = r.audio.body_rms;
+        if (rms <= 0 || r.last < 0) {
+            r.status = CLAY_TAIL_SILENT_BODY;
+            continue;
+        }
+        desired += rms;
+        safe = fmin(safe, 32767 * .8912509381337456 * rms / max(1, r.peak));
+        ++accepted;
+    }
+    const double target = fmin(desired / max(1, accepted), safe);
+    for (int i = g.x; i < g.y; ++i) {
+        clay_tail_result& r = results[i];
+        if (r.status != CLAY_TAIL_OK)
+            continue;
+        if (accepted < minimum) {
+            r.status = CLAY_TAIL_INCOMPLETE_GROUP;
+        } else {
+            r.audio.target_rms = target;
+            r.audio.gain = target / r.audio.body_rms;
+        }
+    }
+}
+
+// Compaction offsets of accepted jobs (src/cuda/quality.cu:211-232).
+extern "C" __global__ void clayLayoutAccepted(const Job* __restrict__ jobs, int count, clay_tail_result* __restrict__ results,
+                                              PcmJob* __restrict__ pcm)
+{
+    if (threadIdx.x || blockIdx.x)
+        return;
+    uint64_t offset = 0;
+    for (int i = 0; i < count; ++i) {
+        clay_tail_result& r = results[i];
+        pcm[i] = {r.offset, jobs[i].blockFirst, 0};
+        if (r.status != CLAY_TAIL_OK || r.last < 0)
+            r.data_frames = 0;
+        r.offset = offset;
+        offset += uint64_t(r.data_frames);
+        if (r.audio.gain > 0) {
+            r.peak = 0;
+            r.first = r.frames;
+            r.last = -1;
+        }
+    }
+}
+
+// Group gain applied while packing accepted PCM (src/cuda/quality.cu:234-280).
+extern "C" __global__ void clayPackAccepted(const PcmJob* __restrict__ jobs, int count, const int16_t* __restrict__ source,
+                                            int16_t* __restrict__ output, clay_tail_result* __restrict__ results,
+                                            int channels, int firstBlock)
+{
+    const int block = firstBlock + blockIdx.x;
+    const int index = ownerOf(count, block, [&](int i) { return jobs[i].blockFirst; });
+    const PcmJob j = jobs[index];
+    const clay_tail_result& r = results[index];
+    const int first = (block - j.blockFirst) * kTailBlock;
+    const int end = min(first + kTailBlock, r.data_frames);
+    if (first >= end)
+        return;
+    const double gain = r.audio.gain;
+    const int16_t* in = source + j.sourceOffset * uint64_t(channels);
+    int16_t* out = output + r.offset * uint64_t(channels);
+    int peak = 0, audibleFirst = r.frames, audibleLast = -1;
+    for (int f = first + int(threadIdx.x); f < end; f += blockDim.x) {
+        int framePeak = 0;
+        for (int c = 0; c < channels; ++c) {
+            const int i = f * channels + c;
+            const int16_t v = gain > 0 ? int16_t(__double2int_rn(in[i] * gain)) : in[i];
+            out[i] = v;
+            framePeak = max(framePeak, abs(int(v)));
+        }
+        peak = max(peak, framePeak);
+        if (framePeak > 3) {
+            audibleFirst = min(audibleFirst, f);
+            audibleLast = f;
+        }
+    }
+    if (gain > 0) {
+        peak = blockReduce(peak, 0, Max{});
+        audibleFirst = blockReduce(audibleFirst, r.frames, Min{});
+        audibleLast = blockReduce(audibleLast, -1, Max{});
+        if (threadIdx.x == 0) {
+            atomicMax(&results[index].peak, peak);
+            if (audibleLast >= 0) {
+                atomicMin(&results[index].first, audibleFirst);
+                atomicMax(&results[index].last, audibleLast);
+            }
+        }
+    }
+}

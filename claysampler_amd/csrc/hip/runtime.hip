@@ -1,0 +1,1057 @@
+// Host runtime of the render core: the C ABI of include/clay_hip.h.
+//
+// Replaces the reference renderer (src/cuda/renderer.cpp). A window of plans
+// is split into batches; each batch is flattened into device records
+// (the role of BatchIndex::append, renderer.cpp:378-462), uploaded once into
+// one device arena, rendered by the kernels of kernels.hip on one HIP stream,
+// finished on the device (tail, quality, group gain, packing) and downloaded
+// as packed PCM16. MI355X's 288 GB of HBM lets a batch keep its whole filter
+// scratch resident, so a batch normally renders as one segment.
+#include <hip/hip_runtime.h>
+
+#include "device_types.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+using namespace clay_dev;
+
+// Kernels (kernels.hip).
+extern "C" __global__ void clayPitchChains(const Span*, const SpanCold*, const int*, const clay_control_point*,
+                                           const Mod*, double*, VoiceState*, int, int, int);
+extern "C" __global__ void clayFilterChains(const Span*, const SpanCold*, const int*, const clay_control_point*,
+                                            const Mod*, float*, VoiceState*, int, int, int);
+template<int T>
+__global__ void clayFilterSources(const int16_t*, const float*, const Span*, const Job*, const FilterTile*,
+                                  const clay_control_point*, const Mod*, const double*, float*, int, int, int);
+template<int T, bool FloatOut>
+__global__ void clayMixTiles(const int16_t*, const float*, const Span*, const Job*, const Tile*, const int*,
+                             const clay_control_point*, const Mod*, const float*, void*, int, int, int);
+extern "C" __global__ void clayScanStereo(const Job*, int, const float2*, StereoStats*, int);
+extern "C" __global__ void claySelectNative(const Job*, int, const float2*, const StereoStats*, clay_tail_result*);
+extern "C" __global__ void clayQuantizeMono(const Job*, int, const float2*, const clay_tail_result*, int16_t*,
+                                            MonoStats*, int, int);
+extern "C" __global__ void claySelectTrim(const Job*, int, const MonoStats*, clay_tail_result*, TailConfig);
+extern "C" __global__ void clayLayoutPcm(const Job*, int, clay_tail_result*, int);
+extern "C" __global__ void clayFinalizePcm(const Job*, int, const int16_t*, int16_t*, clay_tail_result*, int, int, int);
+extern "C" __global__ void clayMeasureBody(const AudioJobDesc*, int, const int16_t*, const clay_tail_result*,
+                                           unsigned long long*, clay_audio_config, int, int);
+extern "C" __global__ void clayMeasureOnsets(const AudioJobDesc*, int, const int*, const int16_t*,
+                                             const clay_tail_result*, unsigned long long*, int, int, int);
+extern "C" __global__ void claySummarizeAudio(const AudioJobDesc*, int, const unsigned long long*,
+                                              unsigned long long*, clay_tail_result*, clay_audio_config, int);
+extern "C" __global__ void clayMatchVolume(const int2*, int, int, clay_tail_result*);
+extern "C" __global__ void clayLayoutAccepted(const Job*, int, clay_tail_result*, PcmJob*);
+extern "C" __global__ void clayPackAccepted(const PcmJob*, int, const int16_t*, int16_t*, clay_tail_result*, int, int);
+
+namespace {
+
+struct Failure : std::runtime_error {
+    int code;
+    Failure(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+void check(hipError_t e, const char* what)
+{
+    if (e == hipErrorOutOfMemory)
+        throw Failure(CLAY_E_MEMORY, std::string(what) + ": out of memory");
+    if (e != hipSuccess)
+        throw Failure(CLAY_E_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+thread_local std::string g_createError;
+
+template<class T>
+size_t bytesOf(const std::vector<T>& v) { return v.size() * sizeof(T); }
+
+uint64_t ceilDiv(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
+
+// std::vector of trivially copyable records -> device, in one arena.
+struct ArenaBuilder {
+    size_t bytes = 0;
+    size_t add(size_t n)
+    {
+        const size_t at = bytes;
+        bytes += (n + 255) & ~size_t(255);
+        return at;
+    }
+};
+
+// Sinc table, tap-major as src/cuda/renderer.cpp:347-376 generalised to `taps`.
+std::vector<float> sincTable(int taps)
+{
+    const double beta = 8.5, pi = 3.14159265358979323846;
+    const auto bessel = [](double v) {
+        double sum = 1, term = 1;
+        for (int i = 1; i < 30; ++i) {
+            term *= v * v / (4 * i * i);
+            sum += term;
+            if (term < 1e-14)
+                break;
+        }
+        return sum;
+    };
+    const double norm = bessel(beta);
+    const int centre = taps / 2 - 1;
+    const double half = taps / 2;
+    std::vector<float> t(size_t(taps) * (kPhases + 1));
+    for (int tap = 0; tap < taps; ++tap)
+        for (int p = 0; p <= kPhases; ++p) {
+            const double x = tap - centre - double(p) / kPhases;
+            const double f = x / half;
+            const double window = bessel(beta * std::sqrt(1 - f * f)) / norm;
+            t[size_t(tap) * (kPhases + 1) + size_t(p)] = float(window * (x == 0 ? 1 : std::sin(pi * x) / (pi * x)));
+        }
+    return t;
+}
+
+struct Timer {
+    std::string name;
+    hipEvent_t begin, end;
+};
+
+} // namespace
+
+struct clay_hip_batch {
+    // Host records.
+    std::vector<Span> spans;
+    std::vector<SpanCold> cold;
+    std::vector<Mod> mods;
+    std::vector<clay_control_point> points;
+    std::vector<Job> jobs;
+    std::vector<Tile> tiles;
+    std::vector<int> voices;
+    std::vector<FilterTile> filterTiles;
+    std::vector<int> pitchHeads, filterHeads;
+    std::vector<AudioJobDesc> audioJobs;
+    std::vector<int> onsets;
+    std::vector<int2> groups;
+    uint64_t frames = 0, filteredSamples = 0, pitchFrames = 0;
+    int blocks = 0, windows = 0, bodyBlocks = 0, chains = 0;
+    bool assess = false;
+    // Device arena and views into it.
+    void* arena = nullptr;
+    size_t arenaBytes = 0;
+    Span* dSpans = nullptr;
+    SpanCold* dCold = nullptr;
+    Mod* dMods = nullptr;
+    clay_control_point* dPoints = nullptr;
+    Job* dJobs = nullptr;
+    Tile* dTiles = nullptr;
+    int* dVoices = nullptr;
+    FilterTile* dFilterTiles = nullptr;
+    int *dPitchHeads = nullptr, *dFilterHeads = nullptr;
+    VoiceState* dStates = nullptr;
+    AudioJobDesc* dAudioJobs = nullptr;
+    int* dOnsets = nullptr;
+    int2* dGroups = nullptr;
+    void* dMix = nullptr;
+    float* dFiltered = nullptr;
+    double* dPhase = nullptr;
+    int16_t* dMono = nullptr;
+    StereoStats* dStereo = nullptr;
+    MonoStats* dWindows = nullptr;
+    unsigned long long *dBody = nullptr, *dOnsetPower = nullptr;
+    PcmJob* dPcmJobs = nullptr;
+    int16_t* dOutput = nullptr;   // packed PCM of the batch
+    clay_tail_result* dResults = nullptr;
+    // Per-kernel timing of the last run.
+    std::map<std::string, std::pair<double, int>> kernelTimes;
+};
+
+struct clay_hip {
+    clay_hip_config cfg{};
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipDeviceProp_t props{};
+    float* sinc = nullptr;       // phase-major taps x (phases+1)
+    int16_t* bank = nullptr;
+    uint64_t bankCount = 0, bankCapacity = 0;
+    std::string error;
+    clay_hip_stats stats{};
+    bool natural = true;
+    TailConfig tail{};
+    bool timing = false;
+    std::vector<Timer> timers;
+    std::vector<hipEvent_t> eventPool;
+    // Pinned PCM slots handed to publish callbacks.
+    struct Slot {
+        void* host = nullptr;
+        size_t capacity = 0;
+        bool busy = false;
+    };
+    std::vector<Slot> slots;
+    size_t residentBytes = 0;
+
+    int channels() const { return natural ? 1 : 2; }
+
+    hipEvent_t event()
+    {
+        if (!eventPool.empty()) {
+            auto e = eventPool.back();
+            eventPool.pop_back();
+            return e;
+        }
+        hipEvent_t e;
+        check(hipEventCreate(&e), "hipEventCreate");
+        return e;
+    }
+
+    // Launch wrapper: chunked grids plus optional per-launch events.
+    template<class F>
+    void launch(const char* name, F&& body)
+    {
+        Timer t{name, nullptr, nullptr};
+        if (timing) {
+            t.begin = event();
+            t.end = event();
+            check(hipEventRecord(t.begin, stream), "hipEventRecord");
+        }
+        body();
+        check(hipGetLastError(), name);
+        ++stats.launches;
+        if (timing) {
+            check(hipEventRecord(t.end, stream), "hipEventRecord");
+            timers.push_back(t);
+        }
+    }
+
+    void collectTimes(std::map<std::string, std::pair<double, int>>& out)
+    {
+        for (auto& t : timers) {
+            float ms = 0;
+            check(hipEventElapsedTime(&ms, t.begin, t.end), "hipEventElapsedTime");
+            auto& slot = out[t.name];
+            slot.first += ms;
+            slot.second += 1;
+            eventPool.push_back(t.begin);
+            eventPool.push_back(t.end);
+        }
+        timers.clear();
+    }
+};
+
+namespace {
+
+void validatePlan(const clay_hip* h, const clay_plan_view& p, size_t index)
+{
+    const auto bad = [&](const std::string& what) {
+        throw Failure(CLAY_E_INVALID, "plan " + std::to_string(index) + ": " + what);
+    };
+    if (p.frames < 0 || p.span_count < 0 || p.track_count < 3 || !p.track_offsets || p.modulation_count < 0 ||
+        p.onset_count < 0 || p.filter_tracks < 0)
+        bad("negative count or fewer than three control tracks");
+    if ((p.span_count && !p.spans) || (p.modulation_count && !p.modulation) || (p.onset_count && !p.onsets))
+        bad("missing array");
+    if (p.track_offsets[0] != 0)
+        bad("track offsets must start at 0");
+    for (int t = 0; t < p.track_count; ++t)
+        if (p.track_offsets[t + 1] <= p.track_offsets[t])
+            bad("empty or unordered control track");
+    if (p.track_offsets[p.track_count] && !p.points)
+        bad("missing control points");
+    for (int m = 0; m < p.modulation_count; ++m) {
+        const auto& b = p.modulation[m];
+        if (b.track >= p.track_count || b.shape_track >= p.track_count || b.target < 0 || b.target > CLAY_MOD_EQ3_GAIN)
+            bad("modulation binding out of range");
+    }
+    for (int s = 0; s < p.span_count; ++s) {
+        const auto& v = p.spans[s];
+        if (v.begin < 0 || v.end <= v.begin || v.end > p.frames)
+            bad("span outside the job");
+        if (v.channels != 1 && v.channels != 2)
+            bad("span channels must be 1 or 2");
+        if (v.first_modulation < 0 || v.modulation_count < 0 || v.first_modulation + v.modulation_count > p.modulation_count)
+            bad("span modulation range");
+        if (v.filter_track >= p.filter_tracks)
+            bad("span filter track");
+        if (v.variable_pitch && v.filter_track < 0)
+            bad("variable pitch span without a voice chain");
+        if (v.generator < CLAY_GEN_NONE || v.generator > CLAY_GEN_NOISE)
+            bad("span generator");
+        if (v.generator == CLAY_GEN_NONE) {
+            if (v.source_end < 0 || v.sample + uint64_t(v.source_end) * uint64_t(v.channels) > h->bankCount)
+                bad("span source outside the sample bank");
+            if (v.loop_end > 0 && (v.loop_first < 0 || v.loop_first >= v.loop_end || v.loop_end > v.source_end))
+                bad("span loop");
+        }
+        for (const auto& f : v.filters)
+            if (f.type < CLAY_FILTER_NONE || f.type > CLAY_FILTER_PEAK)
+                bad("filter type");
+        if (v.filter_track >= 0 && v.sample_rate <= 0)
+            bad("filter sample rate");
+    }
+}
+
+// Flatten plans into device records (renderer.cpp:378-462 semantics).
+void flatten(clay_hip* h, const clay_plan_view* plans, size_t count, clay_hip_batch& b)
+{
+    const bool assess = h->cfg.audio.enabled || h->cfg.group_minimum;
+    b.assess = assess;
+    const int window = h->tail.window;
+    const auto bodyWindows = [&](int frames) {
+        const int chunk = 2 * h->cfg.audio.rate, span = 5 * h->cfg.audio.window;
+        return (frames / chunk) * int(ceilDiv(uint64_t(chunk), uint64_t(span))) + int(ceilDiv(uint64_t(frames % chunk), uint64_t(span)));
+    };
+    for (size_t i = 0; i < count; ++i) {
+        const auto& p = plans[i];
+        validatePlan(h, p, i);
+        const int jobIndex = int(b.jobs.size());
+        // Controls and modulation bindings with absolute ranges.
+        const int pointBase = int(b.points.size());
+        b.points.insert(b.points.end(), p.points, p.points + p.track_offsets[p.track_count]);
+        const auto trackFirst = [&](int t) { return pointBase + p.track_offsets[t]; };
+        const int modBase = int(b.mods.size());
+        for (int m = 0; m < p.modulation_count; ++m) {
+            const auto& src = p.modulation[m];
+            Mod d{-1, 0, src.target, src.depth, -1, 0};
+            if (src.track >= 0) {
+                d.first = trackFirst(src.track);
+                d.last = trackFirst(src.track + 1);
+            }
+            if (src.shape_track >= 0) {
+                d.shapeFirst = trackFirst(src.shape_track);
+                d.shapeLast = trackFirst(src.shape_track + 1);
+            }
+            b.mods.push_back(d);
+        }
+        Job j{};
+        j.mix = b.frames;
+        j.frames = p.frames;
+        for (int t = 0; t < 4; ++t)
+            j.controls[t] = trackFirst(t);
+        j.musicEnd = p.music_end;
+        j.limit = p.limit;
+        j.fade = p.fade_frames;
+        j.blockFirst = b.blocks;
+        j.windowFirst = b.windows;
+        const int body = h->cfg.audio.enabled || p.volume_group ? p.body_frames : 0;
+        j.bodyFrames = body;
+        j.bodyFirst = b.bodyBlocks;
+        j.onsetFirst = int(b.onsets.size());
+        j.onsetCount = assess && h->cfg.audio.enabled ? p.onset_count : 0;
+        b.jobs.push_back(j);
+        if (assess) {
+            b.audioJobs.push_back({body, b.bodyBlocks, j.onsetFirst, j.onsetCount});
+            if (h->cfg.audio.enabled)
+                b.onsets.insert(b.onsets.end(), p.onsets, p.onsets + p.onset_count);
+            b.bodyBlocks += bodyWindows(body);
+        }
+        b.blocks += int(ceilDiv(uint64_t(p.frames), kTailBlock));
+        b.windows += h->natural ? int(ceilDiv(uint64_t(p.frames), uint64_t(window))) : 0;
+        // Tiles and their voice lists (difference array over tile indices).
+        const int tileCount = int(ceilDiv(uint64_t(p.frames), kTileFrames));
+        std::vector<int> perTile(size_t(tileCount) + 1, 0);
+        for (int s = 0; s < p.span_count; ++s) {
+            ++perTile[size_t(p.spans[s].begin / kTileFrames)];
+            --perTile[size_t((p.spans[s].end - 1) / kTileFrames + 1)];
+        }
+        for (int t = 1; t <= tileCount; ++t)
+            perTile[size_t(t)] += perTile[size_t(t - 1)];
+        std::vector<int> cursor(static_cast<size_t>(tileCount), 0);
+        int at = int(b.voices.size());
+        for (int t = 0; t < tileCount; ++t) {
+            const int frame = t * kTileFrames;
+            b.tiles.push_back({jobIndex, frame, std::min(kTileFrames, p.frames - frame), at, perTile[size_t(t)]});
+            cursor[size_t(t)] = at;
+            at += perTile[size_t(t)];
+        }
+        b.voices.resize(size_t(at));
+        // Spans, voice chains, scratch offsets.
+        std::vector<int> chainTail(size_t(p.filter_tracks), -1);
+        const int chainBase = b.chains;
+        b.chains += p.filter_tracks;
+        for (int s = 0; s < p.span_count; ++s) {
+            const auto& v = p.spans[s];
+            const int index = int(b.spans.size());
+            for (int t = v.begin / kTileFrames; t <= (v.end - 1) / kTileFrames; ++t)
+                b.voices[size_t(cursor[size_t(t)]++)] = index;
+            Span d{};
+            d.sample = v.sample;
+            d.position = v.position;
+            d.step = v.step;
+            d.pitchSlope = v.pitch_slope;
+            d.begin = v.begin;
+            d.end = v.end;
+            d.sourceEnd = v.source_end;
+            d.loopFirst = v.loop_first;
+            d.loopEnd = v.loop_end;
+            d.loopCrossfade = v.loop_crossfade;
+            d.channels = v.channels;
+            d.generator = v.generator;
+            d.stereoMatrix = v.stereo_matrix;
+            d.variablePitch = v.variable_pitch;
+            d.noiseSeed = v.noise_seed;
+            d.noiseStart = v.noise_start;
+            d.firstMod = v.first_modulation + modBase;
+            d.modCount = v.modulation_count;
+            d.filterTrack = v.filter_track >= 0 ? chainBase + v.filter_track : -1;
+            d.job = jobIndex;
+            d.next = -1;
+            d.attackStart = v.attack_start;
+            d.holdStart = v.hold_start;
+            d.decayStart = v.decay_start;
+            d.releaseStart = v.release_start;
+            d.fadeStart = v.fade_start;
+            d.start = v.start;
+            d.attackStep = v.attack_step;
+            d.decayRate = v.decay_rate;
+            d.sustain = v.sustain;
+            d.releaseValue = v.release_value;
+            d.releaseRate = v.release_rate;
+            d.fadeValue = v.fade_value;
+            d.fadeStep = v.fade_step;
+            d.gain = v.gain;
+            d.pan = v.pan;
+            d.left = v.left;
+            d.right = v.right;
+            d.widthDirect = v.width_direct;
+            d.widthCross = v.width_cross;
+            d.positionLeft = v.position_left;
+            d.positionRight = v.position_right;
+            d.width = v.width;
+            d.stereoPosition = v.stereo_position;
+            SpanCold c{};
+            std::memcpy(c.filters, v.filters, sizeof c.filters);
+            std::memcpy(c.extra, v.extra_envelope, sizeof c.extra);
+            c.active[0] = v.active_envelope[0];
+            c.active[1] = v.active_envelope[1];
+            c.sampleRate = v.sample_rate;
+            if (v.filter_track >= 0) {
+                int& tail = chainTail[size_t(v.filter_track)];
+                if (tail < 0) {
+                    if (v.variable_pitch)
+                        b.pitchHeads.push_back(index);
+                    bool filters = false;
+                    for (const auto& f : v.filters)
+                        filters |= f.type != CLAY_FILTER_NONE;
+                    if (filters)
+                        b.filterHeads.push_back(index);
+                } else {
+                    b.spans[size_t(tail)].next = index;
+                }
+                tail = index;
+                d.filtered = b.filteredSamples;
+                b.filteredSamples += uint64_t(v.end - v.begin) * uint64_t(v.channels);
+                if (v.variable_pitch) {
+                    d.phase = b.pitchFrames;
+                    b.pitchFrames += uint64_t(v.end - v.begin);
+                }
+                for (int f = v.begin; f < v.end; f += kTileFrames)
+                    b.filterTiles.push_back({index, f});
+            }
+            b.spans.push_back(d);
+            b.cold.push_back(c);
+        }
+        b.frames += uint64_t(p.frames);
+    }
+    // Volume groups: consecutive jobs sharing a nonzero id.
+    if (h->cfg.group_minimum)
+        for (size_t first = 0, end; first < count; first = end) {
+            end = first + 1;
+            if (!plans[first].volume_group)
+                continue;
+            while (end < count && plans[end].volume_group == plans[first].volume_group)
+                ++end;
+            b.groups.push_back(make_int2(int(first), int(end)));
+        }
+}
+
+void release(clay_hip_batch& b)
+{
+    if (b.arena)
+        (void)hipFree(b.arena);
+    b.arena = nullptr;
+}
+
+// One device arena for all records, scratch and finishing buffers.
+void upload(clay_hip* h, clay_hip_batch& b)
+{
+    ArenaBuilder a;
+    const int ch = h->channels();
+    const size_t oSpans = a.add(bytesOf(b.spans)), oCold = a.add(bytesOf(b.cold)), oMods = a.add(bytesOf(b.mods));
+    const size_t oPoints = a.add(bytesOf(b.points)), oJobs = a.add(bytesOf(b.jobs)), oTiles = a.add(bytesOf(b.tiles));
+    const size_t oVoices = a.add(bytesOf(b.voices)), oFilterTiles = a.add(bytesOf(b.filterTiles));
+    const size_t oPitch = a.add(bytesOf(b.pitchHeads)), oFilter = a.add(bytesOf(b.filterHeads));
+    const size_t oStates = a.add(size_t(b.chains) * sizeof(VoiceState));
+    const size_t oAudio = a.add(bytesOf(b.audioJobs)), oOnsets = a.add(bytesOf(b.onsets)), oGroups = a.add(bytesOf(b.groups));
+    const size_t oMix = a.add(size_t(b.frames) * (h->natural ? sizeof(float2) : sizeof(short2)));
+    const size_t oFiltered = a.add(size_t(b.filteredSamples) * sizeof(float));
+    const size_t oPhase = a.add(size_t(b.pitchFrames) * sizeof(double));
+    const size_t oMono = a.add(h->natural ? size_t(b.frames) * sizeof(int16_t) : 0);
+    const size_t oStereo = a.add(h->natural ? size_t(b.blocks) * sizeof(StereoStats) : 0);
+    const size_t oWindows = a.add(size_t(b.windows) * sizeof(MonoStats));
+    const size_t oBody = a.add(size_t(b.bodyBlocks) * sizeof(unsigned long long));
+    const size_t oOnsetPower = a.add(b.onsets.size() * sizeof(unsigned long long));
+    const size_t oPcmJobs = a.add(b.assess ? b.jobs.size() * sizeof(PcmJob) : 0);
+    const size_t oOutput = a.add(b.assess ? size_t(b.frames) * size_t(ch) * sizeof(int16_t) : 0);
+    const size_t oResults = a.add(b.jobs.size() * sizeof(clay_tail_result));
+    check(hipMalloc(&b.arena, a.bytes), "batch arena");
+    b.arenaBytes = a.bytes;
+    auto* base = static_cast<char*>(b.arena);
+    const auto put = [&](size_t offset, const auto& v) {
+        if (!v.empty())
+            check(hipMemcpyAsync(base + offset, v.data(), bytesOf(v), hipMemcpyHostToDevice, h->stream), "batch upload");
+        return base + offset;
+    };
+    b.dSpans = reinterpret_cast<Span*>(put(oSpans, b.spans));
+    b.dCold = reinterpret_cast<SpanCold*>(put(oCold, b.cold));
+    b.dMods = reinterpret_cast<Mod*>(put(oMods, b.mods));
+    b.dPoints = reinterpret_cast<clay_control_point*>(put(oPoints, b.points));
+    b.dJobs = reinterpret_cast<Job*>(put(oJobs, b.jobs));
+    b.dTiles = reinterpret_cast<Tile*>(put(oTiles, b.tiles));
+    b.dVoices = reinterpret_cast<int*>(put(oVoices, b.voices));
+    b.dFilterTiles = reinterpret_cast<FilterTile*>(put(oFilterTiles, b.filterTiles));
+    b.dPitchHeads = reinterpret_cast<int*>(put(oPitch, b.pitchHeads));
+    b.dFilterHeads = reinterpret_cast<int*>(put(oFilter, b.filterHeads));
+    b.dStates = reinterpret_cast<VoiceState*>(base + oStates);
+    b.dAudioJobs = reinterpret_cast<AudioJobDesc*>(put(oAudio, b.audioJobs));
+    b.dOnsets = reinterpret_cast<int*>(put(oOnsets, b.onsets));
+    b.dGroups = reinterpret_cast<int2*>(put(oGroups, b.groups));
+    b.dMix = base + oMix;
+    b.dFiltered = reinterpret_cast<float*>(base + oFiltered);
+    b.dPhase = reinterpret_cast<double*>(base + oPhase);
+    b.dMono = reinterpret_cast<int16_t*>(base + oMono);
+    b.dStereo = reinterpret_cast<StereoStats*>(base + oStereo);
+    b.dWindows = reinterpret_cast<MonoStats*>(base + oWindows);
+    b.dBody = reinterpret_cast<unsigned long long*>(base + oBody);
+    b.dOnsetPower = reinterpret_cast<unsigned long long*>(base + oOnsetPower);
+    b.dPcmJobs = reinterpret_cast<PcmJob*>(base + oPcmJobs);
+    // Natural: the packed mono PCM reuses the float mix once it is quantised.
+    // orig/custom: finalize works in place on the short2 mix.
+    b.dOutput = b.assess ? reinterpret_cast<int16_t*>(base + oOutput) : reinterpret_cast<int16_t*>(b.dMix);
+    b.dResults = reinterpret_cast<clay_tail_result*>(base + oResults);
+    check(hipStreamSynchronize(h->stream), "batch upload");
+    h->residentBytes = a.bytes;
+    h->stats.peak_bytes = std::max<uint64_t>(h->stats.peak_bytes, a.bytes + h->bankCapacity * 2 +
+                                                                      size_t(h->cfg.sinc_taps) * (kPhases + 1) * 4);
+}
+
+constexpr int kMaxBlocksPerLaunch = 1 << 20;
+
+template<class F>
+void chunked(int blocks, F&& launchChunk)
+{
+    for (int first = 0; first < blocks; first += kMaxBlocksPerLaunch)
+        launchChunk(first, std::min(kMaxBlocksPerLaunch, blocks - first));
+}
+
+// Lanes per block for one-lane-per-chain recurrences: spread short chain lists
+// over all CUs (several waves per SIMD) before filling waves.
+int recurrenceWidth(const clay_hip* h, int count)
+{
+    const int cus = std::max(1, h->props.multiProcessorCount);
+    int width = 1;
+    while (width < 64 && count > width * cus * 16)
+        width *= 2;
+    return width;
+}
+
+template<int T>
+void renderSegment(clay_hip* h, clay_hip_batch& b, int segFirst, int segEnd)
+{
+    hipStream_t s = h->stream;
+    const int pitchCount = int(b.pitchHeads.size()), filterCount = int(b.filterHeads.size());
+    if (pitchCount) {
+        const int w = recurrenceWidth(h, pitchCount);
+        h->launch("clayPitchChains", [&] {
+            hipLaunchKernelGGL(clayPitchChains, dim3(ceilDiv(pitchCount, w)), dim3(w), 0, s, b.dSpans, b.dCold,
+                               b.dPitchHeads, b.dPoints, b.dMods, b.dPhase, b.dStates, segFirst, segEnd, pitchCount);
+        });
+    }
+    if (!b.filterTiles.empty())
+        chunked(int(b.filterTiles.size()), [&](int first, int n) {
+            h->launch("clayFilterSources", [&] {
+                hipLaunchKernelGGL(clayFilterSources<T>, dim3(n), dim3(kTileFrames), 0, s, h->bank, h->sinc, b.dSpans,
+                                   b.dJobs, b.dFilterTiles, b.dPoints, b.dMods, b.dPhase, b.dFiltered, segFirst, segEnd,
+                                   first);
+            });
+        });
+    if (filterCount) {
+        const int w = recurrenceWidth(h, filterCount);
+        h->launch("clayFilterChains", [&] {
+            hipLaunchKernelGGL(clayFilterChains, dim3(ceilDiv(filterCount, w)), dim3(w), 0, s, b.dSpans, b.dCold,
+                               b.dFilterHeads, b.dPoints, b.dMods, b.dFiltered, b.dStates, segFirst, segEnd, filterCount);
+        });
+    }
+    chunked(int(b.tiles.size()), [&](int first, int n) {
+        h->launch("clayMixTiles", [&] {
+            if (h->natural)
+                hipLaunchKernelGGL((clayMixTiles<T, true>), dim3(n), dim3(kTileFrames), 0, s, h->bank, h->sinc, b.dSpans,
+                                   b.dJobs, b.dTiles, b.dVoices, b.dPoints, b.dMods, b.dFiltered, b.dMix, segFirst,
+                                   segEnd, first);
+            else
+                hipLaunchKernelGGL((clayMixTiles<T, false>), dim3(n), dim3(kTileFrames), 0, s, h->bank, h->sinc,
+                                   b.dSpans, b.dJobs, b.dTiles, b.dVoices, b.dPoints, b.dMods, b.dFiltered, b.dMix,
+                                   segFirst, segEnd, first);
+        });
+    });
+}
+
+// Render + finish one resident batch (renderer.cpp:1046-1142).
+void runBatch(clay_hip* h, clay_hip_batch& b)
+{
+    hipStream_t s = h->stream;
+    const int count = int(b.jobs.size());
+    const int ch = h->channels();
+    if (b.chains)
+        check(hipMemsetAsync(b.dStates, 0, size_t(b.chains) * sizeof(VoiceState), s), "state reset");
+    int lastFrame = 0;
+    for (const auto& j : b.jobs)
+        lastFrame = std::max(lastFrame, j.frames);
+    switch (h->cfg.sinc_taps) {
+    case 16: renderSegment<16>(h, b, 0, lastFrame); break;
+    case 32: renderSegment<32>(h, b, 0, lastFrame); break;
+    case 48: renderSegment<48>(h, b, 0, lastFrame); break;
+    default: renderSegment<64>(h, b, 0, lastFrame); break;
+    }
+    ++h->stats.segments;
+    const auto mix2 = static_cast<const float2*>(b.dMix);
+    const int lanes = 64;
+    if (h->natural) {
+        chunked(b.blocks, [&](int first, int n) {
+            h->launch("clayScanStereo", [&] {
+                hipLaunchKernelGGL(clayScanStereo, dim3(n), dim3(256), 0, s, b.dJobs, count, mix2, b.dStereo, first);
+            });
+        });
+        h->launch("claySelectNative", [&] {
+            hipLaunchKernelGGL(claySelectNative, dim3(ceilDiv(count, lanes)), dim3(lanes), 0, s, b.dJobs, count, mix2,
+                               b.dStereo, b.dResults);
+        });
+        chunked(b.windows, [&](int first, int n) {
+            h->launch("clayQuantizeMono", [&] {
+                hipLaunchKernelGGL(clayQuantizeMono, dim3(n), dim3(256), 0, s, b.dJobs, count, mix2, b.dResults,
+                                   b.dMono, b.dWindows, h->tail.window, first);
+            });
+        });
+        h->launch("claySelectTrim", [&] {
+            hipLaunchKernelGGL(claySelectTrim, dim3(ceilDiv(count, lanes)), dim3(lanes), 0, s, b.dJobs, count, b.dWindows,
+                               b.dResults, h->tail);
+        });
+    }
+    h->launch("clayLayoutPcm", [&] {
+        hipLaunchKernelGGL(clayLayoutPcm, dim3(1), dim3(1), 0, s, b.dJobs, count, b.dResults, h->natural ? 0 : 1);
+    });
+    // Natural: mono scratch -> packed (mix region). orig/custom: in place on the short2 mix.
+    const int16_t* finalIn = h->natural ? b.dMono : static_cast<const int16_t*>(b.dMix);
+    int16_t* packed = static_cast<int16_t*>(b.dMix);
+    chunked(b.blocks, [&](int first, int n) {
+        h->launch("clayFinalizePcm", [&] {
+            hipLaunchKernelGGL(clayFinalizePcm, dim3(n), dim3(256), 0, s, b.dJobs, count, finalIn, packed, b.dResults, ch,
+                               h->cfg.audio.enabled ? 1 : 0, first);
+        });
+    });
+    if (b.assess) {
+        chunked(b.bodyBlocks, [&](int first, int n) {
+            h->launch("clayMeasureBody", [&] {
+                hipLaunchKernelGGL(clayMeasureBody, dim3(n), dim3(256), 0, s, b.dAudioJobs, count, packed, b.dResults,
+                                   b.dBody, h->cfg.audio, ch, first);
+            });
+        });
+        if (h->cfg.audio.enabled && !b.onsets.empty())
+            chunked(int(b.onsets.size()), [&](int first, int n) {
+                h->launch("clayMeasureOnsets", [&] {
+                    hipLaunchKernelGGL(clayMeasureOnsets, dim3(n), dim3(320), 0, s, b.dAudioJobs, count, b.dOnsets, packed,
+                                       b.dResults, b.dOnsetPower, h->cfg.audio.window, ch, first);
+                });
+            });
+        h->launch("claySummarizeAudio", [&] {
+            hipLaunchKernelGGL(claySummarizeAudio, dim3(ceilDiv(count, lanes)), dim3(lanes), 0, s, b.dAudioJobs, count,
+                               b.dBody, b.dOnsetPower, b.dResults, h->cfg.audio, ch);
+        });
+        if (!b.groups.empty())
+            h->launch("clayMatchVolume", [&] {
+                hipLaunchKernelGGL(clayMatchVolume, dim3(ceilDiv(b.groups.size(), lanes)), dim3(lanes), 0, s, b.dGroups,
+                                   int(b.groups.size()), h->cfg.group_minimum, b.dResults);
+            });
+        h->launch("clayLayoutAccepted", [&] {
+            hipLaunchKernelGGL(clayLayoutAccepted, dim3(1), dim3(1), 0, s, b.dJobs, count, b.dResults, b.dPcmJobs);
+        });
+        chunked(b.blocks, [&](int first, int n) {
+            h->launch("clayPackAccepted", [&] {
+                hipLaunchKernelGGL(clayPackAccepted, dim3(n), dim3(256), 0, s, b.dPcmJobs, count, packed, b.dOutput,
+                                   b.dResults, ch, first);
+            });
+        });
+    }
+}
+
+// PCM16 of a finished batch into a pinned slot (renderer.cpp:672-690).
+void* pinnedSlot(clay_hip* h, size_t bytes)
+{
+    for (auto& slot : h->slots)
+        if (!slot.busy && slot.capacity >= bytes) {
+            slot.busy = true;
+            return slot.host;
+        }
+    for (auto& slot : h->slots)
+        if (!slot.busy) {
+            check(hipHostFree(slot.host), "hipHostFree");
+            slot.host = nullptr;
+            check(hipHostMalloc(&slot.host, std::max<size_t>(bytes, 1), hipHostMallocDefault), "pinned PCM");
+            slot.capacity = std::max<size_t>(bytes, 1);
+            slot.busy = true;
+            return slot.host;
+        }
+    clay_hip::Slot slot;
+    check(hipHostMalloc(&slot.host, std::max<size_t>(bytes, 1), hipHostMallocDefault), "pinned PCM");
+    slot.capacity = std::max<size_t>(bytes, 1);
+    slot.busy = true;
+    h->slots.push_back(slot);
+    return slot.host;
+}
+
+template<class F>
+int guarded(clay_hip* h, F&& body)
+{
+    try {
+        body();
+        return CLAY_OK;
+    } catch (const Failure& f) {
+        if (h)
+            h->error = f.what();
+        return f.code;
+    } catch (const std::exception& e) {
+        if (h)
+            h->error = e.what();
+        return CLAY_E_INVALID;
+    }
+}
+
+void initTail(clay_hip* h)
+{
+    const int rate = h->cfg.audio.rate;
+    h->tail = {h->cfg.audio.window, 2 * rate, int(std::nearbyint(rate / 4.)), int(std::nearbyint(rate / 10.)),
+               std::pow(10., -1.5), 32768 * std::pow(10., -3.9), 32768 * std::pow(10., -3.)};
+}
+
+} // namespace
+
+extern "C" {
+
+int clay_hip_sinc_table(int taps, float* out, size_t capacity)
+{
+    if (taps < 4 || taps > CLAY_MAX_SINC_TAPS || taps % 4 || !out || capacity < size_t(taps) * (kPhases + 1))
+        return CLAY_E_INVALID;
+    const auto t = sincTable(taps);
+    std::memcpy(out, t.data(), t.size() * sizeof(float));
+    return CLAY_OK;
+}
+
+int clay_hip_create(const clay_hip_config* config, clay_hip** out)
+{
+    if (!config || !out) {
+        g_createError = "null argument";
+        return CLAY_E_INVALID;
+    }
+    auto h = std::make_unique<clay_hip>();
+    const int status = guarded(h.get(), [&] {
+        h->cfg = *config;
+        const int taps = config->sinc_taps;
+        if (taps != 16 && taps != 32 && taps != 48 && taps != 64)
+            throw Failure(CLAY_E_INVALID, "sinc_taps must be 16, 32, 48 or 64");
+        if (config->tail_policy < CLAY_POLICY_NATURAL || config->tail_policy > CLAY_POLICY_CUSTOM)
+            throw Failure(CLAY_E_INVALID, "unknown tail policy");
+        if (config->audio.rate <= 0 || config->audio.window <= 0)
+            throw Failure(CLAY_E_INVALID, "audio rate and window must be positive");
+        if (config->batch_jobs == 0)
+            h->cfg.batch_jobs = 64;
+        h->natural = config->tail_policy == CLAY_POLICY_NATURAL;
+        initTail(h.get());
+        int devices = 0;
+        check(hipGetDeviceCount(&devices), "hipGetDeviceCount");
+        if (config->device < 0 || config->device >= devices)
+            throw Failure(CLAY_E_DEVICE, "no HIP device " + std::to_string(config->device));
+        h->device = config->device;
+        check(hipSetDevice(h->device), "hipSetDevice");
+        check(hipGetDeviceProperties(&h->props, h->device), "hipGetDeviceProperties");
+        std::snprintf(h->stats.device, sizeof h->stats.device, "%s", h->props.name);
+        check(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking), "hipStreamCreate");
+        // Phase-major table: each lane reads its taps contiguously.
+        const auto tapMajor = sincTable(taps);
+        std::vector<float> phaseMajor(tapMajor.size());
+        for (int t = 0; t < taps; ++t)
+            for (int p = 0; p <= kPhases; ++p)
+                phaseMajor[size_t(p) * size_t(taps) + size_t(t)] = tapMajor[size_t(t) * (kPhases + 1) + size_t(p)];
+        check(hipMalloc(&h->sinc, bytesOf(phaseMajor)), "sinc table");
+        check(hipMemcpy(h->sinc, phaseMajor.data(), bytesOf(phaseMajor), hipMemcpyHostToDevice), "sinc upload");
+        size_t freeBytes = 0, totalBytes = 0;
+        check(hipMemGetInfo(&freeBytes, &totalBytes), "hipMemGetInfo");
+        h->stats.sample_cache_budget = std::min<uint64_t>(freeBytes - freeBytes / 3, totalBytes / 2);
+    });
+    if (status != CLAY_OK) {
+        g_createError = h->error;
+        if (h->sinc)
+            (void)hipFree(h->sinc);
+        if (h->stream)
+            (void)hipStreamDestroy(h->stream);
+        return status;
+    }
+    *out = h.release();
+    return CLAY_OK;
+}
+
+int clay_hip_destroy(clay_hip* h)
+{
+    if (!h)
+        return CLAY_OK;
+    (void)hipSetDevice(h->device);
+    if (h->stream)
+        (void)hipStreamSynchronize(h->stream);
+    for (auto& slot : h->slots)
+        if (slot.host)
+            (void)hipHostFree(slot.host);
+    for (auto e : h->eventPool)
+        (void)hipEventDestroy(e);
+    if (h->bank)
+        (void)hipFree(h->bank);
+    if (h->sinc)
+        (void)hipFree(h->sinc);
+    if (h->stream)
+        (void)hipStreamDestroy(h->stream);
+    delete h;
+    return CLAY_OK;
+}
+
+int clay_hip_upload_bank(clay_hip* h, const int16_t* pcm, uint64_t count, int fresh)
+{
+    if (!h)
+        return CLAY_E_INVALID;
+    return guarded(h, [&] {
+        check(hipSetDevice(h->device), "hipSetDevice");
+        if (fresh) {
+            check(hipStreamSynchronize(h->stream), "bank reset");
+            if (h->bank)
+                check(hipFree(h->bank), "hipFree");
+            h->bank = nullptr;
+            h->bankCount = h->bankCapacity = 0;
+        }
+        if (count <= h->bankCount && h->bank)
+            return;
+        if (count && !pcm)
+            throw Failure(CLAY_E_INVALID, "null sample bank");
+        if (count > h->bankCapacity || !h->bank) {
+            // Append-only growth with slack (renderer.cpp:553-603).
+            const uint64_t slack = std::min<uint64_t>(count / 2, 64ull << 20);
+            const uint64_t capacity = std::max<uint64_t>(count + slack, 1);
+            int16_t* grown = nullptr;
+            check(hipMalloc(&grown, capacity * sizeof(int16_t)), "sample bank");
+            if (h->bankCount)
+                check(hipMemcpyAsync(grown, h->bank, h->bankCount * sizeof(int16_t), hipMemcpyDeviceToDevice, h->stream),
+                      "sample bank growth");
+            check(hipStreamSynchronize(h->stream), "sample bank growth");
+            if (h->bank)
+                check(hipFree(h->bank), "hipFree");
+            h->bank = grown;
+            h->bankCapacity = capacity;
+        }
+        if (count > h->bankCount)
+            check(hipMemcpy(h->bank + h->bankCount, pcm + h->bankCount, (count - h->bankCount) * sizeof(int16_t),
+                            hipMemcpyHostToDevice), "sample upload");
+        h->bankCount = count;
+    });
+}
+
+int clay_hip_batch_prepare(clay_hip* h, const clay_plan_view* plans, size_t count, clay_hip_batch** out)
+{
+    if (!h || !out || (count && !plans))
+        return CLAY_E_INVALID;
+    auto b = std::make_unique<clay_hip_batch>();
+    const int status = guarded(h, [&] {
+        if (!count)
+            throw Failure(CLAY_E_INVALID, "empty batch");
+        check(hipSetDevice(h->device), "hipSetDevice");
+        flatten(h, plans, count, *b);
+        upload(h, *b);
+    });
+    if (status != CLAY_OK) {
+        release(*b);
+        return status;
+    }
+    *out = b.release();
+    return CLAY_OK;
+}
+
+int clay_hip_batch_run(clay_hip* h, clay_hip_batch* b, double* kernel_seconds)
+{
+    if (!h || !b)
+        return CLAY_E_INVALID;
+    return guarded(h, [&] {
+        check(hipSetDevice(h->device), "hipSetDevice");
+        hipEvent_t begin = h->event(), end = h->event();
+        check(hipEventRecord(begin, h->stream), "hipEventRecord");
+        const char* timing = std::getenv("CLAY_KERNEL_TIMING");
+        h->timing = timing && *timing == '1';
+        runBatch(h, *b);
+        check(hipEventRecord(end, h->stream), "hipEventRecord");
+        check(hipStreamSynchronize(h->stream), "batch run");
+        float ms = 0;
+        check(hipEventElapsedTime(&ms, begin, end), "hipEventElapsedTime");
+        h->eventPool.push_back(begin);
+        h->eventPool.push_back(end);
+        b->kernelTimes.clear();
+        h->collectTimes(b->kernelTimes);
+        h->timing = false;
+        if (kernel_seconds)
+            *kernel_seconds = ms / 1000.0;
+    });
+}
+
+int clay_hip_batch_kernel_time(clay_hip* h, clay_hip_batch* b, const char* kernel, double* ms, int* launches)
+{
+    if (!h || !b || !kernel)
+        return CLAY_E_INVALID;
+    const auto it = b->kernelTimes.find(kernel);
+    if (ms)
+        *ms = it == b->kernelTimes.end() ? 0 : it->second.first;
+    if (launches)
+        *launches = it == b->kernelTimes.end() ? 0 : it->second.second;
+    return CLAY_OK;
+}
+
+int clay_hip_batch_results(clay_hip* h, clay_hip_batch* b, clay_tail_result* results, size_t result_capacity,
+                           int16_t* pcm, uint64_t pcm_capacity, uint64_t* pcm_samples)
+{
+    if (!h || !b)
+        return CLAY_E_INVALID;
+    return guarded(h, [&] {
+        check(hipSetDevice(h->device), "hipSetDevice");
+        std::vector<clay_tail_result> r(b->jobs.size());
+        check(hipMemcpy(r.data(), b->dResults, bytesOf(r), hipMemcpyDeviceToHost), "results download");
+        uint64_t samples = 0;
+        for (const auto& x : r)
+            samples = std::max<uint64_t>(samples, (x.offset + uint64_t(x.data_frames)) * uint64_t(h->channels()));
+        if (pcm_samples)
+            *pcm_samples = samples;
+        if (results) {
+            if (result_capacity < r.size())
+                throw Failure(CLAY_E_INVALID, "result capacity too small");
+            std::memcpy(results, r.data(), bytesOf(r));
+        }
+        if (pcm) {
+            if (pcm_capacity < samples)
+                throw Failure(CLAY_E_INVALID, "pcm capacity too small");
+            if (samples)
+                check(hipMemcpy(pcm, b->dOutput, samples * sizeof(int16_t), hipMemcpyDeviceToHost), "pcm download");
+            h->stats.downloaded_bytes += samples * sizeof(int16_t);
+        }
+    });
+}
+
+int clay_hip_batch_destroy(clay_hip* h, clay_hip_batch* b)
+{
+    if (!b)
+        return CLAY_OK;
+    if (h) {
+        (void)hipSetDevice(h->device);
+        (void)hipStreamSynchronize(h->stream);
+    }
+    release(*b);
+    delete b;
+    return CLAY_OK;
+}
+
+int clay_hip_render_window(clay_hip* h, const clay_plan_view* plans, size_t count, clay_publish_fn publish,
+                           clay_refill_fn refill, void* user)
+{
+    if (!h || !publish || (count && !plans))
+        return CLAY_E_INVALID;
+    return guarded(h, [&] {
+        std::vector<clay_plan_view> queue(plans, plans + count);
+        size_t first = 0;
+        for (;;) {
+            if (first == queue.size()) {
+                const clay_plan_view* more = nullptr;
+                size_t n = 0;
+                if (!refill || !refill(user, 1, &more, &n) || !n)
+                    break;
+                queue.insert(queue.end(), more, more + n);
+            }
+            // A batch holds up to batch_jobs jobs and never splits a volume group.
+            size_t end = std::min(queue.size(), first + size_t(h->cfg.batch_jobs));
+            if (h->cfg.group_minimum)
+                while (end < queue.size() && queue[end].volume_group && queue[end].volume_group == queue[end - 1].volume_group)
+                    ++end;
+            clay_hip_batch* b = nullptr;
+            int status = CLAY_OK;
+            for (;;) {
+                status = clay_hip_batch_prepare(h, queue.data() + first, end - first, &b);
+                if (status != CLAY_E_MEMORY || end - first == 1)
+                    break;
+                size_t half = first + std::max<size_t>(1, (end - first) / 2);
+                end = half;
+            }
+            if (status == CLAY_E_MEMORY) {
+                clay_render_batch rejected{};
+                rejected.failed_scratch_bytes = 1;
+                if (publish(user, &rejected, first))
+                    throw Failure(CLAY_E_CALLBACK, "publish aborted the window");
+                first = end;
+                continue;
+            }
+            if (status != CLAY_OK)
+                throw Failure(status, h->error);
+            double seconds = 0;
+            status = clay_hip_batch_run(h, b, &seconds);
+            std::vector<clay_tail_result> results(end - first);
+            uint64_t samples = 0;
+            if (status == CLAY_OK)
+                status = clay_hip_batch_results(h, b, results.data(), results.size(), nullptr, 0, &samples);
+            void* slot = nullptr;
+            if (status == CLAY_OK) {
+                slot = pinnedSlot(h, samples * sizeof(int16_t));
+                if (samples)
+                    status = guarded(h, [&] {
+                        check(hipMemcpy(slot, b->dOutput, samples * sizeof(int16_t), hipMemcpyDeviceToHost), "pcm download");
+                    });
+                h->stats.downloaded_bytes += samples * sizeof(int16_t);
+            }
+            const std::string error = h->error;
+            clay_hip_batch_destroy(h, b);
+            if (status != CLAY_OK)
+                throw Failure(status, error);
+            clay_render_batch batch{};
+            batch.pcm = static_cast<const int16_t*>(slot);
+            batch.pcm_handle = slot;
+            batch.results = results.data();
+            batch.result_count = results.size();
+            batch.kernel_seconds = seconds;
+            if (publish(user, &batch, first))
+                throw Failure(CLAY_E_CALLBACK, "publish aborted the window");
+            first = end;
+        }
+    });
+}
+
+int clay_hip_release_pcm(clay_hip* h, void* pcm_handle)
+{
+    if (!h)
+        return CLAY_E_INVALID;
+    for (auto& slot : h->slots)
+        if (slot.host == pcm_handle) {
+            slot.busy = false;
+            return CLAY_OK;
+        }
+    h->error = "unknown PCM handle";
+    return CLAY_E_INVALID;
+}
+
+int clay_hip_get_stats(clay_hip* h, clay_hip_stats* out)
+{
+    if (!h || !out)
+        return CLAY_E_INVALID;
+    *out = h->stats;
+    return CLAY_OK;
+}
+
+const char* clay_hip_last_error(const clay_hip* h) { return h ? h->error.c_str() : g_createError.c_str(); }
+
+} // extern "C"

@@ -1,0 +1,155 @@
+"""Shared test fixtures: generated inputs in the style of the reference's CLI
+tests (tests/check_tail.py, check_group_audio.py, check_segments.py), and
+helpers that render one set of plans through the HIP core and the oracle."""
+
+import math
+import os
+import struct
+
+import numpy as np
+
+import claysampler_amd as ca
+from claysampler_amd import abi, synth
+
+RATE = 44100
+
+
+def smf(path, data):
+    """Format 0, PPQ 480 MIDI around raw track bytes (tests/check_tail.py:14-16)."""
+    with open(path, "wb") as f:
+        f.write(b"MThd" + struct.pack(">IHHH", 6, 0, 1, 480) + b"MTrk" + struct.pack(">I", len(data)) + data)
+    return str(path)
+
+
+def wav(path, samples, rate=RATE, channels=1):
+    synth.write_wav(str(path), np.asarray(samples), rate, channels)
+    return str(path)
+
+
+def text(path, content):
+    with open(path, "w") as f:
+        f.write(content)
+    return str(path)
+
+
+def tail_case(root):
+    """tests/check_tail.py:26-36: 0.4 s at 1600 then 3 s at 12, a one-shot region,
+    a 0.1 s note followed by a 5 s end-of-track gap."""
+    wav(root / "sample.wav", [1600] * (RATE * 2 // 5) + [12] * (RATE * 3))
+    sfz = text(root / "sample.sfz", "<region> sample=sample.wav key=60 loop_mode=one_shot amp_veltrack=0\n")
+    mid = smf(root / "source.mid", bytes.fromhex("00 90 3c 64 60 80 3c 00 a5 40 ff 2f 00"))
+    return sfz, mid
+
+
+def pedal_case(root):
+    """tests/check_tail.py:83-90: looping region, pedal and sostenuto MIDIs."""
+    if not os.path.exists(root / "sample.wav"):
+        tail_case(root)
+    loop = text(root / "loop.sfz", "<region> sample=sample.wav key=60 loop_mode=loop_continuous loop_start=0 "
+                                   "loop_end=800 ampeg_release=0.1 amp_veltrack=0\n")
+    mids = {
+        "regular": "00 90 3c 64 60 80 3c 00 00 ff 2f 00",
+        "held": "00 b0 40 7f 00 90 3c 64 60 80 3c 00 00 b0 78 00 a5 40 ff 2f 00",
+        "later": "00 b0 40 7f 00 90 3c 64 60 80 3c 00 82 68 b0 40 00 00 ff 2f 00",
+        "sostenuto": "00 90 3c 64 30 b0 42 7f 30 80 3c 00 82 68 b0 42 00 00 ff 2f 00",
+    }
+    return loop, {k: smf(root / f"{k}.mid", bytes.fromhex(v)) for k, v in mids.items()}
+
+
+def segment_case(root):
+    """tests/check_segments.py:17-28: 3000 sin(0.04 i) loop, filter + CC1 cutoff/pitch."""
+    wav(root / "seg.wav", [round(3000 * math.sin(i * .04)) for i in range(RATE)])
+    sfz = text(root / "voice.sfz", "<region> sample=seg.wav key=60 loop_mode=loop_continuous loop_start=0 "
+                                   "loop_end=44099 amp_veltrack=0 ampeg_release=1.7 cutoff=1800 resonance=3 "
+                                   "cutoff_oncc1=3000 pitch_oncc1=37\n")
+    long_mid = smf(root / "long.mid", bytes.fromhex(
+        "00 b0 40 7f 00 90 3c 64 87 40 b0 01 40 87 40 e0 00 50 87 40 80 3c 00 87 40 b0 40 00 00 ff 2f 00"))
+    short_mid = smf(root / "short.mid", bytes.fromhex("00 90 3c 64 60 80 3c 00 00 ff 2f 00"))
+    return sfz, long_mid, short_mid
+
+
+def group_case(root):
+    """tests/check_group_audio.py:23-31: constant-level one-shot presets and a 0.5 s melody."""
+    melody = smf(root / "melody.mid", bytes.fromhex("00 90 3c 64 83 60 80 3c 00 00 ff 2f 00"))
+    presets = {}
+    for name, value in [("soft", 1600), ("loud", 6400), ("quiet", 100), ("tiny", 8)]:
+        wav(root / f"{name}.wav", [value] * RATE)
+        presets[name] = text(root / f"{name}.sfz",
+                             f"<region> sample={name}.wav key=60 loop_mode=one_shot amp_veltrack=0\n")
+    return presets, melody
+
+
+def rich_case(root):
+    """Stereo sample with pan/width/position, loop crossfade, generators,
+    velocity-modulated volume, a multi-section filter and a pitch-CC voice."""
+    n = np.arange(RATE)
+    left = np.rint(9000 * np.sin(2 * np.pi * 330 * n / RATE) * np.exp(-n / 30000))
+    right = np.rint(7000 * np.sin(2 * np.pi * 495 * n / RATE + 1) * np.exp(-n / 20000))
+    wav(root / "stereo.wav", np.stack([left, right], 1).reshape(-1), channels=2)
+    sfz = text(root / "rich.sfz", (
+        "<group> amp_veltrack=50\n"
+        "<region> sample=stereo.wav lokey=40 hikey=59 pitch_keycenter=50 pan=-30 width=60 position=20 "
+        "loop_mode=loop_continuous loop_start=1000 loop_end=30000 loop_crossfade=0.05 ampeg_release=0.25\n"
+        "<region> sample=stereo.wav lokey=60 hikey=71 pitch_keycenter=64 fil_type=lpf_4p cutoff=900 "
+        "resonance=2 fileg_depth=1200 fileg_attack=0.1 fileg_decay=0.3 fileg_sustain=20 ampeg_attack=0.02\n"
+        "<region> sample=*sine lokey=72 hikey=80 pitch_keycenter=69 ampeg_release=0.1 volume=-6\n"
+        "<region> sample=*noise lokey=81 hikey=84 volume=-30 ampeg_release=0.05\n"
+        "<region> sample=stereo.wav lokey=85 hikey=96 pitch_keycenter=90 pitch_oncc1=300 eq1_freq=2000 "
+        "eq1_gain=6 eq1_bw=1 loop_mode=one_shot\n"))
+    events = []
+    rng = synth.SplitMix64(77)
+    tick = 0
+    for _ in range(24):
+        key, vel = rng.integer(40, 96), rng.integer(30, 127)
+        dur = rng.integer(100, 700)
+        events.append((tick, 2, 0x90, key, vel))
+        events.append((tick + dur, 1, 0x80, key, 0))
+        events.append((tick + dur // 2, 0, 0xB0, 1, rng.integer(0, 127)))
+        events.append((tick + dur // 3, 0, 0xE0, rng.integer(0, 127), rng.integer(0, 127)))
+        tick += rng.integer(60, 400)
+    mid = str(root / "rich.mid")
+    synth.write_midi(mid, events)
+    return sfz, mid
+
+
+def gpu_render(session, policy, taps, quality=False, group_minimum=0, batch_jobs=64):
+    bank_ptr, bank_count = session.bank()
+    plans = session.plans()
+    r = ca.Renderer(policy=policy, taps=taps, quality=quality, group_minimum=group_minimum, batch_jobs=batch_jobs,
+                    rate=session.rate)
+    r.upload_bank(bank_ptr, bank_count)
+    b = r.prepare(plans)
+    b.run()
+    results, pcm = b.results()
+    b.close()
+    r.close()
+    return results, pcm
+
+
+def oracle_render(session, policy, taps, quality=False, group_minimum=0, threads=4):
+    import oracle
+
+    bank_ptr, bank_count = session.bank()
+    plans = session.plans()
+    cfg = oracle.config(policy=policy, taps=taps, quality=quality, group_minimum=group_minimum, threads=threads,
+                        rate=session.rate)
+    return oracle.render(cfg, bank_ptr, bank_count, plans)
+
+
+def result_tuple(r):
+    return (r.status, r.frames, r.data_frames, r.fade, r.offset)
+
+
+def assert_parity(got, want, lsb=1, extent=True):
+    """Frame counts and statuses exact; PCM within `lsb`; peak/first/last consistent."""
+    (gres, gpcm), (wres, wpcm) = got, want
+    assert [result_tuple(r) for r in gres] == [result_tuple(r) for r in wres]
+    assert gpcm.shape == wpcm.shape
+    if gpcm.size:
+        diff = np.abs(gpcm.astype(np.int32) - wpcm.astype(np.int32))
+        assert int(diff.max()) <= lsb, f"max |GPU - oracle| = {int(diff.max())} LSB at {int(diff.argmax())}"
+    if extent:
+        for g, w in zip(gres, wres):
+            assert abs(g.peak - w.peak) <= lsb
+            assert abs(g.first - w.first) <= 64 and abs(g.last - w.last) <= 64
+    return gres, gpcm
