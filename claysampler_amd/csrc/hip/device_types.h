@@ -77,13 +77,17 @@ struct FilterTile {
 };
 
 // Filter and pitch state carried across segments (src/cuda/render_types.h:22-28).
+// The recurrences run in FP64: a float32 biquad with a 200 Hz cutoff at
+// 44.1 kHz (poles near z = 1) amplifies its own rounding by ~1/(1-r)^2, and a
+// float pitch increment drifts the source phase over long notes; FP64 keeps
+// both within the +-1 LSB bar against the FP64 transcription.
 struct VoiceState {
     double position, step;
     int32_t pitchSpan, previousLoopEnd, previousLoopStart;
     int32_t pitchInitialized, filterInitialized;
     int32_t pad;
-    float z1[CLAY_FILTER_SLOTS][3][2], z2[CLAY_FILTER_SLOTS][3][2];
-    float coefficients[CLAY_FILTER_SLOTS][5];
+    double z1[CLAY_FILTER_SLOTS][3][2], z2[CLAY_FILTER_SLOTS][3][2];
+    double coefficients[CLAY_FILTER_SLOTS][5];
 };
 
 struct StereoStats {

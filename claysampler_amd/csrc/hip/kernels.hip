@@ -40,24 +40,6 @@ __device__ __forceinline__ float envelopeSpan(const Span& v, int frame)
     return fmaxf(v.sustain, expf(float(frame - v.decayStart + 1) * v.decayRate));
 }
 
-__device__ __forceinline__ float envelopeShape(const clay_envelope_shape& v, int frame)
-{
-    if (v.release_start >= 0 && frame >= v.release_start) {
-        if (frame >= v.fade_start)
-            return fmaxf(0.f, v.fade_value - float(frame - v.fade_start + 1) * v.fade_step);
-        return v.release_value * expf(float(frame - v.release_start + 1) * v.release_rate);
-    }
-    if (frame < v.attack_start)
-        return v.start;
-    if (frame < v.hold_start)
-        return fminf(1.f, v.start + float(frame - v.attack_start + 1) * v.attack_step);
-    if (frame < v.decay_start)
-        return 1.f;
-    if (v.decay_rate == 0)
-        return v.sustain;
-    return fmaxf(v.sustain, expf(float(frame - v.decay_start + 1) * v.decay_rate));
-}
-
 // ---------------------------------------------------------------- controls
 // Last point at or before `frame` (the first point when none), then linear
 // (src/cuda/render.cu:107-118).
@@ -260,43 +242,7 @@ __device__ float2 voiceAudio(const int16_t* __restrict__ bank, const float* __re
 }
 
 // ---------------------------------------------------------------- filters
-struct Coeff {
-    float b0, b1, b2, a1, a2;
-};
-
-// src/cuda/render.cu:278-310
-__device__ Coeff coefficients(int type, float cutoff, float resonance, int rate, float bandwidth, float gain)
-{
-    const float w = 6.28318530717958647692f * fminf(fminf(20000.f, float(rate) * .499f), fmaxf(1.f, cutoff)) / float(rate);
-    if (type == CLAY_FILTER_LOW1 || type == CLAY_FILTER_HIGH1) {
-        const float p = expf(-w);
-        if (type == CLAY_FILTER_LOW1)
-            return {1 - p, 0, 0, -p, 0};
-        const float h = (1 + p) * .5f;
-        return {h, -h, 0, -p, 0};
-    }
-    float s, c;
-    sincosf(w, &s, &c);
-    if (type == CLAY_FILTER_PEAK) {
-        const float a = exp2f(fminf(60.f, fmaxf(-96.f, gain)) * .083048202372184f);
-        const float width = fminf(12.f, fmaxf(.01f, bandwidth));
-        const float alpha = s * sinhf(fminf(20.f, .346573590279973f * width * w / s));
-        const float inv = 1 / (1 + alpha / a);
-        const float mid = -2 * c * inv;
-        return {(1 + alpha * a) * inv, mid, (1 - alpha * a) * inv, mid, (1 - alpha / a) * inv};
-    }
-    const float alpha = .5f * s * exp2f(-.166096404744368f * fminf(60.f, fmaxf(-60.f, resonance)));
-    const float inv = 1.f / (1 + alpha);
-    const float a1 = -2 * c * inv, a2 = (1 - alpha) * inv;
-    if (type == CLAY_FILTER_BAND2 || type == CLAY_FILTER_BAND4 || type == CLAY_FILTER_BAND6)
-        return {alpha * inv, 0, -alpha * inv, a1, a2};
-    if (type == CLAY_FILTER_NOTCH2)
-        return {inv, a1, inv, a1, a2};
-    const bool high = type == CLAY_FILTER_HIGH2 || type == CLAY_FILTER_HIGH4 || type == CLAY_FILTER_HIGH6;
-    const float b = (1 + (high ? c : -c)) * .5f * inv;
-    return {b, (high ? -2 : 2) * b, b, a1, a2};
-}
-
+// Biquad sections per filter slot (src/cuda/render.cu:312-319).
 __device__ __forceinline__ int sections(int type)
 {
     if (type == CLAY_FILTER_NONE) return 0;
@@ -304,14 +250,6 @@ __device__ __forceinline__ int sections(int type)
     if (type >= CLAY_FILTER_LOW6) return 3;
     if (type >= CLAY_FILTER_LOW4) return 2;
     return 1;
-}
-
-__device__ __forceinline__ float filterSample(float x, const Coeff& c, float& z1, float& z2)
-{
-    const float y = fmaf(c.b0, x, z1);
-    z1 = fmaf(-c.a1, y, fmaf(c.b1, x, z2));
-    z2 = fmaf(-c.a2, y, c.b2 * x);
-    return y;
 }
 
 // ---------------------------------------------------------------- reductions
@@ -373,60 +311,6 @@ __device__ __forceinline__ int uniform(int x) { return __builtin_amdgcn_readfirs
 
 // ====================================================================== render
 
-// Variable-pitch recurrence per voice chain: per-frame source phase
-// (src/cuda/render.cu:222-266). One lane per chain.
-extern "C" __global__ void clayPitchChains(const Span* __restrict__ spans, const SpanCold* __restrict__ cold,
-                                           const int* __restrict__ heads, const clay_control_point* __restrict__ points,
-                                           const Mod* __restrict__ mods, double* __restrict__ phase,
-                                           VoiceState* __restrict__ states, int segFirst, int segEnd, int count)
-{
-    const int track = blockIdx.x * blockDim.x + threadIdx.x;
-    if (track >= count)
-        return;
-    const int head = heads[track];
-    if (!spans[head].variablePitch)
-        return;
-    VoiceState& st = states[spans[head].filterTrack];  // one state per voice chain
-    double position = st.pitchInitialized ? st.position : spans[head].position;
-    int prevLoopEnd = st.previousLoopEnd, prevLoopStart = st.previousLoopStart;
-    for (int s = head; s >= 0; s = spans[s].next) {
-        const Span& v = spans[s];
-        if (v.end <= segFirst || v.begin >= segEnd)
-            continue;
-        const bool resumed = st.pitchInitialized && st.pitchSpan == s;
-        if (!resumed && !v.loopEnd && prevLoopEnd > 0 && position >= prevLoopEnd)
-            position = prevLoopStart + fmod(position - prevLoopStart, double(prevLoopEnd - prevLoopStart));
-        prevLoopEnd = v.loopEnd;
-        prevLoopStart = v.loopFirst;
-        const double slope = exp(v.pitchSlope);
-        double step = resumed ? st.step : v.step;
-        const clay_envelope_shape& pitchEnv = cold[s].extra[1];
-        const bool envActive = cold[s].active[1] != 0;
-        const int first = max(v.begin, segFirst), stop = min(v.end, segEnd);
-        for (int f = first; f < stop; ++f) {
-            phase[v.phase + uint64_t(f - first)] = position;
-            float cents = 0, depth = pitchEnv.depth;
-            for (int j = v.firstMod; j < v.firstMod + v.modCount; ++j) {
-                const Mod m = mods[j];
-                if (m.target == CLAY_MOD_PITCH)
-                    cents += modulationAt(m, points, f);
-                if (m.target == CLAY_MOD_PITCHEG_DEPTH)
-                    depth += modulationAt(m, points, f);
-            }
-            if (envActive)
-                cents += depth * envelopeShape(pitchEnv, f);
-            position += step * exp2f(fminf(24000.f, fmaxf(-24000.f, cents)) / 1200);
-            step *= slope;
-        }
-        st.pitchInitialized = 1;
-        st.pitchSpan = s;
-        st.position = position;
-        st.step = step;
-        st.previousLoopEnd = prevLoopEnd;
-        st.previousLoopStart = prevLoopStart;
-    }
-}
-
 // Voice audio of filtered / variable-pitch spans into scratch, times the job
 // gain, before pan for mono sources (src/cuda/render.cu:195-220).
 template<int T>
@@ -453,10 +337,228 @@ __global__ void __launch_bounds__(kTileFrames) clayFilterSources(
         dst[1] = s.y * g;
 }
 
-// Serial filter recurrence per voice chain, in place on scratch
-// (src/cuda/render.cu:338-448). Coefficients are recomputed only when a
-// modulation input changes; equal inputs give equal coefficients, so this is
-// exact. State persists across segments in VoiceState.
+// ----------------------------------------------------------------- FP64 voice-chain recurrences
+namespace {
+
+// Control track value in FP64 (src/cuda/render.cu:107-118 evaluated in double).
+__device__ __forceinline__ double pointValue(const clay_control_point& p, int frame)
+{
+    return fma(double(p.step), double(frame - p.frame), double(p.value));
+}
+
+// Sequential reader of one control track: frames only move forward inside a
+// chain, so the segment is found by advancing instead of a binary search.
+// It lands on the same point as controlAt's search (last point at or before
+// the frame, the first point when none).
+struct Cursor {
+    int at, last;
+    __device__ void reset(const clay_control_point* __restrict__ points, int first, int end, int frame)
+    {
+        at = first;
+        last = end;
+        int lo = first, hi = end;
+        while (lo + 1 < hi) {
+            const int mid = lo + (hi - lo) / 2;
+            if (points[mid].frame <= frame)
+                lo = mid;
+            else
+                hi = mid;
+        }
+        at = lo;
+    }
+    __device__ __forceinline__ double value(const clay_control_point* __restrict__ points, int frame)
+    {
+        while (at + 1 < last && points[at + 1].frame <= frame)
+            ++at;
+        return pointValue(points[at], frame);
+    }
+};
+
+// Up to kCursors modulation bindings of a span read through cursors; any
+// further ones fall back to the binary search.
+constexpr int kCursors = 6;
+
+struct ModReader {
+    Cursor track[kCursors], shape[kCursors];
+    int first, count;
+
+    __device__ void reset(const Mod* __restrict__ mods, const clay_control_point* __restrict__ points, int firstMod,
+                          int modCount, int frame)
+    {
+        first = firstMod;
+        count = modCount;
+        for (int j = 0; j < min(count, kCursors); ++j) {
+            const Mod m = mods[first + j];
+            if (m.first >= 0)
+                track[j].reset(points, m.first, m.last, frame);
+            if (m.shapeFirst >= 0)
+                shape[j].reset(points, m.shapeFirst, m.shapeLast, frame);
+        }
+    }
+
+    // src/cuda/render.cu:120-126 in double.
+    __device__ double value(const Mod* __restrict__ mods, const clay_control_point* __restrict__ points, int j, int frame)
+    {
+        const Mod m = mods[first + j];
+        double v = m.depth;
+        if (m.first >= 0) {
+            const double x = j < kCursors ? track[j].value(points, frame) : [&] {
+                Cursor c;
+                c.reset(points, m.first, m.last, frame);
+                return pointValue(points[c.at], frame);
+            }();
+            v = double(m.depth) * x;
+        }
+        if (m.shapeFirst >= 0) {
+            const double x = j < kCursors ? shape[j].value(points, frame) : [&] {
+                Cursor c;
+                c.reset(points, m.shapeFirst, m.shapeLast, frame);
+                return pointValue(points[c.at], frame);
+            }();
+            v *= x;
+        }
+        return v;
+    }
+};
+
+__device__ __forceinline__ double envelopeShapeD(const clay_envelope_shape& v, int frame)
+{
+    if (v.release_start >= 0 && frame >= v.release_start) {
+        if (frame >= v.fade_start)
+            return fmax(0.0, double(v.fade_value) - double(frame - v.fade_start + 1) * double(v.fade_step));
+        return double(v.release_value) * exp(double(frame - v.release_start + 1) * double(v.release_rate));
+    }
+    if (frame < v.attack_start)
+        return v.start;
+    if (frame < v.hold_start)
+        return fmin(1.0, double(v.start) + double(frame - v.attack_start + 1) * double(v.attack_step));
+    if (frame < v.decay_start)
+        return 1.0;
+    if (v.decay_rate == 0)
+        return v.sustain;
+    return fmax(double(v.sustain), exp(double(frame - v.decay_start + 1) * double(v.decay_rate)));
+}
+
+struct CoeffD {
+    double b0, b1, b2, a1, a2;
+};
+
+// src/cuda/render.cu:278-310 in double.
+__device__ CoeffD coefficientsD(int type, double cutoff, double resonance, int rate, double bandwidth, double gain)
+{
+    constexpr double pi = 3.14159265358979323846;
+    const double w = 2 * pi * fmin(fmin(20000.0, rate * .499), fmax(1.0, cutoff)) / rate;
+    if (type == CLAY_FILTER_LOW1 || type == CLAY_FILTER_HIGH1) {
+        const double p = exp(-w);
+        if (type == CLAY_FILTER_LOW1)
+            return {1 - p, 0, 0, -p, 0};
+        const double h = (1 + p) * .5;
+        return {h, -h, 0, -p, 0};
+    }
+    double s, c;
+    sincos(w, &s, &c);
+    if (type == CLAY_FILTER_PEAK) {
+        const double a = exp10(fmin(60.0, fmax(-96.0, gain)) / 40);
+        const double width = fmin(12.0, fmax(.01, bandwidth));
+        const double alpha = s * sinh(fmin(20.0, 0.34657359027997264 * width * w / s));
+        const double inv = 1 / (1 + alpha / a);
+        const double mid = -2 * c * inv;
+        return {(1 + alpha * a) * inv, mid, (1 - alpha * a) * inv, mid, (1 - alpha / a) * inv};
+    }
+    const double alpha = .5 * s * exp10(-fmin(60.0, fmax(-60.0, resonance)) / 20);
+    const double inv = 1 / (1 + alpha);
+    const double a1 = -2 * c * inv, a2 = (1 - alpha) * inv;
+    if (type == CLAY_FILTER_BAND2 || type == CLAY_FILTER_BAND4 || type == CLAY_FILTER_BAND6)
+        return {alpha * inv, 0, -alpha * inv, a1, a2};
+    if (type == CLAY_FILTER_NOTCH2)
+        return {inv, a1, inv, a1, a2};
+    const bool high = type == CLAY_FILTER_HIGH2 || type == CLAY_FILTER_HIGH4 || type == CLAY_FILTER_HIGH6;
+    const double b = (1 + (high ? c : -c)) * .5 * inv;
+    return {b, (high ? -2 : 2) * b, b, a1, a2};
+}
+
+// Direct form II transposed section (src/cuda/render.cu:330-336) in double.
+__device__ __forceinline__ double biquad(double x, const CoeffD& c, double& z1, double& z2)
+{
+    const double y = fma(c.b0, x, z1);
+    z1 = fma(-c.a1, y, fma(c.b1, x, z2));
+    z2 = fma(-c.a2, y, c.b2 * x);
+    return y;
+}
+
+__device__ __forceinline__ void smooth(CoeffD& c, const CoeffD& t, double amount)
+{
+    c.b0 = fma(amount, t.b0 - c.b0, c.b0);
+    c.b1 = fma(amount, t.b1 - c.b1, c.b1);
+    c.b2 = fma(amount, t.b2 - c.b2, c.b2);
+    c.a1 = fma(amount, t.a1 - c.a1, c.a1);
+    c.a2 = fma(amount, t.a2 - c.a2, c.a2);
+}
+
+} // namespace
+
+// Variable-pitch recurrence per voice chain: per-frame source phase
+// (src/cuda/render.cu:222-266), in FP64. One lane per chain.
+extern "C" __global__ void clayPitchChains(const Span* __restrict__ spans, const SpanCold* __restrict__ cold,
+                                           const int* __restrict__ heads, const clay_control_point* __restrict__ points,
+                                           const Mod* __restrict__ mods, double* __restrict__ phase,
+                                           VoiceState* __restrict__ states, int segFirst, int segEnd, int count)
+{
+    const int track = blockIdx.x * blockDim.x + threadIdx.x;
+    if (track >= count)
+        return;
+    const int head = heads[track];
+    if (!spans[head].variablePitch)
+        return;
+    VoiceState& st = states[spans[head].filterTrack];  // one state per voice chain
+    double position = st.pitchInitialized ? st.position : spans[head].position;
+    int prevLoopEnd = st.previousLoopEnd, prevLoopStart = st.previousLoopStart;
+    ModReader reader;
+    for (int s = head; s >= 0; s = spans[s].next) {
+        const Span& v = spans[s];
+        if (v.end <= segFirst || v.begin >= segEnd)
+            continue;
+        const bool resumed = st.pitchInitialized && st.pitchSpan == s;
+        if (!resumed && !v.loopEnd && prevLoopEnd > 0 && position >= prevLoopEnd)
+            position = prevLoopStart + fmod(position - prevLoopStart, double(prevLoopEnd - prevLoopStart));
+        prevLoopEnd = v.loopEnd;
+        prevLoopStart = v.loopFirst;
+        const double slope = exp(v.pitchSlope);
+        double step = resumed ? st.step : v.step;
+        const clay_envelope_shape& pitchEnv = cold[s].extra[1];
+        const bool envActive = cold[s].active[1] != 0;
+        const int first = max(v.begin, segFirst), stop = min(v.end, segEnd);
+        reader.reset(mods, points, v.firstMod, v.modCount, first);
+        for (int f = first; f < stop; ++f) {
+            phase[v.phase + uint64_t(f - first)] = position;
+            double cents = 0, depth = pitchEnv.depth;
+            for (int j = 0; j < v.modCount; ++j) {
+                const int target = mods[v.firstMod + j].target;
+                if (target == CLAY_MOD_PITCH)
+                    cents += reader.value(mods, points, j, f);
+                else if (target == CLAY_MOD_PITCHEG_DEPTH)
+                    depth += reader.value(mods, points, j, f);
+            }
+            if (envActive)
+                cents += depth * envelopeShapeD(pitchEnv, f);
+            position += step * exp2(fmin(24000.0, fmax(-24000.0, cents)) / 1200);
+            step *= slope;
+        }
+        st.pitchInitialized = 1;
+        st.pitchSpan = s;
+        st.position = position;
+        st.step = step;
+        st.previousLoopEnd = prevLoopEnd;
+        st.previousLoopStart = prevLoopStart;
+    }
+}
+
+// Serial filter recurrence per voice chain, in place on the float scratch
+// (src/cuda/render.cu:338-448) with FP64 state and coefficients. Target
+// coefficients are recomputed only when their inputs change (equal inputs give
+// equal coefficients). A chain with one biquad section keeps its state in
+// registers; longer cascades use the general loop. State persists across
+// segments in VoiceState.
 extern "C" __global__ void clayFilterChains(const Span* __restrict__ spans, const SpanCold* __restrict__ cold,
                                             const int* __restrict__ heads, const clay_control_point* __restrict__ points,
                                             const Mod* __restrict__ mods, float* __restrict__ filtered,
@@ -472,119 +574,145 @@ extern "C" __global__ void clayFilterChains(const Span* __restrict__ spans, cons
     if (!any)
         return;
     VoiceState& st = states[spans[head].filterTrack];  // one state per voice chain
-    float z1[CLAY_FILTER_SLOTS][3][2], z2[CLAY_FILTER_SLOTS][3][2];
-    Coeff current[CLAY_FILTER_SLOTS];
     bool initialized = st.filterInitialized != 0;
-    for (int f = 0; f < CLAY_FILTER_SLOTS; ++f) {
-        current[f] = {st.coefficients[f][0], st.coefficients[f][1], st.coefficients[f][2], st.coefficients[f][3],
-                      st.coefficients[f][4]};
-        for (int k = 0; k < 3; ++k)
-            for (int c = 0; c < 2; ++c) {
-                z1[f][k][c] = st.z1[f][k][c];
-                z2[f][k][c] = st.z2[f][k][c];
-            }
-    }
+    ModReader reader;
     for (int s = head; s >= 0; s = spans[s].next) {
         const Span& v = spans[s];
         if (v.end <= segFirst || v.begin >= segEnd)
             continue;
         const SpanCold& cv = cold[s];
         const int rate = cv.sampleRate;
-        const float amount = 1 - expf(-1000.f / float(rate));
-        int count[CLAY_FILTER_SLOTS];
-        for (int f = 0; f < CLAY_FILTER_SLOTS; ++f)
+        const double amount = 1 - exp(-1000.0 / rate);
+        int count[CLAY_FILTER_SLOTS], sectionTotal = 0, single = 0;
+        for (int f = 0; f < CLAY_FILTER_SLOTS; ++f) {
             count[f] = sections(cv.filters[f].type);
+            sectionTotal += count[f];
+            if (count[f])
+                single = f;
+        }
         bool moving = cv.active[0] != 0;
         for (int j = v.firstMod; j < v.firstMod + v.modCount; ++j) {
             const int t = mods[j].target;
             moving |= (t >= CLAY_MOD_CUTOFF && t <= CLAY_MOD_RESONANCE2) || t == CLAY_MOD_FILEG_DEPTH ||
                       t >= CLAY_MOD_EQ1_FREQUENCY;
         }
-        // Inputs of the last target computation; targets are reused while they repeat.
-        float lastIn[CLAY_FILTER_SLOTS][3];
-        Coeff target[CLAY_FILTER_SLOTS];
-        bool haveTarget = false;
         const int first = max(v.begin, segFirst), stop = min(v.end, segEnd);
+        reader.reset(mods, points, v.firstMod, v.modCount, first);
+        const bool stereo = v.channels == 2;
         float* x = filtered + v.filtered;
+        // Filter inputs of one frame: cutoff in Hz, resonance, and for EQ slots bandwidth and gain.
+        const auto inputs = [&](int fr, double in[CLAY_FILTER_SLOTS][3]) {
+            double cents[2] = {0, 0}, db[2] = {0, 0}, eq[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+            double depth = cv.extra[0].depth;
+            for (int j = 0; j < v.modCount; ++j) {
+                const int t = mods[v.firstMod + j].target;
+                if (t == CLAY_MOD_CUTOFF) cents[0] += reader.value(mods, points, j, fr);
+                else if (t == CLAY_MOD_CUTOFF2) cents[1] += reader.value(mods, points, j, fr);
+                else if (t == CLAY_MOD_RESONANCE) db[0] += reader.value(mods, points, j, fr);
+                else if (t == CLAY_MOD_RESONANCE2) db[1] += reader.value(mods, points, j, fr);
+                else if (t == CLAY_MOD_FILEG_DEPTH) depth += reader.value(mods, points, j, fr);
+                else if (t >= CLAY_MOD_EQ1_FREQUENCY) {
+                    const int id = t - CLAY_MOD_EQ1_FREQUENCY;
+                    eq[id / 3][id % 3] += reader.value(mods, points, j, fr);
+                }
+            }
+            if (cv.active[0])
+                cents[0] += depth * envelopeShapeD(cv.extra[0], fr);
+            for (int f = 0; f < CLAY_FILTER_SLOTS; ++f) {
+                const clay_voice_filter& flt = cv.filters[f];
+                if (f < 2) {
+                    in[f][0] = double(flt.cutoff) * exp2(cents[f] / 1200);
+                    in[f][1] = double(flt.resonance) + db[f];
+                    in[f][2] = 0;
+                } else {
+                    in[f][0] = double(flt.cutoff) + eq[f - 2][0];
+                    in[f][1] = double(flt.bandwidth) + eq[f - 2][1];
+                    in[f][2] = double(flt.gain) + eq[f - 2][2];
+                }
+            }
+        };
+        const auto target = [&](int f, const double in[3]) {
+            return f < 2 ? coefficientsD(cv.filters[f].type, in[0], in[1], rate, 0, 0)
+                         : coefficientsD(cv.filters[f].type, in[0], 0, rate, in[1], in[2]);
+        };
+        if (sectionTotal == 1) {
+            // One section: state and coefficients in registers.
+            const int f = single;
+            CoeffD c = {st.coefficients[f][0], st.coefficients[f][1], st.coefficients[f][2], st.coefficients[f][3],
+                        st.coefficients[f][4]};
+            double z1l = st.z1[f][0][0], z2l = st.z2[f][0][0], z1r = st.z1[f][0][1], z2r = st.z2[f][0][1];
+            double last[3] = {-1, -1, -1};
+            CoeffD tgt = c;
+            bool have = false;
+            for (int fr = first; fr < stop; ++fr, x += v.channels) {
+                if (moving || !initialized) {
+                    double in[CLAY_FILTER_SLOTS][3];
+                    inputs(fr, in);
+                    if (!have || in[f][0] != last[0] || in[f][1] != last[1] || in[f][2] != last[2]) {
+                        tgt = target(f, in[f]);
+                        last[0] = in[f][0], last[1] = in[f][1], last[2] = in[f][2];
+                        have = true;
+                    }
+                    if (initialized)
+                        smooth(c, tgt, amount);
+                    else
+                        c = tgt;
+                    initialized = true;
+                }
+                x[0] = float(biquad(double(x[0]), c, z1l, z2l));
+                if (stereo)
+                    x[1] = float(biquad(double(x[1]), c, z1r, z2r));
+            }
+            st.coefficients[f][0] = c.b0, st.coefficients[f][1] = c.b1, st.coefficients[f][2] = c.b2;
+            st.coefficients[f][3] = c.a1, st.coefficients[f][4] = c.a2;
+            st.z1[f][0][0] = z1l, st.z2[f][0][0] = z2l, st.z1[f][0][1] = z1r, st.z2[f][0][1] = z2r;
+            continue;
+        }
+        // General cascade: state lives in VoiceState.
+        double last[CLAY_FILTER_SLOTS][3];
+        CoeffD tgt[CLAY_FILTER_SLOTS];
+        bool have = false;
         for (int fr = first; fr < stop; ++fr, x += v.channels) {
             if (moving || !initialized) {
-                float cents[2] = {0, 0}, db[2] = {0, 0}, eq[3][3] = {};
-                float depth = cv.extra[0].depth;
-                for (int j = v.firstMod; j < v.firstMod + v.modCount; ++j) {
-                    const Mod m = mods[j];
-                    const int t = m.target;
-                    if (t == CLAY_MOD_CUTOFF) cents[0] += modulationAt(m, points, fr);
-                    if (t == CLAY_MOD_CUTOFF2) cents[1] += modulationAt(m, points, fr);
-                    if (t == CLAY_MOD_RESONANCE) db[0] += modulationAt(m, points, fr);
-                    if (t == CLAY_MOD_RESONANCE2) db[1] += modulationAt(m, points, fr);
-                    if (t == CLAY_MOD_FILEG_DEPTH) depth += modulationAt(m, points, fr);
-                    if (t >= CLAY_MOD_EQ1_FREQUENCY) {
-                        const int id = t - CLAY_MOD_EQ1_FREQUENCY;
-                        eq[id / 3][id % 3] += modulationAt(m, points, fr);
-                    }
-                }
-                if (cv.active[0])
-                    cents[0] += depth * envelopeShape(cv.extra[0], fr);
+                double in[CLAY_FILTER_SLOTS][3];
+                inputs(fr, in);
                 for (int f = 0; f < CLAY_FILTER_SLOTS; ++f) {
                     if (!count[f])
                         continue;
-                    const clay_voice_filter& flt = cv.filters[f];
-                    float in[3];
-                    if (f < 2) {
-                        in[0] = flt.cutoff * exp2f(cents[f] / 1200);
-                        in[1] = flt.resonance + db[f];
-                        in[2] = 0;
-                    } else {
-                        in[0] = flt.cutoff + eq[f - 2][0];
-                        in[1] = flt.bandwidth + eq[f - 2][1];
-                        in[2] = flt.gain + eq[f - 2][2];
+                    if (!have || in[f][0] != last[f][0] || in[f][1] != last[f][1] || in[f][2] != last[f][2]) {
+                        tgt[f] = target(f, in[f]);
+                        last[f][0] = in[f][0], last[f][1] = in[f][1], last[f][2] = in[f][2];
                     }
-                    if (!haveTarget || in[0] != lastIn[f][0] || in[1] != lastIn[f][1] || in[2] != lastIn[f][2]) {
-                        target[f] = f < 2 ? coefficients(flt.type, in[0], in[1], rate, 0, 0)
-                                          : coefficients(flt.type, in[0], 0, rate, in[1], in[2]);
-                        lastIn[f][0] = in[0];
-                        lastIn[f][1] = in[1];
-                        lastIn[f][2] = in[2];
-                    }
-                    if (initialized) {
-                        Coeff& c = current[f];
-                        c.b0 = fmaf(amount, target[f].b0 - c.b0, c.b0);
-                        c.b1 = fmaf(amount, target[f].b1 - c.b1, c.b1);
-                        c.b2 = fmaf(amount, target[f].b2 - c.b2, c.b2);
-                        c.a1 = fmaf(amount, target[f].a1 - c.a1, c.a1);
-                        c.a2 = fmaf(amount, target[f].a2 - c.a2, c.a2);
-                    } else {
-                        current[f] = target[f];
-                    }
+                    CoeffD c = {st.coefficients[f][0], st.coefficients[f][1], st.coefficients[f][2],
+                                st.coefficients[f][3], st.coefficients[f][4]};
+                    if (initialized)
+                        smooth(c, tgt[f], amount);
+                    else
+                        c = tgt[f];
+                    st.coefficients[f][0] = c.b0, st.coefficients[f][1] = c.b1, st.coefficients[f][2] = c.b2;
+                    st.coefficients[f][3] = c.a1, st.coefficients[f][4] = c.a2;
                 }
-                haveTarget = true;
+                have = true;
                 initialized = true;
             }
-            float l = x[0], r = v.channels == 2 ? x[1] : 0.f;
-            for (int f = 0; f < CLAY_FILTER_SLOTS; ++f)
+            double l = x[0], r = stereo ? x[1] : 0.0;
+            for (int f = 0; f < CLAY_FILTER_SLOTS; ++f) {
+                if (!count[f])
+                    continue;
+                const CoeffD c = {st.coefficients[f][0], st.coefficients[f][1], st.coefficients[f][2],
+                                  st.coefficients[f][3], st.coefficients[f][4]};
                 for (int k = 0; k < count[f]; ++k) {
-                    l = filterSample(l, current[f], z1[f][k][0], z2[f][k][0]);
-                    if (v.channels == 2)
-                        r = filterSample(r, current[f], z1[f][k][1], z2[f][k][1]);
+                    l = biquad(l, c, st.z1[f][k][0], st.z2[f][k][0]);
+                    if (stereo)
+                        r = biquad(r, c, st.z1[f][k][1], st.z2[f][k][1]);
                 }
-            x[0] = l;
-            if (v.channels == 2)
-                x[1] = r;
+            }
+            x[0] = float(l);
+            if (stereo)
+                x[1] = float(r);
         }
     }
     st.filterInitialized = initialized;
-    for (int f = 0; f < CLAY_FILTER_SLOTS; ++f) {
-        st.coefficients[f][0] = current[f].b0;
-        st.coefficients[f][1] = current[f].b1;
-        st.coefficients[f][2] = current[f].b2;
-        st.coefficients[f][3] = current[f].a1;
-        st.coefficients[f][4] = current[f].a2;
-        for (int k = 0; k < 3; ++k)
-            for (int c = 0; c < 2; ++c) {
-                st.z1[f][k][c] = z1[f][k][c];
-                st.z2[f][k][c] = z2[f][k][c];
-            }
-    }
 }
 
 // Per output frame: the dry voices of the tile rendered inline, times the job
