@@ -75,8 +75,9 @@ def kernel_bytes(plans, taps, natural):
         wet_samples += int((n * v["channels"])[wet].sum())
     mix_bytes = frames * (8 if natural else 4)
     return {
-        "clayFilterSources": src["wet"] + 4 * wet_samples,
-        "clayFilterChains": 8 * wet_samples,
+        # source windows of wet spans + one float write per wet sample
+        "clayVoiceChains": src["wet"] + 4 * wet_samples,
+        # dry source windows + wet reads + the mix write + span records
         "clayMixTiles": src["dry"] + 4 * wet_samples + mix_bytes + spans * span_bytes,
     }, voice_frames, frames
 
@@ -148,8 +149,8 @@ def main():
     os.environ["CLAY_KERNEL_TIMING"] = "0"
     bytes_per, voice_frames, frames = kernel_bytes(plans, taps, True)
     kernels = {}
-    for name in ("clayFilterSources", "clayFilterChains", "clayMixTiles", "clayScanStereo", "clayQuantizeMono",
-                 "clayFinalizePcm", "clayPitchChains"):
+    for name in ("clayVoiceChains", "clayMixTiles", "clayScanStereo", "clayQuantizeMono", "clayFinalizePcm",
+                 "clayPitchChains", "claySelectNative", "claySelectTrim"):
         ms, n = batch.kernel_time(name)
         if n:
             kernels[name] = {"ms": ms, "launches": n}

@@ -72,10 +72,6 @@ struct Tile {
     int32_t job, frame, frames, first, count;
 };
 
-struct FilterTile {
-    int32_t span, frame;
-};
-
 // Filter and pitch state carried across segments (src/cuda/render_types.h:22-28).
 // The recurrences run in FP64: a float32 biquad with a 200 Hz cutoff at
 // 44.1 kHz (poles near z = 1) amplifies its own rounding by ~1/(1-r)^2, and a

@@ -311,32 +311,6 @@ __device__ __forceinline__ int uniform(int x) { return __builtin_amdgcn_readfirs
 
 // ====================================================================== render
 
-// Voice audio of filtered / variable-pitch spans into scratch, times the job
-// gain, before pan for mono sources (src/cuda/render.cu:195-220).
-template<int T>
-__global__ void __launch_bounds__(kTileFrames) clayFilterSources(
-    const int16_t* __restrict__ bank, const float* __restrict__ sinc, const Span* __restrict__ spans,
-    const Job* __restrict__ jobs, const FilterTile* __restrict__ tiles, const clay_control_point* __restrict__ points,
-    const Mod* __restrict__ mods, const double* __restrict__ phase, float* __restrict__ filtered, int segFirst, int segEnd,
-    int firstTile)
-{
-    const FilterTile tile = tiles[firstTile + blockIdx.x];
-    const Span& v = spans[uniform(tile.span)];
-    const int frame = tile.frame + int(threadIdx.x);
-    if (frame >= v.end || frame < segFirst || frame >= segEnd)
-        return;
-    const Job& j = jobs[v.job];
-    const int local = frame - max(v.begin, segFirst);
-    const double position = v.variablePitch ? phase[v.phase + uint64_t(local)] : -1.0;
-    const float pan = controlAt(points, j.controls[2], j.controls[3], frame);
-    const float2 s = voiceAudio<T>(bank, sinc, v, points, mods, pan, frame, true, position);
-    const float g = jobGain(points, j, frame);
-    float* dst = filtered + v.filtered + uint64_t(local) * uint64_t(v.channels);
-    dst[0] = s.x * g;
-    if (v.channels == 2)
-        dst[1] = s.y * g;
-}
-
 // ----------------------------------------------------------------- FP64 voice-chain recurrences
 namespace {
 
@@ -477,24 +451,6 @@ __device__ CoeffD coefficientsD(int type, double cutoff, double resonance, int r
     return {b, (high ? -2 : 2) * b, b, a1, a2};
 }
 
-// Direct form II transposed section (src/cuda/render.cu:330-336) in double.
-__device__ __forceinline__ double biquad(double x, const CoeffD& c, double& z1, double& z2)
-{
-    const double y = fma(c.b0, x, z1);
-    z1 = fma(-c.a1, y, fma(c.b1, x, z2));
-    z2 = fma(-c.a2, y, c.b2 * x);
-    return y;
-}
-
-__device__ __forceinline__ void smooth(CoeffD& c, const CoeffD& t, double amount)
-{
-    c.b0 = fma(amount, t.b0 - c.b0, c.b0);
-    c.b1 = fma(amount, t.b1 - c.b1, c.b1);
-    c.b2 = fma(amount, t.b2 - c.b2, c.b2);
-    c.a1 = fma(amount, t.a1 - c.a1, c.a1);
-    c.a2 = fma(amount, t.a2 - c.a2, c.a2);
-}
-
 } // namespace
 
 // Variable-pitch recurrence per voice chain: per-frame source phase
@@ -553,166 +509,312 @@ extern "C" __global__ void clayPitchChains(const Span* __restrict__ spans, const
     }
 }
 
-// Serial filter recurrence per voice chain, in place on the float scratch
-// (src/cuda/render.cu:338-448) with FP64 state and coefficients. Target
-// coefficients are recomputed only when their inputs change (equal inputs give
-// equal coefficients). A chain with one biquad section keeps its state in
-// registers; longer cascades use the general loop. State persists across
-// segments in VoiceState.
-extern "C" __global__ void clayFilterChains(const Span* __restrict__ spans, const SpanCold* __restrict__ cold,
-                                            const int* __restrict__ heads, const clay_control_point* __restrict__ points,
-                                            const Mod* __restrict__ mods, float* __restrict__ filtered,
-                                            VoiceState* __restrict__ states, int segFirst, int segEnd, int count)
+// ----------------------------------------------------------------- wave-parallel voice chains
+namespace {
+
+constexpr int kMaxSections = 9;   // 2 filters x up to 3 sections + 3 EQ bands
+constexpr int kChainWaves = 4;    // voice chains per 256-thread block
+
+// Per-wave carry between 64-frame chunks, in LDS (broadcast reads).
+struct ChainCarry {
+    double z[kMaxSections][2][2];        // [section][channel][z1, z2]
+    double coef[CLAY_FILTER_SLOTS][5];   // current (smoothed) coefficients
+    double target[CLAY_FILTER_SLOTS][5]; // last target
+    double input[CLAY_FILTER_SLOTS][3];  // inputs of the last target
+    int have[CLAY_FILTER_SLOTS];
+};
+
+__device__ __forceinline__ double up(double v, int d) { return __shfl_up(v, d, 64); }
+
+__device__ __forceinline__ double laneValue(double v, int lane) { return __shfl(v, lane, 64); }
+
+} // namespace
+
+// One wave per voice chain (a voice's spans with filters and/or variable pitch),
+// 64 output frames per step, chunks aligned to absolute frame multiples of 64:
+//   1. the 64 frames of voice audio before pan (mono) times the job gain, one per lane;
+//   2. filter targets where a lane's inputs differ from the previous frame's,
+//      copied forward to the following lanes (src/cuda/render.cu:376-406);
+//   3. coefficient smoothing c_n = c_{n-1} + a (t_n - c_{n-1}) as an inclusive
+//      scan of affine maps c -> M c + V over the 64 lanes;
+//   4. each biquad section (direct form II transposed, render.cu:330-336) as a
+//      scan of 2x2 affine maps s_n = A_n s_{n-1} + B_n u_n, in FP64;
+//   5. the filtered frames written to the float scratch the mix kernel reads.
+// State carries across chunks in LDS and across segments in VoiceState.
+template<int T>
+__global__ void __launch_bounds__(64 * kChainWaves) clayVoiceChains(
+    const int16_t* __restrict__ bank, const float* __restrict__ sinc, const Span* __restrict__ spans,
+    const SpanCold* __restrict__ cold, const Job* __restrict__ jobs, const int* __restrict__ heads,
+    const clay_control_point* __restrict__ points, const Mod* __restrict__ mods, const double* __restrict__ phase,
+    float* __restrict__ filtered, VoiceState* __restrict__ states, int segFirst, int segEnd, int count)
 {
-    const int track = blockIdx.x * blockDim.x + threadIdx.x;
-    if (track >= count)
-        return;
-    const int head = heads[track];
-    bool any = false;
-    for (int f = 0; f < CLAY_FILTER_SLOTS; ++f)
-        any |= cold[head].filters[f].type != CLAY_FILTER_NONE;
-    if (!any)
-        return;
-    VoiceState& st = states[spans[head].filterTrack];  // one state per voice chain
+    __shared__ ChainCarry carries[kChainWaves];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int chain = blockIdx.x * kChainWaves + wave;
+    if (chain >= count)
+        return;  // uniform per wave; no block barriers below
+    ChainCarry& carry = carries[wave];
+    const int head = uniform(heads[chain]);
+    VoiceState& st = states[spans[head].filterTrack];
+    int sectionCount[CLAY_FILTER_SLOTS], total = 0;
+    for (int f = 0; f < CLAY_FILTER_SLOTS; ++f) {
+        sectionCount[f] = sections(cold[head].filters[f].type);
+        total += sectionCount[f];
+    }
+    const bool hasFilter = total > 0;
     bool initialized = st.filterInitialized != 0;
-    ModReader reader;
+    if (hasFilter && lane == 0) {
+        int k = 0;
+        for (int f = 0; f < CLAY_FILTER_SLOTS; ++f) {
+            for (int i = 0; i < 5; ++i)
+                carry.coef[f][i] = st.coefficients[f][i];
+            carry.have[f] = 0;
+            for (int s = 0; s < sectionCount[f]; ++s, ++k)
+                for (int c = 0; c < 2; ++c) {
+                    carry.z[k][c][0] = st.z1[f][s][c];
+                    carry.z[k][c][1] = st.z2[f][s][c];
+                }
+        }
+    }
+    __builtin_amdgcn_wave_barrier();
     for (int s = head; s >= 0; s = spans[s].next) {
         const Span& v = spans[s];
         if (v.end <= segFirst || v.begin >= segEnd)
             continue;
         const SpanCold& cv = cold[s];
+        const Job& j = jobs[v.job];
+        const int first = max(v.begin, segFirst), stop = min(v.end, segEnd);
+        const bool stereo = v.channels == 2;
         const int rate = cv.sampleRate;
         const double amount = 1 - exp(-1000.0 / rate);
-        int count[CLAY_FILTER_SLOTS], sectionTotal = 0, single = 0;
-        for (int f = 0; f < CLAY_FILTER_SLOTS; ++f) {
-            count[f] = sections(cv.filters[f].type);
-            sectionTotal += count[f];
-            if (count[f])
-                single = f;
-        }
         bool moving = cv.active[0] != 0;
-        for (int j = v.firstMod; j < v.firstMod + v.modCount; ++j) {
-            const int t = mods[j].target;
+        for (int k = v.firstMod; k < v.firstMod + v.modCount; ++k) {
+            const int t = mods[k].target;
             moving |= (t >= CLAY_MOD_CUTOFF && t <= CLAY_MOD_RESONANCE2) || t == CLAY_MOD_FILEG_DEPTH ||
                       t >= CLAY_MOD_EQ1_FREQUENCY;
         }
-        const int first = max(v.begin, segFirst), stop = min(v.end, segEnd);
-        reader.reset(mods, points, v.firstMod, v.modCount, first);
-        const bool stereo = v.channels == 2;
-        float* x = filtered + v.filtered;
-        // Filter inputs of one frame: cutoff in Hz, resonance, and for EQ slots bandwidth and gain.
-        const auto inputs = [&](int fr, double in[CLAY_FILTER_SLOTS][3]) {
-            double cents[2] = {0, 0}, db[2] = {0, 0}, eq[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
-            double depth = cv.extra[0].depth;
-            for (int j = 0; j < v.modCount; ++j) {
-                const int t = mods[v.firstMod + j].target;
-                if (t == CLAY_MOD_CUTOFF) cents[0] += reader.value(mods, points, j, fr);
-                else if (t == CLAY_MOD_CUTOFF2) cents[1] += reader.value(mods, points, j, fr);
-                else if (t == CLAY_MOD_RESONANCE) db[0] += reader.value(mods, points, j, fr);
-                else if (t == CLAY_MOD_RESONANCE2) db[1] += reader.value(mods, points, j, fr);
-                else if (t == CLAY_MOD_FILEG_DEPTH) depth += reader.value(mods, points, j, fr);
-                else if (t >= CLAY_MOD_EQ1_FREQUENCY) {
-                    const int id = t - CLAY_MOD_EQ1_FREQUENCY;
-                    eq[id / 3][id % 3] += reader.value(mods, points, j, fr);
-                }
+        ModReader reader;
+        reader.reset(mods, points, v.firstMod, v.modCount, max(first, (first & ~63) + lane));
+        for (int f0 = first & ~63; f0 < stop; f0 += 64) {
+            const int frame = f0 + lane;
+            const bool active = frame >= first && frame < stop;
+            const int lo = max(first, f0) - f0, hi = min(stop, f0 + 64) - f0;  // active lanes [lo, hi)
+            float xl = 0.f, xr = 0.f;
+            if (active) {
+                const int local = frame - first;
+                const double ph = v.variablePitch ? phase[v.phase + uint64_t(local)] : -1.0;
+                const float pan = controlAt(points, j.controls[2], j.controls[3], frame);
+                const float2 a = voiceAudio<T>(bank, sinc, v, points, mods, pan, frame, true, ph);
+                const float g = jobGain(points, j, frame);
+                xl = a.x * g;
+                xr = a.y * g;
             }
-            if (cv.active[0])
-                cents[0] += depth * envelopeShapeD(cv.extra[0], fr);
-            for (int f = 0; f < CLAY_FILTER_SLOTS; ++f) {
-                const clay_voice_filter& flt = cv.filters[f];
-                if (f < 2) {
-                    in[f][0] = double(flt.cutoff) * exp2(cents[f] / 1200);
-                    in[f][1] = double(flt.resonance) + db[f];
-                    in[f][2] = 0;
-                } else {
-                    in[f][0] = double(flt.cutoff) + eq[f - 2][0];
-                    in[f][1] = double(flt.bandwidth) + eq[f - 2][1];
-                    in[f][2] = double(flt.gain) + eq[f - 2][2];
-                }
-            }
-        };
-        const auto target = [&](int f, const double in[3]) {
-            return f < 2 ? coefficientsD(cv.filters[f].type, in[0], in[1], rate, 0, 0)
-                         : coefficientsD(cv.filters[f].type, in[0], 0, rate, in[1], in[2]);
-        };
-        if (sectionTotal == 1) {
-            // One section: state and coefficients in registers.
-            const int f = single;
-            CoeffD c = {st.coefficients[f][0], st.coefficients[f][1], st.coefficients[f][2], st.coefficients[f][3],
-                        st.coefficients[f][4]};
-            double z1l = st.z1[f][0][0], z2l = st.z2[f][0][0], z1r = st.z1[f][0][1], z2r = st.z2[f][0][1];
-            double last[3] = {-1, -1, -1};
-            CoeffD tgt = c;
-            bool have = false;
-            for (int fr = first; fr < stop; ++fr, x += v.channels) {
-                if (moving || !initialized) {
-                    double in[CLAY_FILTER_SLOTS][3];
-                    inputs(fr, in);
-                    if (!have || in[f][0] != last[0] || in[f][1] != last[1] || in[f][2] != last[2]) {
-                        tgt = target(f, in[f]);
-                        last[0] = in[f][0], last[1] = in[f][1], last[2] = in[f][2];
-                        have = true;
-                    }
-                    if (initialized)
-                        smooth(c, tgt, amount);
-                    else
-                        c = tgt;
-                    initialized = true;
-                }
-                x[0] = float(biquad(double(x[0]), c, z1l, z2l));
-                if (stereo)
-                    x[1] = float(biquad(double(x[1]), c, z1r, z2r));
-            }
-            st.coefficients[f][0] = c.b0, st.coefficients[f][1] = c.b1, st.coefficients[f][2] = c.b2;
-            st.coefficients[f][3] = c.a1, st.coefficients[f][4] = c.a2;
-            st.z1[f][0][0] = z1l, st.z2[f][0][0] = z2l, st.z1[f][0][1] = z1r, st.z2[f][0][1] = z2r;
-            continue;
-        }
-        // General cascade: state lives in VoiceState.
-        double last[CLAY_FILTER_SLOTS][3];
-        CoeffD tgt[CLAY_FILTER_SLOTS];
-        bool have = false;
-        for (int fr = first; fr < stop; ++fr, x += v.channels) {
-            if (moving || !initialized) {
-                double in[CLAY_FILTER_SLOTS][3];
-                inputs(fr, in);
-                for (int f = 0; f < CLAY_FILTER_SLOTS; ++f) {
-                    if (!count[f])
-                        continue;
-                    if (!have || in[f][0] != last[f][0] || in[f][1] != last[f][1] || in[f][2] != last[f][2]) {
-                        tgt[f] = target(f, in[f]);
-                        last[f][0] = in[f][0], last[f][1] = in[f][1], last[f][2] = in[f][2];
-                    }
-                    CoeffD c = {st.coefficients[f][0], st.coefficients[f][1], st.coefficients[f][2],
-                                st.coefficients[f][3], st.coefficients[f][4]};
-                    if (initialized)
-                        smooth(c, tgt[f], amount);
-                    else
-                        c = tgt[f];
-                    st.coefficients[f][0] = c.b0, st.coefficients[f][1] = c.b1, st.coefficients[f][2] = c.b2;
-                    st.coefficients[f][3] = c.a1, st.coefficients[f][4] = c.a2;
-                }
-                have = true;
-                initialized = true;
-            }
-            double l = x[0], r = stereo ? x[1] : 0.0;
-            for (int f = 0; f < CLAY_FILTER_SLOTS; ++f) {
-                if (!count[f])
-                    continue;
-                const CoeffD c = {st.coefficients[f][0], st.coefficients[f][1], st.coefficients[f][2],
-                                  st.coefficients[f][3], st.coefficients[f][4]};
-                for (int k = 0; k < count[f]; ++k) {
-                    l = biquad(l, c, st.z1[f][k][0], st.z2[f][k][0]);
+            float* dst = filtered + v.filtered + uint64_t(max(frame - first, 0)) * uint64_t(v.channels);
+            if (!hasFilter) {
+                if (active) {
+                    dst[0] = xl;
                     if (stereo)
-                        r = biquad(r, c, st.z1[f][k][1], st.z2[f][k][1]);
+                        dst[1] = xr;
+                }
+                continue;
+            }
+            // Filter inputs of this lane's frame.
+            const bool recompute = moving || !initialized;
+            double in[CLAY_FILTER_SLOTS][3] = {};
+            if (recompute && active) {
+                double cents[2] = {0, 0}, db[2] = {0, 0}, eq[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+                double depth = cv.extra[0].depth;
+                for (int k = 0; k < v.modCount; ++k) {
+                    const int t = mods[v.firstMod + k].target;
+                    if (t == CLAY_MOD_CUTOFF) cents[0] += reader.value(mods, points, k, frame);
+                    else if (t == CLAY_MOD_CUTOFF2) cents[1] += reader.value(mods, points, k, frame);
+                    else if (t == CLAY_MOD_RESONANCE) db[0] += reader.value(mods, points, k, frame);
+                    else if (t == CLAY_MOD_RESONANCE2) db[1] += reader.value(mods, points, k, frame);
+                    else if (t == CLAY_MOD_FILEG_DEPTH) depth += reader.value(mods, points, k, frame);
+                    else if (t >= CLAY_MOD_EQ1_FREQUENCY) {
+                        const int id = t - CLAY_MOD_EQ1_FREQUENCY;
+                        eq[id / 3][id % 3] += reader.value(mods, points, k, frame);
+                    }
+                }
+                if (cv.active[0])
+                    cents[0] += depth * envelopeShapeD(cv.extra[0], frame);
+                for (int f = 0; f < CLAY_FILTER_SLOTS; ++f) {
+                    const clay_voice_filter& flt = cv.filters[f];
+                    if (f < 2) {
+                        in[f][0] = double(flt.cutoff) * exp2(cents[f] / 1200);
+                        in[f][1] = double(flt.resonance) + db[f];
+                    } else {
+                        in[f][0] = double(flt.cutoff) + eq[f - 2][0];
+                        in[f][1] = double(flt.bandwidth) + eq[f - 2][1];
+                        in[f][2] = double(flt.gain) + eq[f - 2][2];
+                    }
                 }
             }
-            x[0] = float(l);
-            if (stereo)
-                x[1] = float(r);
+            int sec = 0;
+            double ul = xl, ur = xr;
+            for (int f = 0; f < CLAY_FILTER_SLOTS; ++f) {
+                if (!sectionCount[f])
+                    continue;
+                double c[5];
+                if (recompute) {
+                    // 2. Targets: recompute where inputs change, copy forward otherwise.
+                    double p0 = up(in[f][0], 1), p1 = up(in[f][1], 1), p2 = up(in[f][2], 1);
+                    bool prevValid = lane > lo;
+                    if (lane == lo) {
+                        p0 = carry.input[f][0], p1 = carry.input[f][1], p2 = carry.input[f][2];
+                        prevValid = carry.have[f] != 0;
+                    }
+                    const bool flag = active && (!prevValid || in[f][0] != p0 || in[f][1] != p1 || in[f][2] != p2);
+                    double t[5] = {0, 0, 0, 0, 0};
+                    if (flag) {
+                        const CoeffD q = f < 2 ? coefficientsD(cv.filters[f].type, in[f][0], in[f][1], rate, 0, 0)
+                                               : coefficientsD(cv.filters[f].type, in[f][0], 0, rate, in[f][1], in[f][2]);
+                        t[0] = q.b0, t[1] = q.b1, t[2] = q.b2, t[3] = q.a1, t[4] = q.a2;
+                    }
+                    int have = flag ? 1 : 0;
+                    for (int d = 1; d < 64; d <<= 1) {
+                        const int hp = __shfl_up(have, d, 64);
+                        double tp[5];
+                        for (int i = 0; i < 5; ++i)
+                            tp[i] = up(t[i], d);
+                        if (!have && lane >= d && hp) {
+                            for (int i = 0; i < 5; ++i)
+                                t[i] = tp[i];
+                            have = 1;
+                        }
+                    }
+                    if (!have)
+                        for (int i = 0; i < 5; ++i)
+                            t[i] = carry.target[f][i];
+                    // 3. Smoothing scan: c_n = M_n c_{n-1} + V_n.
+                    // A filter without modulation takes its coefficients once, unsmoothed.
+                    double M, V[5];
+                    if (!active || (!moving && lane != lo)) {
+                        M = 1;
+                        for (int i = 0; i < 5; ++i)
+                            V[i] = 0;
+                    } else if (!initialized && lane == lo) {
+                        M = 0;
+                        for (int i = 0; i < 5; ++i)
+                            V[i] = t[i];
+                    } else {
+                        M = 1 - amount;
+                        for (int i = 0; i < 5; ++i)
+                            V[i] = amount * t[i];
+                    }
+                    for (int d = 1; d < 64; d <<= 1) {
+                        const double Mp = up(M, d);
+                        double Vp[5];
+                        for (int i = 0; i < 5; ++i)
+                            Vp[i] = up(V[i], d);
+                        if (lane >= d) {
+                            for (int i = 0; i < 5; ++i)
+                                V[i] = fma(M, Vp[i], V[i]);
+                            M *= Mp;
+                        }
+                    }
+                    for (int i = 0; i < 5; ++i)
+                        c[i] = fma(M, carry.coef[f][i], V[i]);
+                    // Carry: values of the last active lane.
+                    const int last = hi - 1;
+                    double nc[5], nt[5], ni[3];
+                    for (int i = 0; i < 5; ++i) {
+                        nc[i] = laneValue(c[i], last);
+                        nt[i] = laneValue(t[i], last);
+                    }
+                    for (int i = 0; i < 3; ++i)
+                        ni[i] = laneValue(in[f][i], last);
+                    __builtin_amdgcn_wave_barrier();
+                    if (lane == 0) {
+                        for (int i = 0; i < 5; ++i) {
+                            carry.coef[f][i] = nc[i];
+                            carry.target[f][i] = nt[i];
+                        }
+                        for (int i = 0; i < 3; ++i)
+                            carry.input[f][i] = ni[i];
+                        carry.have[f] = 1;
+                    }
+                    __builtin_amdgcn_wave_barrier();
+                } else {
+                    for (int i = 0; i < 5; ++i)
+                        c[i] = carry.coef[f][i];
+                }
+                // 4. Biquad sections of this slot: s_n = A_n s_{n-1} + B_n u_n.
+                for (int k = 0; k < sectionCount[f]; ++k, ++sec) {
+                    // A = [[-a1, 1], [-a2, 0]]; inactive lanes are the identity.
+                    double a00 = -c[3], a01 = 1, a10 = -c[4], a11 = 0;
+                    double wl0 = (c[1] - c[3] * c[0]) * ul, wl1 = (c[2] - c[4] * c[0]) * ul;
+                    double wr0 = (c[1] - c[3] * c[0]) * ur, wr1 = (c[2] - c[4] * c[0]) * ur;
+                    if (!active) {
+                        a00 = 1, a01 = 0, a10 = 0, a11 = 1;
+                        wl0 = wl1 = wr0 = wr1 = 0;
+                    }
+                    for (int d = 1; d < 64; d <<= 1) {
+                        const double b00 = up(a00, d), b01 = up(a01, d), b10 = up(a10, d), b11 = up(a11, d);
+                        const double pl0 = up(wl0, d), pl1 = up(wl1, d);
+                        double pr0 = 0, pr1 = 0;
+                        if (stereo) {
+                            pr0 = up(wr0, d);
+                            pr1 = up(wr1, d);
+                        }
+                        if (lane >= d) {
+                            // (A, w) o (B, p) = (A B, A p + w)
+                            wl0 = fma(a00, pl0, fma(a01, pl1, wl0));
+                            wl1 = fma(a10, pl0, fma(a11, pl1, wl1));
+                            if (stereo) {
+                                wr0 = fma(a00, pr0, fma(a01, pr1, wr0));
+                                wr1 = fma(a10, pr0, fma(a11, pr1, wr1));
+                            }
+                            const double n00 = fma(a00, b00, a01 * b10), n01 = fma(a00, b01, a01 * b11);
+                            const double n10 = fma(a10, b00, a11 * b10), n11 = fma(a10, b01, a11 * b11);
+                            a00 = n00, a01 = n01, a10 = n10, a11 = n11;
+                        }
+                    }
+                    const double zl1 = carry.z[sec][0][0], zl2 = carry.z[sec][0][1];
+                    const double zr1 = carry.z[sec][1][0], zr2 = carry.z[sec][1][1];
+                    const double sl1 = fma(a00, zl1, fma(a01, zl2, wl0)), sl2 = fma(a10, zl1, fma(a11, zl2, wl1));
+                    const double sr1 = fma(a00, zr1, fma(a01, zr2, wr0)), sr2 = fma(a10, zr1, fma(a11, zr2, wr1));
+                    // y_n = b0 u_n + z1_{n-1}
+                    double prevL = up(sl1, 1), prevR = up(sr1, 1);
+                    if (lane == lo) {
+                        prevL = zl1;
+                        prevR = zr1;
+                    }
+                    ul = fma(c[0], ul, prevL);
+                    ur = stereo ? fma(c[0], ur, prevR) : 0.0;
+                    const int last = hi - 1;
+                    const double ql1 = laneValue(sl1, last), ql2 = laneValue(sl2, last);
+                    const double qr1 = laneValue(sr1, last), qr2 = laneValue(sr2, last);
+                    __builtin_amdgcn_wave_barrier();
+                    if (lane == 0) {
+                        carry.z[sec][0][0] = ql1, carry.z[sec][0][1] = ql2;
+                        carry.z[sec][1][0] = qr1, carry.z[sec][1][1] = qr2;
+                    }
+                    __builtin_amdgcn_wave_barrier();
+                }
+            }
+            initialized = true;
+            if (active) {
+                dst[0] = float(ul);
+                if (stereo)
+                    dst[1] = float(ur);
+            }
         }
     }
-    st.filterInitialized = initialized;
+    // Save the chain state for the next segment.
+    if (hasFilter && lane == 0) {
+        int k = 0;
+        for (int f = 0; f < CLAY_FILTER_SLOTS; ++f) {
+            for (int i = 0; i < 5; ++i)
+                st.coefficients[f][i] = carry.coef[f][i];
+            for (int s = 0; s < sectionCount[f]; ++s, ++k)
+                for (int c = 0; c < 2; ++c) {
+                    st.z1[f][s][c] = carry.z[k][c][0];
+                    st.z2[f][s][c] = carry.z[k][c][1];
+                }
+        }
+        st.filterInitialized = initialized;
+    }
 }
 
 // Per output frame: the dry voices of the tile rendered inline, times the job
@@ -773,9 +875,9 @@ __global__ void __launch_bounds__(kTileFrames) clayMixTiles(
 }
 
 #define CLAY_INSTANTIATE_TAPS(T)                                                                                       \
-    template __global__ void clayFilterSources<T>(const int16_t*, const float*, const Span*, const Job*,                \
-                                                  const FilterTile*, const clay_control_point*, const Mod*,              \
-                                                  const double*, float*, int, int, int);                                \
+    template __global__ void clayVoiceChains<T>(const int16_t*, const float*, const Span*, const SpanCold*, const Job*, \
+                                                const int*, const clay_control_point*, const Mod*, const double*,      \
+                                                float*, VoiceState*, int, int, int);                                   \
     template __global__ void clayMixTiles<T, true>(const int16_t*, const float*, const Span*, const Job*, const Tile*, \
                                                    const int*, const clay_control_point*, const Mod*, const float*,     \
                                                    void*, int, int, int);                                               \

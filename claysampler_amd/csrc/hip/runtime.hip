@@ -27,11 +27,10 @@ using namespace clay_dev;
 // Kernels (kernels.hip).
 extern "C" __global__ void clayPitchChains(const Span*, const SpanCold*, const int*, const clay_control_point*,
                                            const Mod*, double*, VoiceState*, int, int, int);
-extern "C" __global__ void clayFilterChains(const Span*, const SpanCold*, const int*, const clay_control_point*,
-                                            const Mod*, float*, VoiceState*, int, int, int);
 template<int T>
-__global__ void clayFilterSources(const int16_t*, const float*, const Span*, const Job*, const FilterTile*,
-                                  const clay_control_point*, const Mod*, const double*, float*, int, int, int);
+__global__ void clayVoiceChains(const int16_t*, const float*, const Span*, const SpanCold*, const Job*, const int*,
+                                const clay_control_point*, const Mod*, const double*, float*, VoiceState*, int, int,
+                                int);
 template<int T, bool FloatOut>
 __global__ void clayMixTiles(const int16_t*, const float*, const Span*, const Job*, const Tile*, const int*,
                              const clay_control_point*, const Mod*, const float*, void*, int, int, int);
@@ -129,8 +128,7 @@ struct clay_hip_batch {
     std::vector<Job> jobs;
     std::vector<Tile> tiles;
     std::vector<int> voices;
-    std::vector<FilterTile> filterTiles;
-    std::vector<int> pitchHeads, filterHeads;
+    std::vector<int> pitchHeads, chainHeads;  // variable-pitch chains; every voice chain
     std::vector<AudioJobDesc> audioJobs;
     std::vector<int> onsets;
     std::vector<int2> groups;
@@ -147,8 +145,7 @@ struct clay_hip_batch {
     Job* dJobs = nullptr;
     Tile* dTiles = nullptr;
     int* dVoices = nullptr;
-    FilterTile* dFilterTiles = nullptr;
-    int *dPitchHeads = nullptr, *dFilterHeads = nullptr;
+    int *dPitchHeads = nullptr, *dChainHeads = nullptr;
     VoiceState* dStates = nullptr;
     AudioJobDesc* dAudioJobs = nullptr;
     int* dOnsets = nullptr;
@@ -430,11 +427,7 @@ void flatten(clay_hip* h, const clay_plan_view* plans, size_t count, clay_hip_ba
                 if (tail < 0) {
                     if (v.variable_pitch)
                         b.pitchHeads.push_back(index);
-                    bool filters = false;
-                    for (const auto& f : v.filters)
-                        filters |= f.type != CLAY_FILTER_NONE;
-                    if (filters)
-                        b.filterHeads.push_back(index);
+                    b.chainHeads.push_back(index);
                 } else {
                     b.spans[size_t(tail)].next = index;
                 }
@@ -445,14 +438,23 @@ void flatten(clay_hip* h, const clay_plan_view* plans, size_t count, clay_hip_ba
                     d.phase = b.pitchFrames;
                     b.pitchFrames += uint64_t(v.end - v.begin);
                 }
-                for (int f = v.begin; f < v.end; f += kTileFrames)
-                    b.filterTiles.push_back({index, f});
             }
             b.spans.push_back(d);
             b.cold.push_back(c);
         }
         b.frames += uint64_t(p.frames);
     }
+    // Longest voice chains first: each is one wave's serial work.
+    std::vector<std::pair<int64_t, int>> order;
+    for (int head : b.chainHeads) {
+        int64_t frames = 0;
+        for (int s = head; s >= 0; s = b.spans[size_t(s)].next)
+            frames += b.spans[size_t(s)].end - b.spans[size_t(s)].begin;
+        order.emplace_back(-frames, head);
+    }
+    std::stable_sort(order.begin(), order.end());
+    for (size_t i = 0; i < order.size(); ++i)
+        b.chainHeads[i] = order[i].second;
     // Volume groups: consecutive jobs sharing a nonzero id.
     if (h->cfg.group_minimum)
         for (size_t first = 0, end; first < count; first = end) {
@@ -479,8 +481,8 @@ void upload(clay_hip* h, clay_hip_batch& b)
     const int ch = h->channels();
     const size_t oSpans = a.add(bytesOf(b.spans)), oCold = a.add(bytesOf(b.cold)), oMods = a.add(bytesOf(b.mods));
     const size_t oPoints = a.add(bytesOf(b.points)), oJobs = a.add(bytesOf(b.jobs)), oTiles = a.add(bytesOf(b.tiles));
-    const size_t oVoices = a.add(bytesOf(b.voices)), oFilterTiles = a.add(bytesOf(b.filterTiles));
-    const size_t oPitch = a.add(bytesOf(b.pitchHeads)), oFilter = a.add(bytesOf(b.filterHeads));
+    const size_t oVoices = a.add(bytesOf(b.voices));
+    const size_t oPitch = a.add(bytesOf(b.pitchHeads)), oChains = a.add(bytesOf(b.chainHeads));
     const size_t oStates = a.add(size_t(b.chains) * sizeof(VoiceState));
     const size_t oAudio = a.add(bytesOf(b.audioJobs)), oOnsets = a.add(bytesOf(b.onsets)), oGroups = a.add(bytesOf(b.groups));
     const size_t oMix = a.add(size_t(b.frames) * (h->natural ? sizeof(float2) : sizeof(short2)));
@@ -509,9 +511,8 @@ void upload(clay_hip* h, clay_hip_batch& b)
     b.dJobs = reinterpret_cast<Job*>(put(oJobs, b.jobs));
     b.dTiles = reinterpret_cast<Tile*>(put(oTiles, b.tiles));
     b.dVoices = reinterpret_cast<int*>(put(oVoices, b.voices));
-    b.dFilterTiles = reinterpret_cast<FilterTile*>(put(oFilterTiles, b.filterTiles));
     b.dPitchHeads = reinterpret_cast<int*>(put(oPitch, b.pitchHeads));
-    b.dFilterHeads = reinterpret_cast<int*>(put(oFilter, b.filterHeads));
+    b.dChainHeads = reinterpret_cast<int*>(put(oChains, b.chainHeads));
     b.dStates = reinterpret_cast<VoiceState*>(base + oStates);
     b.dAudioJobs = reinterpret_cast<AudioJobDesc*>(put(oAudio, b.audioJobs));
     b.dOnsets = reinterpret_cast<int*>(put(oOnsets, b.onsets));
@@ -559,7 +560,7 @@ template<int T>
 void renderSegment(clay_hip* h, clay_hip_batch& b, int segFirst, int segEnd)
 {
     hipStream_t s = h->stream;
-    const int pitchCount = int(b.pitchHeads.size()), filterCount = int(b.filterHeads.size());
+    const int pitchCount = int(b.pitchHeads.size()), chainCount = int(b.chainHeads.size());
     if (pitchCount) {
         const int w = recurrenceWidth(h, pitchCount);
         h->launch("clayPitchChains", [&] {
@@ -567,19 +568,12 @@ void renderSegment(clay_hip* h, clay_hip_batch& b, int segFirst, int segEnd)
                                b.dPitchHeads, b.dPoints, b.dMods, b.dPhase, b.dStates, segFirst, segEnd, pitchCount);
         });
     }
-    if (!b.filterTiles.empty())
-        chunked(int(b.filterTiles.size()), [&](int first, int n) {
-            h->launch("clayFilterSources", [&] {
-                hipLaunchKernelGGL(clayFilterSources<T>, dim3(n), dim3(kTileFrames), 0, s, h->bank, h->sinc, b.dSpans,
-                                   b.dJobs, b.dFilterTiles, b.dPoints, b.dMods, b.dPhase, b.dFiltered, segFirst, segEnd,
-                                   first);
-            });
-        });
-    if (filterCount) {
-        const int w = recurrenceWidth(h, filterCount);
-        h->launch("clayFilterChains", [&] {
-            hipLaunchKernelGGL(clayFilterChains, dim3(ceilDiv(filterCount, w)), dim3(w), 0, s, b.dSpans, b.dCold,
-                               b.dFilterHeads, b.dPoints, b.dMods, b.dFiltered, b.dStates, segFirst, segEnd, filterCount);
+    if (chainCount) {
+        constexpr int wavesPerBlock = 4;
+        h->launch("clayVoiceChains", [&] {
+            hipLaunchKernelGGL(clayVoiceChains<T>, dim3(ceilDiv(chainCount, wavesPerBlock)), dim3(64 * wavesPerBlock), 0,
+                               s, h->bank, h->sinc, b.dSpans, b.dCold, b.dJobs, b.dChainHeads, b.dPoints, b.dMods,
+                               b.dPhase, b.dFiltered, b.dStates, segFirst, segEnd, chainCount);
         });
     }
     chunked(int(b.tiles.size()), [&](int first, int n) {
