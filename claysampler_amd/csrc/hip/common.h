@@ -1,0 +1,448 @@
+// Device helpers shared by the render and finishing kernels (gfx950).
+// Reference behaviour: src/cuda/render.cu:1-336, tail.cu:1-51, quality.cu:1-19.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "device_types.h"
+
+using namespace clay_dev;
+
+namespace {
+
+constexpr float kQuarterPi = .7853981633974483f;
+
+// ---------------------------------------------------------------- envelopes
+// src/cuda/render.cu:6-19
+__device__ __forceinline__ float envelopeSpan(const Span& v, int frame)
+{
+    if (v.releaseStart >= 0 && frame >= v.releaseStart) {
+        if (frame >= v.fadeStart)
+            return fmaxf(0.f, v.fadeValue - float(frame - v.fadeStart + 1) * v.fadeStep);
+        return v.releaseValue * expf(float(frame - v.releaseStart + 1) * v.releaseRate);
+    }
+    if (frame < v.attackStart)
+        return v.start;
+    if (frame < v.holdStart)
+        return fminf(1.f, v.start + float(frame - v.attackStart + 1) * v.attackStep);
+    if (frame < v.decayStart)
+        return 1.f;
+    if (v.decayRate == 0)
+        return v.sustain;
+    return fmaxf(v.sustain, expf(float(frame - v.decayStart + 1) * v.decayRate));
+}
+
+// ---------------------------------------------------------------- controls
+// Last point at or before `frame` (the first point when none), then linear
+// (src/cuda/render.cu:107-118).
+__device__ __forceinline__ float controlAt(const clay_control_point* __restrict__ points, int lo, int hi, int frame)
+{
+    while (lo + 1 < hi) {
+        const int mid = lo + (hi - lo) / 2;
+        if (points[mid].frame <= frame)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    const clay_control_point p = points[lo];
+    return fmaf(p.step, float(frame - p.frame), p.value);
+}
+
+__device__ __forceinline__ float modulationAt(const Mod& m, const clay_control_point* __restrict__ points, int frame)
+{
+    float v = m.first < 0 ? m.depth : m.depth * controlAt(points, m.first, m.last, frame);
+    if (m.shapeFirst >= 0)
+        v *= controlAt(points, m.shapeFirst, m.shapeLast, frame);
+    return v;
+}
+
+__device__ __forceinline__ float jobGain(const clay_control_point* __restrict__ points, const Job& j, int frame)
+{
+    return controlAt(points, j.controls[0], j.controls[1], frame) * controlAt(points, j.controls[1], j.controls[2], frame);
+}
+
+// ---------------------------------------------------------------- resampling
+// Source frame of a span, zero outside [0, sourceEnd) (src/cuda/render.cu:21-28).
+// Values are raw int16 (scaled by 2^-15 by the caller).
+__device__ __forceinline__ float2 fetchRaw(const int16_t* __restrict__ bank, const Span& v, int frame)
+{
+    if (frame < 0 || frame >= v.sourceEnd)
+        return make_float2(0.f, 0.f);
+    const uint64_t i = v.sample + uint64_t(frame) * uint64_t(v.channels);
+    const float left = float(bank[i]);
+    return make_float2(left, v.channels == 1 ? left : float(bank[i + 1]));
+}
+
+// Windowed-sinc tap sum at the nearest of 2048 phases (src/cuda/render.cu:30-61).
+// Accumulating the raw int16 values and scaling once by 2^-15 gives the same
+// float result as weighting the pre-scaled float bank of the reference.
+template<int T>
+__device__ __forceinline__ float2 interpolateAt(const int16_t* __restrict__ bank, const float* __restrict__ sinc,
+                                                const Span& v, double pos)
+{
+    constexpr float scale = 1.f / 32768.f;
+    const double floorPos = floor(pos);
+    const int centre = int(floorPos);
+    if (pos == floorPos) {
+        const float2 s = fetchRaw(bank, v, centre);
+        return make_float2(s.x * scale, s.y * scale);
+    }
+    const int phase = __double2int_rn((pos - floorPos) * kPhases);
+    const float4* __restrict__ w4 = reinterpret_cast<const float4*>(sinc + size_t(phase) * T);
+    constexpr int first = T / 2 - 1;
+    float a0 = 0.f, a1 = 0.f, b0 = 0.f, b1 = 0.f;
+    if (centre >= first && centre < v.sourceEnd - T / 2) {
+        const int16_t* __restrict__ src = bank + v.sample + uint64_t(centre - first) * uint64_t(v.channels);
+        if (v.channels == 1) {
+#pragma unroll
+            for (int q = 0; q < T / 4; ++q) {
+                const float4 w = w4[q];
+                a0 = fmaf(w.x, float(src[4 * q + 0]), a0);
+                a1 = fmaf(w.y, float(src[4 * q + 1]), a1);
+                a0 = fmaf(w.z, float(src[4 * q + 2]), a0);
+                a1 = fmaf(w.w, float(src[4 * q + 3]), a1);
+            }
+            const float m = (a0 + a1) * scale;
+            return make_float2(m, m);
+        }
+#pragma unroll
+        for (int q = 0; q < T / 4; ++q) {
+            const float4 w = w4[q];
+            a0 = fmaf(w.x, float(src[8 * q + 0]), a0);
+            b0 = fmaf(w.x, float(src[8 * q + 1]), b0);
+            a1 = fmaf(w.y, float(src[8 * q + 2]), a1);
+            b1 = fmaf(w.y, float(src[8 * q + 3]), b1);
+            a0 = fmaf(w.z, float(src[8 * q + 4]), a0);
+            b0 = fmaf(w.z, float(src[8 * q + 5]), b0);
+            a1 = fmaf(w.w, float(src[8 * q + 6]), a1);
+            b1 = fmaf(w.w, float(src[8 * q + 7]), b1);
+        }
+        return make_float2((a0 + a1) * scale, (b0 + b1) * scale);
+    }
+    for (int q = 0; q < T / 4; ++q) {
+        const float4 w = w4[q];
+        const float ws[4] = {w.x, w.y, w.z, w.w};
+        for (int k = 0; k < 4; ++k) {
+            const float2 s = fetchRaw(bank, v, centre - first + 4 * q + k);
+            a0 = fmaf(ws[k], s.x, a0);
+            b0 = fmaf(ws[k], s.y, b0);
+        }
+    }
+    return make_float2(a0 * scale, b0 * scale);
+}
+
+// Source position, generators, loop wrap and loop crossfade (src/cuda/render.cu:63-100).
+// The position arithmetic uses explicitly rounded double operations so that
+// the phase index equals the one of a plain IEEE double evaluation.
+template<int T>
+__device__ float2 interpolate(const int16_t* __restrict__ bank, const float* __restrict__ sinc, const Span& v,
+                              int frame, double pos)
+{
+    if (pos < 0) {
+        const int elapsed = frame - v.begin;
+        double distance = elapsed;
+        if (v.pitchSlope != 0)
+            distance = __ddiv_rn(expm1(__dmul_rn(v.pitchSlope, double(elapsed))), expm1(v.pitchSlope));
+        pos = __dadd_rn(v.position, __dmul_rn(distance, v.step));
+    }
+    if (v.generator != CLAY_GEN_NONE) {
+        if (v.generator == CLAY_GEN_NOISE) {
+            unsigned bits = v.noiseSeed + unsigned(frame - v.noiseStart);
+            bits = (bits ^ (bits >> 16)) * 0x7feb352du;
+            bits = (bits ^ (bits >> 15)) * 0x846ca68bu;
+            bits ^= bits >> 16;
+            const float x = float(bits >> 8) * (2.f / 16777216.f) - 1.f;
+            return make_float2(x, x);
+        }
+        const float x = v.generator == CLAY_GEN_SINE ? -sinf(6.28318530717958647692f * float(pos - floor(pos))) : 0.f;
+        return make_float2(x, x);
+    }
+    const int period = v.loopEnd - v.loopFirst;
+    if (v.loopEnd > 0 && pos >= v.loopEnd)
+        pos = __dadd_rn(double(v.loopFirst), fmod(__dsub_rn(pos, double(v.loopFirst)), double(period)));
+    if (pos >= v.sourceEnd)
+        return make_float2(0.f, 0.f);
+    float2 s = interpolateAt<T>(bank, sinc, v, pos);
+    const int fadeStart = v.loopEnd - v.loopCrossfade;
+    if (v.loopEnd > 0 && v.loopCrossfade > 0 && pos >= fadeStart) {
+        const float2 entry = interpolateAt<T>(bank, sinc, v, __dsub_rn(pos, double(period)));
+        const float phase = float((pos - fadeStart) / v.loopCrossfade);
+        const float blend = .5f - .5f * cosf(3.14159265358979323846f * phase);
+        s.x = fmaf(blend, entry.x - s.x, s.x);
+        s.y = fmaf(blend, entry.y - s.y, s.y);
+    }
+    return s;
+}
+
+// src/cuda/render.cu:130-136
+__device__ __forceinline__ float2 panGains(const Span& v, float offset)
+{
+    if (offset == 0)
+        return make_float2(v.left, v.right);
+    const float p = fminf(1.f, fmaxf(-1.f, v.pan + offset));
+    return make_float2(v.gain * cosf((1 + p) * kQuarterPi), v.gain * cosf((1 - p) * kQuarterPi));
+}
+
+// Envelope, modulation, resampling, pan and stereo matrix of one voice frame
+// (src/cuda/render.cu:138-187).
+template<int T>
+__device__ float2 voiceAudio(const int16_t* __restrict__ bank, const float* __restrict__ sinc, const Span& v,
+                             const clay_control_point* __restrict__ points, const Mod* __restrict__ mods, float pan,
+                             int frame, bool filterInput, double phase)
+{
+    float level = envelopeSpan(v, frame);
+    if (level <= 0)
+        return make_float2(0.f, 0.f);
+    float db = 0, widthOffset = 0, positionOffset = 0;
+    for (int j = v.firstMod; j < v.firstMod + v.modCount; ++j) {
+        const Mod m = mods[j];
+        if (m.target > CLAY_MOD_POSITION)
+            continue;
+        const float x = modulationAt(m, points, frame);
+        switch (m.target) {
+        case CLAY_MOD_AMPLITUDE: level *= x; break;
+        case CLAY_MOD_VOLUME: db += x; break;
+        case CLAY_MOD_PAN: pan += x; break;
+        case CLAY_MOD_WIDTH: widthOffset += x; break;
+        case CLAY_MOD_POSITION: positionOffset += x; break;
+        default: break;
+        }
+    }
+    if (db != 0)
+        level *= exp2f(db * .166096404744368f);
+    const float2 s = interpolate<T>(bank, sinc, v, frame, phase);
+    if (filterInput && v.channels == 1)
+        return make_float2(s.x * level, s.x * level);
+    const float2 g = panGains(v, pan);
+    const float l = s.x * g.x, r = s.y * g.y;
+    if (v.stereoMatrix) {
+        float direct = v.widthDirect, cross = v.widthCross;
+        if (widthOffset != 0) {
+            const float w = fminf(1.f, fmaxf(-1.f, v.width + widthOffset));
+            direct = cosf((1 - w) * kQuarterPi);
+            cross = cosf((1 + w) * kQuarterPi);
+        }
+        float pl = v.positionLeft, pr = v.positionRight;
+        if (positionOffset != 0) {
+            const float p = fminf(1.f, fmaxf(-1.f, v.stereoPosition + positionOffset));
+            pl = cosf((1 + p) * kQuarterPi);
+            pr = cosf((1 - p) * kQuarterPi);
+        }
+        return make_float2(fmaf(r, cross, l * direct) * (level * pl), fmaf(l, cross, r * direct) * (level * pr));
+    }
+    return make_float2(l * level, r * level);
+}
+
+// ---------------------------------------------------------------- filters
+// Biquad sections per filter slot (src/cuda/render.cu:312-319).
+__device__ __forceinline__ int sections(int type)
+{
+    if (type == CLAY_FILTER_NONE) return 0;
+    if (type == CLAY_FILTER_PEAK) return 1;
+    if (type >= CLAY_FILTER_LOW6) return 3;
+    if (type >= CLAY_FILTER_LOW4) return 2;
+    return 1;
+}
+
+// ---------------------------------------------------------------- reductions
+template<class T, class Op>
+__device__ __forceinline__ T waveReduce(T v, Op op)
+{
+#pragma unroll
+    for (int d = 32; d; d >>= 1)
+        v = op(v, __shfl_xor(v, d, 64));
+    return v;
+}
+
+// Block reduction for up to 1024 threads; every thread gets the result.
+template<class T, class Op>
+__device__ T blockReduce(T v, T identity, Op op)
+{
+    __shared__ T partial[16];
+    v = waveReduce(v, op);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 0)
+        partial[wave] = v;
+    __syncthreads();
+    const int waves = (blockDim.x + 63) >> 6;
+    T r = identity;
+    for (int i = 0; i < waves; ++i)
+        r = op(r, partial[i]);
+    __syncthreads();
+    return r;
+}
+
+struct Add {
+    template<class T> __device__ T operator()(T a, T b) const { return a + b; }
+};
+struct Max {
+    template<class T> __device__ T operator()(T a, T b) const { return a > b ? a : b; }
+};
+struct Min {
+    template<class T> __device__ T operator()(T a, T b) const { return a < b ? a : b; }
+};
+
+// Index of the last entry whose `first` is <= block (src/cuda/tail.cu:32-44).
+template<class F>
+__device__ __forceinline__ int ownerOf(int count, int block, F firstOf)
+{
+    int lo = 0, hi = count;
+    while (lo + 1 < hi) {
+        const int mid = (lo + hi) / 2;
+        if (firstOf(mid) <= block)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ int uniform(int x) { return __builtin_amdgcn_readfirstlane(x); }
+
+} // namespace
+
+// ----------------------------------------------------------------- FP64 voice-chain recurrences
+namespace {
+
+// Control track value in FP64 (src/cuda/render.cu:107-118 evaluated in double).
+__device__ __forceinline__ double pointValue(const clay_control_point& p, int frame)
+{
+    return fma(double(p.step), double(frame - p.frame), double(p.value));
+}
+
+// Sequential reader of one control track: frames only move forward inside a
+// chain, so the segment is found by advancing instead of a binary search.
+// It lands on the same point as controlAt's search (last point at or before
+// the frame, the first point when none).
+struct Cursor {
+    int at, last;
+    __device__ void reset(const clay_control_point* __restrict__ points, int first, int end, int frame)
+    {
+        at = first;
+        last = end;
+        int lo = first, hi = end;
+        while (lo + 1 < hi) {
+            const int mid = lo + (hi - lo) / 2;
+            if (points[mid].frame <= frame)
+                lo = mid;
+            else
+                hi = mid;
+        }
+        at = lo;
+    }
+    __device__ __forceinline__ double value(const clay_control_point* __restrict__ points, int frame)
+    {
+        while (at + 1 < last && points[at + 1].frame <= frame)
+            ++at;
+        return pointValue(points[at], frame);
+    }
+};
+
+// Up to kCursors modulation bindings of a span read through cursors; any
+// further ones fall back to the binary search.
+constexpr int kCursors = 6;
+
+struct ModReader {
+    Cursor track[kCursors], shape[kCursors];
+    int first, count;
+
+    __device__ void reset(const Mod* __restrict__ mods, const clay_control_point* __restrict__ points, int firstMod,
+                          int modCount, int frame)
+    {
+        first = firstMod;
+        count = modCount;
+#pragma unroll
+        for (int j = 0; j < kCursors; ++j) {
+            if (j >= count)
+                break;
+            const Mod m = mods[first + j];
+            if (m.first >= 0)
+                track[j].reset(points, m.first, m.last, frame);
+            if (m.shapeFirst >= 0)
+                shape[j].reset(points, m.shapeFirst, m.shapeLast, frame);
+        }
+    }
+
+    // src/cuda/render.cu:120-126 in double.
+    __device__ double value(const Mod* __restrict__ mods, const clay_control_point* __restrict__ points, int j, int frame)
+    {
+        const Mod m = mods[first + j];
+        double v = m.depth;
+        if (m.first >= 0) {
+            const double x = j < kCursors ? track[j].value(points, frame) : [&] {
+                Cursor c;
+                c.reset(points, m.first, m.last, frame);
+                return pointValue(points[c.at], frame);
+            }();
+            v = double(m.depth) * x;
+        }
+        if (m.shapeFirst >= 0) {
+            const double x = j < kCursors ? shape[j].value(points, frame) : [&] {
+                Cursor c;
+                c.reset(points, m.shapeFirst, m.shapeLast, frame);
+                return pointValue(points[c.at], frame);
+            }();
+            v *= x;
+        }
+        return v;
+    }
+};
+
+__device__ __forceinline__ double envelopeShapeD(const clay_envelope_shape& v, int frame)
+{
+    if (v.release_start >= 0 && frame >= v.release_start) {
+        if (frame >= v.fade_start)
+            return fmax(0.0, double(v.fade_value) - double(frame - v.fade_start + 1) * double(v.fade_step));
+        return double(v.release_value) * exp(double(frame - v.release_start + 1) * double(v.release_rate));
+    }
+    if (frame < v.attack_start)
+        return v.start;
+    if (frame < v.hold_start)
+        return fmin(1.0, double(v.start) + double(frame - v.attack_start + 1) * double(v.attack_step));
+    if (frame < v.decay_start)
+        return 1.0;
+    if (v.decay_rate == 0)
+        return v.sustain;
+    return fmax(double(v.sustain), exp(double(frame - v.decay_start + 1) * double(v.decay_rate)));
+}
+
+struct CoeffD {
+    double b0, b1, b2, a1, a2;
+};
+
+// src/cuda/render.cu:278-310 in double.
+__device__ inline CoeffD coefficientsD(int type, double cutoff, double resonance, int rate, double bandwidth,
+                                       double gain)
+{
+    constexpr double pi = 3.14159265358979323846;
+    const double w = 2 * pi * fmin(fmin(20000.0, rate * .499), fmax(1.0, cutoff)) / rate;
+    if (type == CLAY_FILTER_LOW1 || type == CLAY_FILTER_HIGH1) {
+        const double p = exp(-w);
+        if (type == CLAY_FILTER_LOW1)
+            return {1 - p, 0, 0, -p, 0};
+        const double h = (1 + p) * .5;
+        return {h, -h, 0, -p, 0};
+    }
+    double s, c;
+    sincos(w, &s, &c);
+    if (type == CLAY_FILTER_PEAK) {
+        const double a = exp10(fmin(60.0, fmax(-96.0, gain)) / 40);
+        const double width = fmin(12.0, fmax(.01, bandwidth));
+        const double alpha = s * sinh(fmin(20.0, 0.34657359027997264 * width * w / s));
+        const double inv = 1 / (1 + alpha / a);
+        const double mid = -2 * c * inv;
+        return {(1 + alpha * a) * inv, mid, (1 - alpha * a) * inv, mid, (1 - alpha / a) * inv};
+    }
+    const double alpha = .5 * s * exp10(-fmin(60.0, fmax(-60.0, resonance)) / 20);
+    const double inv = 1 / (1 + alpha);
+    const double a1 = -2 * c * inv, a2 = (1 - alpha) * inv;
+    if (type == CLAY_FILTER_BAND2 || type == CLAY_FILTER_BAND4 || type == CLAY_FILTER_BAND6)
+        return {alpha * inv, 0, -alpha * inv, a1, a2};
+    if (type == CLAY_FILTER_NOTCH2)
+        return {inv, a1, inv, a1, a2};
+    const bool high = type == CLAY_FILTER_HIGH2 || type == CLAY_FILTER_HIGH4 || type == CLAY_FILTER_HIGH6;
+    const double b = (1 + (high ? c : -c)) * .5 * inv;
+    return {b, (high ? -2 : 2) * b, b, a1, a2};
+}
+
+} // namespace

@@ -1,0 +1,489 @@
+// Finishing kernels (tail policies, PCM16, quality gate, group volume) for
+// gfx950. Reference behaviour: src/cuda/tail.cu and src/cuda/quality.cu.
+// Block reductions are wave64 shuffles combined across at most 16 waves.
+#include "common.h"
+
+// ====================================================================== tail
+
+// Power and peak of each 1024-frame block of the float mix (src/cuda/tail.cu:54-81).
+extern "C" __global__ void clayScanStereo(const Job* __restrict__ jobs, int count, const float2* __restrict__ mix,
+                                          StereoStats* __restrict__ stats, int firstBlock)
+{
+    const int block = firstBlock + blockIdx.x;
+    const Job j = jobs[ownerOf(count, block, [&](int i) { return jobs[i].blockFirst; })];
+    const int first = (block - j.blockFirst) * kTailBlock;
+    const int end = min(first + kTailBlock, j.frames);
+    double power = 0;
+    float peak = 0;
+    for (int f = first + int(threadIdx.x); f < end; f += blockDim.x) {
+        const float2 x = mix[j.mix + uint64_t(f)];
+        if (!isfinite(x.x) || !isfinite(x.y)) {
+            peak = __int_as_float(0x7f800000);
+            continue;
+        }
+        peak = fmaxf(peak, fmaxf(fabsf(x.x), fabsf(x.y)));
+        power += double(__fmul_rn(x.x, x.x)) + double(__fmul_rn(x.y, x.y));
+    }
+    power = blockReduce(power, 0.0, Add{});
+    peak = blockReduce(peak, 0.f, Max{});
+    if (threadIdx.x == 0)
+        stats[block] = {power, peak, 0};
+}
+
+// Natural endpoint, silence-beyond-plan rule and peak repair, one lane per job
+// (src/cuda/tail.cu:83-145).
+extern "C" __global__ void claySelectNative(const Job* __restrict__ jobs, int count, const float2* __restrict__ mix,
+                                            const StereoStats* __restrict__ stats, clay_tail_result* __restrict__ results)
+{
+    const int index = blockIdx.x * blockDim.x + threadIdx.x;
+    if (index >= count)
+        return;
+    const Job j = jobs[index];
+    clay_tail_result r{};
+    float peak = 0;
+    int end = 0;
+    for (int first = 0; first < j.frames; first += kTailBlock) {
+        const StereoStats st = stats[j.blockFirst + first / kTailBlock];
+        if (!isfinite(st.peak)) {
+            r.status = CLAY_TAIL_NONFINITE;
+            results[index] = r;
+            return;
+        }
+        peak = fmaxf(peak, st.peak);
+        const int blockEnd = min(j.limit, first + kTailBlock);
+        if (blockEnd < j.musicEnd)
+            continue;
+        const double threshold = 1e-12 * (blockEnd - first) * 2;
+        double power = st.power;
+        // A serial re-sum decides near-ties in a fixed order.
+        if (fabs(power - threshold) <= threshold * 1e-12) {
+            power = 0;
+            for (int f = first; f < min(blockEnd, j.frames); ++f) {
+                const float2 x = mix[j.mix + uint64_t(f)];
+                power += __fmul_rn(x.x, x.x);
+                power += __fmul_rn(x.y, x.y);
+            }
+        }
+        if (power <= threshold) {
+            end = blockEnd;
+            break;
+        }
+    }
+    if (!end) {
+        const int silent = (j.frames + kTailBlock - 1) / kTailBlock * kTailBlock;
+        if (silent < j.limit)
+            end = min(j.limit, max(silent + kTailBlock, (j.musicEnd + kTailBlock - 1) / kTailBlock * kTailBlock));
+    }
+    if (!end) {
+        r.status = CLAY_TAIL_DURATION;
+    } else {
+        const double attenuation = peak >= 1 ? 20 * log10(double(peak)) + 1 : 0;
+        if (attenuation > 5 + 1e-6) {
+            r.status = CLAY_TAIL_PEAK;
+        } else {
+            r.frames = end;
+            r.data_frames = min(end, j.frames);
+            r.scale = float(32768 * pow(10., -attenuation / 20));
+        }
+    }
+    results[index] = r;
+}
+
+// Mono PCM16 (channel rounding then ties-to-even average) and 50 ms window
+// statistics (src/cuda/tail.cu:147-181).
+extern "C" __global__ void clayQuantizeMono(const Job* __restrict__ jobs, int count, const float2* __restrict__ mix,
+                                            const clay_tail_result* __restrict__ results, int16_t* __restrict__ mono,
+                                            MonoStats* __restrict__ stats, int window, int firstBlock)
+{
+    const int block = firstBlock + blockIdx.x;
+    const int index = ownerOf(count, block, [&](int i) { return jobs[i].windowFirst; });
+    const Job j = jobs[index];
+    const clay_tail_result r = results[index];
+    const int first = (block - j.windowFirst) * window;
+    const int end = min(first + window, r.data_frames);
+    if (first >= end)
+        return;
+    double power = 0;
+    int peak = 0;
+    for (int f = first + int(threadIdx.x); f < end; f += blockDim.x) {
+        const float2 x = mix[j.mix + uint64_t(f)];
+        const int l = __float2int_rn(fminf(32767.f, fmaxf(-32768.f, x.x * r.scale)));
+        const int rr = __float2int_rn(fminf(32767.f, fmaxf(-32768.f, x.y * r.scale)));
+        const int sum = l + rr;
+        const int v = (sum >> 1) + ((sum & 1) & ((sum >> 1) & 1));
+        mono[j.mix + uint64_t(f)] = int16_t(v);
+        power += double(v) * v;
+        peak = max(peak, abs(v));
+    }
+    power = blockReduce(power, 0.0, Add{});
+    peak = blockReduce(peak, 0, Max{});
+    if (threadIdx.x == 0)
+        stats[block] = {power, peak, 0};
+}
+
+// Weak-tail trim, one lane per job (src/cuda/tail.cu:183-213).
+extern "C" __global__ void claySelectTrim(const Job* __restrict__ jobs, int count, const MonoStats* __restrict__ stats,
+                                          clay_tail_result* __restrict__ results, TailConfig cfg)
+{
+    const int index = blockIdx.x * blockDim.x + threadIdx.x;
+    if (index >= count || results[index].status != CLAY_TAIL_OK)
+        return;
+    clay_tail_result& r = results[index];
+    const MonoStats* w = stats + jobs[index].windowFirst;
+    const int windows = (r.data_frames + cfg.window - 1) / cfg.window;
+    double maxPower = 0;
+    for (int i = 0; i < windows; ++i)
+        maxPower = fmax(maxPower, w[i].power);
+    const double threshold = fmin(cfg.ceiling, fmax(cfg.floor, sqrt(maxPower / cfg.window) * cfg.relative));
+    for (int i = windows - 1; i >= 0; --i) {
+        if (sqrt(w[i].power / cfg.window) > threshold || w[i].peak > threshold * 4) {
+            const int lastSound = min(r.frames, (i + 1) * cfg.window);
+            if (r.frames - lastSound >= cfg.quietFrames) {
+                r.frames = lastSound + cfg.padding;
+                r.data_frames = min(r.frames, r.data_frames);
+                r.fade = min(r.frames, cfg.fade);
+            }
+            break;
+        }
+    }
+}
+
+// Output offsets: exclusive prefix of dataFrames (src/cuda/tail.cu:216-238).
+// orig/custom results are initialised from the plan here.
+extern "C" __global__ void clayLayoutPcm(const Job* __restrict__ jobs, int count, clay_tail_result* __restrict__ results,
+                                         int fromPlan)
+{
+    if (threadIdx.x || blockIdx.x)
+        return;
+    uint64_t offset = 0;
+    for (int i = 0; i < count; ++i) {
+        clay_tail_result& r = results[i];
+        if (fromPlan) {
+            r = clay_tail_result{};
+            r.frames = jobs[i].frames;
+            r.data_frames = jobs[i].frames;
+            r.fade = jobs[i].fade;
+        }
+        r.offset = offset;
+        r.peak = 0;
+        r.first = r.frames;
+        r.last = -1;
+        offset += uint64_t(r.data_frames);
+    }
+}
+
+// Fade, pack, peak, audible extent and clip count per 1024-frame block
+// (src/cuda/tail.cu:240-297). `channels` is 1 (natural) or 2.
+extern "C" __global__ void clayFinalizePcm(const Job* __restrict__ jobs, int count, const int16_t* __restrict__ input,
+                                           int16_t* __restrict__ output, clay_tail_result* __restrict__ results,
+                                           int channels, int countClipped, int firstBlock)
+{
+    const int block = firstBlock + blockIdx.x;
+    const int index = ownerOf(count, block, [&](int i) { return jobs[i].blockFirst; });
+    const Job j = jobs[index];
+    const uint64_t outAt = results[index].offset;
+    const int frames = results[index].frames, fade = results[index].fade;
+    const int first = (block - j.blockFirst) * kTailBlock;
+    const int end = min(first + kTailBlock, results[index].data_frames);
+    if (first >= end)
+        return;
+    int peak = 0, audibleFirst = frames, audibleLast = -1, clipped = 0;
+    const int fadeStart = frames - fade;
+    constexpr double pi = 3.14159265358979323846;
+    for (int f = first + int(threadIdx.x); f < end; f += blockDim.x) {
+        int framePeak = 0;
+        for (int c = 0; c < channels; ++c) {
+            int v = input[(j.mix + uint64_t(f)) * uint64_t(channels) + uint64_t(c)];
+            if (f >= fadeStart) {
+                const int i = f - fadeStart;
+                const double gain = fade == 1 ? 0 : .5 * (1 + cos(pi * i / (fade - 1)));
+                v = int16_t(__double2int_rn(v * gain));
+            }
+            output[(outAt + uint64_t(f)) * uint64_t(channels) + uint64_t(c)] = int16_t(v);
+            framePeak = max(framePeak, abs(v));
+            clipped += abs(v) >= 32767;
+        }
+        peak = max(peak, framePeak);
+        if (framePeak > 3) {
+            audibleFirst = min(audibleFirst, f);
+            audibleLast = f;
+        }
+    }
+    peak = blockReduce(peak, 0, Max{});
+    audibleFirst = blockReduce(audibleFirst, frames, Min{});
+    audibleLast = blockReduce(audibleLast, -1, Max{});
+    if (countClipped) {
+        clipped = blockReduce(clipped, 0, Add{});
+        if (threadIdx.x == 0 && clipped)
+            atomicAdd(&results[index].audio.clipped, clipped);
+    }
+    if (threadIdx.x == 0 && audibleLast >= 0) {
+        atomicMax(&results[index].peak, peak);
+        atomicMin(&results[index].first, audibleFirst);
+        atomicMax(&results[index].last, audibleLast);
+    }
+}
+
+// ====================================================================== quality
+
+namespace {
+
+__device__ __forceinline__ unsigned long long monoPower(const int16_t* pcm, int channels, int frame)
+{
+    const int v = channels == 1 ? 2 * int(pcm[frame]) : int(pcm[2 * frame]) + int(pcm[2 * frame + 1]);
+    return (unsigned long long)((long long)v * v);
+}
+
+__device__ __forceinline__ int bodyBlockStart(int block, const clay_audio_config& a)
+{
+    const int chunk = 2 * a.rate, span = 5 * a.window;
+    const int windows = (chunk + span - 1) / span;
+    return (block / windows) * chunk + (block % windows) * span;
+}
+
+__device__ double levelOf(unsigned long long power, int window)
+{
+    return power ? fmax(-120., 10 * log10(double(power) / (window * 4294967296.))) : -120.;
+}
+
+} // namespace
+
+// Sum of (L+R)^2 over 250 ms body windows (src/cuda/quality.cu:21-45).
+extern "C" __global__ void clayMeasureBody(const AudioJobDesc* __restrict__ jobs, int count, const int16_t* __restrict__ pcm,
+                                           const clay_tail_result* __restrict__ results,
+                                           unsigned long long* __restrict__ powers, clay_audio_config cfg, int channels,
+                                           int firstBlock)
+{
+    const int block = firstBlock + blockIdx.x;
+    const int index = ownerOf(count, block, [&](int i) { return jobs[i].bodyFirst; });
+    const clay_tail_result& r = results[index];
+    if (r.status != CLAY_TAIL_OK)
+        return;
+    const AudioJobDesc j = jobs[index];
+    const int w = block - j.bodyFirst;
+    const int first = bodyBlockStart(w, cfg);
+    const int end = min(min(j.bodyFrames, r.data_frames), bodyBlockStart(w + 1, cfg));
+    const int16_t* audio = pcm + r.offset * uint64_t(channels);
+    unsigned long long power = 0;
+    for (int f = first + int(threadIdx.x); f < end; f += blockDim.x)
+        power += monoPower(audio, channels, f);
+    power = blockReduce(power, 0ull, Add{});
+    if (threadIdx.x == 0)
+        powers[block] = power;
+}
+
+// Loudest of five 50 ms windows after each onset; one wave per window
+// (src/cuda/quality.cu:47-77). Launched with 5 waves (320 threads).
+extern "C" __global__ void clayMeasureOnsets(const AudioJobDesc* __restrict__ jobs, int count, const int* __restrict__ onsets,
+                                             const int16_t* __restrict__ pcm, const clay_tail_result* __restrict__ results,
+                                             unsigned long long* __restrict__ powers, int window, int channels,
+                                             int firstBlock)
+{
+    const int block = firstBlock + blockIdx.x;
+    const int index = ownerOf(count, block, [&](int i) { return jobs[i].onsetFirst; });
+    const clay_tail_result& r = results[index];
+    if (r.status != CLAY_TAIL_OK)
+        return;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int first = onsets[block] + wave * window;
+    const int end = min(first + window, r.data_frames);
+    const int16_t* audio = pcm + r.offset * uint64_t(channels);
+    unsigned long long power = 0;
+    for (int f = first + lane; f < end; f += 64)
+        power += monoPower(audio, channels, f);
+    power = waveReduce(power, Add{});
+    __shared__ unsigned long long maxima[5];
+    if (lane == 0)
+        maxima[wave] = power;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long best = maxima[0];
+        for (int k = 1; k < 5; ++k)
+            best = best > maxima[k] ? best : maxima[k];
+        powers[block] = best;
+    }
+}
+
+// Body RMS, silence, onset percentiles and the verdict, one lane per job
+// (src/cuda/quality.cu:79-170). Onset powers are sorted in place.
+extern "C" __global__ void claySummarizeAudio(const AudioJobDesc* __restrict__ jobs, int count,
+                                              const unsigned long long* __restrict__ body,
+                                              unsigned long long* __restrict__ onsets,
+                                              clay_tail_result* __restrict__ results, clay_audio_config cfg, int channels)
+{
+    const int index = blockIdx.x * blockDim.x + threadIdx.x;
+    if (index >= count || results[index].status != CLAY_TAIL_OK)
+        return;
+    const AudioJobDesc j = jobs[index];
+    clay_tail_result& r = results[index];
+    clay_audio_metrics& st = r.audio;
+    const int end = min(r.frames, j.bodyFrames);
+    unsigned long long total = 0;
+    int silent = 0, run = 0;
+    for (int w = 0, first = 0; first < end; ++w) {
+        const int next = min(end, bodyBlockStart(w + 1, cfg));
+        const unsigned long long wp = body[j.bodyFirst + w];
+        total += wp;
+        if (cfg.enabled) {
+            if (double(wp) < (next - first) * cfg.silence_power) {
+                silent += next - first;
+                run += next - first;
+                st.longest_silence = max(run, st.longest_silence);
+            } else {
+                run = 0;
+            }
+        }
+        first = next;
+    }
+    st.body_rms = j.bodyFrames ? sqrt(double(total) / j.bodyFrames) * .5 : 0;
+    if (!cfg.enabled)
+        return;
+    st.measured = 1;
+    st.median = st.quartile = -120;
+    st.silent_fraction = double(silent) / max(1, end);
+    st.clipped_fraction = double(st.clipped) / (double(max(1, r.frames)) * channels);
+    if (j.onsetCount) {
+        unsigned long long* v = onsets + j.onsetFirst;
+        const int n = j.onsetCount;
+        // Heap sort, ascending.
+        const auto sift = [&](int root, int size) {
+            const unsigned long long value = v[root];
+            for (int child = 2 * root + 1; child < size; child = 2 * root + 1) {
+                if (child + 1 < size && v[child] < v[child + 1])
+                    ++child;
+                if (value >= v[child])
+                    break;
+                v[root] = v[child];
+                root = child;
+            }
+            v[root] = value;
+        };
+        for (int root = n / 2; root-- > 0;)
+            sift(root, n);
+        for (int last = n - 1; last > 0; --last) {
+            const unsigned long long top = v[0];
+            v[0] = v[last];
+            v[last] = top;
+            sift(0, last);
+        }
+        const auto percentile = [&](double fraction) {
+            const double position = (n - 1) * fraction;
+            const int lo = int(position);
+            const double lower = levelOf(v[lo], cfg.window);
+            return lo + 1 < n ? lower + (levelOf(v[lo + 1], cfg.window) - lower) * (position - lo) : lower;
+        };
+        st.median = percentile(.5);
+        st.quartile = percentile(.25);
+    }
+    if (r.frames > cfg.max_frames) r.status = CLAY_TAIL_AUDIO_DURATION;
+    else if (!j.onsetCount || st.median < cfg.min_median || st.quartile < cfg.min_quartile) r.status = CLAY_TAIL_QUIET_PERFORMANCE;
+    else if (st.silent_fraction > cfg.max_silent_fraction || st.longest_silence > cfg.max_silent_frames) r.status = CLAY_TAIL_EXCESSIVE_SILENCE;
+    else if (r.peak < cfg.min_peak) r.status = CLAY_TAIL_SILENT_AUDIO;
+    else if (st.clipped_fraction > cfg.max_clipped_fraction) r.status = CLAY_TAIL_CLIPPING;
+    st.verdict = r.status;
+}
+
+// Shared gain per volume group (src/cuda/quality.cu:172-209). `groups` holds
+// [first, end) batch job ranges.
+extern "C" __global__ void clayMatchVolume(const int2* __restrict__ groups, int count, int minimum,
+                                           clay_tail_result* __restrict__ results)
+{
+    const int index = blockIdx.x * blockDim.x + threadIdx.x;
+    if (index >= count)
+        return;
+    const int2 g = groups[index];
+    double desired = 0, safe = 1e30;
+    int accepted = 0;
+    for (int i = g.x; i < g.y; ++i) {
+        clay_tail_result& r = results[i];
+        if (r.status != CLAY_TAIL_OK)
+            continue;
+        const double rms = r.audio.body_rms;
+        if (rms <= 0 || r.last < 0) {
+            r.status = CLAY_TAIL_SILENT_BODY;
+            continue;
+        }
+        desired += rms;
+        safe = fmin(safe, 32767 * .8912509381337456 * rms / max(1, r.peak));
+        ++accepted;
+    }
+    const double target = fmin(desired / max(1, accepted), safe);
+    for (int i = g.x; i < g.y; ++i) {
+        clay_tail_result& r = results[i];
+        if (r.status != CLAY_TAIL_OK)
+            continue;
+        if (accepted < minimum) {
+            r.status = CLAY_TAIL_INCOMPLETE_GROUP;
+        } else {
+            r.audio.target_rms = target;
+            r.audio.gain = target / r.audio.body_rms;
+        }
+    }
+}
+
+// Compaction offsets of accepted jobs (src/cuda/quality.cu:211-232).
+extern "C" __global__ void clayLayoutAccepted(const Job* __restrict__ jobs, int count, clay_tail_result* __restrict__ results,
+                                              PcmJob* __restrict__ pcm)
+{
+    if (threadIdx.x || blockIdx.x)
+        return;
+    uint64_t offset = 0;
+    for (int i = 0; i < count; ++i) {
+        clay_tail_result& r = results[i];
+        pcm[i] = {r.offset, jobs[i].blockFirst, 0};
+        if (r.status != CLAY_TAIL_OK || r.last < 0)
+            r.data_frames = 0;
+        r.offset = offset;
+        offset += uint64_t(r.data_frames);
+        if (r.audio.gain > 0) {
+            r.peak = 0;
+            r.first = r.frames;
+            r.last = -1;
+        }
+    }
+}
+
+// Group gain applied while packing accepted PCM (src/cuda/quality.cu:234-280).
+extern "C" __global__ void clayPackAccepted(const PcmJob* __restrict__ jobs, int count, const int16_t* __restrict__ source,
+                                            int16_t* __restrict__ output, clay_tail_result* __restrict__ results,
+                                            int channels, int firstBlock)
+{
+    const int block = firstBlock + blockIdx.x;
+    const int index = ownerOf(count, block, [&](int i) { return jobs[i].blockFirst; });
+    const PcmJob j = jobs[index];
+    const clay_tail_result& r = results[index];
+    const int first = (block - j.blockFirst) * kTailBlock;
+    const int end = min(first + kTailBlock, r.data_frames);
+    if (first >= end)
+        return;
+    const double gain = r.audio.gain;
+    const int16_t* in = source + j.sourceOffset * uint64_t(channels);
+    int16_t* out = output + r.offset * uint64_t(channels);
+    int peak = 0, audibleFirst = r.frames, audibleLast = -1;
+    for (int f = first + int(threadIdx.x); f < end; f += blockDim.x) {
+        int framePeak = 0;
+        for (int c = 0; c < channels; ++c) {
+            const int i = f * channels + c;
+            const int16_t v = gain > 0 ? int16_t(__double2int_rn(in[i] * gain)) : in[i];
+            out[i] = v;
+            framePeak = max(framePeak, abs(int(v)));
+        }
+        peak = max(peak, framePeak);
+        if (framePeak > 3) {
+            audibleFirst = min(audibleFirst, f);
+            audibleLast = f;
+        }
+    }
+    if (gain > 0) {
+        peak = blockReduce(peak, 0, Max{});
+        audibleFirst = blockReduce(audibleFirst, r.frames, Min{});
+        audibleLast = blockReduce(audibleLast, -1, Max{});
+        if (threadIdx.x == 0) {
+            atomicMax(&results[index].peak, peak);
+            if (audibleLast >= 0) {
+                atomicMin(&results[index].first, audibleFirst);
+                atomicMax(&results[index].last, audibleLast);
+            }
+        }
+    }
+}

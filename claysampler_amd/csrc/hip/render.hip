@@ -1,0 +1,476 @@
+// Render kernels of the ClaySampler core for gfx950 (MI355X), per segment:
+//   clayPitchChains -> clayVoiceChains -> clayMixTiles
+// Reference behaviour: src/cuda/render.cu. Written for wave64: per-voice
+// parameters are wave-uniform (scalar loads), voice chains are wave-parallel
+// FP64 scans, and the mix sums voices in span order.
+#include "common.h"
+
+// ====================================================================== render
+
+// Variable-pitch recurrence per voice chain: per-frame source phase
+// (src/cuda/render.cu:222-266), in FP64. One lane per chain.
+extern "C" __global__ void clayPitchChains(const Span* __restrict__ spans, const SpanCold* __restrict__ cold,
+                                           const int* __restrict__ heads, const clay_control_point* __restrict__ points,
+                                           const Mod* __restrict__ mods, double* __restrict__ phase,
+                                           VoiceState* __restrict__ states, int segFirst, int segEnd, int count)
+{
+    const int track = blockIdx.x * blockDim.x + threadIdx.x;
+    if (track >= count)
+        return;
+    const int head = heads[track];
+    if (!spans[head].variablePitch)
+        return;
+    VoiceState& st = states[spans[head].filterTrack];  // one state per voice chain
+    double position = st.pitchInitialized ? st.position : spans[head].position;
+    int prevLoopEnd = st.previousLoopEnd, prevLoopStart = st.previousLoopStart;
+    ModReader reader;
+    for (int s = head; s >= 0; s = spans[s].next) {
+        const Span& v = spans[s];
+        if (v.end <= segFirst || v.begin >= segEnd)
+            continue;
+        const bool resumed = st.pitchInitialized && st.pitchSpan == s;
+        if (!resumed && !v.loopEnd && prevLoopEnd > 0 && position >= prevLoopEnd)
+            position = prevLoopStart + fmod(position - prevLoopStart, double(prevLoopEnd - prevLoopStart));
+        prevLoopEnd = v.loopEnd;
+        prevLoopStart = v.loopFirst;
+        const double slope = exp(v.pitchSlope);
+        double step = resumed ? st.step : v.step;
+        const clay_envelope_shape& pitchEnv = cold[s].extra[1];
+        const bool envActive = cold[s].active[1] != 0;
+        const int first = max(v.begin, segFirst), stop = min(v.end, segEnd);
+        reader.reset(mods, points, v.firstMod, v.modCount, first);
+        for (int f = first; f < stop; ++f) {
+            phase[v.phase + uint64_t(f - first)] = position;
+            double cents = 0, depth = pitchEnv.depth;
+            for (int j = 0; j < v.modCount; ++j) {
+                const int target = mods[v.firstMod + j].target;
+                if (target == CLAY_MOD_PITCH)
+                    cents += reader.value(mods, points, j, f);
+                else if (target == CLAY_MOD_PITCHEG_DEPTH)
+                    depth += reader.value(mods, points, j, f);
+            }
+            if (envActive)
+                cents += depth * envelopeShapeD(pitchEnv, f);
+            position += step * exp2(fmin(24000.0, fmax(-24000.0, cents)) / 1200);
+            step *= slope;
+        }
+        st.pitchInitialized = 1;
+        st.pitchSpan = s;
+        st.position = position;
+        st.step = step;
+        st.previousLoopEnd = prevLoopEnd;
+        st.previousLoopStart = prevLoopStart;
+    }
+}
+
+// ----------------------------------------------------------------- wave-parallel voice chains
+namespace {
+
+constexpr int kMaxSections = 9;   // 2 filters x up to 3 sections + 3 EQ bands
+constexpr int kChainWaves = 4;    // voice chains per 256-thread block
+
+// Per-wave carry between 64-frame chunks, in LDS (broadcast reads).
+struct ChainCarry {
+    double z[kMaxSections][2][2];        // [section][channel][z1, z2]
+    double coef[CLAY_FILTER_SLOTS][5];   // current (smoothed) coefficients
+    double target[CLAY_FILTER_SLOTS][5]; // last target
+    double input[CLAY_FILTER_SLOTS][3];  // inputs of the last target
+    int have[CLAY_FILTER_SLOTS];
+};
+
+__device__ __forceinline__ double up(double v, int d) { return __shfl_up(v, d, 64); }
+
+__device__ __forceinline__ double laneValue(double v, int lane) { return __shfl(v, lane, 64); }
+
+} // namespace
+
+// One wave per voice chain (a voice's spans with filters and/or variable pitch),
+// 64 output frames per step, chunks aligned to absolute frame multiples of 64:
+//   1. the 64 frames of voice audio before pan (mono) times the job gain, one per lane;
+//   2. filter targets where a lane's inputs differ from the previous frame's,
+//      copied forward to the following lanes (src/cuda/render.cu:376-406);
+//   3. coefficient smoothing c_n = c_{n-1} + a (t_n - c_{n-1}) as an inclusive
+//      scan of affine maps c -> M c + V over the 64 lanes;
+//   4. each biquad section (direct form II transposed, render.cu:330-336) as a
+//      scan of 2x2 affine maps s_n = A_n s_{n-1} + B_n u_n, in FP64;
+//   5. the filtered frames written to the float scratch the mix kernel reads.
+// State carries across chunks in LDS and across segments in VoiceState.
+template<int T>
+__global__ void __launch_bounds__(64 * kChainWaves) clayVoiceChains(
+    const int16_t* __restrict__ bank, const float* __restrict__ sinc, const Span* __restrict__ spans,
+    const SpanCold* __restrict__ cold, const Job* __restrict__ jobs, const int* __restrict__ heads,
+    const clay_control_point* __restrict__ points, const Mod* __restrict__ mods, const double* __restrict__ phase,
+    float* __restrict__ filtered, VoiceState* __restrict__ states, int segFirst, int segEnd, int count)
+{
+    __shared__ ChainCarry carries[kChainWaves];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int chain = blockIdx.x * kChainWaves + wave;
+    if (chain >= count)
+        return;  // uniform per wave; no block barriers below
+    ChainCarry& carry = carries[wave];
+    const int head = uniform(heads[chain]);
+    VoiceState& st = states[spans[head].filterTrack];
+    int sectionCount[CLAY_FILTER_SLOTS], total = 0;
+    for (int f = 0; f < CLAY_FILTER_SLOTS; ++f) {
+        sectionCount[f] = sections(cold[head].filters[f].type);
+        total += sectionCount[f];
+    }
+    const bool hasFilter = total > 0;
+    bool initialized = st.filterInitialized != 0;
+    if (hasFilter && lane == 0) {
+        int k = 0;
+        for (int f = 0; f < CLAY_FILTER_SLOTS; ++f) {
+            for (int i = 0; i < 5; ++i)
+                carry.coef[f][i] = st.coefficients[f][i];
+            carry.have[f] = 0;
+            for (int s = 0; s < sectionCount[f]; ++s, ++k)
+                for (int c = 0; c < 2; ++c) {
+                    carry.z[k][c][0] = st.z1[f][s][c];
+                    carry.z[k][c][1] = st.z2[f][s][c];
+                }
+        }
+    }
+    __builtin_amdgcn_wave_barrier();
+    for (int s = head; s >= 0; s = spans[s].next) {
+        const Span& v = spans[s];
+        if (v.end <= segFirst || v.begin >= segEnd)
+            continue;
+        const SpanCold& cv = cold[s];
+        const Job& j = jobs[v.job];
+        const int first = max(v.begin, segFirst), stop = min(v.end, segEnd);
+        const bool stereo = v.channels == 2;
+        const int rate = cv.sampleRate;
+        const double amount = 1 - exp(-1000.0 / rate);
+        bool moving = cv.active[0] != 0;
+        for (int k = v.firstMod; k < v.firstMod + v.modCount; ++k) {
+            const int t = mods[k].target;
+            moving |= (t >= CLAY_MOD_CUTOFF && t <= CLAY_MOD_RESONANCE2) || t == CLAY_MOD_FILEG_DEPTH ||
+                      t >= CLAY_MOD_EQ1_FREQUENCY;
+        }
+        ModReader reader;
+        reader.reset(mods, points, v.firstMod, v.modCount, max(first, (first & ~63) + lane));
+        for (int f0 = first & ~63; f0 < stop; f0 += 64) {
+            const int frame = f0 + lane;
+            const bool active = frame >= first && frame < stop;
+            const int lo = max(first, f0) - f0, hi = min(stop, f0 + 64) - f0;  // active lanes [lo, hi)
+            float xl = 0.f, xr = 0.f;
+            if (active) {
+                const int local = frame - first;
+                const double ph = v.variablePitch ? phase[v.phase + uint64_t(local)] : -1.0;
+                const float pan = controlAt(points, j.controls[2], j.controls[3], frame);
+                const float2 a = voiceAudio<T>(bank, sinc, v, points, mods, pan, frame, true, ph);
+                const float g = jobGain(points, j, frame);
+                xl = a.x * g;
+                xr = a.y * g;
+            }
+            float* dst = filtered + v.filtered + uint64_t(max(frame - first, 0)) * uint64_t(v.channels);
+            if (!hasFilter) {
+                if (active) {
+                    dst[0] = xl;
+                    if (stereo)
+                        dst[1] = xr;
+                }
+                continue;
+            }
+            // Filter inputs of this lane's frame for slot f: cutoff (Hz) and
+            // resonance, or for EQ slots frequency, bandwidth and gain.
+            const auto slotInputs = [&](int f, double& in0, double& in1, double& in2) {
+                double cents = 0, db = 0, depth = cv.extra[0].depth, eq0 = 0, eq1 = 0, eq2 = 0;
+                const auto add = [&](int k) {
+                    const int t = mods[v.firstMod + k].target;
+                    if (f == 0 && t == CLAY_MOD_CUTOFF) cents += reader.value(mods, points, k, frame);
+                    else if (f == 0 && t == CLAY_MOD_RESONANCE) db += reader.value(mods, points, k, frame);
+                    else if (f == 0 && t == CLAY_MOD_FILEG_DEPTH) depth += reader.value(mods, points, k, frame);
+                    else if (f == 1 && t == CLAY_MOD_CUTOFF2) cents += reader.value(mods, points, k, frame);
+                    else if (f == 1 && t == CLAY_MOD_RESONANCE2) db += reader.value(mods, points, k, frame);
+                    else if (f >= 2 && t >= CLAY_MOD_EQ1_FREQUENCY && (t - CLAY_MOD_EQ1_FREQUENCY) / 3 == f - 2) {
+                        const int field = (t - CLAY_MOD_EQ1_FREQUENCY) % 3;
+                        const double x = reader.value(mods, points, k, frame);
+                        if (field == 0) eq0 += x;
+                        else if (field == 1) eq1 += x;
+                        else eq2 += x;
+                    }
+                };
+                // Unrolled over the cursor slots so the cursors stay in registers.
+#pragma unroll
+                for (int k = 0; k < kCursors; ++k)
+                    if (k < v.modCount)
+                        add(k);
+                for (int k = kCursors; k < v.modCount; ++k)
+                    add(k);
+                const double eq[3] = {eq0, eq1, eq2};
+                const clay_voice_filter& flt = cv.filters[f];
+                if (f < 2) {
+                    if (f == 0 && cv.active[0])
+                        cents += depth * envelopeShapeD(cv.extra[0], frame);
+                    in0 = double(flt.cutoff) * exp2(cents / 1200);
+                    in1 = double(flt.resonance) + db;
+                    in2 = 0;
+                } else {
+                    in0 = double(flt.cutoff) + eq[0];
+                    in1 = double(flt.bandwidth) + eq[1];
+                    in2 = double(flt.gain) + eq[2];
+                }
+            };
+            const bool recompute = moving || !initialized;
+            int sec = 0;
+            double ul = xl, ur = xr;
+            for (int f = 0; f < CLAY_FILTER_SLOTS; ++f) {
+                const int slotSections = sections(cold[head].filters[f].type);
+                if (!slotSections)
+                    continue;
+                double c[5];
+                if (recompute) {
+                    double in0 = 0, in1 = 0, in2 = 0;
+                    if (active)
+                        slotInputs(f, in0, in1, in2);
+                    // 2. Targets: recompute where inputs change, copy forward otherwise.
+                    double p0 = up(in0, 1), p1 = up(in1, 1), p2 = up(in2, 1);
+                    bool prevValid = lane > lo;
+                    if (lane == lo) {
+                        p0 = carry.input[f][0], p1 = carry.input[f][1], p2 = carry.input[f][2];
+                        prevValid = carry.have[f] != 0;
+                    }
+                    const bool flag = active && (!prevValid || in0 != p0 || in1 != p1 || in2 != p2);
+                    const bool anyFlag = __ballot(flag) != 0;
+                    double t[5] = {0, 0, 0, 0, 0};
+                    if (!anyFlag) {
+                        for (int i = 0; i < 5; ++i)
+                            t[i] = carry.target[f][i];
+                    } else {
+                        if (flag) {
+                            const CoeffD q = f < 2 ? coefficientsD(cv.filters[f].type, in0, in1, rate, 0, 0)
+                                                   : coefficientsD(cv.filters[f].type, in0, 0, rate, in1, in2);
+                            t[0] = q.b0, t[1] = q.b1, t[2] = q.b2, t[3] = q.a1, t[4] = q.a2;
+                        }
+                        int have = flag ? 1 : 0;
+                        for (int d = 1; d < 64; d <<= 1) {
+                            const int hp = __shfl_up(have, d, 64);
+                            double tp[5];
+                            for (int i = 0; i < 5; ++i)
+                                tp[i] = up(t[i], d);
+                            if (!have && lane >= d && hp) {
+                                for (int i = 0; i < 5; ++i)
+                                    t[i] = tp[i];
+                                have = 1;
+                            }
+                        }
+                        if (!have)
+                            for (int i = 0; i < 5; ++i)
+                                t[i] = carry.target[f][i];
+                    }
+                    if (!anyFlag && moving && initialized) {
+                        // 3a. Constant target over the chunk: c_n = t + q^k (c - t), q = 1 - a,
+                        // k = frames smoothed so far in this chunk.
+                        const int k = lane - lo + 1;
+                        double qk = 1, qp = 1 - amount;
+                        for (int bit = 1; bit <= 64; bit <<= 1, qp *= qp)
+                            if (k & bit)
+                                qk *= qp;
+                        for (int i = 0; i < 5; ++i)
+                            c[i] = fma(qk, carry.coef[f][i] - t[i], t[i]);
+                    } else {
+                        // 3b. Smoothing scan: c_n = M_n c_{n-1} + V_n.
+                        // A filter without modulation takes its coefficients once, unsmoothed.
+                        double M, V[5];
+                        if (!active || (!moving && lane != lo)) {
+                            M = 1;
+                            for (int i = 0; i < 5; ++i)
+                                V[i] = 0;
+                        } else if (!initialized && lane == lo) {
+                            M = 0;
+                            for (int i = 0; i < 5; ++i)
+                                V[i] = t[i];
+                        } else {
+                            M = 1 - amount;
+                            for (int i = 0; i < 5; ++i)
+                                V[i] = amount * t[i];
+                        }
+                        for (int d = 1; d < 64; d <<= 1) {
+                            const double Mp = up(M, d);
+                            double Vp[5];
+                            for (int i = 0; i < 5; ++i)
+                                Vp[i] = up(V[i], d);
+                            if (lane >= d) {
+                                for (int i = 0; i < 5; ++i)
+                                    V[i] = fma(M, Vp[i], V[i]);
+                                M *= Mp;
+                            }
+                        }
+                        for (int i = 0; i < 5; ++i)
+                            c[i] = fma(M, carry.coef[f][i], V[i]);
+                    }
+                    // Carry: values of the last active lane.
+                    const int last = hi - 1;
+                    double nc[5], nt[5], ni[3];
+                    for (int i = 0; i < 5; ++i) {
+                        nc[i] = laneValue(c[i], last);
+                        nt[i] = laneValue(t[i], last);
+                    }
+                    ni[0] = laneValue(in0, last);
+                    ni[1] = laneValue(in1, last);
+                    ni[2] = laneValue(in2, last);
+                    __builtin_amdgcn_wave_barrier();
+                    if (lane == 0) {
+                        for (int i = 0; i < 5; ++i) {
+                            carry.coef[f][i] = nc[i];
+                            carry.target[f][i] = nt[i];
+                        }
+                        for (int i = 0; i < 3; ++i)
+                            carry.input[f][i] = ni[i];
+                        carry.have[f] = 1;
+                    }
+                    __builtin_amdgcn_wave_barrier();
+                } else {
+                    for (int i = 0; i < 5; ++i)
+                        c[i] = carry.coef[f][i];
+                }
+                // 4. Biquad sections of this slot: s_n = A_n s_{n-1} + B_n u_n.
+                for (int k = 0; k < slotSections; ++k, ++sec) {
+                    // A = [[-a1, 1], [-a2, 0]]; inactive lanes are the identity.
+                    double a00 = -c[3], a01 = 1, a10 = -c[4], a11 = 0;
+                    double wl0 = (c[1] - c[3] * c[0]) * ul, wl1 = (c[2] - c[4] * c[0]) * ul;
+                    double wr0 = (c[1] - c[3] * c[0]) * ur, wr1 = (c[2] - c[4] * c[0]) * ur;
+                    if (!active) {
+                        a00 = 1, a01 = 0, a10 = 0, a11 = 1;
+                        wl0 = wl1 = wr0 = wr1 = 0;
+                    }
+                    for (int d = 1; d < 64; d <<= 1) {
+                        const double b00 = up(a00, d), b01 = up(a01, d), b10 = up(a10, d), b11 = up(a11, d);
+                        const double pl0 = up(wl0, d), pl1 = up(wl1, d);
+                        double pr0 = 0, pr1 = 0;
+                        if (stereo) {
+                            pr0 = up(wr0, d);
+                            pr1 = up(wr1, d);
+                        }
+                        if (lane >= d) {
+                            // (A, w) o (B, p) = (A B, A p + w)
+                            wl0 = fma(a00, pl0, fma(a01, pl1, wl0));
+                            wl1 = fma(a10, pl0, fma(a11, pl1, wl1));
+                            if (stereo) {
+                                wr0 = fma(a00, pr0, fma(a01, pr1, wr0));
+                                wr1 = fma(a10, pr0, fma(a11, pr1, wr1));
+                            }
+                            const double n00 = fma(a00, b00, a01 * b10), n01 = fma(a00, b01, a01 * b11);
+                            const double n10 = fma(a10, b00, a11 * b10), n11 = fma(a10, b01, a11 * b11);
+                            a00 = n00, a01 = n01, a10 = n10, a11 = n11;
+                        }
+                    }
+                    const double zl1 = carry.z[sec][0][0], zl2 = carry.z[sec][0][1];
+                    const double zr1 = carry.z[sec][1][0], zr2 = carry.z[sec][1][1];
+                    const double sl1 = fma(a00, zl1, fma(a01, zl2, wl0)), sl2 = fma(a10, zl1, fma(a11, zl2, wl1));
+                    const double sr1 = fma(a00, zr1, fma(a01, zr2, wr0)), sr2 = fma(a10, zr1, fma(a11, zr2, wr1));
+                    // y_n = b0 u_n + z1_{n-1}
+                    double prevL = up(sl1, 1), prevR = up(sr1, 1);
+                    if (lane == lo) {
+                        prevL = zl1;
+                        prevR = zr1;
+                    }
+                    ul = fma(c[0], ul, prevL);
+                    ur = stereo ? fma(c[0], ur, prevR) : 0.0;
+                    const int last = hi - 1;
+                    const double ql1 = laneValue(sl1, last), ql2 = laneValue(sl2, last);
+                    const double qr1 = laneValue(sr1, last), qr2 = laneValue(sr2, last);
+                    __builtin_amdgcn_wave_barrier();
+                    if (lane == 0) {
+                        carry.z[sec][0][0] = ql1, carry.z[sec][0][1] = ql2;
+                        carry.z[sec][1][0] = qr1, carry.z[sec][1][1] = qr2;
+                    }
+                    __builtin_amdgcn_wave_barrier();
+                }
+            }
+            initialized = true;
+            if (active) {
+                dst[0] = float(ul);
+                if (stereo)
+                    dst[1] = float(ur);
+            }
+        }
+    }
+    // Save the chain state for the next segment.
+    if (hasFilter && lane == 0) {
+        int k = 0;
+        for (int f = 0; f < CLAY_FILTER_SLOTS; ++f) {
+            for (int i = 0; i < 5; ++i)
+                st.coefficients[f][i] = carry.coef[f][i];
+            for (int s = 0; s < sectionCount[f]; ++s, ++k)
+                for (int c = 0; c < 2; ++c) {
+                    st.z1[f][s][c] = carry.z[k][c][0];
+                    st.z2[f][s][c] = carry.z[k][c][1];
+                }
+        }
+        st.filterInitialized = initialized;
+    }
+}
+
+// Per output frame: the dry voices of the tile rendered inline, times the job
+// gain, plus the filtered voices from scratch (pan applied for mono). Voices
+// are summed in span order, so every execution form yields the same bits
+// (src/cuda/render.cu:458-523). Natural writes float2, orig/custom short2.
+template<int T, bool FloatOut>
+__global__ void __launch_bounds__(kTileFrames) clayMixTiles(
+    const int16_t* __restrict__ bank, const float* __restrict__ sinc, const Span* __restrict__ spans,
+    const Job* __restrict__ jobs, const Tile* __restrict__ tiles, const int* __restrict__ voices,
+    const clay_control_point* __restrict__ points, const Mod* __restrict__ mods, const float* __restrict__ filtered,
+    void* __restrict__ out, int segFirst, int segEnd, int firstTile)
+{
+    const Tile tile = tiles[firstTile + blockIdx.x];
+    const int lane = int(threadIdx.x);
+    const int frame = tile.frame + lane;
+    if (lane >= tile.frames || frame < segFirst || frame >= segEnd)
+        return;
+    const Job& j = jobs[uniform(tile.job)];
+    const float pan = controlAt(points, j.controls[2], j.controls[3], frame);
+    float left = 0, right = 0, wetLeft = 0, wetRight = 0;
+    for (int i = tile.first; i < tile.first + tile.count; ++i) {
+        const Span& v = spans[uniform(voices[i])];
+        if (frame < v.begin || frame >= v.end)
+            continue;
+        if (v.filterTrack >= 0) {
+            const float* src = filtered + v.filtered + uint64_t(frame - max(v.begin, segFirst)) * uint64_t(v.channels);
+            float2 s = make_float2(src[0], v.channels == 1 ? src[0] : src[1]);
+            if (v.channels == 1) {
+                float offset = pan;
+                for (int k = v.firstMod; k < v.firstMod + v.modCount; ++k) {
+                    const Mod m = mods[k];
+                    if (m.target == CLAY_MOD_PAN)
+                        offset += modulationAt(m, points, frame);
+                }
+                const float2 g = panGains(v, offset);
+                s.x *= g.x;
+                s.y *= g.y;
+            }
+            wetLeft += s.x;
+            wetRight += s.y;
+        } else {
+            const float2 s = voiceAudio<T>(bank, sinc, v, points, mods, pan, frame, false, -1.0);
+            left += s.x;
+            right += s.y;
+        }
+    }
+    const float amp = jobGain(points, j, frame);
+    const float2 mixed = make_float2(fmaf(left, amp, wetLeft), fmaf(right, amp, wetRight));
+    const uint64_t at = j.mix + uint64_t(frame);
+    if (FloatOut) {
+        static_cast<float2*>(out)[at] = mixed;
+    } else {
+        const short l = short(__float2int_rn(fminf(32767.f, fmaxf(-32768.f, mixed.x * 32768.f))));
+        const short r = short(__float2int_rn(fminf(32767.f, fmaxf(-32768.f, mixed.y * 32768.f))));
+        static_cast<short2*>(out)[at] = make_short2(l, r);
+    }
+}
+
+#define CLAY_INSTANTIATE_TAPS(T)                                                                                       \
+    template __global__ void clayVoiceChains<T>(const int16_t*, const float*, const Span*, const SpanCold*, const Job*, \
+                                                const int*, const clay_control_point*, const Mod*, const double*,      \
+                                                float*, VoiceState*, int, int, int);                                   \
+    template __global__ void clayMixTiles<T, true>(const int16_t*, const float*, const Span*, const Job*, const Tile*, \
+                                                   const int*, const clay_control_point*, const Mod*, const float*,     \
+                                                   void*, int, int, int);                                               \
+    template __global__ void clayMixTiles<T, false>(const int16_t*, const float*, const Span*, const Job*,             \
+                                                    const Tile*, const int*, const clay_control_point*, const Mod*,     \
+                                                    const float*, void*, int, int, int);
+CLAY_INSTANTIATE_TAPS(16)
+CLAY_INSTANTIATE_TAPS(32)
+CLAY_INSTANTIATE_TAPS(48)
+CLAY_INSTANTIATE_TAPS(64)
