@@ -155,10 +155,11 @@ class Renderer:
     """The HIP render core (clay_hip handle)."""
 
     def __init__(self, policy=abi.POLICY_NATURAL, taps=48, rate=44100, quality=False, group_minimum=0,
-                 batch_jobs=64, device=0, form=abi.FORM_PLAIN, audio=None):
+                 batch_jobs=64, device=0, form=abi.FORM_PLAIN, audio=None, segment_frames=0, flags=0):
         self.lib = hip()
         audio = audio if audio is not None else abi.default_audio_config(rate, quality)
-        self.cfg = abi.HipConfig(device, policy, taps, form, audio, group_minimum, 256 << 20, batch_jobs)
+        self.cfg = abi.HipConfig(device, policy, taps, form, audio, group_minimum, 256 << 20, batch_jobs,
+                                 segment_frames, flags)
         self.handle = C.c_void_p()
         if self.lib.clay_hip_create(C.byref(self.cfg), C.byref(self.handle)) != abi.OK:
             raise ClayError(self.lib.clay_hip_last_error(None).decode())
@@ -181,9 +182,21 @@ class Renderer:
         self._check(self.lib.clay_hip_get_stats(self.handle, C.byref(s)))
         return s
 
-    def render_window(self, plans, on_batch):
-        """Render through clay_hip_render_window; on_batch(results, pcm, first) per published batch."""
+    def render_window(self, plans, on_batch, refills=()):
+        """Render through clay_hip_render_window; on_batch(results, pcm, first, failed) per published
+        batch. `refills` are further PlanView arrays handed out one per refill call."""
         errors = []
+        pending = list(refills)
+        handed = []
+
+        def refill(user, wait, out_plans, out_count):
+            if not pending:
+                return 0
+            arr = pending.pop(0)
+            handed.append(arr)
+            out_plans[0] = C.cast(arr, C.POINTER(abi.PlanView))
+            out_count[0] = len(arr)
+            return 1
 
         def publish(user, batch_ptr, first):
             try:
@@ -202,7 +215,8 @@ class Renderer:
                 return 1
 
         cb = abi.PUBLISH_FN(publish)
-        status = self.lib.clay_hip_render_window(self.handle, plans, len(plans), cb, abi.REFILL_FN(), None)
+        rf = abi.REFILL_FN(refill) if refills else abi.REFILL_FN()
+        status = self.lib.clay_hip_render_window(self.handle, plans, len(plans), cb, rf, None)
         if errors:
             raise errors[0]
         self._check(status)

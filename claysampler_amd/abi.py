@@ -103,7 +103,11 @@ class TailResult(C.Structure):
 class HipConfig(C.Structure):
     _fields_ = [("device", C.c_int32), ("tail_policy", C.c_int32), ("sinc_taps", C.c_int32), ("form", C.c_int32),
                 ("audio", AudioConfig), ("group_minimum", C.c_int32), ("waiting_limit", C.c_uint64),
-                ("batch_jobs", C.c_uint64)]
+                ("batch_jobs", C.c_uint64), ("segment_frames", C.c_int32), ("flags", C.c_int32)]
+
+
+FLAG_NO_GRAPH = 1
+FLAG_NO_TUNE = 2
 
 
 class RenderBatch(C.Structure):
@@ -115,7 +119,8 @@ class RenderBatch(C.Structure):
 class HipStats(C.Structure):
     _fields_ = [("device", C.c_char * 256)] + [(n, C.c_uint64) for n in (
         "peak_bytes", "sample_cache_budget", "peak_waiting_bytes", "spilled_bytes", "downloaded_bytes",
-        "rounds", "admissions", "segments", "launches")]
+        "rounds", "admissions", "segments", "launches", "graph_launches")] + [
+        ("segment_frames", C.c_int32), ("mix_blocks", C.c_int32)]
 
 
 class SessionConfig(C.Structure):

@@ -77,6 +77,9 @@ struct Tile {
 // 44.1 kHz (poles near z = 1) amplifies its own rounding by ~1/(1-r)^2, and a
 // float pitch increment drifts the source phase over long notes; FP64 keeps
 // both within the +-1 LSB bar against the FP64 transcription.
+// Besides the reference fields, the last filter target of each slot and the
+// inputs it was computed from are kept, so a chunk after a segment boundary
+// takes exactly the path it takes without the boundary (bit-identical forms).
 struct VoiceState {
     double position, step;
     int32_t pitchSpan, previousLoopEnd, previousLoopStart;
@@ -84,6 +87,40 @@ struct VoiceState {
     int32_t pad;
     double z1[CLAY_FILTER_SLOTS][3][2], z2[CLAY_FILTER_SLOTS][3][2];
     double coefficients[CLAY_FILTER_SLOTS][5];
+    double target[CLAY_FILTER_SLOTS][5];
+    double input[CLAY_FILTER_SLOTS][3];
+    int32_t have[CLAY_FILTER_SLOTS];
+    int32_t pad2[3];
+};
+
+// Device pointers and the frame window of one resident unit (a batch of one
+// or more jobs) for one round of render launches. Tiles of a unit are sorted
+// by frame, so the tiles of a window are one contiguous range.
+struct UnitView {
+    const Span* spans;
+    const SpanCold* cold;
+    const Mod* mods;
+    const clay_control_point* points;
+    const Job* jobs;
+    const Tile* tiles;
+    const int* voices;
+    const int* chainHeads;
+    const int* pitchHeads;
+    double* phase;
+    float* filtered;
+    VoiceState* states;
+    void* mix;
+    int32_t segFirst, segEnd;           // frame window of this round
+    int32_t tileFirst, tileCount;       // tiles overlapping the window
+    int32_t chainCount, pitchCount;
+    // Exclusive prefix sums over the round's units (work item -> unit search).
+    int32_t tileBase, chainBase, pitchBase;
+    int32_t pad;
+};
+
+// Totals of one round, read by the grid-stride loops of the render kernels.
+struct RoundHeader {
+    int32_t units, tiles, chains, pitches;
 };
 
 struct StereoStats {

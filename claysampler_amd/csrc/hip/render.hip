@@ -1,5 +1,13 @@
-// Render kernels of the ClaySampler core for gfx950 (MI355X), per segment:
-//   clayPitchChains -> clayVoiceChains -> clayMixTiles
+// Render kernels of the ClaySampler core for gfx950 (MI355X). One round of
+//   pitch chains -> voice chains -> mix tiles
+// renders a frame window of one or more units (a batch of jobs):
+//  - plain form: clayPitchChains, clayVoiceChains, clayMixTiles take one
+//    unit's pointers as __restrict__ kernel arguments (scalar loads of the
+//    uniform span, modulation and control records) and exact grids;
+//  - persistent form: clayRound* walk the round's unit descriptors with fixed
+//    grids, grid-stride, so a captured round replays unchanged.
+// Both inline the same per-item functions; with expression-level FMA
+// contraction (-ffp-contract=on) they compute bit-identical audio.
 // Reference behaviour: src/cuda/render.cu. Written for wave64: per-voice
 // parameters are wave-uniform (scalar loads), voice chains are wave-parallel
 // FP64 scans, and the mix sums voices in span order.
@@ -7,17 +15,49 @@
 
 // ====================================================================== render
 
-// Variable-pitch recurrence per voice chain: per-frame source phase
-// (src/cuda/render.cu:222-266), in FP64. One lane per chain.
-extern "C" __global__ void clayPitchChains(const Span* __restrict__ spans, const SpanCold* __restrict__ cold,
-                                           const int* __restrict__ heads, const clay_control_point* __restrict__ points,
-                                           const Mod* __restrict__ mods, double* __restrict__ phase,
-                                           VoiceState* __restrict__ states, int segFirst, int segEnd, int count)
+// The round descriptors are read through the constant address space: the
+// compiler then takes the pointers stored in them for global memory (global
+// and, for uniform addresses of read-only records, scalar loads instead of
+// flat loads). The per-item functions take them as __restrict__ parameters,
+// which is what licenses the scalar loads next to the stores of one kernel.
+#define CLAY_CONST __attribute__((address_space(4)))
+using UnitsPtr = const UnitView* __restrict__;
+using RoundPtr = const RoundHeader* __restrict__;
+
+__device__ __forceinline__ const CLAY_CONST UnitView* constant(const UnitView* p)
 {
-    const int track = blockIdx.x * blockDim.x + threadIdx.x;
-    if (track >= count)
-        return;
-    const int head = heads[track];
+    return (const CLAY_CONST UnitView*)p;
+}
+
+template<class P>
+__device__ __forceinline__ P* global(P* p)
+{
+    return p;
+}
+
+// Resident unit of work item `item` given the exclusive prefix sums of the
+// round (units are few; the search runs on scalar registers).
+template<class F>
+__device__ __forceinline__ int unitOf(const RoundHeader& header, int item, F baseOf)
+{
+    int lo = 0, hi = header.units;
+    while (lo + 1 < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (baseOf(mid) <= item)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+// Variable-pitch recurrence of one voice chain: per-frame source phase
+// (src/cuda/render.cu:222-266), in FP64.
+__device__ void pitchChain(const Span* __restrict__ spans, const SpanCold* __restrict__ cold,
+                           const clay_control_point* __restrict__ points, const Mod* __restrict__ mods,
+                           double* __restrict__ phase, VoiceState* __restrict__ states, int segFirst, int segEnd,
+                           int head)
+{
     if (!spans[head].variablePitch)
         return;
     VoiceState& st = states[spans[head].filterTrack];  // one state per voice chain
@@ -63,6 +103,30 @@ extern "C" __global__ void clayPitchChains(const Span* __restrict__ spans, const
     }
 }
 
+// One lane per variable-pitch chain of the round, grid-stride.
+extern "C" __global__ void clayRoundPitchChains(UnitsPtr units, RoundPtr round)
+{
+    const RoundHeader header = *round;
+    for (int item = int(blockIdx.x * blockDim.x + threadIdx.x); item < header.pitches;
+         item += int(gridDim.x * blockDim.x)) {
+        const int unit = unitOf(header, item, [&](int i) { return units[i].pitchBase; });
+        const auto& u = constant(units)[unit];
+        pitchChain(global(u.spans), global(u.cold), global(u.points), global(u.mods), global(u.phase),
+                   global(u.states), u.segFirst, u.segEnd, global(u.pitchHeads)[item - u.pitchBase]);
+    }
+}
+
+// Plain form: one unit, its pointers as kernel arguments, one lane per chain.
+extern "C" __global__ void clayPitchChains(const Span* __restrict__ spans, const SpanCold* __restrict__ cold,
+                                           const int* __restrict__ heads, const clay_control_point* __restrict__ points,
+                                           const Mod* __restrict__ mods, double* __restrict__ phase,
+                                           VoiceState* __restrict__ states, int segFirst, int segEnd, int count)
+{
+    const int item = int(blockIdx.x * blockDim.x + threadIdx.x);
+    if (item < count)
+        pitchChain(spans, cold, points, mods, phase, states, segFirst, segEnd, heads[item]);
+}
+
 // ----------------------------------------------------------------- wave-parallel voice chains
 namespace {
 
@@ -96,19 +160,13 @@ __device__ __forceinline__ double laneValue(double v, int lane) { return __shfl(
 //   5. the filtered frames written to the float scratch the mix kernel reads.
 // State carries across chunks in LDS and across segments in VoiceState.
 template<int T>
-__global__ void __launch_bounds__(64 * kChainWaves) clayVoiceChains(
-    const int16_t* __restrict__ bank, const float* __restrict__ sinc, const Span* __restrict__ spans,
-    const SpanCold* __restrict__ cold, const Job* __restrict__ jobs, const int* __restrict__ heads,
-    const clay_control_point* __restrict__ points, const Mod* __restrict__ mods, const double* __restrict__ phase,
-    float* __restrict__ filtered, VoiceState* __restrict__ states, int segFirst, int segEnd, int count)
+__device__ __forceinline__ void voiceChain(const int16_t* __restrict__ bank, const float* __restrict__ sinc,
+                                           const Span* __restrict__ spans, const SpanCold* __restrict__ cold,
+                                           const Job* __restrict__ jobs, const clay_control_point* __restrict__ points,
+                                           const Mod* __restrict__ mods, const double* __restrict__ phase,
+                                           float* __restrict__ filtered, VoiceState* __restrict__ states,
+                                           int segFirst, int segEnd, int head, ChainCarry& carry, int lane)
 {
-    __shared__ ChainCarry carries[kChainWaves];
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int chain = blockIdx.x * kChainWaves + wave;
-    if (chain >= count)
-        return;  // uniform per wave; no block barriers below
-    ChainCarry& carry = carries[wave];
-    const int head = uniform(heads[chain]);
     VoiceState& st = states[spans[head].filterTrack];
     int sectionCount[CLAY_FILTER_SLOTS], total = 0;
     for (int f = 0; f < CLAY_FILTER_SLOTS; ++f) {
@@ -117,12 +175,17 @@ __global__ void __launch_bounds__(64 * kChainWaves) clayVoiceChains(
     }
     const bool hasFilter = total > 0;
     bool initialized = st.filterInitialized != 0;
+    __builtin_amdgcn_wave_barrier();
     if (hasFilter && lane == 0) {
         int k = 0;
         for (int f = 0; f < CLAY_FILTER_SLOTS; ++f) {
-            for (int i = 0; i < 5; ++i)
+            for (int i = 0; i < 5; ++i) {
                 carry.coef[f][i] = st.coefficients[f][i];
-            carry.have[f] = 0;
+                carry.target[f][i] = st.target[f][i];
+            }
+            for (int i = 0; i < 3; ++i)
+                carry.input[f][i] = st.input[f][i];
+            carry.have[f] = st.have[f];
             for (int s = 0; s < sectionCount[f]; ++s, ++k)
                 for (int c = 0; c < 2; ++c) {
                     carry.z[k][c][0] = st.z1[f][s][c];
@@ -388,11 +451,17 @@ __global__ void __launch_bounds__(64 * kChainWaves) clayVoiceChains(
         }
     }
     // Save the chain state for the next segment.
+    __builtin_amdgcn_wave_barrier();
     if (hasFilter && lane == 0) {
         int k = 0;
         for (int f = 0; f < CLAY_FILTER_SLOTS; ++f) {
-            for (int i = 0; i < 5; ++i)
+            for (int i = 0; i < 5; ++i) {
                 st.coefficients[f][i] = carry.coef[f][i];
+                st.target[f][i] = carry.target[f][i];
+            }
+            for (int i = 0; i < 3; ++i)
+                st.input[f][i] = carry.input[f][i];
+            st.have[f] = carry.have[f];
             for (int s = 0; s < sectionCount[f]; ++s, ++k)
                 for (int c = 0; c < 2; ++c) {
                     st.z1[f][s][c] = carry.z[k][c][0];
@@ -403,19 +472,52 @@ __global__ void __launch_bounds__(64 * kChainWaves) clayVoiceChains(
     }
 }
 
+// One wave per voice chain of the round, grid-stride; 4 chains per block.
+template<int T>
+__global__ void __launch_bounds__(64 * kChainWaves) clayRoundVoiceChains(
+    const int16_t* __restrict__ bank, const float* __restrict__ sinc, UnitsPtr units, RoundPtr round)
+{
+    __shared__ ChainCarry carries[kChainWaves];
+    const int wave = uniform(int(threadIdx.x >> 6)), lane = threadIdx.x & 63;
+    const RoundHeader header = *round;
+    // Uniform per wave: no block barriers below.
+    for (int item = int(blockIdx.x) * kChainWaves + wave; item < header.chains; item += int(gridDim.x) * kChainWaves) {
+        const int unit = unitOf(header, item, [&](int i) { return units[i].chainBase; });
+        const auto& u = constant(units)[unit];
+        voiceChain<T>(bank, sinc, global(u.spans), global(u.cold), global(u.jobs), global(u.points), global(u.mods),
+                      global(u.phase), global(u.filtered), global(u.states), u.segFirst, u.segEnd,
+                      uniform(global(u.chainHeads)[item - u.chainBase]), carries[wave], lane);
+    }
+}
+
+// Plain form: one unit, its pointers as kernel arguments, one wave per chain.
+template<int T>
+__global__ void __launch_bounds__(64 * kChainWaves) clayVoiceChains(
+    const int16_t* __restrict__ bank, const float* __restrict__ sinc, const Span* __restrict__ spans,
+    const SpanCold* __restrict__ cold, const Job* __restrict__ jobs, const int* __restrict__ heads,
+    const clay_control_point* __restrict__ points, const Mod* __restrict__ mods, const double* __restrict__ phase,
+    float* __restrict__ filtered, VoiceState* __restrict__ states, int segFirst, int segEnd, int count)
+{
+    __shared__ ChainCarry carries[kChainWaves];
+    const int wave = uniform(int(threadIdx.x >> 6)), lane = threadIdx.x & 63;
+    const int item = int(blockIdx.x) * kChainWaves + wave;
+    if (item >= count)
+        return;  // uniform per wave; no block barriers below
+    voiceChain<T>(bank, sinc, spans, cold, jobs, points, mods, phase, filtered, states, segFirst, segEnd,
+                  uniform(heads[item]), carries[wave], lane);
+}
+
 // Per output frame: the dry voices of the tile rendered inline, times the job
 // gain, plus the filtered voices from scratch (pan applied for mono). Voices
 // are summed in span order, so every execution form yields the same bits
 // (src/cuda/render.cu:458-523). Natural writes float2, orig/custom short2.
 template<int T, bool FloatOut>
-__global__ void __launch_bounds__(kTileFrames) clayMixTiles(
-    const int16_t* __restrict__ bank, const float* __restrict__ sinc, const Span* __restrict__ spans,
-    const Job* __restrict__ jobs, const Tile* __restrict__ tiles, const int* __restrict__ voices,
-    const clay_control_point* __restrict__ points, const Mod* __restrict__ mods, const float* __restrict__ filtered,
-    void* __restrict__ out, int segFirst, int segEnd, int firstTile)
+__device__ __forceinline__ void mixFrame(const int16_t* __restrict__ bank, const float* __restrict__ sinc,
+                                         const Span* __restrict__ spans, const Job* __restrict__ jobs,
+                                         const int* __restrict__ voices, const clay_control_point* __restrict__ points,
+                                         const Mod* __restrict__ mods, const float* __restrict__ filtered,
+                                         void* __restrict__ out, int segFirst, int segEnd, const Tile& tile, int lane)
 {
-    const Tile tile = tiles[firstTile + blockIdx.x];
-    const int lane = int(threadIdx.x);
     const int frame = tile.frame + lane;
     if (lane >= tile.frames || frame < segFirst || frame >= segEnd)
         return;
@@ -460,7 +562,40 @@ __global__ void __launch_bounds__(kTileFrames) clayMixTiles(
     }
 }
 
+// One block per tile of the round (256 frames, one per thread), grid-stride.
+template<int T, bool FloatOut>
+__global__ void __launch_bounds__(kTileFrames) clayRoundMixTiles(const int16_t* __restrict__ bank,
+                                                                 const float* __restrict__ sinc, UnitsPtr units,
+                                                                 RoundPtr round)
+{
+    const RoundHeader header = *round;
+    for (int item = int(blockIdx.x); item < header.tiles; item += int(gridDim.x)) {
+        const int unit = unitOf(header, item, [&](int i) { return units[i].tileBase; });
+        const auto& u = constant(units)[unit];
+        const Tile tile = global(u.tiles)[u.tileFirst + item - u.tileBase];
+        mixFrame<T, FloatOut>(bank, sinc, global(u.spans), global(u.jobs), global(u.voices), global(u.points),
+                              global(u.mods), global(u.filtered), global(u.mix), u.segFirst, u.segEnd, tile,
+                              int(threadIdx.x));
+    }
+}
+
+// Plain form: one unit, its pointers as kernel arguments, one block per tile.
+template<int T, bool FloatOut>
+__global__ void __launch_bounds__(kTileFrames) clayMixTiles(
+    const int16_t* __restrict__ bank, const float* __restrict__ sinc, const Span* __restrict__ spans,
+    const Job* __restrict__ jobs, const Tile* __restrict__ tiles, const int* __restrict__ voices,
+    const clay_control_point* __restrict__ points, const Mod* __restrict__ mods, const float* __restrict__ filtered,
+    void* __restrict__ out, int segFirst, int segEnd, int firstTile)
+{
+    const Tile tile = tiles[firstTile + int(blockIdx.x)];
+    mixFrame<T, FloatOut>(bank, sinc, spans, jobs, voices, points, mods, filtered, out, segFirst, segEnd, tile,
+                          int(threadIdx.x));
+}
+
 #define CLAY_INSTANTIATE_TAPS(T)                                                                                       \
+    template __global__ void clayRoundVoiceChains<T>(const int16_t*, const float*, UnitsPtr, RoundPtr);                \
+    template __global__ void clayRoundMixTiles<T, true>(const int16_t*, const float*, UnitsPtr, RoundPtr);             \
+    template __global__ void clayRoundMixTiles<T, false>(const int16_t*, const float*, UnitsPtr, RoundPtr);            \
     template __global__ void clayVoiceChains<T>(const int16_t*, const float*, const Span*, const SpanCold*, const Job*, \
                                                 const int*, const clay_control_point*, const Mod*, const double*,      \
                                                 float*, VoiceState*, int, int, int);                                   \

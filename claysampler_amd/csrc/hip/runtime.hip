@@ -3,10 +3,21 @@
 // Replaces the reference renderer (src/cuda/renderer.cpp). A window of plans
 // is split into batches; each batch is flattened into device records
 // (the role of BatchIndex::append, renderer.cpp:378-462), uploaded once into
-// one device arena, rendered by the kernels of kernels.hip on one HIP stream,
-// finished on the device (tail, quality, group gain, packing) and downloaded
-// as packed PCM16. MI355X's 288 GB of HBM lets a batch keep its whole filter
-// scratch resident, so a batch normally renders as one segment.
+// one device arena, rendered by the kernels of render.hip on one HIP stream,
+// finished on the device by the kernels of finish.hip (tail, quality, group
+// gain, packing) and downloaded as packed PCM16. MI355X's 288 GB of HBM lets a
+// batch keep its whole filter scratch resident, so a plain batch normally
+// renders as one segment.
+//
+// Two forms (bit-identical PCM):
+//  - plain: batches of up to batch_jobs jobs, one round of render launches per
+//    segment, then the batch's finishing launches (renderer.cpp:787-1142);
+//  - persistent: up to batch_jobs resident units (a job, or a whole volume
+//    group) advance by segment_frames per round; a round is one replay of a
+//    captured hipGraph whose kernels walk the round's unit descriptors with
+//    fixed grids; finished units are finished, downloaded and published while
+//    the others keep rendering, and queued jobs are admitted in their place
+//    (the role of renderRolling, renderer.cpp:1351-1746).
 #include <hip/hip_runtime.h>
 
 #include "device_types.h"
@@ -24,7 +35,7 @@
 
 using namespace clay_dev;
 
-// Kernels (kernels.hip).
+// Render kernels (render.hip). Plain form: one unit, pointers as arguments.
 extern "C" __global__ void clayPitchChains(const Span*, const SpanCold*, const int*, const clay_control_point*,
                                            const Mod*, double*, VoiceState*, int, int, int);
 template<int T>
@@ -34,6 +45,13 @@ __global__ void clayVoiceChains(const int16_t*, const float*, const Span*, const
 template<int T, bool FloatOut>
 __global__ void clayMixTiles(const int16_t*, const float*, const Span*, const Job*, const Tile*, const int*,
                              const clay_control_point*, const Mod*, const float*, void*, int, int, int);
+// Persistent form: the work of every resident unit listed in the round descriptors.
+extern "C" __global__ void clayRoundPitchChains(const UnitView*, const RoundHeader*);
+template<int T>
+__global__ void clayRoundVoiceChains(const int16_t*, const float*, const UnitView*, const RoundHeader*);
+template<int T, bool FloatOut>
+__global__ void clayRoundMixTiles(const int16_t*, const float*, const UnitView*, const RoundHeader*);
+// Finishing kernels (finish.hip).
 extern "C" __global__ void clayScanStereo(const Job*, int, const float2*, StereoStats*, int);
 extern "C" __global__ void claySelectNative(const Job*, int, const float2*, const StereoStats*, clay_tail_result*);
 extern "C" __global__ void clayQuantizeMono(const Job*, int, const float2*, const clay_tail_result*, int16_t*,
@@ -128,7 +146,9 @@ struct clay_hip_batch {
     std::vector<Job> jobs;
     std::vector<Tile> tiles;
     std::vector<int> voices;
-    std::vector<int> pitchHeads, chainHeads;  // variable-pitch chains; every voice chain
+    std::vector<int> pitchHeads, chainHeads;  // variable-pitch chains; every voice chain (longest first)
+    std::vector<int> chainByStart;            // voice chains by first frame (windowed rounds)
+    std::vector<int2> chainExtent;            // [first, end) frames of chainByStart[i]
     std::vector<AudioJobDesc> audioJobs;
     std::vector<int> onsets;
     std::vector<int2> groups;
@@ -145,7 +165,9 @@ struct clay_hip_batch {
     Job* dJobs = nullptr;
     Tile* dTiles = nullptr;
     int* dVoices = nullptr;
-    int *dPitchHeads = nullptr, *dChainHeads = nullptr;
+    int *dPitchHeads = nullptr, *dChainHeads = nullptr, *dChainByStart = nullptr;
+    int lastFrame = 0;
+    bool streamOrdered = false;  // arena from hipMallocAsync (persistent form)
     VoiceState* dStates = nullptr;
     AudioJobDesc* dAudioJobs = nullptr;
     int* dOnsets = nullptr;
@@ -187,6 +209,25 @@ struct clay_hip {
     };
     std::vector<Slot> slots;
     size_t residentBytes = 0;
+    // Round descriptors: a header and up to unitCapacity unit views on the
+    // device, written through a ring of pinned staging buffers.
+    static constexpr int kStages = 8;
+    struct Stage {
+        void* host = nullptr;
+        hipEvent_t copied = nullptr;
+        bool used = false;
+    };
+    Stage stages[kStages];
+    int stageNext = 0;
+    char* dRound = nullptr;
+    size_t unitCapacity = 0;
+    // Captured persistent rounds by launch shape.
+    struct Graph {
+        hipGraphExec_t exec;
+        int mixBlocks, chainBlocks;
+        const void* bank;
+    };
+    std::vector<Graph> graphs;
 
     int channels() const { return natural ? 1 : 2; }
 
@@ -455,6 +496,23 @@ void flatten(clay_hip* h, const clay_plan_view* plans, size_t count, clay_hip_ba
     std::stable_sort(order.begin(), order.end());
     for (size_t i = 0; i < order.size(); ++i)
         b.chainHeads[i] = order[i].second;
+    // Windowed rounds: chains by first frame, with their extents.
+    std::vector<std::pair<int2, int>> byStart;
+    for (int head : b.chainHeads) {
+        int end = 0;
+        for (int s = head; s >= 0; s = b.spans[size_t(s)].next)
+            end = std::max(end, b.spans[size_t(s)].end);
+        byStart.push_back({make_int2(b.spans[size_t(head)].begin, end), head});
+    }
+    std::stable_sort(byStart.begin(), byStart.end(), [](const auto& x, const auto& y) { return x.first.x < y.first.x; });
+    for (const auto& c : byStart) {
+        b.chainByStart.push_back(c.second);
+        b.chainExtent.push_back(c.first);
+    }
+    // Tiles by frame: the tiles of a frame window form one contiguous range.
+    std::stable_sort(b.tiles.begin(), b.tiles.end(), [](const Tile& x, const Tile& y) { return x.frame < y.frame; });
+    for (const auto& j : b.jobs)
+        b.lastFrame = std::max(b.lastFrame, j.frames);
     // Volume groups: consecutive jobs sharing a nonzero id.
     if (h->cfg.group_minimum)
         for (size_t first = 0, end; first < count; first = end) {
@@ -467,15 +525,21 @@ void flatten(clay_hip* h, const clay_plan_view* plans, size_t count, clay_hip_ba
         }
 }
 
-void release(clay_hip_batch& b)
+void release(clay_hip* h, clay_hip_batch& b)
 {
-    if (b.arena)
-        (void)hipFree(b.arena);
+    if (b.arena) {
+        if (b.streamOrdered && h)
+            (void)hipFreeAsync(b.arena, h->stream);
+        else
+            (void)hipFree(b.arena);
+    }
     b.arena = nullptr;
 }
 
-// One device arena for all records, scratch and finishing buffers.
-void upload(clay_hip* h, clay_hip_batch& b)
+// One device arena for all records, scratch and finishing buffers. The
+// persistent form allocates it stream-ordered and does not wait for the
+// copies (the host records live as long as the batch).
+void upload(clay_hip* h, clay_hip_batch& b, bool streamOrdered = false)
 {
     ArenaBuilder a;
     const int ch = h->channels();
@@ -483,6 +547,7 @@ void upload(clay_hip* h, clay_hip_batch& b)
     const size_t oPoints = a.add(bytesOf(b.points)), oJobs = a.add(bytesOf(b.jobs)), oTiles = a.add(bytesOf(b.tiles));
     const size_t oVoices = a.add(bytesOf(b.voices));
     const size_t oPitch = a.add(bytesOf(b.pitchHeads)), oChains = a.add(bytesOf(b.chainHeads));
+    const size_t oByStart = a.add(bytesOf(b.chainByStart));
     const size_t oStates = a.add(size_t(b.chains) * sizeof(VoiceState));
     const size_t oAudio = a.add(bytesOf(b.audioJobs)), oOnsets = a.add(bytesOf(b.onsets)), oGroups = a.add(bytesOf(b.groups));
     const size_t oMix = a.add(size_t(b.frames) * (h->natural ? sizeof(float2) : sizeof(short2)));
@@ -496,7 +561,11 @@ void upload(clay_hip* h, clay_hip_batch& b)
     const size_t oPcmJobs = a.add(b.assess ? b.jobs.size() * sizeof(PcmJob) : 0);
     const size_t oOutput = a.add(b.assess ? size_t(b.frames) * size_t(ch) * sizeof(int16_t) : 0);
     const size_t oResults = a.add(b.jobs.size() * sizeof(clay_tail_result));
-    check(hipMalloc(&b.arena, a.bytes), "batch arena");
+    if (streamOrdered)
+        check(hipMallocAsync(&b.arena, a.bytes, h->stream), "batch arena");
+    else
+        check(hipMalloc(&b.arena, a.bytes), "batch arena");
+    b.streamOrdered = streamOrdered;
     b.arenaBytes = a.bytes;
     auto* base = static_cast<char*>(b.arena);
     const auto put = [&](size_t offset, const auto& v) {
@@ -513,6 +582,7 @@ void upload(clay_hip* h, clay_hip_batch& b)
     b.dVoices = reinterpret_cast<int*>(put(oVoices, b.voices));
     b.dPitchHeads = reinterpret_cast<int*>(put(oPitch, b.pitchHeads));
     b.dChainHeads = reinterpret_cast<int*>(put(oChains, b.chainHeads));
+    b.dChainByStart = reinterpret_cast<int*>(put(oByStart, b.chainByStart));
     b.dStates = reinterpret_cast<VoiceState*>(base + oStates);
     b.dAudioJobs = reinterpret_cast<AudioJobDesc*>(put(oAudio, b.audioJobs));
     b.dOnsets = reinterpret_cast<int*>(put(oOnsets, b.onsets));
@@ -530,7 +600,10 @@ void upload(clay_hip* h, clay_hip_batch& b)
     // orig/custom: finalize works in place on the short2 mix.
     b.dOutput = b.assess ? reinterpret_cast<int16_t*>(base + oOutput) : reinterpret_cast<int16_t*>(b.dMix);
     b.dResults = reinterpret_cast<clay_tail_result*>(base + oResults);
-    check(hipStreamSynchronize(h->stream), "batch upload");
+    if (b.chains)
+        check(hipMemsetAsync(b.dStates, 0, size_t(b.chains) * sizeof(VoiceState), h->stream), "state reset");
+    if (!streamOrdered)
+        check(hipStreamSynchronize(h->stream), "batch upload");
     h->residentBytes = a.bytes;
     h->stats.peak_bytes = std::max<uint64_t>(h->stats.peak_bytes, a.bytes + h->bankCapacity * 2 +
                                                                       size_t(h->cfg.sinc_taps) * (kPhases + 1) * 4);
@@ -545,69 +618,255 @@ void chunked(int blocks, F&& launchChunk)
         launchChunk(first, std::min(kMaxBlocksPerLaunch, blocks - first));
 }
 
-// Lanes per block for one-lane-per-chain recurrences: spread short chain lists
-// over all CUs (several waves per SIMD) before filling waves.
-int recurrenceWidth(const clay_hip* h, int count)
+// ---------------------------------------------------------------- rounds
+
+// A resident unit and the frame window it renders in one round.
+struct Window {
+    clay_hip_batch* b;
+    int first, end;
+    int chainLo, chainHi;  // range of b->chainByStart (windowed); -1 = every chain, longest first
+};
+
+// Chains of a window [first, end) in start order: from the first chain that
+// has not ended before `first` (every earlier one has) to the first chain that
+// starts at or after `end`. `lo` only moves forward across a unit's rounds.
+void windowChains(const clay_hip_batch& b, int first, int end, int& lo, int& hi)
 {
-    const int cus = std::max(1, h->props.multiProcessorCount);
-    int width = 1;
-    while (width < 64 && count > width * cus * 16)
-        width *= 2;
-    return width;
+    const int n = int(b.chainExtent.size());
+    while (lo < n && b.chainExtent[size_t(lo)].y <= first)
+        ++lo;
+    hi = lo;
+    while (hi < n && b.chainExtent[size_t(hi)].x < end)
+        ++hi;
 }
 
+UnitView viewOf(const Window& w)
+{
+    const clay_hip_batch& b = *w.b;
+    UnitView v{};
+    v.spans = b.dSpans;
+    v.cold = b.dCold;
+    v.mods = b.dMods;
+    v.points = b.dPoints;
+    v.jobs = b.dJobs;
+    v.tiles = b.dTiles;
+    v.voices = b.dVoices;
+    v.pitchHeads = b.dPitchHeads;
+    v.phase = b.dPhase;
+    v.filtered = b.dFiltered;
+    v.states = b.dStates;
+    v.mix = b.dMix;
+    v.segFirst = w.first;
+    v.segEnd = w.end;
+    // Window bounds are multiples of the tile size (or the batch end).
+    v.tileFirst = int(std::lower_bound(b.tiles.begin(), b.tiles.end(), w.first,
+                                       [](const Tile& t, int f) { return t.frame < f; }) -
+                      b.tiles.begin());
+    v.tileCount = int(std::lower_bound(b.tiles.begin() + v.tileFirst, b.tiles.end(), w.end,
+                                       [](const Tile& t, int f) { return t.frame < f; }) -
+                      b.tiles.begin()) - v.tileFirst;
+    if (w.chainLo < 0) {
+        v.chainHeads = b.dChainHeads;
+        v.chainCount = int(b.chainHeads.size());
+    } else {
+        v.chainHeads = b.dChainByStart + w.chainLo;
+        v.chainCount = w.chainHi - w.chainLo;
+    }
+    v.pitchCount = int(b.pitchHeads.size());
+    return v;
+}
+
+// Launch shape of a round: grids of the three render kernels (0 = sized to the work).
+struct Shape {
+    int mixBlocks = 0, chainBlocks = 0, pitchBlocks = 0;
+};
+
+constexpr int kChainWavesPerBlock = 4;  // render.hip: voice chains per 256-thread block
+
+// Persistent form: the round kernels with the fixed grids of `shape`.
 template<int T>
-void renderSegment(clay_hip* h, clay_hip_batch& b, int segFirst, int segEnd)
+void launchRound(clay_hip* h, const Shape& shape, bool timed)
 {
     hipStream_t s = h->stream;
-    const int pitchCount = int(b.pitchHeads.size()), chainCount = int(b.chainHeads.size());
-    if (pitchCount) {
-        const int w = recurrenceWidth(h, pitchCount);
+    const auto* units = reinterpret_cast<const UnitView*>(h->dRound + 256);
+    const auto* round = reinterpret_cast<const RoundHeader*>(h->dRound);
+    const auto run = [&](const char* name, auto&& body) {
+        if (timed)
+            h->launch(name, body);
+        else {
+            body();
+            check(hipGetLastError(), name);
+            ++h->stats.launches;
+        }
+    };
+    run("clayRoundPitchChains", [&] {
+        hipLaunchKernelGGL(clayRoundPitchChains, dim3(shape.pitchBlocks), dim3(64), 0, s, units, round);
+    });
+    run("clayRoundVoiceChains", [&] {
+        hipLaunchKernelGGL(clayRoundVoiceChains<T>, dim3(shape.chainBlocks), dim3(64 * kChainWavesPerBlock), 0, s,
+                           h->bank, h->sinc, units, round);
+    });
+    run("clayRoundMixTiles", [&] {
+        if (h->natural)
+            hipLaunchKernelGGL((clayRoundMixTiles<T, true>), dim3(shape.mixBlocks), dim3(kTileFrames), 0, s, h->bank,
+                               h->sinc, units, round);
+        else
+            hipLaunchKernelGGL((clayRoundMixTiles<T, false>), dim3(shape.mixBlocks), dim3(kTileFrames), 0, s, h->bank,
+                               h->sinc, units, round);
+    });
+}
+
+// Plain form: one unit's window with the direct kernels and exact grids.
+template<int T>
+void launchWindow(clay_hip* h, const Window& w)
+{
+    hipStream_t s = h->stream;
+    clay_hip_batch& b = *w.b;
+    const UnitView v = viewOf(w);
+    const int pitchCount = v.pitchCount, chainCount = v.chainCount;
+    if (pitchCount)
         h->launch("clayPitchChains", [&] {
-            hipLaunchKernelGGL(clayPitchChains, dim3(ceilDiv(pitchCount, w)), dim3(w), 0, s, b.dSpans, b.dCold,
-                               b.dPitchHeads, b.dPoints, b.dMods, b.dPhase, b.dStates, segFirst, segEnd, pitchCount);
+            hipLaunchKernelGGL(clayPitchChains, dim3(ceilDiv(pitchCount, 64)), dim3(64), 0, s, b.dSpans, b.dCold,
+                               b.dPitchHeads, b.dPoints, b.dMods, b.dPhase, b.dStates, w.first, w.end, pitchCount);
         });
-    }
-    if (chainCount) {
-        constexpr int wavesPerBlock = 4;
+    if (chainCount)
         h->launch("clayVoiceChains", [&] {
-            hipLaunchKernelGGL(clayVoiceChains<T>, dim3(ceilDiv(chainCount, wavesPerBlock)), dim3(64 * wavesPerBlock), 0,
-                               s, h->bank, h->sinc, b.dSpans, b.dCold, b.dJobs, b.dChainHeads, b.dPoints, b.dMods,
-                               b.dPhase, b.dFiltered, b.dStates, segFirst, segEnd, chainCount);
+            hipLaunchKernelGGL(clayVoiceChains<T>, dim3(ceilDiv(chainCount, kChainWavesPerBlock)),
+                               dim3(64 * kChainWavesPerBlock), 0, s, h->bank, h->sinc, b.dSpans, b.dCold, b.dJobs,
+                               v.chainHeads, b.dPoints, b.dMods, b.dPhase, b.dFiltered, b.dStates, w.first, w.end,
+                               chainCount);
         });
-    }
-    chunked(int(b.tiles.size()), [&](int first, int n) {
+    chunked(v.tileCount, [&](int first, int n) {
         h->launch("clayMixTiles", [&] {
             if (h->natural)
                 hipLaunchKernelGGL((clayMixTiles<T, true>), dim3(n), dim3(kTileFrames), 0, s, h->bank, h->sinc, b.dSpans,
-                                   b.dJobs, b.dTiles, b.dVoices, b.dPoints, b.dMods, b.dFiltered, b.dMix, segFirst,
-                                   segEnd, first);
+                                   b.dJobs, b.dTiles, b.dVoices, b.dPoints, b.dMods, b.dFiltered, b.dMix, w.first, w.end,
+                                   v.tileFirst + first);
             else
                 hipLaunchKernelGGL((clayMixTiles<T, false>), dim3(n), dim3(kTileFrames), 0, s, h->bank, h->sinc,
                                    b.dSpans, b.dJobs, b.dTiles, b.dVoices, b.dPoints, b.dMods, b.dFiltered, b.dMix,
-                                   segFirst, segEnd, first);
+                                   w.first, w.end, v.tileFirst + first);
         });
     });
 }
 
-// Render + finish one resident batch (renderer.cpp:1046-1142).
-void runBatch(clay_hip* h, clay_hip_batch& b)
+void launchRoundTaps(clay_hip* h, const Shape& shape, bool timed)
+{
+    switch (h->cfg.sinc_taps) {
+    case 16: launchRound<16>(h, shape, timed); break;
+    case 32: launchRound<32>(h, shape, timed); break;
+    case 48: launchRound<48>(h, shape, timed); break;
+    default: launchRound<64>(h, shape, timed); break;
+    }
+}
+
+// Copies the round's descriptors to the device through the next pinned stage.
+RoundHeader stageRound(clay_hip* h, const std::vector<Window>& windows)
+{
+    if (windows.size() > h->unitCapacity)
+        throw Failure(CLAY_E_INVALID, "round holds more units than the renderer was created for");
+    auto& stage = h->stages[h->stageNext];
+    h->stageNext = (h->stageNext + 1) % clay_hip::kStages;
+    if (stage.used)
+        check(hipEventSynchronize(stage.copied), "round staging");
+    RoundHeader header{int(windows.size()), 0, 0, 0};
+    auto* views = reinterpret_cast<UnitView*>(static_cast<char*>(stage.host) + 256);
+    for (size_t i = 0; i < windows.size(); ++i) {
+        UnitView v = viewOf(windows[i]);
+        v.tileBase = header.tiles;
+        v.chainBase = header.chains;
+        v.pitchBase = header.pitches;
+        header.tiles += v.tileCount;
+        header.chains += v.chainCount;
+        header.pitches += v.pitchCount;
+        views[i] = v;
+    }
+    *static_cast<RoundHeader*>(stage.host) = header;
+    check(hipMemcpyAsync(h->dRound, stage.host, 256 + windows.size() * sizeof(UnitView), hipMemcpyHostToDevice,
+                         h->stream), "round descriptors");
+    check(hipEventRecord(stage.copied, h->stream), "hipEventRecord");
+    stage.used = true;
+    return header;
+}
+
+// One segment of a plain batch.
+void renderRound(clay_hip* h, const Window& w)
+{
+    switch (h->cfg.sinc_taps) {
+    case 16: launchWindow<16>(h, w); break;
+    case 32: launchWindow<32>(h, w); break;
+    case 48: launchWindow<48>(h, w); break;
+    default: launchWindow<64>(h, w); break;
+    }
+    ++h->stats.segments;
+}
+
+// One round with fixed grids (persistent form): a replay of the captured
+// round for this launch shape, captured on first use.
+void renderRoundFixed(clay_hip* h, const std::vector<Window>& windows, const Shape& shape, bool useGraph)
+{
+    stageRound(h, windows);
+    ++h->stats.segments;
+    ++h->stats.rounds;
+    if (!useGraph) {
+        launchRoundTaps(h, shape, true);
+        return;
+    }
+    const clay_hip::Graph* found = nullptr;
+    for (const auto& g : h->graphs)
+        if (g.mixBlocks == shape.mixBlocks && g.chainBlocks == shape.chainBlocks && g.bank == h->bank)
+            found = &g;
+    if (!found) {
+        hipGraph_t graph = nullptr;
+        check(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+        try {
+            launchRoundTaps(h, shape, false);
+        } catch (...) {
+            (void)hipStreamEndCapture(h->stream, &graph);
+            if (graph)
+                (void)hipGraphDestroy(graph);
+            throw;
+        }
+        check(hipStreamEndCapture(h->stream, &graph), "hipStreamEndCapture");
+        hipGraphExec_t exec = nullptr;
+        const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        check(e, "hipGraphInstantiate");
+        h->graphs.push_back({exec, shape.mixBlocks, shape.chainBlocks, h->bank});
+        found = &h->graphs.back();
+    }
+    if (h->timing) {
+        h->launch("round", [&] { check(hipGraphLaunch(found->exec, h->stream), "hipGraphLaunch"); });
+    } else {
+        check(hipGraphLaunch(found->exec, h->stream), "hipGraphLaunch");
+    }
+    ++h->stats.graph_launches;
+}
+
+// Segments of a plain batch: the whole batch, or windows of segment_frames.
+void renderPlain(clay_hip* h, clay_hip_batch& b)
+{
+    if (b.chains)
+        check(hipMemsetAsync(b.dStates, 0, size_t(b.chains) * sizeof(VoiceState), h->stream), "state reset");
+    const int step = h->cfg.segment_frames;
+    if (step <= 0 || step >= b.lastFrame) {
+        renderRound(h, Window{&b, 0, b.lastFrame, -1, -1});
+        return;
+    }
+    int lo = 0, hi = 0;
+    for (int first = 0; first < b.lastFrame; first += step) {
+        const int end = std::min(b.lastFrame, first + step);
+        windowChains(b, first, end, lo, hi);
+        renderRound(h, Window{&b, first, end, lo, hi});
+    }
+}
+
+// Tail, quality, group gain and packing of a rendered batch (renderer.cpp:1046-1142).
+void finishBatch(clay_hip* h, clay_hip_batch& b)
 {
     hipStream_t s = h->stream;
     const int count = int(b.jobs.size());
     const int ch = h->channels();
-    if (b.chains)
-        check(hipMemsetAsync(b.dStates, 0, size_t(b.chains) * sizeof(VoiceState), s), "state reset");
-    int lastFrame = 0;
-    for (const auto& j : b.jobs)
-        lastFrame = std::max(lastFrame, j.frames);
-    switch (h->cfg.sinc_taps) {
-    case 16: renderSegment<16>(h, b, 0, lastFrame); break;
-    case 32: renderSegment<32>(h, b, 0, lastFrame); break;
-    case 48: renderSegment<48>(h, b, 0, lastFrame); break;
-    default: renderSegment<64>(h, b, 0, lastFrame); break;
-    }
-    ++h->stats.segments;
     const auto mix2 = static_cast<const float2*>(b.dMix);
     const int lanes = 64;
     if (h->natural) {
@@ -758,6 +1017,10 @@ int clay_hip_create(const clay_hip_config* config, clay_hip** out)
             throw Failure(CLAY_E_INVALID, "audio rate and window must be positive");
         if (config->batch_jobs == 0)
             h->cfg.batch_jobs = 64;
+        if (config->segment_frames < 0 || config->segment_frames % kTileFrames)
+            throw Failure(CLAY_E_INVALID, "segment_frames must be a non-negative multiple of 256");
+        if (config->form != CLAY_FORM_PLAIN && config->form != CLAY_FORM_PERSISTENT)
+            throw Failure(CLAY_E_INVALID, "unknown execution form");
         h->natural = config->tail_policy == CLAY_POLICY_NATURAL;
         initTail(h.get());
         int devices = 0;
@@ -777,16 +1040,27 @@ int clay_hip_create(const clay_hip_config* config, clay_hip** out)
                 phaseMajor[size_t(p) * size_t(taps) + size_t(t)] = tapMajor[size_t(t) * (kPhases + 1) + size_t(p)];
         check(hipMalloc(&h->sinc, bytesOf(phaseMajor)), "sinc table");
         check(hipMemcpy(h->sinc, phaseMajor.data(), bytesOf(phaseMajor), hipMemcpyHostToDevice), "sinc upload");
+        // Round descriptors: header + one view per resident unit.
+        h->unitCapacity = size_t(std::min<uint64_t>(h->cfg.batch_jobs, 1u << 16)) + 1;
+        const size_t roundBytes = 256 + h->unitCapacity * sizeof(UnitView);
+        check(hipMalloc(&h->dRound, roundBytes), "round descriptors");
+        for (auto& stage : h->stages) {
+            check(hipHostMalloc(&stage.host, roundBytes, hipHostMallocDefault), "round staging");
+            check(hipEventCreateWithFlags(&stage.copied, hipEventDisableTiming), "hipEventCreate");
+        }
+        // Persistent units allocate stream-ordered; keep freed blocks in the pool.
+        hipMemPool_t pool = nullptr;
+        if (hipDeviceGetDefaultMemPool(&pool, h->device) == hipSuccess && pool) {
+            uint64_t threshold = UINT64_MAX;
+            (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &threshold);
+        }
         size_t freeBytes = 0, totalBytes = 0;
         check(hipMemGetInfo(&freeBytes, &totalBytes), "hipMemGetInfo");
         h->stats.sample_cache_budget = std::min<uint64_t>(freeBytes - freeBytes / 3, totalBytes / 2);
     });
     if (status != CLAY_OK) {
         g_createError = h->error;
-        if (h->sinc)
-            (void)hipFree(h->sinc);
-        if (h->stream)
-            (void)hipStreamDestroy(h->stream);
+        clay_hip_destroy(h.release());
         return status;
     }
     *out = h.release();
@@ -805,6 +1079,16 @@ int clay_hip_destroy(clay_hip* h)
             (void)hipHostFree(slot.host);
     for (auto e : h->eventPool)
         (void)hipEventDestroy(e);
+    for (auto& g : h->graphs)
+        (void)hipGraphExecDestroy(g.exec);
+    for (auto& stage : h->stages) {
+        if (stage.host)
+            (void)hipHostFree(stage.host);
+        if (stage.copied)
+            (void)hipEventDestroy(stage.copied);
+    }
+    if (h->dRound)
+        (void)hipFree(h->dRound);
     if (h->bank)
         (void)hipFree(h->bank);
     if (h->sinc)
@@ -867,7 +1151,7 @@ int clay_hip_batch_prepare(clay_hip* h, const clay_plan_view* plans, size_t coun
         upload(h, *b);
     });
     if (status != CLAY_OK) {
-        release(*b);
+        release(h, *b);
         return status;
     }
     *out = b.release();
@@ -884,7 +1168,8 @@ int clay_hip_batch_run(clay_hip* h, clay_hip_batch* b, double* kernel_seconds)
         check(hipEventRecord(begin, h->stream), "hipEventRecord");
         const char* timing = std::getenv("CLAY_KERNEL_TIMING");
         h->timing = timing && *timing == '1';
-        runBatch(h, *b);
+        renderPlain(h, *b);
+        finishBatch(h, *b);
         check(hipEventRecord(end, h->stream), "hipEventRecord");
         check(hipStreamSynchronize(h->stream), "batch run");
         float ms = 0;
@@ -948,16 +1233,20 @@ int clay_hip_batch_destroy(clay_hip* h, clay_hip_batch* b)
         (void)hipSetDevice(h->device);
         (void)hipStreamSynchronize(h->stream);
     }
-    release(*b);
+    release(h, *b);
     delete b;
     return CLAY_OK;
 }
 
-int clay_hip_render_window(clay_hip* h, const clay_plan_view* plans, size_t count, clay_publish_fn publish,
-                           clay_refill_fn refill, void* user)
+} // extern "C"
+
+namespace {
+
+// Plain form: batches of up to batch_jobs jobs, never splitting a volume
+// group, halving a batch that does not fit (renderer.cpp:1748-1820, 958-968).
+int renderWindowPlain(clay_hip* h, const clay_plan_view* plans, size_t count, clay_publish_fn publish,
+                      clay_refill_fn refill, void* user)
 {
-    if (!h || !publish || (count && !plans))
-        return CLAY_E_INVALID;
     return guarded(h, [&] {
         std::vector<clay_plan_view> queue(plans, plans + count);
         size_t first = 0;
@@ -1023,6 +1312,296 @@ int clay_hip_render_window(clay_hip* h, const clay_plan_view* plans, size_t coun
             first = end;
         }
     });
+}
+
+
+// Persistent form (the role of renderRolling, renderer.cpp:1351-1746).
+// Up to batch_jobs units are resident; each round advances every unit by
+// `segment` frames with one replay of the captured round. A unit whose last
+// window was rendered is finished on the device right behind that round; its
+// results, then its PCM, come back through pinned memory and it is published
+// while later rounds run. Its place is filled from the queue (and refill) at
+// the next round. The host only waits when a staging buffer is still in use,
+// or when nothing is resident and completions are outstanding.
+struct RoundTuner {
+    std::vector<std::pair<int, int>> candidates;  // (segment frames, mix blocks)
+    std::vector<double> cost;                     // ms per rendered frame, summed over trials
+    std::vector<int> trials;
+    size_t next = 0;
+    int segment = 0, mixBlocks = 0;
+    bool tuning = false;
+    static constexpr int kTrials = 2;
+
+    void init(const clay_hip* h)
+    {
+        const int cus = std::max(1, h->props.multiProcessorCount);
+        const int fixedSegment = h->cfg.segment_frames;
+        segment = fixedSegment > 0 ? fixedSegment : 1 << 17;
+        mixBlocks = cus * 16;
+        tuning = !(h->cfg.flags & CLAY_FLAG_NO_TUNE);
+        if (!tuning)
+            return;
+        for (int seg : {1 << 16, 1 << 17, 1 << 18}) {
+            if (fixedSegment > 0 && seg != fixedSegment)
+                continue;
+            for (int blocks : {cus * 8, cus * 32})
+                candidates.emplace_back(seg, blocks);
+        }
+        if (fixedSegment > 0 && candidates.empty())
+            for (int blocks : {cus * 8, cus * 32})
+                candidates.emplace_back(fixedSegment, blocks);
+        cost.assign(candidates.size(), 0);
+        trials.assign(candidates.size(), 0);
+    }
+
+    // Launch shape of the next round.
+    std::pair<int, int> pick() const
+    {
+        if (tuning && next < candidates.size())
+            return candidates[next];
+        return {segment, mixBlocks};
+    }
+
+    void record(double ms, uint64_t frames)
+    {
+        if (!tuning || next >= candidates.size() || !frames)
+            return;
+        cost[next] += ms / double(frames);
+        if (++trials[next] == kTrials)
+            ++next;
+        if (next == candidates.size()) {
+            size_t best = 0;
+            for (size_t i = 1; i < cost.size(); ++i)
+                if (cost[i] < cost[best])
+                    best = i;
+            segment = candidates[best].first;
+            mixBlocks = candidates[best].second;
+            tuning = false;
+        }
+    }
+};
+
+int renderWindowPersistent(clay_hip* h, const clay_plan_view* plans, size_t count, clay_publish_fn publish,
+                           clay_refill_fn refill, void* user)
+{
+    struct Unit {
+        clay_hip_batch* b = nullptr;
+        size_t first = 0, jobs = 0;
+        int cursor = 0, chainLo = 0;
+        int stage = 0;  // 0 rendering, 1 results in flight, 2 PCM in flight
+        hipEvent_t ready = nullptr, begin = nullptr;
+        clay_tail_result* results = nullptr;  // pinned
+        void* slot = nullptr;
+        uint64_t samples = 0;
+    };
+    std::vector<clay_plan_view> queue(plans, plans + count);
+    std::vector<std::unique_ptr<Unit>> resident, completing;
+    const auto drop = [&](Unit& u) {
+        if (u.b) {
+            release(h, *u.b);
+            delete u.b;
+            u.b = nullptr;
+        }
+        if (u.results)
+            (void)hipHostFree(u.results);
+        u.results = nullptr;
+        if (u.ready)
+            h->eventPool.push_back(u.ready);
+        if (u.begin)
+            h->eventPool.push_back(u.begin);
+        u.ready = u.begin = nullptr;
+    };
+    const int status = guarded(h, [&] {
+        check(hipSetDevice(h->device), "hipSetDevice");
+        RoundTuner tuner;
+        tuner.init(h);
+        const int cus = std::max(1, h->props.multiProcessorCount);
+        const bool useGraph = !(h->cfg.flags & CLAY_FLAG_NO_GRAPH);
+        size_t next = 0;
+        bool exhausted = !refill;
+        std::vector<Window> windows;
+        hipEvent_t roundBegin = h->event(), roundEnd = h->event();
+        try {
+            for (;;) {
+                // 1. Admission: fill free places from the queue, refilling it when empty.
+                while (resident.size() < h->cfg.batch_jobs) {
+                    if (next == queue.size()) {
+                        if (exhausted)
+                            break;
+                        const clay_plan_view* more = nullptr;
+                        size_t n = 0;
+                        const int wait = resident.empty() && completing.empty();
+                        if (!refill(user, wait, &more, &n) || !n) {
+                            if (wait)
+                                exhausted = true;
+                            break;
+                        }
+                        queue.insert(queue.end(), more, more + n);
+                    }
+                    size_t end = next + 1;
+                    if (h->cfg.group_minimum)
+                        while (end < queue.size() && queue[next].volume_group &&
+                               queue[end].volume_group == queue[next].volume_group)
+                            ++end;
+                    auto u = std::make_unique<Unit>();
+                    u->first = next;
+                    u->jobs = end - next;
+                    u->b = new clay_hip_batch();
+                    int st = guarded(h, [&] {
+                        flatten(h, queue.data() + next, end - next, *u->b);
+                        upload(h, *u->b, true);
+                    });
+                    if (st == CLAY_E_MEMORY && (!resident.empty() || !completing.empty())) {
+                        drop(*u);  // retry once memory is returned
+                        break;
+                    }
+                    if (st == CLAY_E_MEMORY) {
+                        drop(*u);
+                        clay_render_batch rejected{};
+                        rejected.failed_scratch_bytes = 1;
+                        if (publish(user, &rejected, next))
+                            throw Failure(CLAY_E_CALLBACK, "publish aborted the window");
+                        next = end;
+                        continue;
+                    }
+                    if (st != CLAY_OK) {
+                        drop(*u);
+                        throw Failure(st, h->error);
+                    }
+                    u->begin = h->event();
+                    check(hipEventRecord(u->begin, h->stream), "hipEventRecord");
+                    ++h->stats.admissions;
+                    next = end;
+                    resident.push_back(std::move(u));
+                }
+                if (resident.empty() && completing.empty() && (exhausted || next == queue.size()))
+                    break;
+                // 2. One round over every resident unit.
+                if (!resident.empty()) {
+                    const auto shape = tuner.pick();
+                    windows.clear();
+                    uint64_t frames = 0;
+                    for (auto& u : resident) {
+                        const int first = u->cursor, end = std::min(u->b->lastFrame, first + shape.first);
+                        int hi = 0;
+                        windowChains(*u->b, first, end, u->chainLo, hi);
+                        windows.push_back({u->b, first, end, u->chainLo, hi});
+                        frames += uint64_t(end - first) * u->b->jobs.size();
+                        u->cursor = end;
+                    }
+                    Shape fixed;
+                    fixed.mixBlocks = shape.second;
+                    fixed.chainBlocks = cus * 2;
+                    fixed.pitchBlocks = cus;
+                    const bool measure = tuner.tuning && h->stats.rounds > 0;
+                    if (measure)
+                        check(hipEventRecord(roundBegin, h->stream), "hipEventRecord");
+                    renderRoundFixed(h, windows, fixed, useGraph);
+                    if (measure) {
+                        check(hipEventRecord(roundEnd, h->stream), "hipEventRecord");
+                        check(hipEventSynchronize(roundEnd), "round timing");
+                        float ms = 0;
+                        check(hipEventElapsedTime(&ms, roundBegin, roundEnd), "hipEventElapsedTime");
+                        tuner.record(ms, frames);
+                    }
+                    h->stats.segment_frames = shape.first;
+                    h->stats.mix_blocks = shape.second;
+                    // Units whose last window was rendered: finish and fetch results.
+                    for (size_t i = 0; i < resident.size();) {
+                        auto& u = resident[i];
+                        if (u->cursor < u->b->lastFrame) {
+                            ++i;
+                            continue;
+                        }
+                        finishBatch(h, *u->b);
+                        check(hipHostMalloc(reinterpret_cast<void**>(&u->results),
+                                            u->jobs * sizeof(clay_tail_result), hipHostMallocDefault), "pinned results");
+                        check(hipMemcpyAsync(u->results, u->b->dResults, u->jobs * sizeof(clay_tail_result),
+                                             hipMemcpyDeviceToHost, h->stream), "results download");
+                        u->ready = h->event();
+                        check(hipEventRecord(u->ready, h->stream), "hipEventRecord");
+                        u->stage = 1;
+                        completing.push_back(std::move(u));
+                        resident.erase(resident.begin() + std::ptrdiff_t(i));
+                    }
+                }
+                // 3. Completions, oldest first; block only when nothing else can run.
+                for (size_t i = 0; i < completing.size();) {
+                    auto& u = completing[i];
+                    hipError_t q = hipEventQuery(u->ready);
+                    if (q == hipErrorNotReady && resident.empty() && i == 0 &&
+                        (exhausted || next == queue.size() || resident.size() >= h->cfg.batch_jobs)) {
+                        check(hipEventSynchronize(u->ready), "completion");
+                        q = hipSuccess;
+                    }
+                    if (q == hipErrorNotReady) {
+                        ++i;
+                        continue;
+                    }
+                    check(q, "completion");
+                    if (u->stage == 1) {
+                        for (size_t k = 0; k < u->jobs; ++k)
+                            u->samples = std::max<uint64_t>(
+                                u->samples, (u->results[k].offset + uint64_t(u->results[k].data_frames)) * uint64_t(h->channels()));
+                        u->slot = pinnedSlot(h, u->samples * sizeof(int16_t));
+                        if (u->samples)
+                            check(hipMemcpyAsync(u->slot, u->b->dOutput, u->samples * sizeof(int16_t),
+                                                 hipMemcpyDeviceToHost, h->stream), "pcm download");
+                        h->stats.downloaded_bytes += u->samples * sizeof(int16_t);
+                        check(hipEventRecord(u->ready, h->stream), "hipEventRecord");
+                        u->stage = 2;
+                        ++i;
+                        continue;
+                    }
+                    float ms = 0;
+                    (void)hipEventElapsedTime(&ms, u->begin, u->ready);
+                    clay_render_batch batch{};
+                    batch.pcm = static_cast<const int16_t*>(u->slot);
+                    batch.pcm_handle = u->slot;
+                    batch.results = u->results;
+                    batch.result_count = u->jobs;
+                    batch.kernel_seconds = ms / 1000.0;
+                    const size_t first = u->first;
+                    const int aborted = publish(user, &batch, first);
+                    drop(*u);
+                    completing.erase(completing.begin() + std::ptrdiff_t(i));
+                    if (aborted)
+                        throw Failure(CLAY_E_CALLBACK, "publish aborted the window");
+                }
+            }
+        } catch (...) {
+            h->eventPool.push_back(roundBegin);
+            h->eventPool.push_back(roundEnd);
+            throw;
+        }
+        h->eventPool.push_back(roundBegin);
+        h->eventPool.push_back(roundEnd);
+    });
+    if (status != CLAY_OK) {
+        (void)hipStreamSynchronize(h->stream);
+        for (auto& u : resident)
+            drop(*u);
+        for (auto& u : completing) {
+            if (u->slot)
+                (void)clay_hip_release_pcm(h, u->slot);
+            drop(*u);
+        }
+    }
+    return status;
+}
+
+} // namespace
+
+extern "C" {
+
+int clay_hip_render_window(clay_hip* h, const clay_plan_view* plans, size_t count, clay_publish_fn publish,
+                           clay_refill_fn refill, void* user)
+{
+    if (!h || !publish || (count && !plans))
+        return CLAY_E_INVALID;
+    if (h->cfg.form == CLAY_FORM_PERSISTENT)
+        return renderWindowPersistent(h, plans, count, publish, refill, user);
+    return renderWindowPlain(h, plans, count, publish, refill, user);
 }
 
 int clay_hip_release_pcm(clay_hip* h, void* pcm_handle)

@@ -200,7 +200,17 @@ typedef struct clay_hip_config {
     int32_t group_minimum;         /* 0 = no volume groups */
     uint64_t waiting_limit;        /* bytes of retained group PCM on the device */
     uint64_t batch_jobs;           /* --gpu-batch-jobs */
+    /* Frames per segment, a multiple of 256. Plain form: 0 renders each batch as
+     * one segment (the reference splits only under scratch pressure,
+     * renderer.cpp:904-930). Persistent form: frames each resident job advances
+     * per round; 0 lets the tuner choose. Output never depends on it. */
+    int32_t segment_frames;
+    int32_t flags;                 /* CLAY_FLAG_* */
 } clay_hip_config;
+
+/* clay_hip_config.flags */
+#define CLAY_FLAG_NO_GRAPH 1       /* persistent form: launch rounds directly instead of replaying a hipGraph */
+#define CLAY_FLAG_NO_TUNE 2        /* persistent form: fixed launch shape instead of the round tuner */
 
 /* One published batch (src/cuda/renderer.h:12-18). `pcm` stays valid until
  * clay_hip_release_pcm(h, pcm_handle); results/job_indices only during the call. */
@@ -219,6 +229,9 @@ typedef struct clay_hip_stats {
     char device[256];
     uint64_t peak_bytes, sample_cache_budget, peak_waiting_bytes, spilled_bytes, downloaded_bytes;
     uint64_t rounds, admissions, segments, launches;
+    uint64_t graph_launches;       /* rounds replayed from a captured hipGraph */
+    int32_t segment_frames;        /* persistent form: frames per round chosen (tuned or configured) */
+    int32_t mix_blocks;            /* persistent form: grid of the mix kernel chosen */
 } clay_hip_stats;
 
 /* publish(user, batch, first): return 0 to continue, nonzero to abort the window. */

@@ -44,7 +44,7 @@ def test_create_without_device_fails_cleanly():
         return
     lib = ca.hip()
     handle = C.c_void_p()
-    cfg = abi.HipConfig(0, abi.POLICY_NATURAL, 48, 0, abi.default_audio_config(), 0, 0, 64)
+    cfg = abi.HipConfig(0, abi.POLICY_NATURAL, 48, 0, abi.default_audio_config(), 0, 0, 64, 0, 0)
     assert lib.clay_hip_create(C.byref(cfg), C.byref(handle)) != abi.OK
     assert lib.clay_hip_last_error(None)
 
