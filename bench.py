@@ -101,6 +101,89 @@ def cpu_baseline(sfz, mids, taps, jobs, threads):
                       f"{threads} threads, {dt:.1f} s"}
 
 
+def window_pass(policy, taps, form, batch_jobs, plans, bank, checksums=False, flags=0):
+    """One clay_hip_render_window over every plan. Returns (wall s, per-job
+    checksums or None, stats). The publish callback only releases the PCM
+    (and optionally checksums it), so the time is the render core's."""
+    import ctypes as C
+
+    r = ca.Renderer(policy=policy, taps=taps, form=form, batch_jobs=batch_jobs, flags=flags)
+    r.upload_bank(*bank)
+    sums = {} if checksums else None
+    published = [0]
+
+    def publish(user, batch_ptr, first):
+        b = batch_ptr.contents
+        if sums is not None:
+            for k in range(b.result_count):
+                res = b.results[k]
+                n = res.data_frames * r.channels
+                view = np.ctypeslib.as_array(C.cast(C.addressof(b.pcm.contents) + 2 * res.offset * r.channels,
+                                                    C.POINTER(C.c_int16)), shape=(n,)) if n else np.zeros(0, np.int16)
+                sums[first + k] = (res.status, res.frames, res.data_frames, int(view.astype(np.int64).sum()),
+                                   int((view.astype(np.int64) * np.arange(n, dtype=np.int64) % 65521).sum()))
+        published[0] += max(1, b.result_count)
+        if b.pcm_handle:
+            r.lib.clay_hip_release_pcm(r.handle, b.pcm_handle)
+        return 0
+
+    cb = abi.PUBLISH_FN(publish)
+    t0 = time.perf_counter()
+    status = r.lib.clay_hip_render_window(r.handle, plans, len(plans), cb, abi.REFILL_FN(), None)
+    wall = time.perf_counter() - t0
+    r._check(status)
+    stats = r.stats()
+    r.close()
+    assert published[0] == len(plans)
+    return wall, sums, stats
+
+
+def forms_main(args, jobs, sfz, mids, taps, plans, bank):
+    """Configuration 3: the plain per-segment form vs the persistent rolling
+    form on the same jobs, whole-window wall time, identical PCM16 required."""
+    frames = sum(p.frames for p in plans)
+    audio_seconds = frames / 44100.0
+    warm = (abi.PlanView * min(8, len(plans)))(*plans[:8])
+    for form in (abi.FORM_PLAIN, abi.FORM_PERSISTENT):
+        window_pass(abi.POLICY_NATURAL, taps, form, args.batch_jobs, warm, bank)
+    log("warmup done")
+    plain, _, plain_stats = window_pass(abi.POLICY_NATURAL, taps, abi.FORM_PLAIN, args.batch_jobs, plans, bank)
+    log(f"plain: {plain:.3f} s, {plain_stats.segments} segments")
+    rolled, _, stats = window_pass(abi.POLICY_NATURAL, taps, abi.FORM_PERSISTENT, args.batch_jobs, plans, bank)
+    log(f"persistent: {rolled:.3f} s, {stats.rounds} rounds, segment {stats.segment_frames}, "
+        f"mix blocks {stats.mix_blocks}, graph launches {stats.graph_launches}")
+    _, want, _ = window_pass(abi.POLICY_NATURAL, taps, abi.FORM_PLAIN, args.batch_jobs, plans, bank, True)
+    _, got, _ = window_pass(abi.POLICY_NATURAL, taps, abi.FORM_PERSISTENT, args.batch_jobs, plans, bank, True)
+    identical = want == got
+    log(f"identical PCM16: {identical}")
+    line = {
+        "metric": "rendered audio-sec/sec and voice-samples/sec on 1 GPU; % of HBM roofline",
+        "value": audio_seconds / rolled,
+        "unit": "audio-s/s",
+        "n_gpus": 1,
+        "steps": 1,
+        "warmup": 1,
+        "ms_per_step": rolled * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"config 3: {jobs} jobs, lognormal 5-300 s, polyphony 4-64, 16-tap sinc, "
+                               f"--gpu-batch-jobs {args.batch_jobs}, whole window incl. host planning handoff, "
+                               "finishing and PCM download",
+                   "jobs": jobs, "sinc_taps": taps, "tail_policy": "natural", "rate": 44100,
+                   "form": "persistent vs plain"},
+        "audio_seconds": audio_seconds,
+        "plain_value": audio_seconds / plain,
+        "speedup_vs_plain": plain / rolled,
+        "identical_pcm16": identical,
+        "persistent": {"rounds": stats.rounds, "segment_frames": stats.segment_frames,
+                       "mix_blocks": stats.mix_blocks, "graph_launches": stats.graph_launches},
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -112,6 +195,9 @@ def main():
     ap.add_argument("--synth-dir", default=os.environ.get("CLAY_SYNTH_DIR", "/tmp/clay_synth"))
     ap.add_argument("--cpu-jobs", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--forms", action="store_true",
+                    help="plain vs persistent form through render_window (configuration 3)")
+    ap.add_argument("--batch-jobs", type=int, default=128)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (jobs shard by job; see DESIGN.md)",
@@ -129,6 +215,9 @@ def main():
     plans = session.plans()
     bank_ptr, bank_count = session.bank()
     log(f"planned {jobs} jobs, {sum(p.span_count for p in plans)} spans in {time.perf_counter() - t0:.1f} s")
+    if args.forms:
+        forms_main(args, jobs, sfz, mids, taps, plans, (bank_ptr, bank_count))
+        return
 
     renderer = ca.Renderer(policy=abi.POLICY_NATURAL, taps=taps)
     renderer.upload_bank(bank_ptr, bank_count)

@@ -26,6 +26,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <deque>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -228,6 +229,101 @@ struct clay_hip {
         const void* bank;
     };
     std::vector<Graph> graphs;
+    // Pinned staging of persistent-form uploads: a ring whose regions are
+    // reused once the copy that read them has completed.
+    struct Ring {
+        char* base = nullptr;
+        size_t capacity = 0, head = 0;
+        struct Use {
+            size_t begin, end;
+            hipEvent_t done;
+        };
+        std::deque<Use> uses;
+        size_t pendingBegin = 0, pendingEnd = 0;
+
+        void retire(const Use& u)
+        {
+            check(hipEventSynchronize(u.done), "staging ring");
+            (void)hipEventDestroy(u.done);
+        }
+
+        char* take(size_t bytes)
+        {
+            bytes = std::max<size_t>(bytes, 1);
+            if (bytes > capacity) {
+                for (auto& u : uses)
+                    retire(u);
+                uses.clear();
+                if (base)
+                    check(hipHostFree(base), "hipHostFree");
+                base = nullptr;
+                capacity = std::max<size_t>(bytes, size_t(64) << 20);
+                check(hipHostMalloc(reinterpret_cast<void**>(&base), capacity, hipHostMallocDefault), "staging ring");
+                head = 0;
+            }
+            if (head + bytes > capacity)
+                head = 0;
+            const size_t begin = head, end = head + bytes;
+            // Every earlier region that overlaps must have been read by its
+            // copy; after wrap-arounds such regions need not be the oldest.
+            for (auto it = uses.begin(); it != uses.end();) {
+                if (it->begin < end && begin < it->end) {
+                    retire(*it);
+                    it = uses.erase(it);
+                } else {
+                    ++it;
+                }
+            }
+            pendingBegin = begin;
+            pendingEnd = end;
+            head = end;
+            return base + begin;
+        }
+
+        void commit(hipStream_t stream)
+        {
+            hipEvent_t e = nullptr;
+            check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
+            check(hipEventRecord(e, stream), "hipEventRecord");
+            uses.push_back({pendingBegin, pendingEnd, e});
+        }
+
+        void release()
+        {
+            for (auto& u : uses) {
+                (void)hipEventSynchronize(u.done);
+                (void)hipEventDestroy(u.done);
+            }
+            uses.clear();
+            if (base)
+                (void)hipHostFree(base);
+            base = nullptr;
+        }
+    } ring;
+    // Small pinned buffers (per-unit results) kept for reuse.
+    std::vector<std::pair<void*, size_t>> pinnedSpare;
+
+    void* pinned(size_t bytes)
+    {
+        for (size_t i = 0; i < pinnedSpare.size(); ++i)
+            if (pinnedSpare[i].second >= bytes) {
+                void* p = pinnedSpare[i].first;
+                pinnedSpare.erase(pinnedSpare.begin() + std::ptrdiff_t(i));
+                return p;
+            }
+        void* p = nullptr;
+        const size_t capacity = std::max<size_t>(bytes, 64 * sizeof(clay_tail_result));
+        check(hipHostMalloc(&p, capacity, hipHostMallocDefault), "pinned results");
+        pinnedSizes[p] = capacity;
+        return p;
+    }
+
+    void unpin(void* p)
+    {
+        if (p)
+            pinnedSpare.emplace_back(p, pinnedSizes[p]);
+    }
+    std::map<void*, size_t> pinnedSizes;
 
     int channels() const { return natural ? 1 : 2; }
 
@@ -548,8 +644,9 @@ void upload(clay_hip* h, clay_hip_batch& b, bool streamOrdered = false)
     const size_t oVoices = a.add(bytesOf(b.voices));
     const size_t oPitch = a.add(bytesOf(b.pitchHeads)), oChains = a.add(bytesOf(b.chainHeads));
     const size_t oByStart = a.add(bytesOf(b.chainByStart));
-    const size_t oStates = a.add(size_t(b.chains) * sizeof(VoiceState));
     const size_t oAudio = a.add(bytesOf(b.audioJobs)), oOnsets = a.add(bytesOf(b.onsets)), oGroups = a.add(bytesOf(b.groups));
+    const size_t recordBytes = a.bytes;  // records first: one host image, one copy
+    const size_t oStates = a.add(size_t(b.chains) * sizeof(VoiceState));
     const size_t oMix = a.add(size_t(b.frames) * (h->natural ? sizeof(float2) : sizeof(short2)));
     const size_t oFiltered = a.add(size_t(b.filteredSamples) * sizeof(float));
     const size_t oPhase = a.add(size_t(b.pitchFrames) * sizeof(double));
@@ -568,9 +665,19 @@ void upload(clay_hip* h, clay_hip_batch& b, bool streamOrdered = false)
     b.streamOrdered = streamOrdered;
     b.arenaBytes = a.bytes;
     auto* base = static_cast<char*>(b.arena);
+    // Host image of the records: pinned ring (persistent form, the copy stays
+    // asynchronous) or a pageable buffer (plain form, which waits anyway).
+    std::vector<char> pageable;
+    char* image = nullptr;
+    if (streamOrdered) {
+        image = h->ring.take(recordBytes);
+    } else {
+        pageable.resize(recordBytes);
+        image = pageable.data();
+    }
     const auto put = [&](size_t offset, const auto& v) {
         if (!v.empty())
-            check(hipMemcpyAsync(base + offset, v.data(), bytesOf(v), hipMemcpyHostToDevice, h->stream), "batch upload");
+            std::memcpy(image + offset, v.data(), bytesOf(v));
         return base + offset;
     };
     b.dSpans = reinterpret_cast<Span*>(put(oSpans, b.spans));
@@ -583,10 +690,13 @@ void upload(clay_hip* h, clay_hip_batch& b, bool streamOrdered = false)
     b.dPitchHeads = reinterpret_cast<int*>(put(oPitch, b.pitchHeads));
     b.dChainHeads = reinterpret_cast<int*>(put(oChains, b.chainHeads));
     b.dChainByStart = reinterpret_cast<int*>(put(oByStart, b.chainByStart));
-    b.dStates = reinterpret_cast<VoiceState*>(base + oStates);
     b.dAudioJobs = reinterpret_cast<AudioJobDesc*>(put(oAudio, b.audioJobs));
     b.dOnsets = reinterpret_cast<int*>(put(oOnsets, b.onsets));
     b.dGroups = reinterpret_cast<int2*>(put(oGroups, b.groups));
+    check(hipMemcpyAsync(base, image, recordBytes, hipMemcpyHostToDevice, h->stream), "batch upload");
+    if (streamOrdered)
+        h->ring.commit(h->stream);
+    b.dStates = reinterpret_cast<VoiceState*>(base + oStates);
     b.dMix = base + oMix;
     b.dFiltered = reinterpret_cast<float*>(base + oFiltered);
     b.dPhase = reinterpret_cast<double*>(base + oPhase);
@@ -1081,6 +1191,9 @@ int clay_hip_destroy(clay_hip* h)
         (void)hipEventDestroy(e);
     for (auto& g : h->graphs)
         (void)hipGraphExecDestroy(g.exec);
+    h->ring.release();
+    for (auto& p : h->pinnedSizes)
+        (void)hipHostFree(p.first);
     for (auto& stage : h->stages) {
         if (stage.host)
             (void)hipHostFree(stage.host);
@@ -1402,8 +1515,7 @@ int renderWindowPersistent(clay_hip* h, const clay_plan_view* plans, size_t coun
             delete u.b;
             u.b = nullptr;
         }
-        if (u.results)
-            (void)hipHostFree(u.results);
+        h->unpin(u.results);
         u.results = nullptr;
         if (u.ready)
             h->eventPool.push_back(u.ready);
@@ -1514,8 +1626,7 @@ int renderWindowPersistent(clay_hip* h, const clay_plan_view* plans, size_t coun
                             continue;
                         }
                         finishBatch(h, *u->b);
-                        check(hipHostMalloc(reinterpret_cast<void**>(&u->results),
-                                            u->jobs * sizeof(clay_tail_result), hipHostMallocDefault), "pinned results");
+                        u->results = static_cast<clay_tail_result*>(h->pinned(u->jobs * sizeof(clay_tail_result)));
                         check(hipMemcpyAsync(u->results, u->b->dResults, u->jobs * sizeof(clay_tail_result),
                                              hipMemcpyDeviceToHost, h->stream), "results download");
                         u->ready = h->event();
