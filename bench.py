@@ -82,6 +82,20 @@ def kernel_bytes(plans, taps, natural):
     }, voice_frames, frames
 
 
+def measured_traffic(config, kernel):
+    """HBM bytes per launch of `kernel` from the committed counter summary of
+    this workload (profiles/traffic_<config>.txt: FETCH_SIZE and WRITE_SIZE
+    passes scaled by a copy kernel of known size), or None."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{config}.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        parts = line.split()
+        if parts and parts[0] == kernel and not line.startswith("#"):
+            return float(parts[-1])
+    return None
+
+
 def cpu_baseline(sfz, mids, taps, jobs, threads):
     """FP64 oracle on a bounded sample of the same workload, on host cores."""
     import oracle
@@ -203,7 +217,8 @@ def main():
         print("bench.py measures one GPU: this build has no multi-GPU path (jobs shard by job; see DESIGN.md)",
               file=sys.stderr)
         sys.exit(2)
-    jobs = args.jobs or {"c1": 1, "c2": 64, "c3": 512}[args.config]
+    default_jobs = {"c1": 1, "c2": 64, "c3": 512}[args.config]
+    jobs = args.jobs or default_jobs
 
     t0 = time.perf_counter()
     sfz, mids, taps = workload(args.config, args.synth_dir, jobs, args.seconds)
@@ -273,7 +288,8 @@ def main():
         "kernels_ms": {k: round(v["ms"], 3) for k, v in kernels.items()},
         "statuses": statuses,
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(args.config, dominant) if jobs == default_jobs else None},
     }
     batch.close()
     renderer.close()
