@@ -201,7 +201,8 @@ class Renderer:
         def publish(user, batch_ptr, first):
             try:
                 b = batch_ptr.contents
-                results = [b.results[i] for i in range(b.result_count)] if b.result_count else []
+                # Copies: the results array belongs to the library and is reused after the call.
+                results = [abi.TailResult.from_buffer_copy(b.results[i]) for i in range(b.result_count)]
                 samples = 0
                 for r in results:
                     samples = max(samples, (r.offset + r.data_frames) * self.channels)

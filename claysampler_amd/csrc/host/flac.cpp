@@ -192,7 +192,7 @@ void writeSubframe(BitWriter& w, const std::vector<int64_t>& x, int bps, const S
             w.putSigned(v, bps);
         return;
     }
-    w.put(uint64_t(0x10 | s.order) << 1, 8);  // 0 + 001ooo + 0
+    w.put(uint64_t(0x08 | s.order) << 1, 8);  // 0 + 001ooo + 0
     for (int i = 0; i < s.order; ++i)
         w.putSigned(x[size_t(i)], bps);
     w.put(0, 2);
