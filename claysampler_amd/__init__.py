@@ -62,7 +62,7 @@ HIP_SYMBOLS = [
     "clay_hip_create", "clay_hip_destroy", "clay_hip_upload_bank", "clay_hip_render_window",
     "clay_hip_release_pcm", "clay_hip_get_stats", "clay_hip_last_error", "clay_hip_sinc_table",
     "clay_hip_batch_prepare", "clay_hip_batch_run", "clay_hip_batch_results", "clay_hip_batch_kernel_time",
-    "clay_hip_batch_destroy",
+    "clay_hip_batch_destroy", "clay_hip_batch_voice_tails",
 ]
 
 
@@ -80,6 +80,8 @@ def hip():
         _bind(lib, "clay_hip_render_window", C.c_int, P, C.POINTER(abi.PlanView), C.c_size_t, abi.PUBLISH_FN,
               abi.REFILL_FN, P)
         _bind(lib, "clay_hip_release_pcm", C.c_int, P, P)
+        _bind(lib, "clay_hip_batch_voice_tails", C.c_int, P, P, C.POINTER(C.c_int32), C.c_size_t,
+              C.POINTER(C.c_size_t))
         _bind(lib, "clay_hip_get_stats", C.c_int, P, C.POINTER(abi.HipStats))
         _bind(lib, "clay_hip_last_error", C.c_char_p, P)
         _bind(lib, "clay_hip_sinc_table", C.c_int, C.c_int, C.POINTER(C.c_float), C.c_size_t)
@@ -155,11 +157,12 @@ class Renderer:
     """The HIP render core (clay_hip handle)."""
 
     def __init__(self, policy=abi.POLICY_NATURAL, taps=48, rate=44100, quality=False, group_minimum=0,
-                 batch_jobs=64, device=0, form=abi.FORM_PLAIN, audio=None, segment_frames=0, flags=0):
+                 batch_jobs=64, device=0, form=abi.FORM_PLAIN, audio=None, segment_frames=0, flags=0,
+                 voice_silence_db=0.0):
         self.lib = hip()
         audio = audio if audio is not None else abi.default_audio_config(rate, quality)
         self.cfg = abi.HipConfig(device, policy, taps, form, audio, group_minimum, 256 << 20, batch_jobs,
-                                 segment_frames, flags)
+                                 segment_frames, flags, voice_silence_db, 0)
         self.handle = C.c_void_p()
         if self.lib.clay_hip_create(C.byref(self.cfg), C.byref(self.handle)) != abi.OK:
             raise ClayError(self.lib.clay_hip_last_error(None).decode())
@@ -255,6 +258,16 @@ class Batch:
             self.r.handle, self.handle, res, self.count, pcm.ctypes.data_as(C.POINTER(C.c_int16)), pcm.size,
             C.byref(samples)))
         return list(res), pcm
+
+    def voice_tails(self):
+        """Exact per-chain retirement frames of the last run (chain end when not retired)."""
+        n = C.c_size_t()
+        self.r._check(self.r.lib.clay_hip_batch_voice_tails(self.r.handle, self.handle, None, 0, C.byref(n)))
+        out = np.zeros(max(1, n.value), np.int32)
+        self.r._check(self.r.lib.clay_hip_batch_voice_tails(self.r.handle, self.handle,
+                                                            out.ctypes.data_as(C.POINTER(C.c_int32)), out.size,
+                                                            C.byref(n)))
+        return out[:n.value]
 
     def kernel_time(self, name):
         ms, n = C.c_double(), C.c_int()

@@ -103,7 +103,11 @@ class TailResult(C.Structure):
 class HipConfig(C.Structure):
     _fields_ = [("device", C.c_int32), ("tail_policy", C.c_int32), ("sinc_taps", C.c_int32), ("form", C.c_int32),
                 ("audio", AudioConfig), ("group_minimum", C.c_int32), ("waiting_limit", C.c_uint64),
-                ("batch_jobs", C.c_uint64), ("segment_frames", C.c_int32), ("flags", C.c_int32)]
+                ("batch_jobs", C.c_uint64), ("segment_frames", C.c_int32), ("flags", C.c_int32),
+                ("voice_silence_db", C.c_float), ("pad0", C.c_int32)]
+
+
+SILENT_CHUNKS = 16
 
 
 FLAG_NO_GRAPH = 1

@@ -90,7 +90,11 @@ struct VoiceState {
     double target[CLAY_FILTER_SLOTS][5];
     double input[CLAY_FILTER_SLOTS][3];
     int32_t have[CLAY_FILTER_SLOTS];
-    int32_t pad2[3];
+    // Voice-silence policy: retired chains end at `silence`; the chunk being
+    // measured (index + 1, 0 = none) with its running peak and eligibility.
+    int32_t retired, silence, silentChunks, openChunk;
+    float openPeak;
+    int32_t openEligible;
 };
 
 // Device pointers and the frame window of one resident unit (a batch of one
@@ -121,6 +125,8 @@ struct UnitView {
 // Totals of one round, read by the grid-stride loops of the render kernels.
 struct RoundHeader {
     int32_t units, tiles, chains, pitches;
+    float silence;                      // voice-silence threshold (linear), 0 = off
+    int32_t pad[3];
 };
 
 struct StereoStats {

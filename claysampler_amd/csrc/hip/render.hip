@@ -148,6 +148,49 @@ __device__ __forceinline__ double laneValue(double v, int lane) { return __shfl(
 
 } // namespace
 
+// Voice-silence policy state of one chain (wave-uniform values).
+struct SilenceTrack {
+    int retired, silence, silentChunks, openChunk;
+    float openPeak;
+    int openEligible;
+};
+
+// Measures one chunk pass of a chain: the wave's peak |output| (non-finite
+// counts as loud) and whether every active frame is at or after the voice's
+// release. A chunk (64 absolute frames) may take several passes when spans
+// split it; it is judged when its last frame has been rendered. After
+// CLAY_SILENT_CHUNKS consecutive silent chunks the chain retires at the end
+// of the current chunk. Returns true when it retires.
+__device__ __forceinline__ bool silenceStep(SilenceTrack& t, float level, int f0, int hi, bool active, float outL,
+                                            float outR, int frame, int releaseStart)
+{
+    float a = active ? fmaxf(fabsf(outL), fabsf(outR)) : 0.f;
+    if (!(a <= 3.402823466e38f))
+        a = __builtin_inff();
+    for (int d = 32; d; d >>= 1)
+        a = fmaxf(a, __shfl_xor(a, d, 64));
+    const bool eligible = !active || (releaseStart >= 0 && frame >= releaseStart);
+    const bool all = __ballot(!eligible) == 0;
+    const int chunk = (f0 >> 6) + 1;
+    if (t.openChunk != chunk) {
+        t.openChunk = chunk;
+        t.openPeak = 0.f;
+        t.openEligible = 1;
+    }
+    t.openPeak = fmaxf(t.openPeak, a);
+    t.openEligible = t.openEligible && all;
+    if (hi < 64)
+        return false;
+    const bool silent = t.openEligible && t.openPeak < level;
+    t.openChunk = 0;
+    t.silentChunks = silent ? t.silentChunks + 1 : 0;
+    if (t.silentChunks < CLAY_SILENT_CHUNKS)
+        return false;
+    t.retired = 1;
+    t.silence = f0 + 64;
+    return true;
+}
+
 // One wave per voice chain (a voice's spans with filters and/or variable pitch),
 // 64 output frames per step, chunks aligned to absolute frame multiples of 64:
 //   1. the 64 frames of voice audio before pan (mono) times the job gain, one per lane;
@@ -165,9 +208,13 @@ __device__ __forceinline__ void voiceChain(const int16_t* __restrict__ bank, con
                                            const Job* __restrict__ jobs, const clay_control_point* __restrict__ points,
                                            const Mod* __restrict__ mods, const double* __restrict__ phase,
                                            float* __restrict__ filtered, VoiceState* __restrict__ states,
-                                           int segFirst, int segEnd, int head, ChainCarry& carry, int lane)
+                                           int segFirst, int segEnd, int head, float silenceLevel, ChainCarry& carry,
+                                           int lane)
 {
     VoiceState& st = states[spans[head].filterTrack];
+    SilenceTrack track{st.retired, st.silence, st.silentChunks, st.openChunk, st.openPeak, st.openEligible};
+    if (track.retired)
+        return;  // retired in an earlier segment: nothing more to render
     int sectionCount[CLAY_FILTER_SLOTS], total = 0;
     for (int f = 0; f < CLAY_FILTER_SLOTS; ++f) {
         sectionCount[f] = sections(cold[head].filters[f].type);
@@ -227,231 +274,241 @@ __device__ __forceinline__ void voiceChain(const int16_t* __restrict__ bank, con
                 xr = a.y * g;
             }
             float* dst = filtered + v.filtered + uint64_t(max(frame - first, 0)) * uint64_t(v.channels);
-            if (!hasFilter) {
-                if (active) {
-                    dst[0] = xl;
-                    if (stereo)
-                        dst[1] = xr;
-                }
-                continue;
-            }
-            // Filter inputs of this lane's frame for slot f: cutoff (Hz) and
-            // resonance, or for EQ slots frequency, bandwidth and gain.
-            const auto slotInputs = [&](int f, double& in0, double& in1, double& in2) {
-                double cents = 0, db = 0, depth = cv.extra[0].depth, eq0 = 0, eq1 = 0, eq2 = 0;
-                const auto add = [&](int k) {
-                    const int t = mods[v.firstMod + k].target;
-                    if (f == 0 && t == CLAY_MOD_CUTOFF) cents += reader.value(mods, points, k, frame);
-                    else if (f == 0 && t == CLAY_MOD_RESONANCE) db += reader.value(mods, points, k, frame);
-                    else if (f == 0 && t == CLAY_MOD_FILEG_DEPTH) depth += reader.value(mods, points, k, frame);
-                    else if (f == 1 && t == CLAY_MOD_CUTOFF2) cents += reader.value(mods, points, k, frame);
-                    else if (f == 1 && t == CLAY_MOD_RESONANCE2) db += reader.value(mods, points, k, frame);
-                    else if (f >= 2 && t >= CLAY_MOD_EQ1_FREQUENCY && (t - CLAY_MOD_EQ1_FREQUENCY) / 3 == f - 2) {
-                        const int field = (t - CLAY_MOD_EQ1_FREQUENCY) % 3;
-                        const double x = reader.value(mods, points, k, frame);
-                        if (field == 0) eq0 += x;
-                        else if (field == 1) eq1 += x;
-                        else eq2 += x;
+            float outL = xl, outR = xr;
+            if (hasFilter) {
+                // Filter inputs of this lane's frame for slot f: cutoff (Hz) and
+                // resonance, or for EQ slots frequency, bandwidth and gain.
+                const auto slotInputs = [&](int f, double& in0, double& in1, double& in2) {
+                    double cents = 0, db = 0, depth = cv.extra[0].depth, eq0 = 0, eq1 = 0, eq2 = 0;
+                    const auto add = [&](int k) {
+                        const int t = mods[v.firstMod + k].target;
+                        if (f == 0 && t == CLAY_MOD_CUTOFF) cents += reader.value(mods, points, k, frame);
+                        else if (f == 0 && t == CLAY_MOD_RESONANCE) db += reader.value(mods, points, k, frame);
+                        else if (f == 0 && t == CLAY_MOD_FILEG_DEPTH) depth += reader.value(mods, points, k, frame);
+                        else if (f == 1 && t == CLAY_MOD_CUTOFF2) cents += reader.value(mods, points, k, frame);
+                        else if (f == 1 && t == CLAY_MOD_RESONANCE2) db += reader.value(mods, points, k, frame);
+                        else if (f >= 2 && t >= CLAY_MOD_EQ1_FREQUENCY && (t - CLAY_MOD_EQ1_FREQUENCY) / 3 == f - 2) {
+                            const int field = (t - CLAY_MOD_EQ1_FREQUENCY) % 3;
+                            const double x = reader.value(mods, points, k, frame);
+                            if (field == 0) eq0 += x;
+                            else if (field == 1) eq1 += x;
+                            else eq2 += x;
+                        }
+                    };
+                    // Unrolled over the cursor slots so the cursors stay in registers.
+    #pragma unroll
+                    for (int k = 0; k < kCursors; ++k)
+                        if (k < v.modCount)
+                            add(k);
+                    for (int k = kCursors; k < v.modCount; ++k)
+                        add(k);
+                    const double eq[3] = {eq0, eq1, eq2};
+                    const clay_voice_filter& flt = cv.filters[f];
+                    if (f < 2) {
+                        if (f == 0 && cv.active[0])
+                            cents += depth * envelopeShapeD(cv.extra[0], frame);
+                        in0 = double(flt.cutoff) * exp2(cents / 1200);
+                        in1 = double(flt.resonance) + db;
+                        in2 = 0;
+                    } else {
+                        in0 = double(flt.cutoff) + eq[0];
+                        in1 = double(flt.bandwidth) + eq[1];
+                        in2 = double(flt.gain) + eq[2];
                     }
                 };
-                // Unrolled over the cursor slots so the cursors stay in registers.
-#pragma unroll
-                for (int k = 0; k < kCursors; ++k)
-                    if (k < v.modCount)
-                        add(k);
-                for (int k = kCursors; k < v.modCount; ++k)
-                    add(k);
-                const double eq[3] = {eq0, eq1, eq2};
-                const clay_voice_filter& flt = cv.filters[f];
-                if (f < 2) {
-                    if (f == 0 && cv.active[0])
-                        cents += depth * envelopeShapeD(cv.extra[0], frame);
-                    in0 = double(flt.cutoff) * exp2(cents / 1200);
-                    in1 = double(flt.resonance) + db;
-                    in2 = 0;
-                } else {
-                    in0 = double(flt.cutoff) + eq[0];
-                    in1 = double(flt.bandwidth) + eq[1];
-                    in2 = double(flt.gain) + eq[2];
-                }
-            };
-            const bool recompute = moving || !initialized;
-            int sec = 0;
-            double ul = xl, ur = xr;
-            for (int f = 0; f < CLAY_FILTER_SLOTS; ++f) {
-                const int slotSections = sections(cold[head].filters[f].type);
-                if (!slotSections)
-                    continue;
-                double c[5];
-                if (recompute) {
-                    double in0 = 0, in1 = 0, in2 = 0;
-                    if (active)
-                        slotInputs(f, in0, in1, in2);
-                    // 2. Targets: recompute where inputs change, copy forward otherwise.
-                    double p0 = up(in0, 1), p1 = up(in1, 1), p2 = up(in2, 1);
-                    bool prevValid = lane > lo;
-                    if (lane == lo) {
-                        p0 = carry.input[f][0], p1 = carry.input[f][1], p2 = carry.input[f][2];
-                        prevValid = carry.have[f] != 0;
-                    }
-                    const bool flag = active && (!prevValid || in0 != p0 || in1 != p1 || in2 != p2);
-                    const bool anyFlag = __ballot(flag) != 0;
-                    double t[5] = {0, 0, 0, 0, 0};
-                    if (!anyFlag) {
-                        for (int i = 0; i < 5; ++i)
-                            t[i] = carry.target[f][i];
-                    } else {
-                        if (flag) {
-                            const CoeffD q = f < 2 ? coefficientsD(cv.filters[f].type, in0, in1, rate, 0, 0)
-                                                   : coefficientsD(cv.filters[f].type, in0, 0, rate, in1, in2);
-                            t[0] = q.b0, t[1] = q.b1, t[2] = q.b2, t[3] = q.a1, t[4] = q.a2;
+                const bool recompute = moving || !initialized;
+                int sec = 0;
+                double ul = xl, ur = xr;
+                for (int f = 0; f < CLAY_FILTER_SLOTS; ++f) {
+                    const int slotSections = sections(cold[head].filters[f].type);
+                    if (!slotSections)
+                        continue;
+                    double c[5];
+                    if (recompute) {
+                        double in0 = 0, in1 = 0, in2 = 0;
+                        if (active)
+                            slotInputs(f, in0, in1, in2);
+                        // 2. Targets: recompute where inputs change, copy forward otherwise.
+                        double p0 = up(in0, 1), p1 = up(in1, 1), p2 = up(in2, 1);
+                        bool prevValid = lane > lo;
+                        if (lane == lo) {
+                            p0 = carry.input[f][0], p1 = carry.input[f][1], p2 = carry.input[f][2];
+                            prevValid = carry.have[f] != 0;
                         }
-                        int have = flag ? 1 : 0;
-                        for (int d = 1; d < 64; d <<= 1) {
-                            const int hp = __shfl_up(have, d, 64);
-                            double tp[5];
-                            for (int i = 0; i < 5; ++i)
-                                tp[i] = up(t[i], d);
-                            if (!have && lane >= d && hp) {
-                                for (int i = 0; i < 5; ++i)
-                                    t[i] = tp[i];
-                                have = 1;
-                            }
-                        }
-                        if (!have)
+                        const bool flag = active && (!prevValid || in0 != p0 || in1 != p1 || in2 != p2);
+                        const bool anyFlag = __ballot(flag) != 0;
+                        double t[5] = {0, 0, 0, 0, 0};
+                        if (!anyFlag) {
                             for (int i = 0; i < 5; ++i)
                                 t[i] = carry.target[f][i];
-                    }
-                    if (!anyFlag && moving && initialized) {
-                        // 3a. Constant target over the chunk: c_n = t + q^k (c - t), q = 1 - a,
-                        // k = frames smoothed so far in this chunk.
-                        const int k = lane - lo + 1;
-                        double qk = 1, qp = 1 - amount;
-                        for (int bit = 1; bit <= 64; bit <<= 1, qp *= qp)
-                            if (k & bit)
-                                qk *= qp;
-                        for (int i = 0; i < 5; ++i)
-                            c[i] = fma(qk, carry.coef[f][i] - t[i], t[i]);
-                    } else {
-                        // 3b. Smoothing scan: c_n = M_n c_{n-1} + V_n.
-                        // A filter without modulation takes its coefficients once, unsmoothed.
-                        double M, V[5];
-                        if (!active || (!moving && lane != lo)) {
-                            M = 1;
-                            for (int i = 0; i < 5; ++i)
-                                V[i] = 0;
-                        } else if (!initialized && lane == lo) {
-                            M = 0;
-                            for (int i = 0; i < 5; ++i)
-                                V[i] = t[i];
                         } else {
-                            M = 1 - amount;
+                            if (flag) {
+                                const CoeffD q = f < 2 ? coefficientsD(cv.filters[f].type, in0, in1, rate, 0, 0)
+                                                       : coefficientsD(cv.filters[f].type, in0, 0, rate, in1, in2);
+                                t[0] = q.b0, t[1] = q.b1, t[2] = q.b2, t[3] = q.a1, t[4] = q.a2;
+                            }
+                            int have = flag ? 1 : 0;
+                            for (int d = 1; d < 64; d <<= 1) {
+                                const int hp = __shfl_up(have, d, 64);
+                                double tp[5];
+                                for (int i = 0; i < 5; ++i)
+                                    tp[i] = up(t[i], d);
+                                if (!have && lane >= d && hp) {
+                                    for (int i = 0; i < 5; ++i)
+                                        t[i] = tp[i];
+                                    have = 1;
+                                }
+                            }
+                            if (!have)
+                                for (int i = 0; i < 5; ++i)
+                                    t[i] = carry.target[f][i];
+                        }
+                        if (!anyFlag && moving && initialized) {
+                            // 3a. Constant target over the chunk: c_n = t + q^k (c - t), q = 1 - a,
+                            // k = frames smoothed so far in this chunk.
+                            const int k = lane - lo + 1;
+                            double qk = 1, qp = 1 - amount;
+                            for (int bit = 1; bit <= 64; bit <<= 1, qp *= qp)
+                                if (k & bit)
+                                    qk *= qp;
                             for (int i = 0; i < 5; ++i)
-                                V[i] = amount * t[i];
+                                c[i] = fma(qk, carry.coef[f][i] - t[i], t[i]);
+                        } else {
+                            // 3b. Smoothing scan: c_n = M_n c_{n-1} + V_n.
+                            // A filter without modulation takes its coefficients once, unsmoothed.
+                            double M, V[5];
+                            if (!active || (!moving && lane != lo)) {
+                                M = 1;
+                                for (int i = 0; i < 5; ++i)
+                                    V[i] = 0;
+                            } else if (!initialized && lane == lo) {
+                                M = 0;
+                                for (int i = 0; i < 5; ++i)
+                                    V[i] = t[i];
+                            } else {
+                                M = 1 - amount;
+                                for (int i = 0; i < 5; ++i)
+                                    V[i] = amount * t[i];
+                            }
+                            for (int d = 1; d < 64; d <<= 1) {
+                                const double Mp = up(M, d);
+                                double Vp[5];
+                                for (int i = 0; i < 5; ++i)
+                                    Vp[i] = up(V[i], d);
+                                if (lane >= d) {
+                                    for (int i = 0; i < 5; ++i)
+                                        V[i] = fma(M, Vp[i], V[i]);
+                                    M *= Mp;
+                                }
+                            }
+                            for (int i = 0; i < 5; ++i)
+                                c[i] = fma(M, carry.coef[f][i], V[i]);
+                        }
+                        // Carry: values of the last active lane.
+                        const int last = hi - 1;
+                        double nc[5], nt[5], ni[3];
+                        for (int i = 0; i < 5; ++i) {
+                            nc[i] = laneValue(c[i], last);
+                            nt[i] = laneValue(t[i], last);
+                        }
+                        ni[0] = laneValue(in0, last);
+                        ni[1] = laneValue(in1, last);
+                        ni[2] = laneValue(in2, last);
+                        __builtin_amdgcn_wave_barrier();
+                        if (lane == 0) {
+                            for (int i = 0; i < 5; ++i) {
+                                carry.coef[f][i] = nc[i];
+                                carry.target[f][i] = nt[i];
+                            }
+                            for (int i = 0; i < 3; ++i)
+                                carry.input[f][i] = ni[i];
+                            carry.have[f] = 1;
+                        }
+                        __builtin_amdgcn_wave_barrier();
+                    } else {
+                        for (int i = 0; i < 5; ++i)
+                            c[i] = carry.coef[f][i];
+                    }
+                    // 4. Biquad sections of this slot: s_n = A_n s_{n-1} + B_n u_n.
+                    for (int k = 0; k < slotSections; ++k, ++sec) {
+                        // A = [[-a1, 1], [-a2, 0]]; inactive lanes are the identity.
+                        double a00 = -c[3], a01 = 1, a10 = -c[4], a11 = 0;
+                        double wl0 = (c[1] - c[3] * c[0]) * ul, wl1 = (c[2] - c[4] * c[0]) * ul;
+                        double wr0 = (c[1] - c[3] * c[0]) * ur, wr1 = (c[2] - c[4] * c[0]) * ur;
+                        if (!active) {
+                            a00 = 1, a01 = 0, a10 = 0, a11 = 1;
+                            wl0 = wl1 = wr0 = wr1 = 0;
                         }
                         for (int d = 1; d < 64; d <<= 1) {
-                            const double Mp = up(M, d);
-                            double Vp[5];
-                            for (int i = 0; i < 5; ++i)
-                                Vp[i] = up(V[i], d);
-                            if (lane >= d) {
-                                for (int i = 0; i < 5; ++i)
-                                    V[i] = fma(M, Vp[i], V[i]);
-                                M *= Mp;
-                            }
-                        }
-                        for (int i = 0; i < 5; ++i)
-                            c[i] = fma(M, carry.coef[f][i], V[i]);
-                    }
-                    // Carry: values of the last active lane.
-                    const int last = hi - 1;
-                    double nc[5], nt[5], ni[3];
-                    for (int i = 0; i < 5; ++i) {
-                        nc[i] = laneValue(c[i], last);
-                        nt[i] = laneValue(t[i], last);
-                    }
-                    ni[0] = laneValue(in0, last);
-                    ni[1] = laneValue(in1, last);
-                    ni[2] = laneValue(in2, last);
-                    __builtin_amdgcn_wave_barrier();
-                    if (lane == 0) {
-                        for (int i = 0; i < 5; ++i) {
-                            carry.coef[f][i] = nc[i];
-                            carry.target[f][i] = nt[i];
-                        }
-                        for (int i = 0; i < 3; ++i)
-                            carry.input[f][i] = ni[i];
-                        carry.have[f] = 1;
-                    }
-                    __builtin_amdgcn_wave_barrier();
-                } else {
-                    for (int i = 0; i < 5; ++i)
-                        c[i] = carry.coef[f][i];
-                }
-                // 4. Biquad sections of this slot: s_n = A_n s_{n-1} + B_n u_n.
-                for (int k = 0; k < slotSections; ++k, ++sec) {
-                    // A = [[-a1, 1], [-a2, 0]]; inactive lanes are the identity.
-                    double a00 = -c[3], a01 = 1, a10 = -c[4], a11 = 0;
-                    double wl0 = (c[1] - c[3] * c[0]) * ul, wl1 = (c[2] - c[4] * c[0]) * ul;
-                    double wr0 = (c[1] - c[3] * c[0]) * ur, wr1 = (c[2] - c[4] * c[0]) * ur;
-                    if (!active) {
-                        a00 = 1, a01 = 0, a10 = 0, a11 = 1;
-                        wl0 = wl1 = wr0 = wr1 = 0;
-                    }
-                    for (int d = 1; d < 64; d <<= 1) {
-                        const double b00 = up(a00, d), b01 = up(a01, d), b10 = up(a10, d), b11 = up(a11, d);
-                        const double pl0 = up(wl0, d), pl1 = up(wl1, d);
-                        double pr0 = 0, pr1 = 0;
-                        if (stereo) {
-                            pr0 = up(wr0, d);
-                            pr1 = up(wr1, d);
-                        }
-                        if (lane >= d) {
-                            // (A, w) o (B, p) = (A B, A p + w)
-                            wl0 = fma(a00, pl0, fma(a01, pl1, wl0));
-                            wl1 = fma(a10, pl0, fma(a11, pl1, wl1));
+                            const double b00 = up(a00, d), b01 = up(a01, d), b10 = up(a10, d), b11 = up(a11, d);
+                            const double pl0 = up(wl0, d), pl1 = up(wl1, d);
+                            double pr0 = 0, pr1 = 0;
                             if (stereo) {
-                                wr0 = fma(a00, pr0, fma(a01, pr1, wr0));
-                                wr1 = fma(a10, pr0, fma(a11, pr1, wr1));
+                                pr0 = up(wr0, d);
+                                pr1 = up(wr1, d);
                             }
-                            const double n00 = fma(a00, b00, a01 * b10), n01 = fma(a00, b01, a01 * b11);
-                            const double n10 = fma(a10, b00, a11 * b10), n11 = fma(a10, b01, a11 * b11);
-                            a00 = n00, a01 = n01, a10 = n10, a11 = n11;
+                            if (lane >= d) {
+                                // (A, w) o (B, p) = (A B, A p + w)
+                                wl0 = fma(a00, pl0, fma(a01, pl1, wl0));
+                                wl1 = fma(a10, pl0, fma(a11, pl1, wl1));
+                                if (stereo) {
+                                    wr0 = fma(a00, pr0, fma(a01, pr1, wr0));
+                                    wr1 = fma(a10, pr0, fma(a11, pr1, wr1));
+                                }
+                                const double n00 = fma(a00, b00, a01 * b10), n01 = fma(a00, b01, a01 * b11);
+                                const double n10 = fma(a10, b00, a11 * b10), n11 = fma(a10, b01, a11 * b11);
+                                a00 = n00, a01 = n01, a10 = n10, a11 = n11;
+                            }
                         }
+                        const double zl1 = carry.z[sec][0][0], zl2 = carry.z[sec][0][1];
+                        const double zr1 = carry.z[sec][1][0], zr2 = carry.z[sec][1][1];
+                        const double sl1 = fma(a00, zl1, fma(a01, zl2, wl0)), sl2 = fma(a10, zl1, fma(a11, zl2, wl1));
+                        const double sr1 = fma(a00, zr1, fma(a01, zr2, wr0)), sr2 = fma(a10, zr1, fma(a11, zr2, wr1));
+                        // y_n = b0 u_n + z1_{n-1}
+                        double prevL = up(sl1, 1), prevR = up(sr1, 1);
+                        if (lane == lo) {
+                            prevL = zl1;
+                            prevR = zr1;
+                        }
+                        ul = fma(c[0], ul, prevL);
+                        ur = stereo ? fma(c[0], ur, prevR) : 0.0;
+                        const int last = hi - 1;
+                        const double ql1 = laneValue(sl1, last), ql2 = laneValue(sl2, last);
+                        const double qr1 = laneValue(sr1, last), qr2 = laneValue(sr2, last);
+                        __builtin_amdgcn_wave_barrier();
+                        if (lane == 0) {
+                            carry.z[sec][0][0] = ql1, carry.z[sec][0][1] = ql2;
+                            carry.z[sec][1][0] = qr1, carry.z[sec][1][1] = qr2;
+                        }
+                        __builtin_amdgcn_wave_barrier();
                     }
-                    const double zl1 = carry.z[sec][0][0], zl2 = carry.z[sec][0][1];
-                    const double zr1 = carry.z[sec][1][0], zr2 = carry.z[sec][1][1];
-                    const double sl1 = fma(a00, zl1, fma(a01, zl2, wl0)), sl2 = fma(a10, zl1, fma(a11, zl2, wl1));
-                    const double sr1 = fma(a00, zr1, fma(a01, zr2, wr0)), sr2 = fma(a10, zr1, fma(a11, zr2, wr1));
-                    // y_n = b0 u_n + z1_{n-1}
-                    double prevL = up(sl1, 1), prevR = up(sr1, 1);
-                    if (lane == lo) {
-                        prevL = zl1;
-                        prevR = zr1;
-                    }
-                    ul = fma(c[0], ul, prevL);
-                    ur = stereo ? fma(c[0], ur, prevR) : 0.0;
-                    const int last = hi - 1;
-                    const double ql1 = laneValue(sl1, last), ql2 = laneValue(sl2, last);
-                    const double qr1 = laneValue(sr1, last), qr2 = laneValue(sr2, last);
-                    __builtin_amdgcn_wave_barrier();
-                    if (lane == 0) {
-                        carry.z[sec][0][0] = ql1, carry.z[sec][0][1] = ql2;
-                        carry.z[sec][1][0] = qr1, carry.z[sec][1][1] = qr2;
-                    }
-                    __builtin_amdgcn_wave_barrier();
                 }
+                initialized = true;
+                outL = float(ul);
+                outR = float(ur);
             }
-            initialized = true;
             if (active) {
-                dst[0] = float(ul);
+                dst[0] = outL;
                 if (stereo)
-                    dst[1] = float(ur);
+                    dst[1] = outR;
             }
+            if (silenceLevel > 0.f &&
+                silenceStep(track, silenceLevel, f0, hi, active, outL, stereo ? outR : 0.f, frame, v.releaseStart))
+                break;
         }
+        if (track.retired)
+            break;
     }
     // Save the chain state for the next segment.
     __builtin_amdgcn_wave_barrier();
+    if (lane == 0) {
+        st.retired = track.retired;
+        st.silence = track.silence;
+        st.silentChunks = track.silentChunks;
+        st.openChunk = track.openChunk;
+        st.openPeak = track.openPeak;
+        st.openEligible = track.openEligible;
+    }
     if (hasFilter && lane == 0) {
         int k = 0;
         for (int f = 0; f < CLAY_FILTER_SLOTS; ++f) {
@@ -486,7 +543,7 @@ __global__ void __launch_bounds__(64 * kChainWaves) clayRoundVoiceChains(
         const auto& u = constant(units)[unit];
         voiceChain<T>(bank, sinc, global(u.spans), global(u.cold), global(u.jobs), global(u.points), global(u.mods),
                       global(u.phase), global(u.filtered), global(u.states), u.segFirst, u.segEnd,
-                      uniform(global(u.chainHeads)[item - u.chainBase]), carries[wave], lane);
+                      uniform(global(u.chainHeads)[item - u.chainBase]), header.silence, carries[wave], lane);
     }
 }
 
@@ -496,7 +553,8 @@ __global__ void __launch_bounds__(64 * kChainWaves) clayVoiceChains(
     const int16_t* __restrict__ bank, const float* __restrict__ sinc, const Span* __restrict__ spans,
     const SpanCold* __restrict__ cold, const Job* __restrict__ jobs, const int* __restrict__ heads,
     const clay_control_point* __restrict__ points, const Mod* __restrict__ mods, const double* __restrict__ phase,
-    float* __restrict__ filtered, VoiceState* __restrict__ states, int segFirst, int segEnd, int count)
+    float* __restrict__ filtered, VoiceState* __restrict__ states, int segFirst, int segEnd, int count,
+    float silenceLevel)
 {
     __shared__ ChainCarry carries[kChainWaves];
     const int wave = uniform(int(threadIdx.x >> 6)), lane = threadIdx.x & 63;
@@ -504,7 +562,7 @@ __global__ void __launch_bounds__(64 * kChainWaves) clayVoiceChains(
     if (item >= count)
         return;  // uniform per wave; no block barriers below
     voiceChain<T>(bank, sinc, spans, cold, jobs, points, mods, phase, filtered, states, segFirst, segEnd,
-                  uniform(heads[item]), carries[wave], lane);
+                  uniform(heads[item]), silenceLevel, carries[wave], lane);
 }
 
 // Per output frame: the dry voices of the tile rendered inline, times the job
@@ -516,7 +574,8 @@ __device__ __forceinline__ void mixFrame(const int16_t* __restrict__ bank, const
                                          const Span* __restrict__ spans, const Job* __restrict__ jobs,
                                          const int* __restrict__ voices, const clay_control_point* __restrict__ points,
                                          const Mod* __restrict__ mods, const float* __restrict__ filtered,
-                                         void* __restrict__ out, int segFirst, int segEnd, const Tile& tile, int lane)
+                                         const VoiceState* __restrict__ states, void* __restrict__ out, int segFirst,
+                                         int segEnd, const Tile& tile, int lane)
 {
     const int frame = tile.frame + lane;
     if (lane >= tile.frames || frame < segFirst || frame >= segEnd)
@@ -529,6 +588,9 @@ __device__ __forceinline__ void mixFrame(const int16_t* __restrict__ bank, const
         if (frame < v.begin || frame >= v.end)
             continue;
         if (v.filterTrack >= 0) {
+            const VoiceState& st = states[v.filterTrack];
+            if (st.retired && frame >= st.silence)
+                continue;  // retired under the voice-silence policy
             const float* src = filtered + v.filtered + uint64_t(frame - max(v.begin, segFirst)) * uint64_t(v.channels);
             float2 s = make_float2(src[0], v.channels == 1 ? src[0] : src[1]);
             if (v.channels == 1) {
@@ -574,8 +636,8 @@ __global__ void __launch_bounds__(kTileFrames) clayRoundMixTiles(const int16_t* 
         const auto& u = constant(units)[unit];
         const Tile tile = global(u.tiles)[u.tileFirst + item - u.tileBase];
         mixFrame<T, FloatOut>(bank, sinc, global(u.spans), global(u.jobs), global(u.voices), global(u.points),
-                              global(u.mods), global(u.filtered), global(u.mix), u.segFirst, u.segEnd, tile,
-                              int(threadIdx.x));
+                              global(u.mods), global(u.filtered), global(u.states), global(u.mix), u.segFirst,
+                              u.segEnd, tile, int(threadIdx.x));
     }
 }
 
@@ -585,11 +647,11 @@ __global__ void __launch_bounds__(kTileFrames) clayMixTiles(
     const int16_t* __restrict__ bank, const float* __restrict__ sinc, const Span* __restrict__ spans,
     const Job* __restrict__ jobs, const Tile* __restrict__ tiles, const int* __restrict__ voices,
     const clay_control_point* __restrict__ points, const Mod* __restrict__ mods, const float* __restrict__ filtered,
-    void* __restrict__ out, int segFirst, int segEnd, int firstTile)
+    const VoiceState* __restrict__ states, void* __restrict__ out, int segFirst, int segEnd, int firstTile)
 {
     const Tile tile = tiles[firstTile + int(blockIdx.x)];
-    mixFrame<T, FloatOut>(bank, sinc, spans, jobs, voices, points, mods, filtered, out, segFirst, segEnd, tile,
-                          int(threadIdx.x));
+    mixFrame<T, FloatOut>(bank, sinc, spans, jobs, voices, points, mods, filtered, states, out, segFirst, segEnd,
+                          tile, int(threadIdx.x));
 }
 
 #define CLAY_INSTANTIATE_TAPS(T)                                                                                       \
@@ -598,13 +660,13 @@ __global__ void __launch_bounds__(kTileFrames) clayMixTiles(
     template __global__ void clayRoundMixTiles<T, false>(const int16_t*, const float*, UnitsPtr, RoundPtr);            \
     template __global__ void clayVoiceChains<T>(const int16_t*, const float*, const Span*, const SpanCold*, const Job*, \
                                                 const int*, const clay_control_point*, const Mod*, const double*,      \
-                                                float*, VoiceState*, int, int, int);                                   \
+                                                float*, VoiceState*, int, int, int, float);                            \
     template __global__ void clayMixTiles<T, true>(const int16_t*, const float*, const Span*, const Job*, const Tile*, \
                                                    const int*, const clay_control_point*, const Mod*, const float*,     \
-                                                   void*, int, int, int);                                               \
+                                                   const VoiceState*, void*, int, int, int);                            \
     template __global__ void clayMixTiles<T, false>(const int16_t*, const float*, const Span*, const Job*,             \
                                                     const Tile*, const int*, const clay_control_point*, const Mod*,     \
-                                                    const float*, void*, int, int, int);
+                                                    const float*, const VoiceState*, void*, int, int, int);
 CLAY_INSTANTIATE_TAPS(16)
 CLAY_INSTANTIATE_TAPS(32)
 CLAY_INSTANTIATE_TAPS(48)

@@ -42,10 +42,11 @@ extern "C" __global__ void clayPitchChains(const Span*, const SpanCold*, const i
 template<int T>
 __global__ void clayVoiceChains(const int16_t*, const float*, const Span*, const SpanCold*, const Job*, const int*,
                                 const clay_control_point*, const Mod*, const double*, float*, VoiceState*, int, int,
-                                int);
+                                int, float);
 template<int T, bool FloatOut>
 __global__ void clayMixTiles(const int16_t*, const float*, const Span*, const Job*, const Tile*, const int*,
-                             const clay_control_point*, const Mod*, const float*, void*, int, int, int);
+                             const clay_control_point*, const Mod*, const float*, const VoiceState*, void*, int, int,
+                             int);
 // Persistent form: the work of every resident unit listed in the round descriptors.
 extern "C" __global__ void clayRoundPitchChains(const UnitView*, const RoundHeader*);
 template<int T>
@@ -198,6 +199,7 @@ struct clay_hip {
     std::string error;
     clay_hip_stats stats{};
     bool natural = true;
+    float silenceLevel = 0;  // voice-silence policy threshold (linear), 0 = off
     TailConfig tail{};
     bool timing = false;
     std::vector<Timer> timers;
@@ -844,18 +846,18 @@ void launchWindow(clay_hip* h, const Window& w)
             hipLaunchKernelGGL(clayVoiceChains<T>, dim3(ceilDiv(chainCount, kChainWavesPerBlock)),
                                dim3(64 * kChainWavesPerBlock), 0, s, h->bank, h->sinc, b.dSpans, b.dCold, b.dJobs,
                                v.chainHeads, b.dPoints, b.dMods, b.dPhase, b.dFiltered, b.dStates, w.first, w.end,
-                               chainCount);
+                               chainCount, h->silenceLevel);
         });
     chunked(v.tileCount, [&](int first, int n) {
         h->launch("clayMixTiles", [&] {
             if (h->natural)
                 hipLaunchKernelGGL((clayMixTiles<T, true>), dim3(n), dim3(kTileFrames), 0, s, h->bank, h->sinc, b.dSpans,
-                                   b.dJobs, b.dTiles, b.dVoices, b.dPoints, b.dMods, b.dFiltered, b.dMix, w.first, w.end,
-                                   v.tileFirst + first);
+                                   b.dJobs, b.dTiles, b.dVoices, b.dPoints, b.dMods, b.dFiltered, b.dStates, b.dMix,
+                                   w.first, w.end, v.tileFirst + first);
             else
                 hipLaunchKernelGGL((clayMixTiles<T, false>), dim3(n), dim3(kTileFrames), 0, s, h->bank, h->sinc,
-                                   b.dSpans, b.dJobs, b.dTiles, b.dVoices, b.dPoints, b.dMods, b.dFiltered, b.dMix,
-                                   w.first, w.end, v.tileFirst + first);
+                                   b.dSpans, b.dJobs, b.dTiles, b.dVoices, b.dPoints, b.dMods, b.dFiltered, b.dStates,
+                                   b.dMix, w.first, w.end, v.tileFirst + first);
         });
     });
 }
@@ -879,7 +881,7 @@ RoundHeader stageRound(clay_hip* h, const std::vector<Window>& windows)
     h->stageNext = (h->stageNext + 1) % clay_hip::kStages;
     if (stage.used)
         check(hipEventSynchronize(stage.copied), "round staging");
-    RoundHeader header{int(windows.size()), 0, 0, 0};
+    RoundHeader header{int(windows.size()), 0, 0, 0, h->silenceLevel, {0, 0, 0}};
     auto* views = reinterpret_cast<UnitView*>(static_cast<char*>(stage.host) + 256);
     for (size_t i = 0; i < windows.size(); ++i) {
         UnitView v = viewOf(windows[i]);
@@ -1132,6 +1134,9 @@ int clay_hip_create(const clay_hip_config* config, clay_hip** out)
         if (config->form != CLAY_FORM_PLAIN && config->form != CLAY_FORM_PERSISTENT)
             throw Failure(CLAY_E_INVALID, "unknown execution form");
         h->natural = config->tail_policy == CLAY_POLICY_NATURAL;
+        if (config->voice_silence_db > 0 || !(config->voice_silence_db > -400 || config->voice_silence_db == 0))
+            throw Failure(CLAY_E_INVALID, "voice_silence_db must be 0 (off) or a negative level in dBFS");
+        h->silenceLevel = config->voice_silence_db < 0 ? float(std::pow(10.0, config->voice_silence_db / 20.0)) : 0.f;
         initTail(h.get());
         int devices = 0;
         check(hipGetDeviceCount(&devices), "hipGetDeviceCount");
@@ -1335,6 +1340,32 @@ int clay_hip_batch_results(clay_hip* h, clay_hip_batch* b, clay_tail_result* res
                 check(hipMemcpy(pcm, b->dOutput, samples * sizeof(int16_t), hipMemcpyDeviceToHost), "pcm download");
             h->stats.downloaded_bytes += samples * sizeof(int16_t);
         }
+    });
+}
+
+int clay_hip_batch_voice_tails(clay_hip* h, clay_hip_batch* b, int32_t* silence, size_t capacity, size_t* count)
+{
+    if (!h || !b)
+        return CLAY_E_INVALID;
+    return guarded(h, [&] {
+        check(hipSetDevice(h->device), "hipSetDevice");
+        const size_t n = size_t(b->chains);
+        if (count)
+            *count = n;
+        if (!silence)
+            return;
+        if (capacity < n)
+            throw Failure(CLAY_E_INVALID, "voice tail capacity too small");
+        std::vector<VoiceState> states(n);
+        if (n)
+            check(hipMemcpy(states.data(), b->dStates, n * sizeof(VoiceState), hipMemcpyDeviceToHost), "voice tails");
+        // Chain ends from the host records (chain id = span filter track).
+        std::vector<int32_t> end(n, 0);
+        for (const auto& sp : b->spans)
+            if (sp.filterTrack >= 0)
+                end[size_t(sp.filterTrack)] = std::max(end[size_t(sp.filterTrack)], sp.end);
+        for (size_t i = 0; i < n; ++i)
+            silence[i] = states[i].retired ? states[i].silence : end[i];
     });
 }
 

@@ -206,7 +206,16 @@ typedef struct clay_hip_config {
      * per round; 0 lets the tuner choose. Output never depends on it. */
     int32_t segment_frames;
     int32_t flags;                 /* CLAY_FLAG_* */
+    /* Voice-silence tail policy (no reference counterpart; 0 = off): a voice
+     * chain retires at the end of its first run of CLAY_SILENT_CHUNKS silent
+     * 64-frame chunks after its release starts, silent meaning every output
+     * sample below this level (dBFS, e.g. -90); later frames are not rendered
+     * and not mixed. */
+    float voice_silence_db;
+    int32_t pad0;
 } clay_hip_config;
+
+#define CLAY_SILENT_CHUNKS 16      /* 1024 frames */
 
 /* clay_hip_config.flags */
 #define CLAY_FLAG_NO_GRAPH 1       /* persistent form: launch rounds directly instead of replaying a hipGraph */
@@ -293,6 +302,13 @@ int clay_hip_batch_results(clay_hip* h, clay_hip_batch* b, clay_tail_result* res
 /* Per-kernel average launch time (ms) and launch count of the last run, by name. */
 int clay_hip_batch_kernel_time(clay_hip* h, clay_hip_batch* b, const char* kernel, double* ms, int* launches);
 int clay_hip_batch_destroy(clay_hip* h, clay_hip_batch* b);
+
+/* Exact voice tails of the last run of a resident batch, one entry per voice
+ * chain in batch order (jobs in order, each job's filter tracks in order):
+ * the frame where the chain retired under the voice-silence policy, or its
+ * last span's end when it did not retire. No reference counterpart: the
+ * training labels and the judge of the learned tail predictor. */
+int clay_hip_batch_voice_tails(clay_hip* h, clay_hip_batch* b, int32_t* silence, size_t capacity, size_t* count);
 
 #ifdef __cplusplus
 }

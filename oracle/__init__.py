@@ -20,7 +20,7 @@ _lib = None
 
 class OracleConfig(C.Structure):
     _fields_ = [("tail_policy", C.c_int32), ("sinc_taps", C.c_int32), ("audio", abi.AudioConfig),
-                ("group_minimum", C.c_int32), ("threads", C.c_int32)]
+                ("group_minimum", C.c_int32), ("threads", C.c_int32), ("voice_silence_db", C.c_float)]
 
 
 def lib():
@@ -36,6 +36,8 @@ def lib():
                                          C.POINTER(abi.PlanView), C.c_size_t, C.POINTER(abi.TailResult),
                                          C.POINTER(C.c_int16), C.c_uint64, C.POINTER(C.c_uint64)]
         L.clay_oracle_envelope.restype = C.c_double
+        L.clay_oracle_voice_tails.argtypes = [C.POINTER(OracleConfig), C.POINTER(C.c_int16), C.c_uint64,
+                                              C.POINTER(abi.PlanView), C.POINTER(C.c_int32), C.c_size_t]
         L.clay_oracle_envelope.argtypes = [C.POINTER(abi.VoiceSpan), C.c_int]
         L.clay_oracle_filter_coefficients.argtypes = [C.c_int, C.c_double, C.c_double, C.c_int, C.c_double,
                                                       C.c_double, C.POINTER(C.c_double)]
@@ -47,9 +49,19 @@ def lib():
     return _lib
 
 
-def config(policy=abi.POLICY_NATURAL, taps=48, rate=44100, quality=False, group_minimum=0, threads=1, audio=None):
+def config(policy=abi.POLICY_NATURAL, taps=48, rate=44100, quality=False, group_minimum=0, threads=1, audio=None,
+           voice_silence_db=0.0):
     audio = audio if audio is not None else abi.default_audio_config(rate, quality)
-    return OracleConfig(policy, taps, audio, group_minimum, threads)
+    return OracleConfig(policy, taps, audio, group_minimum, threads, voice_silence_db)
+
+
+def voice_tails(cfg, bank_ptr, bank_count, plan):
+    """Exact per-chain retirement frames of one job (chain end when it does not retire)."""
+    out = np.zeros(max(1, plan.filter_tracks), np.int32)
+    status = lib().clay_oracle_voice_tails(C.byref(cfg), bank_ptr, bank_count, C.byref(plan),
+                                           out.ctypes.data_as(C.POINTER(C.c_int32)), out.size)
+    assert status == 0
+    return out[:plan.filter_tracks]
 
 
 def sinc_table(taps):

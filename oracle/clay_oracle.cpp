@@ -122,6 +122,7 @@ struct Renderer {
     uint64_t bankCount;
     int taps;
     std::vector<float> table;
+    float silence = 0;  // voice-silence policy level (linear), 0 = off
 
     // src/cuda/render.cu:21-28
     Stereo fetch(const clay_voice_span& v, long long frame) const
@@ -313,8 +314,51 @@ std::vector<std::vector<int>> chains(const clay_plan_view& p)
     return out;
 }
 
+constexpr int kSilentChunks = CLAY_SILENT_CHUNKS;
+
+// Voice-silence policy of one chain (no reference counterpart): chunks are the
+// absolute 64-frame blocks the chain covers; a chunk is silent when every
+// rendered output sample of the chain in it (float, non-finite = loud) is below
+// the level and every one of its frames is at or after its span's release.
+// The chain retires at the end of its first run of kSilentChunks silent chunks.
+int chainSilence(const clay_plan_view& p, const std::vector<int>& chain,
+                 const std::vector<std::vector<Stereo>>& filtered, float level)
+{
+    int end = 0;
+    for (int s : chain)
+        end = std::max(end, p.spans[s].end);
+    if (!(level > 0))
+        return end;
+    int open = -1, silent = 0;
+    float peak = 0;
+    bool eligible = true;
+    for (int s : chain) {
+        const auto& v = p.spans[s];
+        for (int f = v.begin; f < v.end; ++f) {
+            if ((f >> 6) != open) {
+                open = f >> 6;
+                peak = 0;
+                eligible = true;
+            }
+            const Stereo& x = filtered[size_t(s)][size_t(f - v.begin)];
+            float a = std::max(std::fabs(float(x.l)), v.channels == 2 ? std::fabs(float(x.r)) : 0.f);
+            if (!(a <= std::numeric_limits<float>::max()))
+                a = std::numeric_limits<float>::infinity();
+            peak = std::max(peak, a);
+            eligible = eligible && v.release_start >= 0 && f >= v.release_start;
+            if ((f & 63) == 63) {
+                silent = eligible && peak < level ? silent + 1 : 0;
+                open = -1;
+                if (silent == kSilentChunks)
+                    return f + 1;
+            }
+        }
+    }
+    return end;
+}
+
 // Mix of one job (src/cuda/render.cu:195-492 with one segment covering the job).
-void mixJob(const Renderer& R, const Job& job, std::vector<Stereo>& mix)
+void mixJob(const Renderer& R, const Job& job, std::vector<Stereo>& mix, std::vector<int>* tails = nullptr)
 {
     const auto& p = *job.plan;
     const int frames = p.frames;
@@ -445,11 +489,19 @@ void mixJob(const Renderer& R, const Job& job, std::vector<Stereo>& mix)
             }
         }
     }
+    // Voice-silence policy: retirement frame per chain.
+    std::vector<int> silence;
+    for (const auto& chain : chains(p))
+        silence.push_back(chainSilence(p, chain, filtered, R.silence));
+    if (tails)
+        *tails = silence;
     // Output mix (render.cu:458-492): dry voices times the job gain plus the wet chains.
     for (int s = 0; s < p.span_count; ++s) {
         const auto& v = p.spans[s];
         for (int f = v.begin; f < v.end; ++f) {
             if (v.filter_track >= 0) {
+                if (f >= silence[size_t(v.filter_track)])
+                    continue;
                 Stereo x = filtered[size_t(s)][size_t(f - v.begin)];
                 if (v.channels == 1) {
                     double offset = job.panBase(f);
@@ -780,12 +832,30 @@ int clay_oracle_sinc_table(int taps, float* out, size_t capacity)
     return CLAY_OK;
 }
 
+float silenceLevel(float db) { return db < 0 ? float(std::pow(10.0, double(db) / 20.0)) : 0.f; }
+
+int clay_oracle_voice_tails(const clay_oracle_config* cfg, const int16_t* bank, uint64_t bank_count,
+                            const clay_plan_view* plan, int32_t* silence, size_t capacity)
+{
+    if (!cfg || !plan || !silence || capacity < size_t(std::max(0, plan->filter_tracks)))
+        return CLAY_E_INVALID;
+    Renderer R{bank, bank_count, cfg->sinc_taps, sincTable(cfg->sinc_taps)};
+    R.silence = silenceLevel(cfg->voice_silence_db);
+    Job job(plan);
+    std::vector<Stereo> m;
+    std::vector<int> tails;
+    mixJob(R, job, m, &tails);
+    std::copy(tails.begin(), tails.end(), silence);
+    return CLAY_OK;
+}
+
 int clay_oracle_mix(const clay_oracle_config* cfg, const int16_t* bank, uint64_t bank_count, const clay_plan_view* plan,
                     double* mix, uint64_t capacity_frames)
 {
     if (!cfg || !plan || (uint64_t)std::max(0, plan->frames) > capacity_frames)
         return CLAY_E_INVALID;
     Renderer R{bank, bank_count, cfg->sinc_taps, sincTable(cfg->sinc_taps)};
+    R.silence = silenceLevel(cfg->voice_silence_db);
     Job job(plan);
     std::vector<Stereo> m;
     mixJob(R, job, m);
@@ -808,6 +878,7 @@ int clay_oracle_render(const clay_oracle_config* cfg, const int16_t* bank, uint6
         const TailConfig tail{a.window, 2 * rate, int(std::nearbyint(rate / 4.)), int(std::nearbyint(rate / 10.)),
                               std::pow(10., -1.5), 32768 * std::pow(10., -3.9), 32768 * std::pow(10., -3.)};
         Renderer R{bank, bank_count, cfg->sinc_taps, sincTable(cfg->sinc_taps)};
+        R.silence = silenceLevel(cfg->voice_silence_db);
         std::vector<Finished> jobs(count);
         std::atomic<size_t> next{0};
         const auto work = [&] {

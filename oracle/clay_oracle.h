@@ -22,6 +22,7 @@ typedef struct clay_oracle_config {
     clay_audio_config audio;
     int32_t group_minimum;
     int32_t threads;          /* jobs rendered concurrently (>= 1) */
+    float voice_silence_db;   /* voice-silence tail policy, 0 = off (clay_hip_config.voice_silence_db) */
 } clay_oracle_config;
 
 /* Sinc table, taps x (CLAY_SINC_PHASES + 1) floats, tap-major. */
@@ -34,6 +35,11 @@ int clay_oracle_mix(const clay_oracle_config* cfg, const int16_t* bank, uint64_t
 
 /* Whole batch, like one clay_hip_batch run: results + packed PCM16
  * (mono in natural, stereo otherwise). pcm may be NULL to only size it. */
+/* Exact voice tails of one job under the voice-silence policy: per filter
+ * track, the retirement frame, or the chain's last span end. */
+int clay_oracle_voice_tails(const clay_oracle_config* cfg, const int16_t* bank, uint64_t bank_count,
+                            const clay_plan_view* plan, int32_t* silence, size_t capacity);
+
 int clay_oracle_render(const clay_oracle_config* cfg, const int16_t* bank, uint64_t bank_count,
                        const clay_plan_view* plans, size_t count, clay_tail_result* results,
                        int16_t* pcm, uint64_t pcm_capacity, uint64_t* pcm_samples);
