@@ -62,7 +62,8 @@ HIP_SYMBOLS = [
     "clay_hip_create", "clay_hip_destroy", "clay_hip_upload_bank", "clay_hip_render_window",
     "clay_hip_release_pcm", "clay_hip_get_stats", "clay_hip_last_error", "clay_hip_sinc_table",
     "clay_hip_batch_prepare", "clay_hip_batch_run", "clay_hip_batch_results", "clay_hip_batch_kernel_time",
-    "clay_hip_batch_destroy", "clay_hip_batch_voice_tails",
+    "clay_hip_batch_destroy", "clay_hip_batch_voice_tails", "clay_mlp_init", "clay_hip_mlp_gradient",
+    "clay_hip_mlp_train", "clay_hip_mlp_predict",
 ]
 
 
@@ -82,6 +83,12 @@ def hip():
         _bind(lib, "clay_hip_release_pcm", C.c_int, P, P)
         _bind(lib, "clay_hip_batch_voice_tails", C.c_int, P, P, C.POINTER(C.c_int32), C.c_size_t,
               C.POINTER(C.c_size_t))
+        F = C.POINTER(C.c_float)
+        _bind(lib, "clay_mlp_init", C.c_int, C.POINTER(abi.TailModel), F, F, C.c_size_t, C.c_uint64)
+        _bind(lib, "clay_hip_mlp_gradient", C.c_int, P, C.POINTER(abi.TailModel), F, F, C.c_size_t, F, F)
+        _bind(lib, "clay_hip_mlp_train", C.c_int, P, C.POINTER(abi.TailModel), F, F, C.c_size_t,
+              C.POINTER(abi.MlpTrainConfig), F)
+        _bind(lib, "clay_hip_mlp_predict", C.c_int, P, C.POINTER(abi.TailModel), F, C.c_size_t, F)
         _bind(lib, "clay_hip_get_stats", C.c_int, P, C.POINTER(abi.HipStats))
         _bind(lib, "clay_hip_last_error", C.c_char_p, P)
         _bind(lib, "clay_hip_sinc_table", C.c_int, C.c_int, C.POINTER(C.c_float), C.c_size_t)

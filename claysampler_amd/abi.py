@@ -149,3 +149,19 @@ def default_audio_config(rate=44100, enabled=False):
     power = lambda dbfs: 4294967296.0 * 10 ** (dbfs / 10)
     return AudioConfig(rate, int(round(rate / 20)), 375 * rate, 32, 1 if enabled else 0,
                        -38.0, -45.0, power(-70), 0.75, 10.0 * rate, 0.0001)
+
+
+# Learned voice-tail predictor (include/clay_hip.h clay_tail_model).
+MLP_INPUTS, MLP_HIDDEN = 12, 64
+MLP_PARAMS = MLP_INPUTS * MLP_HIDDEN + MLP_HIDDEN + MLP_HIDDEN * MLP_HIDDEN + MLP_HIDDEN + MLP_HIDDEN + 1
+MOD_CUTOFF = 15
+
+
+class TailModel(C.Structure):
+    _fields_ = [("input_mean", C.c_float * MLP_INPUTS), ("input_scale", C.c_float * MLP_INPUTS),
+                ("output_mean", C.c_float), ("output_scale", C.c_float), ("params", C.c_float * MLP_PARAMS)]
+
+
+class MlpTrainConfig(C.Structure):
+    _fields_ = [("epochs", C.c_int32), ("batch", C.c_int32), ("lr", C.c_float), ("beta1", C.c_float),
+                ("beta2", C.c_float), ("eps", C.c_float), ("seed", C.c_uint64)]

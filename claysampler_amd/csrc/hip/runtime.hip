@@ -1770,3 +1770,8 @@ int clay_hip_get_stats(clay_hip* h, clay_hip_stats* out)
 const char* clay_hip_last_error(const clay_hip* h) { return h ? h->error.c_str() : g_createError.c_str(); }
 
 } // extern "C"
+
+// Accessors for the predictor's host code (predictor.hip).
+hipStream_t clayStreamOf(clay_hip* h) { return h->stream; }
+int clayDeviceOf(clay_hip* h) { return h->device; }
+void claySetError(clay_hip* h, const char* message) { h->error = message; }

@@ -287,6 +287,42 @@ const char* clay_hip_last_error(const clay_hip* h);
  * like src/cuda/renderer.cpp:347-376 generalised to `taps` (centre taps/2 - 1). */
 int clay_hip_sinc_table(int taps, float* out, size_t capacity);
 
+/* ---- learned voice-tail predictor ------------------------------------------
+ * No reference counterpart. A 12 -> 64 -> 64 -> 1 ReLU MLP (FP32) predicting
+ * log1p((tail - release) / 64) of a voice chain under the voice-silence policy
+ * from features known at planning time; trained and evaluated by this
+ * library's HIP kernels. Parameters: W1[12][64], b1[64], W2[64][64], b2[64],
+ * w3[64], b3. Inputs and the target are standardised with the stored
+ * mean/scale (x' = (x - mean) * scale). */
+#define CLAY_MLP_INPUTS 12
+#define CLAY_MLP_HIDDEN 64
+#define CLAY_MLP_PARAMS (CLAY_MLP_INPUTS * CLAY_MLP_HIDDEN + CLAY_MLP_HIDDEN + CLAY_MLP_HIDDEN * CLAY_MLP_HIDDEN + \
+                         CLAY_MLP_HIDDEN + CLAY_MLP_HIDDEN + 1)
+
+typedef struct clay_tail_model {
+    float input_mean[CLAY_MLP_INPUTS], input_scale[CLAY_MLP_INPUTS];
+    float output_mean, output_scale;
+    float params[CLAY_MLP_PARAMS];
+} clay_tail_model;
+
+typedef struct clay_mlp_train_config {
+    int32_t epochs, batch;         /* batch: samples per Adam step (multiple of 32) */
+    float lr, beta1, beta2, eps;
+    uint64_t seed;                 /* shuffles */
+} clay_mlp_train_config;
+
+/* Standardisation from (x, y) and He-normal initial weights from `seed` (host only). */
+int clay_mlp_init(clay_tail_model* model, const float* x, const float* y, size_t n, uint64_t seed);
+/* Mean squared error over the n samples (standardised target) and its gradient
+ * with respect to the parameters (CLAY_MLP_PARAMS floats). */
+int clay_hip_mlp_gradient(clay_hip* h, const clay_tail_model* model, const float* x, const float* y, size_t n,
+                          float* gradient, float* loss);
+/* Adam training in place; epoch_loss (epochs floats, may be NULL) = mean loss per epoch. */
+int clay_hip_mlp_train(clay_hip* h, clay_tail_model* model, const float* x, const float* y, size_t n,
+                       const clay_mlp_train_config* config, float* epoch_loss);
+/* Predictions in target units (de-standardised). */
+int clay_hip_mlp_predict(clay_hip* h, const clay_tail_model* model, const float* x, size_t n, float* y);
+
 /*
  * Resident batch: the jobs' plans and the bank already on the device, so a
  * caller can time the render itself (src/cuda/renderer.cpp:1046-1131 is the
