@@ -198,12 +198,129 @@ def forms_main(args, jobs, sfz, mids, taps, plans, bank):
     print(json.dumps(line), flush=True)
 
 
+def predictor_main(args, taps=32, db=-90.0):
+    """Configuration 4: the learned voice-tail predictor on configuration 2's
+    workload under the -90 dBFS voice-silence policy. Exact tails from the
+    plain renderer (the chain kernel's judge) for args.tail_batches batches of
+    64 jobs x 60 s, GPU training, then on a held-out batch: frame error against
+    the analytic envelope estimate, post-release scratch for 99 % coverage,
+    throughput with and without voice retirement, and plain vs persistent
+    PCM16 identity with the policy on."""
+    from claysampler_amd import tails
+
+    sfz = synth.c2_instrument(os.path.join(args.synth_dir, "c2"))
+
+    def batch(seeds):
+        s = ca.Session()
+        for seed in seeds:
+            path = os.path.join(args.synth_dir, "c2", f"c4_{seed}.mid")
+            if not os.path.exists(path):
+                synth.c2_midi(path, seed, 60.0)
+            s.add_job(sfz, path)
+        return s
+
+    def exact_tails(s, plans):
+        r = ca.Renderer(policy=abi.POLICY_NATURAL, taps=taps, voice_silence_db=db)
+        r.upload_bank(*s.bank())
+        b = r.prepare(plans)
+        b.run()
+        out = b.voice_tails().astype(np.int64)
+        b.close()
+        r.close()
+        return out
+
+    xs, ys = [], []
+    t0 = time.perf_counter()
+    for k in range(args.tail_batches):
+        s = batch(range(100000 + 64 * k, 100000 + 64 * (k + 1)))
+        plans = s.plans()
+        exact = exact_tails(s, plans)
+        x, rel, end, ok = tails.chain_records(plans, s.bank_array())
+        xs.append(x[ok])
+        ys.append(tails.to_target(exact[ok], rel[ok]))
+        log(f"tail batch {k + 1}/{args.tail_batches}: {ok.sum()} voices, {time.perf_counter() - t0:.1f} s")
+    x, y = np.concatenate(xs), np.concatenate(ys)
+    renderer = ca.Renderer(policy=abi.POLICY_NATURAL, taps=taps, voice_silence_db=db)
+    model = tails.init(x, y, seed=1)
+    t0 = time.perf_counter()
+    losses = tails.train(renderer, model, x, y, epochs=args.epochs, batch=4096, lr=2e-3, seed=2)
+    train_s = time.perf_counter() - t0
+    log(f"trained on {len(y)} tails in {train_s:.1f} s, loss {losses[0]:.4f} -> {losses[-1]:.4f}")
+    held = batch(range(900000, 900064))
+    plans = held.plans()
+    bank = held.bank()
+    exact = exact_tails(held, plans)
+    xt, rel, end, ok = tails.chain_records(plans, held.bank_array())
+    xt, rel, end, exact_ok = xt[ok], rel[ok], end[ok], exact[ok]
+    pred = tails.from_target(tails.predict(renderer, model, xt), rel, end)
+    base = tails.envelope_baseline(xt, rel, end, db)
+    truth = tails.to_target(exact_ok, rel)
+    frames_err = {name: float(np.median(np.abs(p - exact_ok))) for name, p in (("network", pred), ("envelope", base))}
+    log_err = {name: float(np.abs(tails.to_target(p, rel) - truth).mean()) for name, p in (("network", pred),
+                                                                                          ("envelope", base))}
+    planned = 4.0 * float((end - rel).sum())
+    scratch = {"planned": planned}
+    for name, p in (("network", pred), ("envelope", base)):
+        scratch[name], margin = tails.scratch_for_coverage(p, exact_ok, rel, end)
+        scratch[name + "_margin"] = margin
+    renderer.close()
+    # Throughput on the held-out batch, policy off and on (plain form, resident batch).
+    rates = {}
+    frames = sum(p.frames for p in plans)
+    for name, level in (("all_frames", 0.0), ("voice_retirement", db)):
+        r = ca.Renderer(policy=abi.POLICY_NATURAL, taps=taps, voice_silence_db=level)
+        r.upload_bank(*bank)
+        b = r.prepare(plans)
+        b.run()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            b.run()
+        rates[name] = frames / 44100.0 / ((time.perf_counter() - t0) / args.steps)
+        b.close()
+        r.close()
+    # Plain vs persistent with the policy on: identical PCM16 per job.
+    sums = []
+    for form, flags in ((abi.FORM_PLAIN, 0), (abi.FORM_PERSISTENT, 0)):
+        r = ca.Renderer(policy=abi.POLICY_NATURAL, taps=taps, form=form, batch_jobs=16, flags=flags,
+                        voice_silence_db=db)
+        r.upload_bank(*bank)
+        got = {}
+        r.render_window(plans, lambda res, pcm, first, failed: got.update(
+            {first + k: (x.status, x.frames, x.data_frames, int(pcm[x.offset:x.offset + x.data_frames].astype(
+                np.int64).sum())) for k, x in enumerate(res)}))
+        r.close()
+        sums.append(got)
+    line = {
+        "metric": "rendered audio-sec/sec and voice-samples/sec on 1 GPU; % of HBM roofline",
+        "value": rates["voice_retirement"],
+        "unit": "audio-s/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": 1,
+        "ms_per_step": 1e3 * frames / 44100.0 / rates["voice_retirement"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "config 4: config-2 workload (64 jobs x 60 s, 32 taps) with the -90 dBFS voice-silence "
+                               "policy; learned tail predictor trained on exact tails of the plain renderer",
+                   "training_tails": int(len(y)), "epochs": args.epochs, "held_out_voices": int(len(truth))},
+        "predictor": {"train_seconds": train_s, "loss_first_epoch": float(losses[0]),
+                      "loss_last_epoch": float(losses[-1]), "median_abs_frame_error": frames_err,
+                      "mean_abs_log_error": log_err, "post_release_scratch_bytes_99pct": scratch},
+        "throughput_audio_s_per_s": rates,
+        "identical_pcm16_plain_vs_persistent": sums[0] == sums[1],
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3"])
+    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4"])
     ap.add_argument("--jobs", type=int, default=None)
     ap.add_argument("--seconds", type=float, default=60.0)
     ap.add_argument("--synth-dir", default=os.environ.get("CLAY_SYNTH_DIR", "/tmp/clay_synth"))
@@ -212,11 +329,16 @@ def main():
     ap.add_argument("--forms", action="store_true",
                     help="plain vs persistent form through render_window (configuration 3)")
     ap.add_argument("--batch-jobs", type=int, default=128)
+    ap.add_argument("--tail-batches", type=int, default=32, help="configuration 4: training batches of 64 jobs")
+    ap.add_argument("--epochs", type=int, default=20, help="configuration 4: training epochs")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (jobs shard by job; see DESIGN.md)",
               file=sys.stderr)
         sys.exit(2)
+    if args.config == "c4":
+        predictor_main(args)
+        return
     default_jobs = {"c1": 1, "c2": 64, "c3": 512}[args.config]
     jobs = args.jobs or default_jobs
 

@@ -58,34 +58,34 @@ def chain_records(plans, bank=None):
         head = np.full(n, -1, np.int64)
         last = np.full(n, -1, np.int64)
         end = np.zeros(n, np.int64)
-        for i in chained:  # spans are in plan order; the last one of a chain carries its release
-            t = track[i]
-            if head[t] < 0:
-                head[t] = i
-            last[t] = i
-            end[t] = max(end[t], sp["end"][i])
+        # Spans are in plan order; the last one of a chain carries its release.
+        t = track[chained]
+        np.maximum.at(end, t, sp["end"][chained])
+        last[t] = chained                      # later spans overwrite earlier ones
+        head[t[::-1]] = chained[::-1]          # earlier spans overwrite later ones
         ok = head >= 0
         h, l = sp[np.maximum(head, 0)], sp[np.maximum(last, 0)]
         release = l["release_start"].astype(np.int64)
         ok &= (release >= 0) & (release < end)
         rate = np.maximum(l["sample_rate"], 1).astype(np.float64)
         relf = np.clip(release, 0, None)
-        # Cutoff at release: static cutoff x 2^(cents/1200) of the cutoff bindings.
+        # Cutoff at release: static cutoff x 2^(cents/1200) of the cutoff bindings;
+        # every (chain, binding) pair at once, control tracks evaluated per track.
         cents = np.zeros(n)
         if mods is not None:
-            for t in range(n):
-                if not ok[t]:
-                    continue
-                first, count = l["first_modulation"][t], l["modulation_count"][t]
-                for m in mods[first:first + count]:
-                    if m["target"] != abi.MOD_CUTOFF:
-                        continue
-                    v = float(m["depth"])
-                    if m["track"] >= 0:
-                        v *= float(_track_value(plan, points, m["track"], np.array([relf[t]]))[0])
-                    if m["shape_track"] >= 0:
-                        v *= float(_track_value(plan, points, m["shape_track"], np.array([relf[t]]))[0])
-                    cents[t] += v
+            counts = np.where(ok, l["modulation_count"], 0).astype(np.int64)
+            chain_of = np.repeat(np.arange(n), counts)
+            within = np.arange(counts.sum()) - np.repeat(np.cumsum(counts) - counts, counts)
+            mod_of = np.repeat(l["first_modulation"].astype(np.int64), counts) + within
+            sel = mods["target"][mod_of] == abi.MOD_CUTOFF
+            chain_of, mod_of = chain_of[sel], mod_of[sel]
+            value = mods["depth"][mod_of].astype(np.float64)
+            for field in ("track", "shape_track"):
+                tracks = mods[field][mod_of]
+                for tr in np.unique(tracks[tracks >= 0]):
+                    m = tracks == tr
+                    value[m] *= _track_value(plan, points, int(tr), relf[chain_of[m]])
+            np.add.at(cents, chain_of, value)
         filt = l["filters"][:, 0]
         cutoff = np.where(filt["type"] > 0, filt["cutoff"] * np.exp2(cents / 1200.0), 1000.0)
         tau = np.where(l["release_rate"] < 0, -1.0 / np.minimum(l["release_rate"], -1e-12), 1e7) / rate
@@ -93,17 +93,19 @@ def chain_records(plans, bank=None):
         job_gain = _track_value(plan, points, 0, relf) * _track_value(plan, points, 1, relf)
         peak = np.ones(n)
         if bank is not None:
-            for t in range(n):
-                if not ok[t]:
-                    continue
-                a, chans = int(l["sample"][t]), int(l["channels"][t])
-                lo, hi = (int(l["loop_first"][t]), int(l["loop_end"][t])) if l["loop_end"][t] > 0 else (
-                    0, int(l["source_end"][t]))
-                key = (a, chans, lo, hi)
+            looped = l["loop_end"] > 0
+            region = np.stack([l["sample"].astype(np.int64), l["channels"].astype(np.int64),
+                               np.where(looped, l["loop_first"], 0).astype(np.int64),
+                               np.where(looped, l["loop_end"], l["source_end"]).astype(np.int64)], 1)
+            keys, inverse = np.unique(region, axis=0, return_inverse=True)
+            values = np.zeros(len(keys))
+            for k, (a, chans, lo, hi) in enumerate(keys):
+                key = (int(a), int(chans), int(lo), int(hi))
                 if key not in peaks:
-                    seg = bank[a + lo * chans:a + hi * chans]
+                    seg = bank[key[0] + key[2] * key[1]:key[0] + key[3] * key[1]]
                     peaks[key] = float(np.abs(seg.astype(np.int32)).max()) / 32768.0 if seg.size else 0.0
-                peak[t] = peaks[key]
+                values[k] = peaks[key]
+            peak = values[inverse.reshape(-1)]
         x = np.stack([
             np.log2(np.maximum(h["step"], 1e-9)),
             np.log10(np.maximum(l["gain"], 1e-6)),
