@@ -132,6 +132,10 @@ namespace {
 
 constexpr int kMaxSections = 9;   // 2 filters x up to 3 sections + 3 EQ bands
 constexpr int kChainWaves = 4;    // voice chains per 256-thread block
+// Waves per SIMD the chain kernels are compiled for: 3 caps them at 168 VGPRs
+// (80 B of spills) and hides the scans' LDS latency better than 2 (203 VGPRs,
+// 19 % slower on configuration 2); 4 spills too much to gain.
+constexpr int kChainOccupancy = 3;
 
 // Per-wave carry between 64-frame chunks, in LDS (broadcast reads).
 struct ChainCarry {
@@ -531,7 +535,8 @@ __device__ __forceinline__ void voiceChain(const int16_t* __restrict__ bank, con
 
 // One wave per voice chain of the round, grid-stride; 4 chains per block.
 template<int T>
-__global__ void __launch_bounds__(64 * kChainWaves) clayRoundVoiceChains(
+__global__ void __launch_bounds__(64 * kChainWaves) __attribute__((amdgpu_waves_per_eu(kChainOccupancy)))
+clayRoundVoiceChains(
     const int16_t* __restrict__ bank, const float* __restrict__ sinc, UnitsPtr units, RoundPtr round)
 {
     __shared__ ChainCarry carries[kChainWaves];
@@ -549,7 +554,8 @@ __global__ void __launch_bounds__(64 * kChainWaves) clayRoundVoiceChains(
 
 // Plain form: one unit, its pointers as kernel arguments, one wave per chain.
 template<int T>
-__global__ void __launch_bounds__(64 * kChainWaves) clayVoiceChains(
+__global__ void __launch_bounds__(64 * kChainWaves) __attribute__((amdgpu_waves_per_eu(kChainOccupancy)))
+clayVoiceChains(
     const int16_t* __restrict__ bank, const float* __restrict__ sinc, const Span* __restrict__ spans,
     const SpanCold* __restrict__ cold, const Job* __restrict__ jobs, const int* __restrict__ heads,
     const clay_control_point* __restrict__ points, const Mod* __restrict__ mods, const double* __restrict__ phase,
