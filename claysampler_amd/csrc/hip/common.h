@@ -135,8 +135,8 @@ __device__ __forceinline__ float2 interpolateAt(const int16_t* __restrict__ bank
 // The position arithmetic uses explicitly rounded double operations so that
 // the phase index equals the one of a plain IEEE double evaluation.
 template<int T>
-__device__ float2 interpolate(const int16_t* __restrict__ bank, const float* __restrict__ sinc, const Span& v,
-                              int frame, double pos)
+__device__ __forceinline__ float2 interpolate(const int16_t* __restrict__ bank, const float* __restrict__ sinc,
+                                              const Span& v, int frame, double pos)
 {
     if (pos < 0) {
         const int elapsed = frame - v.begin;
@@ -186,9 +186,10 @@ __device__ __forceinline__ float2 panGains(const Span& v, float offset)
 // Envelope, modulation, resampling, pan and stereo matrix of one voice frame
 // (src/cuda/render.cu:138-187).
 template<int T>
-__device__ float2 voiceAudio(const int16_t* __restrict__ bank, const float* __restrict__ sinc, const Span& v,
-                             const clay_control_point* __restrict__ points, const Mod* __restrict__ mods, float pan,
-                             int frame, bool filterInput, double phase)
+__device__ __forceinline__ float2 voiceAudio(const int16_t* __restrict__ bank, const float* __restrict__ sinc,
+                                             const Span& v, const clay_control_point* __restrict__ points,
+                                             const Mod* __restrict__ mods, float pan, int frame, bool filterInput,
+                                             double phase)
 {
     float level = envelopeSpan(v, frame);
     if (level <= 0)
@@ -314,12 +315,11 @@ __device__ __forceinline__ double pointValue(const clay_control_point& p, int fr
 // chain, so the segment is found by advancing instead of a binary search.
 // It lands on the same point as controlAt's search (last point at or before
 // the frame, the first point when none).
+// Only the position is per lane; the track end comes from the (uniform) binding.
 struct Cursor {
-    int at, last;
+    int at;
     __device__ void reset(const clay_control_point* __restrict__ points, int first, int end, int frame)
     {
-        at = first;
-        last = end;
         int lo = first, hi = end;
         while (lo + 1 < hi) {
             const int mid = lo + (hi - lo) / 2;
@@ -330,7 +330,7 @@ struct Cursor {
         }
         at = lo;
     }
-    __device__ __forceinline__ double value(const clay_control_point* __restrict__ points, int frame)
+    __device__ __forceinline__ double value(const clay_control_point* __restrict__ points, int last, int frame)
     {
         while (at + 1 < last && points[at + 1].frame <= frame)
             ++at;
@@ -340,7 +340,7 @@ struct Cursor {
 
 // Up to kCursors modulation bindings of a span read through cursors; any
 // further ones fall back to the binary search.
-constexpr int kCursors = 6;
+constexpr int kCursors = 4;
 
 struct ModReader {
     Cursor track[kCursors], shape[kCursors];
@@ -369,7 +369,7 @@ struct ModReader {
         const Mod m = mods[first + j];
         double v = m.depth;
         if (m.first >= 0) {
-            const double x = j < kCursors ? track[j].value(points, frame) : [&] {
+            const double x = j < kCursors ? track[j].value(points, m.last, frame) : [&] {
                 Cursor c;
                 c.reset(points, m.first, m.last, frame);
                 return pointValue(points[c.at], frame);
@@ -377,7 +377,7 @@ struct ModReader {
             v = double(m.depth) * x;
         }
         if (m.shapeFirst >= 0) {
-            const double x = j < kCursors ? shape[j].value(points, frame) : [&] {
+            const double x = j < kCursors ? shape[j].value(points, m.shapeLast, frame) : [&] {
                 Cursor c;
                 c.reset(points, m.shapeFirst, m.shapeLast, frame);
                 return pointValue(points[c.at], frame);
