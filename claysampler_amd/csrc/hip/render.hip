@@ -150,9 +150,56 @@ struct ChainCarry {
     int have[CLAY_FILTER_SLOTS];
 };
 
-__device__ __forceinline__ double up(double v, int d) { return __shfl_up(v, d, 64); }
+// Cross-lane moves on DPP (GFX9 data-parallel primitives: VALU moves, no LDS
+// traffic). Lanes without a source lane (or outside `RowMask`) take `fill`.
+constexpr int kRowShr1 = 0x111, kRowShr2 = 0x112, kRowShr4 = 0x114, kRowShr8 = 0x118;
+constexpr int kRowBcast15 = 0x142, kRowBcast31 = 0x143, kWaveShr1 = 0x138;
 
-__device__ __forceinline__ double laneValue(double v, int lane) { return __shfl(v, lane, 64); }
+template<int Ctrl, int RowMask = 0xf>
+__device__ __forceinline__ double shiftD(double x, double fill)
+{
+    const int lo = __builtin_amdgcn_update_dpp(__double2loint(fill), __double2loint(x), Ctrl, RowMask, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(__double2hiint(fill), __double2hiint(x), Ctrl, RowMask, 0xf, false);
+    return __hiloint2double(hi, lo);
+}
+
+template<int Ctrl, int RowMask = 0xf>
+__device__ __forceinline__ int shiftI(int x, int fill)
+{
+    return __builtin_amdgcn_update_dpp(fill, x, Ctrl, RowMask, 0xf, false);
+}
+
+template<int C, int M>
+struct ScanStep {
+    static constexpr int ctrl = C, rowMask = M;
+};
+
+// Inclusive scan of a wave (64 lanes) with an associative operator, as six
+// steps: prefixes within each row of 16 lanes (shifts by 1, 2, 4, 8), then row
+// 0's total into row 1 and row 2's into row 3 (row_bcast:15), then rows 0-1's
+// total into rows 2 and 3 (row_bcast:31). `step` receives a ScanStep and
+// combines each lane with the value moved to it (the identity where none).
+template<class F>
+__device__ __forceinline__ void scanWave(F&& step)
+{
+    step(ScanStep<kRowShr1, 0xf>{});
+    step(ScanStep<kRowShr2, 0xf>{});
+    step(ScanStep<kRowShr4, 0xf>{});
+    step(ScanStep<kRowShr8, 0xf>{});
+    step(ScanStep<kRowBcast15, 0xa>{});
+    step(ScanStep<kRowBcast31, 0xc>{});
+}
+
+// Value of the previous lane (lane 0 gets `fill`).
+__device__ __forceinline__ double previousLane(double x, double fill) { return shiftD<kWaveShr1>(x, fill); }
+
+// Value of a uniform lane index, as a scalar.
+__device__ __forceinline__ double laneValue(double v, int lane)
+{
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
+    return __hiloint2double(hi, lo);
+}
 
 } // namespace
 
@@ -205,7 +252,7 @@ __device__ __forceinline__ bool silenceStep(SilenceTrack& t, float level, int f0
 //   2. filter targets where a lane's inputs differ from the previous frame's,
 //      copied forward to the following lanes (src/cuda/render.cu:376-406);
 //   3. coefficient smoothing c_n = c_{n-1} + a (t_n - c_{n-1}) as an inclusive
-//      scan of affine maps c -> M c + V over the 64 lanes;
+//      scan of affine maps c -> M c + V over the 64 lanes (DPP moves);
 //   4. each biquad section (direct form II transposed, render.cu:330-336) as a
 //      scan of 2x2 affine maps s_n = A_n s_{n-1} + B_n u_n, in FP64;
 //   5. the filtered frames written to the float scratch the mix kernel reads.
@@ -337,7 +384,7 @@ __device__ __forceinline__ void voiceChain(const int16_t* __restrict__ bank, con
                         if (active)
                             slotInputs(f, in0, in1, in2);
                         // 2. Targets: recompute where inputs change, copy forward otherwise.
-                        double p0 = up(in0, 1), p1 = up(in1, 1), p2 = up(in2, 1);
+                        double p0 = previousLane(in0, 0.0), p1 = previousLane(in1, 0.0), p2 = previousLane(in2, 0.0);
                         bool prevValid = lane > lo;
                         if (lane == lo) {
                             p0 = carry.input[f][0], p1 = carry.input[f][1], p2 = carry.input[f][2];
@@ -355,18 +402,20 @@ __device__ __forceinline__ void voiceChain(const int16_t* __restrict__ bank, con
                                                        : coefficientsD(cv.filters[f].type, in0, 0, rate, in1, in2);
                                 t[0] = q.b0, t[1] = q.b1, t[2] = q.b2, t[3] = q.a1, t[4] = q.a2;
                             }
+                            // Nearest flagged lane at or before each lane (the later one wins).
                             int have = flag ? 1 : 0;
-                            for (int d = 1; d < 64; d <<= 1) {
-                                const int hp = __shfl_up(have, d, 64);
+                            scanWave([&](auto st) {
+                                constexpr int C = decltype(st)::ctrl, R = decltype(st)::rowMask;
+                                const int hp = shiftI<C, R>(have, 0);
                                 double tp[5];
                                 for (int i = 0; i < 5; ++i)
-                                    tp[i] = up(t[i], d);
-                                if (!have && lane >= d && hp) {
+                                    tp[i] = shiftD<C, R>(t[i], 0.0);
+                                if (!have && hp) {
                                     for (int i = 0; i < 5; ++i)
                                         t[i] = tp[i];
                                     have = 1;
                                 }
-                            }
+                            });
                             if (!have)
                                 for (int i = 0; i < 5; ++i)
                                     t[i] = carry.target[f][i];
@@ -398,17 +447,14 @@ __device__ __forceinline__ void voiceChain(const int16_t* __restrict__ bank, con
                                 for (int i = 0; i < 5; ++i)
                                     V[i] = amount * t[i];
                             }
-                            for (int d = 1; d < 64; d <<= 1) {
-                                const double Mp = up(M, d);
-                                double Vp[5];
+                            // (M, V) o (Mp, Vp) = (M Mp, M Vp + V); identity (1, 0).
+                            scanWave([&](auto st) {
+                                constexpr int C = decltype(st)::ctrl, R = decltype(st)::rowMask;
+                                const double Mp = shiftD<C, R>(M, 1.0);
                                 for (int i = 0; i < 5; ++i)
-                                    Vp[i] = up(V[i], d);
-                                if (lane >= d) {
-                                    for (int i = 0; i < 5; ++i)
-                                        V[i] = fma(M, Vp[i], V[i]);
-                                    M *= Mp;
-                                }
-                            }
+                                    V[i] = fma(M, shiftD<C, R>(V[i], 0.0), V[i]);
+                                M *= Mp;
+                            });
                             for (int i = 0; i < 5; ++i)
                                 c[i] = fma(M, carry.coef[f][i], V[i]);
                         }
@@ -447,33 +493,29 @@ __device__ __forceinline__ void voiceChain(const int16_t* __restrict__ bank, con
                             a00 = 1, a01 = 0, a10 = 0, a11 = 1;
                             wl0 = wl1 = wr0 = wr1 = 0;
                         }
-                        for (int d = 1; d < 64; d <<= 1) {
-                            const double b00 = up(a00, d), b01 = up(a01, d), b10 = up(a10, d), b11 = up(a11, d);
-                            const double pl0 = up(wl0, d), pl1 = up(wl1, d);
-                            double pr0 = 0, pr1 = 0;
+                        // (A, w) o (B, p) = (A B, A p + w); identity (I, 0).
+                        scanWave([&](auto st) {
+                            constexpr int C = decltype(st)::ctrl, R = decltype(st)::rowMask;
+                            const double b00 = shiftD<C, R>(a00, 1.0), b01 = shiftD<C, R>(a01, 0.0);
+                            const double b10 = shiftD<C, R>(a10, 0.0), b11 = shiftD<C, R>(a11, 1.0);
+                            const double pl0 = shiftD<C, R>(wl0, 0.0), pl1 = shiftD<C, R>(wl1, 0.0);
+                            wl0 = fma(a00, pl0, fma(a01, pl1, wl0));
+                            wl1 = fma(a10, pl0, fma(a11, pl1, wl1));
                             if (stereo) {
-                                pr0 = up(wr0, d);
-                                pr1 = up(wr1, d);
+                                const double pr0 = shiftD<C, R>(wr0, 0.0), pr1 = shiftD<C, R>(wr1, 0.0);
+                                wr0 = fma(a00, pr0, fma(a01, pr1, wr0));
+                                wr1 = fma(a10, pr0, fma(a11, pr1, wr1));
                             }
-                            if (lane >= d) {
-                                // (A, w) o (B, p) = (A B, A p + w)
-                                wl0 = fma(a00, pl0, fma(a01, pl1, wl0));
-                                wl1 = fma(a10, pl0, fma(a11, pl1, wl1));
-                                if (stereo) {
-                                    wr0 = fma(a00, pr0, fma(a01, pr1, wr0));
-                                    wr1 = fma(a10, pr0, fma(a11, pr1, wr1));
-                                }
-                                const double n00 = fma(a00, b00, a01 * b10), n01 = fma(a00, b01, a01 * b11);
-                                const double n10 = fma(a10, b00, a11 * b10), n11 = fma(a10, b01, a11 * b11);
-                                a00 = n00, a01 = n01, a10 = n10, a11 = n11;
-                            }
-                        }
+                            const double n00 = fma(a00, b00, a01 * b10), n01 = fma(a00, b01, a01 * b11);
+                            const double n10 = fma(a10, b00, a11 * b10), n11 = fma(a10, b01, a11 * b11);
+                            a00 = n00, a01 = n01, a10 = n10, a11 = n11;
+                        });
                         const double zl1 = carry.z[sec][0][0], zl2 = carry.z[sec][0][1];
                         const double zr1 = carry.z[sec][1][0], zr2 = carry.z[sec][1][1];
                         const double sl1 = fma(a00, zl1, fma(a01, zl2, wl0)), sl2 = fma(a10, zl1, fma(a11, zl2, wl1));
                         const double sr1 = fma(a00, zr1, fma(a01, zr2, wr0)), sr2 = fma(a10, zr1, fma(a11, zr2, wr1));
                         // y_n = b0 u_n + z1_{n-1}
-                        double prevL = up(sl1, 1), prevR = up(sr1, 1);
+                        double prevL = previousLane(sl1, zl1), prevR = previousLane(sr1, zr1);
                         if (lane == lo) {
                             prevL = zl1;
                             prevR = zr1;
