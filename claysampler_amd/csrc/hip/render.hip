@@ -132,10 +132,10 @@ namespace {
 
 constexpr int kMaxSections = 9;   // 2 filters x up to 3 sections + 3 EQ bands
 constexpr int kChainWaves = 4;    // voice chains per 256-thread block
-// Waves per SIMD the chain kernels are compiled for: 3 caps them at 168 VGPRs
-// (80 B of spills) and hides the scans' LDS latency better than 2 (203 VGPRs,
-// 19 % slower on configuration 2); 4 spills too much to gain.
-constexpr int kChainOccupancy = 3;
+// Waves per SIMD the chain kernels are compiled for: 4 caps them at 128 VGPRs
+// (88 B of spills), about 2.5 % faster on configuration 2 than 3 (154 VGPRs)
+// and well ahead of the unconstrained 2.
+constexpr int kChainOccupancy = 4;
 // The mix kernels: 8 waves per SIMD (64 VGPRs) keep the wet path's scratch
 // reads in flight (7.2 ms vs 12.9 ms at the unconstrained 5 on configuration
 // 2) at no cost to the dry path (configuration 3 unchanged).
