@@ -76,7 +76,9 @@ def kernel_bytes(plans, taps, natural):
     mix_bytes = frames * (8 if natural else 4)
     return {
         # source windows of wet spans + one float write per wet sample
-        "clayVoiceChains": src["wet"] + 4 * wet_samples,
+        "clayChainAudio": src["wet"] + 4 * wet_samples,
+        # one float read and one float write per wet sample
+        "clayVoiceChains": 8 * wet_samples,
         # dry source windows + wet reads + the mix write + span records
         "clayMixTiles": src["dry"] + 4 * wet_samples + mix_bytes + spans * span_bytes,
     }, voice_frames, frames
@@ -375,7 +377,7 @@ def main():
     os.environ["CLAY_KERNEL_TIMING"] = "0"
     bytes_per, voice_frames, frames = kernel_bytes(plans, taps, True)
     kernels = {}
-    for name in ("clayVoiceChains", "clayMixTiles", "clayScanStereo", "clayQuantizeMono", "clayFinalizePcm",
+    for name in ("clayChainAudio", "clayVoiceChains", "clayMixTiles", "clayScanStereo", "clayQuantizeMono", "clayFinalizePcm",
                  "clayPitchChains", "claySelectNative", "claySelectTrim"):
         ms, n = batch.kernel_time(name)
         if n:

@@ -73,12 +73,56 @@ __device__ __forceinline__ float2 fetchRaw(const int16_t* __restrict__ bank, con
     return make_float2(left, v.channels == 1 ? left : float(bank[i + 1]));
 }
 
+// Sinc weights of one phase, four taps at a time, from the phase-major table
+// in global memory ((kPhases + 1) rows of T floats).
+template<int T>
+struct GlobalTaps {
+    const float* __restrict__ sinc;
+    struct Row {
+        const float4* __restrict__ w;
+        __device__ __forceinline__ float4 operator[](int q) const { return w[q]; }
+    };
+    __device__ __forceinline__ Row row(int phase) const
+    {
+        return {reinterpret_cast<const float4*>(sinc + size_t(phase) * T)};
+    }
+};
+
+// The same weights from the half table in LDS. The table is mirror-symmetric
+// bit for bit (weight(tap, p) == weight(T - 1 - tap, kPhases - p): the window
+// and sinc are even functions of an exactly negated argument; the renderer
+// checks it when it builds the table), so rows 0..kPhases/2 hold every phase
+// and a row p > kPhases/2 is row kPhases - p read backwards. Rows are padded
+// by four floats so that rows of different lanes start on different banks.
+template<int T>
+struct SharedTaps {
+    static constexpr int kPitch = T + 4;
+    static constexpr int kRows = kPhases / 2 + 1;
+    const float* table;
+    struct Row {
+        const float4* w;
+        bool mirror;
+        __device__ __forceinline__ float4 operator[](int q) const
+        {
+            if (!mirror)
+                return w[q];
+            const float4 r = w[T / 4 - 1 - q];
+            return make_float4(r.w, r.z, r.y, r.x);
+        }
+    };
+    __device__ __forceinline__ Row row(int phase) const
+    {
+        const bool mirror = phase > kPhases / 2;
+        return {reinterpret_cast<const float4*>(table + (mirror ? kPhases - phase : phase) * kPitch), mirror};
+    }
+};
+
 // Windowed-sinc tap sum at the nearest of 2048 phases (src/cuda/render.cu:30-61).
 // Accumulating the raw int16 values and scaling once by 2^-15 gives the same
 // float result as weighting the pre-scaled float bank of the reference.
-template<int T>
-__device__ __forceinline__ float2 interpolateAt(const int16_t* __restrict__ bank, const float* __restrict__ sinc,
-                                                const Span& v, double pos)
+template<int T, class Taps>
+__device__ __forceinline__ float2 interpolateAt(const int16_t* __restrict__ bank, const Taps& taps, const Span& v,
+                                                double pos)
 {
     constexpr float scale = 1.f / 32768.f;
     const double floorPos = floor(pos);
@@ -88,7 +132,7 @@ __device__ __forceinline__ float2 interpolateAt(const int16_t* __restrict__ bank
         return make_float2(s.x * scale, s.y * scale);
     }
     const int phase = __double2int_rn((pos - floorPos) * kPhases);
-    const float4* __restrict__ w4 = reinterpret_cast<const float4*>(sinc + size_t(phase) * T);
+    const auto w4 = taps.row(phase);
     constexpr int first = T / 2 - 1;
     float a0 = 0.f, a1 = 0.f, b0 = 0.f, b1 = 0.f;
     if (centre >= first && centre < v.sourceEnd - T / 2) {
@@ -134,9 +178,9 @@ __device__ __forceinline__ float2 interpolateAt(const int16_t* __restrict__ bank
 // Source position, generators, loop wrap and loop crossfade (src/cuda/render.cu:63-100).
 // The position arithmetic uses explicitly rounded double operations so that
 // the phase index equals the one of a plain IEEE double evaluation.
-template<int T>
-__device__ __forceinline__ float2 interpolate(const int16_t* __restrict__ bank, const float* __restrict__ sinc,
-                                              const Span& v, int frame, double pos)
+template<int T, class Taps>
+__device__ __forceinline__ float2 interpolate(const int16_t* __restrict__ bank, const Taps& taps, const Span& v,
+                                              int frame, double pos)
 {
     if (pos < 0) {
         const int elapsed = frame - v.begin;
@@ -162,10 +206,10 @@ __device__ __forceinline__ float2 interpolate(const int16_t* __restrict__ bank, 
         pos = __dadd_rn(double(v.loopFirst), fmod(__dsub_rn(pos, double(v.loopFirst)), double(period)));
     if (pos >= v.sourceEnd)
         return make_float2(0.f, 0.f);
-    float2 s = interpolateAt<T>(bank, sinc, v, pos);
+    float2 s = interpolateAt<T>(bank, taps, v, pos);
     const int fadeStart = v.loopEnd - v.loopCrossfade;
     if (v.loopEnd > 0 && v.loopCrossfade > 0 && pos >= fadeStart) {
-        const float2 entry = interpolateAt<T>(bank, sinc, v, __dsub_rn(pos, double(period)));
+        const float2 entry = interpolateAt<T>(bank, taps, v, __dsub_rn(pos, double(period)));
         const float phase = float((pos - fadeStart) / v.loopCrossfade);
         const float blend = .5f - .5f * cosf(3.14159265358979323846f * phase);
         s.x = fmaf(blend, entry.x - s.x, s.x);
@@ -185,9 +229,9 @@ __device__ __forceinline__ float2 panGains(const Span& v, float offset)
 
 // Envelope, modulation, resampling, pan and stereo matrix of one voice frame
 // (src/cuda/render.cu:138-187).
-template<int T>
-__device__ __forceinline__ float2 voiceAudio(const int16_t* __restrict__ bank, const float* __restrict__ sinc,
-                                             const Span& v, const clay_control_point* __restrict__ points,
+template<int T, class Taps>
+__device__ __forceinline__ float2 voiceAudio(const int16_t* __restrict__ bank, const Taps& taps, const Span& v,
+                                             const clay_control_point* __restrict__ points,
                                              const Mod* __restrict__ mods, float pan, int frame, bool filterInput,
                                              double phase)
 {
@@ -211,7 +255,7 @@ __device__ __forceinline__ float2 voiceAudio(const int16_t* __restrict__ bank, c
     }
     if (db != 0)
         level *= exp2f(db * .166096404744368f);
-    const float2 s = interpolate<T>(bank, sinc, v, frame, phase);
+    const float2 s = interpolate<T>(bank, taps, v, frame, phase);
     if (filterInput && v.channels == 1)
         return make_float2(s.x * level, s.x * level);
     const float2 g = panGains(v, pan);

@@ -136,6 +136,10 @@ constexpr int kChainWaves = 4;    // voice chains per 256-thread block
 // (88 B of spills), about 2.5 % faster on configuration 2 than 3 (154 VGPRs)
 // and well ahead of the unconstrained 2.
 constexpr int kChainOccupancy = 4;
+// Audio pass: 16 chains (waves) per 1024-thread block; the sinc table sits in
+// LDS for up to 32 taps.
+constexpr int kAudioWaves = 16;
+constexpr int kSharedTapsMax = 32;
 // The mix kernels: 8 waves per SIMD (64 VGPRs) keep the wet path's scratch
 // reads in flight (7.2 ms vs 12.9 ms at the unconstrained 5 on configuration
 // 2) at no cost to the dry path (configuration 3 unchanged).
@@ -147,6 +151,7 @@ struct ChainCarry {
     double coef[CLAY_FILTER_SLOTS][5];   // current (smoothed) coefficients
     double target[CLAY_FILTER_SLOTS][5]; // last target
     double input[CLAY_FILTER_SLOTS][3];  // inputs of the last target
+    double cents[2];                     // cutoff offsets (cents) behind input[0..1][0]
     int have[CLAY_FILTER_SLOTS];
 };
 
@@ -246,9 +251,43 @@ __device__ __forceinline__ bool silenceStep(SilenceTrack& t, float level, int f0
     return true;
 }
 
+// Voice audio of a chain's spans over the segment (before pan for mono, times
+// the job gain), one wave per chain and one lane per frame, written to the float
+// scratch that the filter pass then reads and overwrites. Frames carry no state
+// here, so this pass runs at full occupancy apart from the serial filters.
+template<int T, class Taps>
+__device__ __forceinline__ void chainAudio(const int16_t* __restrict__ bank, const Taps& taps,
+                                           const Span* __restrict__ spans, const Job* __restrict__ jobs,
+                                           const clay_control_point* __restrict__ points, const Mod* __restrict__ mods,
+                                           const double* __restrict__ phase, float* __restrict__ filtered,
+                                           const VoiceState* __restrict__ states, int segFirst, int segEnd, int head,
+                                           int lane)
+{
+    if (states[spans[head].filterTrack].retired)
+        return;  // retired in an earlier segment: nothing more to render
+    for (int s = head; s >= 0; s = spans[s].next) {
+        const Span& v = spans[s];
+        if (v.end <= segFirst || v.begin >= segEnd)
+            continue;
+        const Job& j = jobs[v.job];
+        const int first = max(v.begin, segFirst), stop = min(v.end, segEnd);
+        for (int frame = first + lane; frame < stop; frame += 64) {
+            const int local = frame - first;
+            const double ph = v.variablePitch ? phase[v.phase + uint64_t(local)] : -1.0;
+            const float pan = controlAt(points, j.controls[2], j.controls[3], frame);
+            const float2 a = voiceAudio<T>(bank, taps, v, points, mods, pan, frame, true, ph);
+            const float g = jobGain(points, j, frame);
+            float* dst = filtered + v.filtered + uint64_t(local) * uint64_t(v.channels);
+            dst[0] = a.x * g;
+            if (v.channels == 2)
+                dst[1] = a.y * g;
+        }
+    }
+}
+
 // One wave per voice chain (a voice's spans with filters and/or variable pitch),
 // 64 output frames per step, chunks aligned to absolute frame multiples of 64:
-//   1. the 64 frames of voice audio before pan (mono) times the job gain, one per lane;
+//   1. the 64 frames of voice audio (chainAudio) read back, one per lane;
 //   2. filter targets where a lane's inputs differ from the previous frame's,
 //      copied forward to the following lanes (src/cuda/render.cu:376-406);
 //   3. coefficient smoothing c_n = c_{n-1} + a (t_n - c_{n-1}) as an inclusive
@@ -257,11 +296,8 @@ __device__ __forceinline__ bool silenceStep(SilenceTrack& t, float level, int f0
 //      scan of 2x2 affine maps s_n = A_n s_{n-1} + B_n u_n, in FP64;
 //   5. the filtered frames written to the float scratch the mix kernel reads.
 // State carries across chunks in LDS and across segments in VoiceState.
-template<int T>
-__device__ __forceinline__ void voiceChain(const int16_t* __restrict__ bank, const float* __restrict__ sinc,
-                                           const Span* __restrict__ spans, const SpanCold* __restrict__ cold,
-                                           const Job* __restrict__ jobs, const clay_control_point* __restrict__ points,
-                                           const Mod* __restrict__ mods, const double* __restrict__ phase,
+__device__ __forceinline__ void voiceChain(const Span* __restrict__ spans, const SpanCold* __restrict__ cold,
+                                           const clay_control_point* __restrict__ points, const Mod* __restrict__ mods,
                                            float* __restrict__ filtered, VoiceState* __restrict__ states,
                                            int segFirst, int segEnd, int head, float silenceLevel, ChainCarry& carry,
                                            int lane)
@@ -276,6 +312,8 @@ __device__ __forceinline__ void voiceChain(const int16_t* __restrict__ bank, con
         total += sectionCount[f];
     }
     const bool hasFilter = total > 0;
+    if (!hasFilter && silenceLevel <= 0.f)
+        return;  // pitch only: the audio pass wrote the final frames
     bool initialized = st.filterInitialized != 0;
     __builtin_amdgcn_wave_barrier();
     if (hasFilter && lane == 0) {
@@ -301,7 +339,6 @@ __device__ __forceinline__ void voiceChain(const int16_t* __restrict__ bank, con
         if (v.end <= segFirst || v.begin >= segEnd)
             continue;
         const SpanCold& cv = cold[s];
-        const Job& j = jobs[v.job];
         const int first = max(v.begin, segFirst), stop = min(v.end, segEnd);
         const bool stereo = v.channels == 2;
         const int rate = cv.sampleRate;
@@ -314,21 +351,19 @@ __device__ __forceinline__ void voiceChain(const int16_t* __restrict__ bank, con
         }
         ModReader reader;
         reader.reset(mods, points, v.firstMod, v.modCount, max(first, (first & ~63) + lane));
+        // Bit f: carry.cents[f] holds the cents behind carry.input[f][0], computed in this span.
+        int centsValid = 0;
         for (int f0 = first & ~63; f0 < stop; f0 += 64) {
             const int frame = f0 + lane;
             const bool active = frame >= first && frame < stop;
             const int lo = max(first, f0) - f0, hi = min(stop, f0 + 64) - f0;  // active lanes [lo, hi)
+            float* dst = filtered + v.filtered + uint64_t(max(frame - first, 0)) * uint64_t(v.channels);
             float xl = 0.f, xr = 0.f;
             if (active) {
-                const int local = frame - first;
-                const double ph = v.variablePitch ? phase[v.phase + uint64_t(local)] : -1.0;
-                const float pan = controlAt(points, j.controls[2], j.controls[3], frame);
-                const float2 a = voiceAudio<T>(bank, sinc, v, points, mods, pan, frame, true, ph);
-                const float g = jobGain(points, j, frame);
-                xl = a.x * g;
-                xr = a.y * g;
+                xl = dst[0];
+                if (stereo)
+                    xr = dst[1];
             }
-            float* dst = filtered + v.filtered + uint64_t(max(frame - first, 0)) * uint64_t(v.channels);
             float outL = xl, outR = xr;
             if (hasFilter) {
                 // Filter inputs of this lane's frame for slot f: cutoff (Hz) and
@@ -362,7 +397,7 @@ __device__ __forceinline__ void voiceChain(const int16_t* __restrict__ bank, con
                     if (f < 2) {
                         if (f == 0 && cv.active[0])
                             cents += depth * envelopeShapeD(cv.extra[0], frame);
-                        in0 = double(flt.cutoff) * exp2(cents / 1200);
+                        in0 = cents;  // the caller turns it into the cutoff
                         in1 = double(flt.resonance) + db;
                         in2 = 0;
                     } else {
@@ -383,6 +418,17 @@ __device__ __forceinline__ void voiceChain(const int16_t* __restrict__ bank, con
                         double in0 = 0, in1 = 0, in2 = 0;
                         if (active)
                             slotInputs(f, in0, in1, in2);
+                        double cents = 0;
+                        if (f < 2) {
+                            // Cutoff = static cutoff x 2^(cents / 1200). When every active lane has
+                            // the cents of the span's last evaluated frame, the carried cutoff is
+                            // the value the formula gives: skip the FP64 exp2.
+                            cents = in0;
+                            if ((centsValid >> f & 1) && __ballot(active && cents != carry.cents[f]) == 0)
+                                in0 = carry.input[f][0];
+                            else if (active)
+                                in0 = double(cv.filters[f].cutoff) * exp2(cents / 1200);
+                        }
                         // 2. Targets: recompute where inputs change, copy forward otherwise.
                         double p0 = previousLane(in0, 0.0), p1 = previousLane(in1, 0.0), p2 = previousLane(in2, 0.0);
                         bool prevValid = lane > lo;
@@ -468,8 +514,11 @@ __device__ __forceinline__ void voiceChain(const int16_t* __restrict__ bank, con
                         ni[0] = laneValue(in0, last);
                         ni[1] = laneValue(in1, last);
                         ni[2] = laneValue(in2, last);
+                        const double nCents = laneValue(cents, last);
                         __builtin_amdgcn_wave_barrier();
                         if (lane == 0) {
+                            if (f < 2)
+                                carry.cents[f] = nCents;
                             for (int i = 0; i < 5; ++i) {
                                 carry.coef[f][i] = nc[i];
                                 carry.target[f][i] = nt[i];
@@ -478,6 +527,8 @@ __device__ __forceinline__ void voiceChain(const int16_t* __restrict__ bank, con
                                 carry.input[f][i] = ni[i];
                             carry.have[f] = 1;
                         }
+                        if (f < 2)
+                            centsValid |= 1 << f;
                         __builtin_amdgcn_wave_barrier();
                     } else {
                         for (int i = 0; i < 5; ++i)
@@ -537,7 +588,7 @@ __device__ __forceinline__ void voiceChain(const int16_t* __restrict__ bank, con
                 outL = float(ul);
                 outR = float(ur);
             }
-            if (active) {
+            if (active && hasFilter) {
                 dst[0] = outL;
                 if (stereo)
                     dst[1] = outR;
@@ -579,11 +630,69 @@ __device__ __forceinline__ void voiceChain(const int16_t* __restrict__ bank, con
     }
 }
 
-// One wave per voice chain of the round, grid-stride; 4 chains per block.
+// Runs body(taps) with the sinc weights the audio pass reads: for T <= 32 the
+// half table staged in this block's LDS (148 KB at 32 taps, so one block per
+// CU), which turns the per-lane gathers of random 128-byte rows from L2 into
+// LDS reads; for wider tables the global table.
+template<int T, class Body>
+__device__ __forceinline__ void withAudioTaps(const float* __restrict__ sinc, Body&& body)
+{
+    if constexpr (T <= kSharedTapsMax) {
+        using S = SharedTaps<T>;
+        __shared__ float4 table[S::kRows * S::kPitch / 4];
+        for (int i = int(threadIdx.x); i < S::kRows * (T / 4); i += int(blockDim.x)) {
+            const int row = i / (T / 4), q = i % (T / 4);
+            table[row * (S::kPitch / 4) + q] = reinterpret_cast<const float4*>(sinc + size_t(row) * T)[q];
+        }
+        __syncthreads();
+        body(S{reinterpret_cast<const float*>(table)});
+    } else {
+        body(GlobalTaps<T>{sinc});
+    }
+}
+
+// Audio pass: one wave per voice chain of the round, grid-stride; 16 chains per block.
 template<int T>
+__global__ void __launch_bounds__(64 * kAudioWaves) clayRoundChainAudio(const int16_t* __restrict__ bank,
+                                                                         const float* __restrict__ sinc,
+                                                                         UnitsPtr units, RoundPtr round)
+{
+    const int wave = uniform(int(threadIdx.x >> 6)), lane = threadIdx.x & 63;
+    const RoundHeader header = *round;
+    withAudioTaps<T>(sinc, [&](const auto& taps) {
+        // Uniform per wave: no block barriers below.
+        for (int item = int(blockIdx.x) * kAudioWaves + wave; item < header.chains;
+             item += int(gridDim.x) * kAudioWaves) {
+            const int unit = unitOf(header, item, [&](int i) { return units[i].chainBase; });
+            const auto& u = constant(units)[unit];
+            chainAudio<T>(bank, taps, global(u.spans), global(u.jobs), global(u.points), global(u.mods),
+                          global(u.phase), global(u.filtered), global(u.states), u.segFirst, u.segEnd,
+                          uniform(global(u.chainHeads)[item - u.chainBase]), lane);
+        }
+    });
+}
+
+// Audio pass, plain form: one unit, its pointers as kernel arguments, one wave
+// per chain, grid-stride (the grid is at most one block per CU when the taps
+// sit in LDS).
+template<int T>
+__global__ void __launch_bounds__(64 * kAudioWaves) clayChainAudio(
+    const int16_t* __restrict__ bank, const float* __restrict__ sinc, const Span* __restrict__ spans,
+    const Job* __restrict__ jobs, const int* __restrict__ heads, const clay_control_point* __restrict__ points,
+    const Mod* __restrict__ mods, const double* __restrict__ phase, float* __restrict__ filtered,
+    const VoiceState* __restrict__ states, int segFirst, int segEnd, int count)
+{
+    const int wave = uniform(int(threadIdx.x >> 6)), lane = threadIdx.x & 63;
+    withAudioTaps<T>(sinc, [&](const auto& taps) {
+        for (int item = int(blockIdx.x) * kAudioWaves + wave; item < count; item += int(gridDim.x) * kAudioWaves)
+            chainAudio<T>(bank, taps, spans, jobs, points, mods, phase, filtered, states, segFirst, segEnd,
+                          uniform(heads[item]), lane);
+    });
+}
+
+// Filter pass: one wave per voice chain of the round, grid-stride; 4 chains per block.
 __global__ void __launch_bounds__(64 * kChainWaves) __attribute__((amdgpu_waves_per_eu(kChainOccupancy)))
-clayRoundVoiceChains(
-    const int16_t* __restrict__ bank, const float* __restrict__ sinc, UnitsPtr units, RoundPtr round)
+clayRoundVoiceChains(UnitsPtr units, RoundPtr round)
 {
     __shared__ ChainCarry carries[kChainWaves];
     const int wave = uniform(int(threadIdx.x >> 6)), lane = threadIdx.x & 63;
@@ -592,29 +701,26 @@ clayRoundVoiceChains(
     for (int item = int(blockIdx.x) * kChainWaves + wave; item < header.chains; item += int(gridDim.x) * kChainWaves) {
         const int unit = unitOf(header, item, [&](int i) { return units[i].chainBase; });
         const auto& u = constant(units)[unit];
-        voiceChain<T>(bank, sinc, global(u.spans), global(u.cold), global(u.jobs), global(u.points), global(u.mods),
-                      global(u.phase), global(u.filtered), global(u.states), u.segFirst, u.segEnd,
-                      uniform(global(u.chainHeads)[item - u.chainBase]), header.silence, carries[wave], lane);
+        voiceChain(global(u.spans), global(u.cold), global(u.points), global(u.mods), global(u.filtered),
+                   global(u.states), u.segFirst, u.segEnd, uniform(global(u.chainHeads)[item - u.chainBase]),
+                   header.silence, carries[wave], lane);
     }
 }
 
-// Plain form: one unit, its pointers as kernel arguments, one wave per chain.
-template<int T>
+// Filter pass, plain form: one unit, its pointers as kernel arguments, one wave per chain.
 __global__ void __launch_bounds__(64 * kChainWaves) __attribute__((amdgpu_waves_per_eu(kChainOccupancy)))
-clayVoiceChains(
-    const int16_t* __restrict__ bank, const float* __restrict__ sinc, const Span* __restrict__ spans,
-    const SpanCold* __restrict__ cold, const Job* __restrict__ jobs, const int* __restrict__ heads,
-    const clay_control_point* __restrict__ points, const Mod* __restrict__ mods, const double* __restrict__ phase,
-    float* __restrict__ filtered, VoiceState* __restrict__ states, int segFirst, int segEnd, int count,
-    float silenceLevel)
+clayVoiceChains(const Span* __restrict__ spans, const SpanCold* __restrict__ cold, const int* __restrict__ heads,
+                const clay_control_point* __restrict__ points, const Mod* __restrict__ mods,
+                float* __restrict__ filtered, VoiceState* __restrict__ states, int segFirst, int segEnd, int count,
+                float silenceLevel)
 {
     __shared__ ChainCarry carries[kChainWaves];
     const int wave = uniform(int(threadIdx.x >> 6)), lane = threadIdx.x & 63;
     const int item = int(blockIdx.x) * kChainWaves + wave;
     if (item >= count)
         return;  // uniform per wave; no block barriers below
-    voiceChain<T>(bank, sinc, spans, cold, jobs, points, mods, phase, filtered, states, segFirst, segEnd,
-                  uniform(heads[item]), silenceLevel, carries[wave], lane);
+    voiceChain(spans, cold, points, mods, filtered, states, segFirst, segEnd, uniform(heads[item]), silenceLevel,
+               carries[wave], lane);
 }
 
 // Per output frame: the dry voices of the tile rendered inline, times the job
@@ -659,7 +765,7 @@ __device__ __forceinline__ void mixFrame(const int16_t* __restrict__ bank, const
             wetLeft += s.x;
             wetRight += s.y;
         } else {
-            const float2 s = voiceAudio<T>(bank, sinc, v, points, mods, pan, frame, false, -1.0);
+            const float2 s = voiceAudio<T>(bank, GlobalTaps<T>{sinc}, v, points, mods, pan, frame, false, -1.0);
             left += s.x;
             right += s.y;
         }
@@ -706,12 +812,12 @@ __global__ void __launch_bounds__(kTileFrames) __attribute__((amdgpu_waves_per_e
 }
 
 #define CLAY_INSTANTIATE_TAPS(T)                                                                                       \
-    template __global__ void clayRoundVoiceChains<T>(const int16_t*, const float*, UnitsPtr, RoundPtr);                \
+    template __global__ void clayRoundChainAudio<T>(const int16_t*, const float*, UnitsPtr, RoundPtr);                 \
     template __global__ void clayRoundMixTiles<T, true>(const int16_t*, const float*, UnitsPtr, RoundPtr);             \
     template __global__ void clayRoundMixTiles<T, false>(const int16_t*, const float*, UnitsPtr, RoundPtr);            \
-    template __global__ void clayVoiceChains<T>(const int16_t*, const float*, const Span*, const SpanCold*, const Job*, \
-                                                const int*, const clay_control_point*, const Mod*, const double*,      \
-                                                float*, VoiceState*, int, int, int, float);                            \
+    template __global__ void clayChainAudio<T>(const int16_t*, const float*, const Span*, const Job*, const int*,      \
+                                               const clay_control_point*, const Mod*, const double*, float*,           \
+                                               const VoiceState*, int, int, int);                                      \
     template __global__ void clayMixTiles<T, true>(const int16_t*, const float*, const Span*, const Job*, const Tile*, \
                                                    const int*, const clay_control_point*, const Mod*, const float*,     \
                                                    const VoiceState*, void*, int, int, int);                            \

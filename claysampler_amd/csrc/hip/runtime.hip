@@ -40,9 +40,11 @@ using namespace clay_dev;
 extern "C" __global__ void clayPitchChains(const Span*, const SpanCold*, const int*, const clay_control_point*,
                                            const Mod*, double*, VoiceState*, int, int, int);
 template<int T>
-__global__ void clayVoiceChains(const int16_t*, const float*, const Span*, const SpanCold*, const Job*, const int*,
-                                const clay_control_point*, const Mod*, const double*, float*, VoiceState*, int, int,
-                                int, float);
+__global__ void clayChainAudio(const int16_t*, const float*, const Span*, const Job*, const int*,
+                               const clay_control_point*, const Mod*, const double*, float*, const VoiceState*, int,
+                               int, int);
+__global__ void clayVoiceChains(const Span*, const SpanCold*, const int*, const clay_control_point*, const Mod*, float*,
+                                VoiceState*, int, int, int, float);
 template<int T, bool FloatOut>
 __global__ void clayMixTiles(const int16_t*, const float*, const Span*, const Job*, const Tile*, const int*,
                              const clay_control_point*, const Mod*, const float*, const VoiceState*, void*, int, int,
@@ -50,7 +52,8 @@ __global__ void clayMixTiles(const int16_t*, const float*, const Span*, const Jo
 // Persistent form: the work of every resident unit listed in the round descriptors.
 extern "C" __global__ void clayRoundPitchChains(const UnitView*, const RoundHeader*);
 template<int T>
-__global__ void clayRoundVoiceChains(const int16_t*, const float*, const UnitView*, const RoundHeader*);
+__global__ void clayRoundChainAudio(const int16_t*, const float*, const UnitView*, const RoundHeader*);
+__global__ void clayRoundVoiceChains(const UnitView*, const RoundHeader*);
 template<int T, bool FloatOut>
 __global__ void clayRoundMixTiles(const int16_t*, const float*, const UnitView*, const RoundHeader*);
 // Finishing kernels (finish.hip).
@@ -790,10 +793,12 @@ UnitView viewOf(const Window& w)
 
 // Launch shape of a round: grids of the three render kernels (0 = sized to the work).
 struct Shape {
-    int mixBlocks = 0, chainBlocks = 0, pitchBlocks = 0;
+    int mixBlocks = 0, chainBlocks = 0, pitchBlocks = 0, audioBlocks = 0;
 };
 
-constexpr int kChainWavesPerBlock = 4;  // render.hip: voice chains per 256-thread block
+constexpr int kChainWavesPerBlock = 4;   // render.hip: voice chains per 256-thread filter block
+constexpr int kAudioWavesPerBlock = 16;  // render.hip: voice chains per 1024-thread audio block
+constexpr int kSharedTapsMax = 32;       // render.hip: widest sinc table the audio pass keeps in LDS
 
 // Persistent form: the round kernels with the fixed grids of `shape`.
 template<int T>
@@ -814,9 +819,13 @@ void launchRound(clay_hip* h, const Shape& shape, bool timed)
     run("clayRoundPitchChains", [&] {
         hipLaunchKernelGGL(clayRoundPitchChains, dim3(shape.pitchBlocks), dim3(64), 0, s, units, round);
     });
-    run("clayRoundVoiceChains", [&] {
-        hipLaunchKernelGGL(clayRoundVoiceChains<T>, dim3(shape.chainBlocks), dim3(64 * kChainWavesPerBlock), 0, s,
+    run("clayRoundChainAudio", [&] {
+        hipLaunchKernelGGL(clayRoundChainAudio<T>, dim3(shape.audioBlocks), dim3(64 * kAudioWavesPerBlock), 0, s,
                            h->bank, h->sinc, units, round);
+    });
+    run("clayRoundVoiceChains", [&] {
+        hipLaunchKernelGGL(clayRoundVoiceChains, dim3(shape.chainBlocks), dim3(64 * kChainWavesPerBlock), 0, s, units,
+                           round);
     });
     run("clayRoundMixTiles", [&] {
         if (h->natural)
@@ -841,13 +850,22 @@ void launchWindow(clay_hip* h, const Window& w)
             hipLaunchKernelGGL(clayPitchChains, dim3(ceilDiv(pitchCount, 64)), dim3(64), 0, s, b.dSpans, b.dCold,
                                b.dPitchHeads, b.dPoints, b.dMods, b.dPhase, b.dStates, w.first, w.end, pitchCount);
         });
-    if (chainCount)
-        h->launch("clayVoiceChains", [&] {
-            hipLaunchKernelGGL(clayVoiceChains<T>, dim3(ceilDiv(chainCount, kChainWavesPerBlock)),
-                               dim3(64 * kChainWavesPerBlock), 0, s, h->bank, h->sinc, b.dSpans, b.dCold, b.dJobs,
-                               v.chainHeads, b.dPoints, b.dMods, b.dPhase, b.dFiltered, b.dStates, w.first, w.end,
-                               chainCount, h->silenceLevel);
+    if (chainCount) {
+        // With the taps in LDS one audio block fits a CU: at most one block per CU, grid-stride.
+        int audioBlocks = ceilDiv(chainCount, kAudioWavesPerBlock);
+        if (T <= kSharedTapsMax)
+            audioBlocks = std::min(audioBlocks, std::max(1, h->props.multiProcessorCount));
+        h->launch("clayChainAudio", [&] {
+            hipLaunchKernelGGL(clayChainAudio<T>, dim3(audioBlocks), dim3(64 * kAudioWavesPerBlock), 0, s, h->bank,
+                               h->sinc, b.dSpans, b.dJobs, v.chainHeads, b.dPoints, b.dMods, b.dPhase, b.dFiltered,
+                               b.dStates, w.first, w.end, chainCount);
         });
+        h->launch("clayVoiceChains", [&] {
+            hipLaunchKernelGGL(clayVoiceChains, dim3(ceilDiv(chainCount, kChainWavesPerBlock)),
+                               dim3(64 * kChainWavesPerBlock), 0, s, b.dSpans, b.dCold, v.chainHeads, b.dPoints, b.dMods,
+                               b.dFiltered, b.dStates, w.first, w.end, chainCount, h->silenceLevel);
+        });
+    }
     chunked(v.tileCount, [&](int first, int n) {
         h->launch("clayMixTiles", [&] {
             if (h->natural)
@@ -1149,6 +1167,12 @@ int clay_hip_create(const clay_hip_config* config, clay_hip** out)
         check(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking), "hipStreamCreate");
         // Phase-major table: each lane reads its taps contiguously.
         const auto tapMajor = sincTable(taps);
+        // The audio pass reads phases above kPhases / 2 as mirrored rows of the lower half.
+        for (int t = 0; t < taps; ++t)
+            for (int p = 0; p <= kPhases; ++p)
+                if (tapMajor[size_t(t) * (kPhases + 1) + size_t(p)] !=
+                    tapMajor[size_t(taps - 1 - t) * (kPhases + 1) + size_t(kPhases - p)])
+                    throw Failure(CLAY_E_DEVICE, "sinc table is not mirror-symmetric");
         std::vector<float> phaseMajor(tapMajor.size());
         for (int t = 0; t < taps; ++t)
             for (int p = 0; p <= kPhases; ++p)
@@ -1636,6 +1660,7 @@ int renderWindowPersistent(clay_hip* h, const clay_plan_view* plans, size_t coun
                     fixed.mixBlocks = shape.second;
                     fixed.chainBlocks = cus * 2;
                     fixed.pitchBlocks = cus;
+                    fixed.audioBlocks = cus;
                     const bool measure = tuner.tuning && h->stats.rounds > 0;
                     if (measure)
                         check(hipEventRecord(roundBegin, h->stream), "hipEventRecord");
