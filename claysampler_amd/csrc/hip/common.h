@@ -374,11 +374,18 @@ struct Cursor {
         }
         at = lo;
     }
+    // The current point and the next one are loaded together (the index of the
+    // next is clamped into the track), so a frame that stays in its segment
+    // costs one round trip.
     __device__ __forceinline__ double value(const clay_control_point* __restrict__ points, int last, int frame)
     {
-        while (at + 1 < last && points[at + 1].frame <= frame)
+        clay_control_point p = points[at], n = points[min(at + 1, last - 1)];
+        while (at + 1 < last && n.frame <= frame) {
             ++at;
-        return pointValue(points[at], frame);
+            p = n;
+            n = points[min(at + 1, last - 1)];
+        }
+        return pointValue(p, frame);
     }
 };
 

@@ -131,11 +131,15 @@ extern "C" __global__ void clayPitchChains(const Span* __restrict__ spans, const
 namespace {
 
 constexpr int kMaxSections = 9;   // 2 filters x up to 3 sections + 3 EQ bands
-constexpr int kChainWaves = 4;    // voice chains per 256-thread block
-// Waves per SIMD the chain kernels are compiled for: 4 caps them at 128 VGPRs
-// (88 B of spills), about 2.5 % faster on configuration 2 than 3 (154 VGPRs)
-// and well ahead of the unconstrained 2.
-constexpr int kChainOccupancy = 4;
+constexpr int kChainWaves = 2;    // voice chains per 128-thread filter block (LDS: 22 KB per block)
+#ifndef CLAY_LANE_FRAMES
+#define CLAY_LANE_FRAMES 4
+#endif
+constexpr int kLaneFrames = CLAY_LANE_FRAMES;  // consecutive frames per lane in the filter pass
+constexpr int kPassFrames = 64 * kLaneFrames;  // frames per filter pass (segments are multiples of it)
+// Waves per SIMD the filter kernels are compiled for: 3 (168 VGPRs), which
+// the per-wave LDS carry and stage (11 KB) also allow.
+constexpr int kChainOccupancy = 3;
 // Audio pass: 16 chains (waves) per 1024-thread block; the sinc table sits in
 // LDS for up to 32 taps.
 constexpr int kAudioWaves = 16;
@@ -153,6 +157,7 @@ struct ChainCarry {
     double input[CLAY_FILTER_SLOTS][3];  // inputs of the last target
     double cents[2];                     // cutoff offsets (cents) behind input[0..1][0]
     int have[CLAY_FILTER_SLOTS];
+    double stage[5][kLaneFrames][64];    // per-frame targets, then coefficients, after a target change
 };
 
 // Cross-lane moves on DPP (GFX9 data-parallel primitives: VALU moves, no LDS
@@ -215,42 +220,6 @@ struct SilenceTrack {
     int openEligible;
 };
 
-// Measures one chunk pass of a chain: the wave's peak |output| (non-finite
-// counts as loud) and whether every active frame is at or after the voice's
-// release. A chunk (64 absolute frames) may take several passes when spans
-// split it; it is judged when its last frame has been rendered. After
-// CLAY_SILENT_CHUNKS consecutive silent chunks the chain retires at the end
-// of the current chunk. Returns true when it retires.
-__device__ __forceinline__ bool silenceStep(SilenceTrack& t, float level, int f0, int hi, bool active, float outL,
-                                            float outR, int frame, int releaseStart)
-{
-    float a = active ? fmaxf(fabsf(outL), fabsf(outR)) : 0.f;
-    if (!(a <= 3.402823466e38f))
-        a = __builtin_inff();
-    for (int d = 32; d; d >>= 1)
-        a = fmaxf(a, __shfl_xor(a, d, 64));
-    const bool eligible = !active || (releaseStart >= 0 && frame >= releaseStart);
-    const bool all = __ballot(!eligible) == 0;
-    const int chunk = (f0 >> 6) + 1;
-    if (t.openChunk != chunk) {
-        t.openChunk = chunk;
-        t.openPeak = 0.f;
-        t.openEligible = 1;
-    }
-    t.openPeak = fmaxf(t.openPeak, a);
-    t.openEligible = t.openEligible && all;
-    if (hi < 64)
-        return false;
-    const bool silent = t.openEligible && t.openPeak < level;
-    t.openChunk = 0;
-    t.silentChunks = silent ? t.silentChunks + 1 : 0;
-    if (t.silentChunks < CLAY_SILENT_CHUNKS)
-        return false;
-    t.retired = 1;
-    t.silence = f0 + 64;
-    return true;
-}
-
 // Voice audio of a chain's spans over the segment (before pan for mono, times
 // the job gain), one wave per chain and one lane per frame, written to the float
 // scratch that the filter pass then reads and overwrites. Frames carry no state
@@ -285,23 +254,84 @@ __device__ __forceinline__ void chainAudio(const int16_t* __restrict__ bank, con
     }
 }
 
+// Voice-silence bookkeeping of one filter pass (kPassFrames frames, lane l
+// holding frames kLaneFrames * l ...): each row of 16 lanes is one chunk of 64
+// absolute frames. A chunk may take several passes when spans split it; it is
+// judged when its last frame has been rendered. After CLAY_SILENT_CHUNKS
+// consecutive silent chunks (peak |output| below the level, non-finite counting
+// as loud, every active frame at or after the voice's release) the chain
+// retires at the end of the chunk. Returns true when it retires.
+__device__ __forceinline__ bool silencePass(SilenceTrack& t, float level, int f0, int lo, int hi,
+                                            const bool (&act)[kLaneFrames], const float (&outL)[kLaneFrames],
+                                            const float (&outR)[kLaneFrames], int base, int releaseStart)
+{
+    float a = 0.f;
+    bool eligible = true;
+#pragma unroll
+    for (int k = 0; k < kLaneFrames; ++k)
+        if (act[k]) {
+            float x = fmaxf(fabsf(outL[k]), fabsf(outR[k]));
+            if (!(x <= 3.402823466e38f))
+                x = __builtin_inff();
+            a = fmaxf(a, x);
+            eligible = eligible && releaseStart >= 0 && f0 + base + k >= releaseStart;
+        }
+    for (int d = 8; d; d >>= 1)
+        a = fmaxf(a, __shfl_xor(a, d, 64));  // peak of each row of 16 lanes
+    const uint64_t loud = __ballot(!eligible);
+    for (int j = 0; j < kLaneFrames; ++j) {
+        const int c0 = 64 * j;
+        if (hi <= c0 || lo >= c0 + 64)
+            continue;  // no frame of this chunk in the pass
+        const float peak = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(a), 16 * j));
+        const bool all = ((loud >> (16 * j)) & 0xffffu) == 0;
+        const int chunk = ((f0 + c0) >> 6) + 1;
+        if (t.openChunk != chunk) {
+            t.openChunk = chunk;
+            t.openPeak = 0.f;
+            t.openEligible = 1;
+        }
+        t.openPeak = fmaxf(t.openPeak, peak);
+        t.openEligible = t.openEligible && all;
+        if (hi < c0 + 64)
+            return false;  // the chunk continues in a later span
+        const bool silent = t.openEligible && t.openPeak < level;
+        t.openChunk = 0;
+        t.silentChunks = silent ? t.silentChunks + 1 : 0;
+        if (t.silentChunks >= CLAY_SILENT_CHUNKS) {
+            t.retired = 1;
+            t.silence = f0 + c0 + 64;
+            return true;
+        }
+    }
+    return false;
+}
+
 // One wave per voice chain (a voice's spans with filters and/or variable pitch),
-// 64 output frames per step, chunks aligned to absolute frame multiples of 64:
-//   1. the 64 frames of voice audio (chainAudio) read back, one per lane;
-//   2. filter targets where a lane's inputs differ from the previous frame's,
-//      copied forward to the following lanes (src/cuda/render.cu:376-406);
-//   3. coefficient smoothing c_n = c_{n-1} + a (t_n - c_{n-1}) as an inclusive
-//      scan of affine maps c -> M c + V over the 64 lanes (DPP moves);
-//   4. each biquad section (direct form II transposed, render.cu:330-336) as a
-//      scan of 2x2 affine maps s_n = A_n s_{n-1} + B_n u_n, in FP64;
-//   5. the filtered frames written to the float scratch the mix kernel reads.
-// State carries across chunks in LDS and across segments in VoiceState.
+// kPassFrames = 256 frames per pass, passes aligned to absolute frame multiples
+// of 256, lane l holding the 4 consecutive frames 4l .. 4l + 3 of the pass:
+//   1. the voice audio (chainAudio) read back;
+//   2. filter inputs per frame; a frame whose inputs differ from the previous
+//      frame's gets new targets, copied forward to the following frames
+//      (src/cuda/render.cu:376-406);
+//   3. coefficient smoothing c_n = c_{n-1} + a (t_n - c_{n-1}): with unchanged
+//      targets in closed form c_n = t + (1 - a)^n (c_0 - t); otherwise the
+//      lane composes the affine maps c -> M c + V of its frames, a scan over
+//      the lanes (DPP moves) gives the map before each lane, and the lane
+//      applies its frames in order;
+//   4. each biquad section (direct form II transposed, render.cu:330-336) the
+//      same way with 2x2 affine maps s -> A s + d u in FP64: compose the lane's
+//      frames, scan over lanes, then run the lane's frames from its entry state;
+//   5. the filtered frames written back over the audio for the mix kernel.
+// Composing four frames per lane in registers leaves one wave scan per 256
+// frames. State carries across passes in LDS and across segments in VoiceState.
 __device__ __forceinline__ void voiceChain(const Span* __restrict__ spans, const SpanCold* __restrict__ cold,
                                            const clay_control_point* __restrict__ points, const Mod* __restrict__ mods,
                                            float* __restrict__ filtered, VoiceState* __restrict__ states,
                                            int segFirst, int segEnd, int head, float silenceLevel, ChainCarry& carry,
                                            int lane)
 {
+    constexpr int K = kLaneFrames;
     VoiceState& st = states[spans[head].filterTrack];
     SilenceTrack track{st.retired, st.silence, st.silentChunks, st.openChunk, st.openPeak, st.openEligible};
     if (track.retired)
@@ -342,158 +372,285 @@ __device__ __forceinline__ void voiceChain(const Span* __restrict__ spans, const
         const int first = max(v.begin, segFirst), stop = min(v.end, segEnd);
         const bool stereo = v.channels == 2;
         const int rate = cv.sampleRate;
-        const double amount = 1 - exp(-1000.0 / rate);
+        const double amount = 1 - exp(-1000.0 / rate), keep = 1 - amount;
         bool moving = cv.active[0] != 0;
         for (int k = v.firstMod; k < v.firstMod + v.modCount; ++k) {
             const int t = mods[k].target;
             moving |= (t >= CLAY_MOD_CUTOFF && t <= CLAY_MOD_RESONANCE2) || t == CLAY_MOD_FILEG_DEPTH ||
                       t >= CLAY_MOD_EQ1_FREQUENCY;
         }
+        const int base = K * lane;  // the lane's first frame within a pass
         ModReader reader;
-        reader.reset(mods, points, v.firstMod, v.modCount, max(first, (first & ~63) + lane));
+        reader.reset(mods, points, v.firstMod, v.modCount, max(first, (first & ~(kPassFrames - 1)) + base));
         // Bit f: carry.cents[f] holds the cents behind carry.input[f][0], computed in this span.
         int centsValid = 0;
-        for (int f0 = first & ~63; f0 < stop; f0 += 64) {
-            const int frame = f0 + lane;
-            const bool active = frame >= first && frame < stop;
-            const int lo = max(first, f0) - f0, hi = min(stop, f0 + 64) - f0;  // active lanes [lo, hi)
-            float* dst = filtered + v.filtered + uint64_t(max(frame - first, 0)) * uint64_t(v.channels);
-            float xl = 0.f, xr = 0.f;
-            if (active) {
-                xl = dst[0];
-                if (stereo)
-                    xr = dst[1];
+        // Where modulation target t feeds filter slot f: 0 nowhere, 1 cutoff
+        // (cents), 2 resonance (dB), 3 filter-envelope depth, 4..6 EQ frequency,
+        // bandwidth and gain.
+        const auto routeOf = [](int f, int t) {
+            if (f == 0 && t == CLAY_MOD_CUTOFF) return 1;
+            if (f == 0 && t == CLAY_MOD_RESONANCE) return 2;
+            if (f == 0 && t == CLAY_MOD_FILEG_DEPTH) return 3;
+            if (f == 1 && t == CLAY_MOD_CUTOFF2) return 1;
+            if (f == 1 && t == CLAY_MOD_RESONANCE2) return 2;
+            if (f >= 2 && t >= CLAY_MOD_EQ1_FREQUENCY && (t - CLAY_MOD_EQ1_FREQUENCY) / 3 == f - 2)
+                return 4 + (t - CLAY_MOD_EQ1_FREQUENCY) % 3;
+            return 0;
+        };
+        // Filter inputs of one frame for slot f: cutoff offset (cents, turned
+        // into Hz by the caller) and resonance, or for EQ slots frequency,
+        // bandwidth and gain. `routes` holds routeOf for the cursor slots, 3
+        // bits each, so the per-frame code branches on one uniform word.
+        const auto slotInputs = [&](int f, int routes, int frame, double& in0, double& in1, double& in2) {
+            double cents = 0, db = 0, depth = cv.extra[0].depth, eq0 = 0, eq1 = 0, eq2 = 0;
+            const auto add = [&](int route, double x) {
+                switch (route) {
+                case 1: cents += x; break;
+                case 2: db += x; break;
+                case 3: depth += x; break;
+                case 4: eq0 += x; break;
+                case 5: eq1 += x; break;
+                case 6: eq2 += x; break;
+                default: break;
+                }
+            };
+            // Unrolled over the cursor slots so the cursors stay in registers.
+#pragma unroll
+            for (int k = 0; k < kCursors; ++k) {
+                const int route = routes >> (3 * k) & 7;
+                if (route)
+                    add(route, reader.value(mods, points, k, frame));
             }
-            float outL = xl, outR = xr;
+            for (int k = kCursors; k < v.modCount; ++k) {
+                const int route = routeOf(f, mods[v.firstMod + k].target);
+                if (route)
+                    add(route, reader.value(mods, points, k, frame));
+            }
+            const clay_voice_filter& flt = cv.filters[f];
+            if (f < 2) {
+                if (f == 0 && cv.active[0])
+                    cents += depth * envelopeShapeD(cv.extra[0], frame);
+                in0 = cents;
+                in1 = double(flt.resonance) + db;
+                in2 = 0;
+            } else {
+                in0 = double(flt.cutoff) + eq0;
+                in1 = double(flt.bandwidth) + eq1;
+                in2 = double(flt.gain) + eq2;
+            }
+        };
+        for (int f0 = first & ~(kPassFrames - 1); f0 < stop; f0 += kPassFrames) {
+            const int lo = max(first, f0) - f0, hi = min(stop, f0 + kPassFrames) - f0;  // active frames [lo, hi)
+            const int lastLane = (hi - 1) / K, lastSlot = (hi - 1) % K;                 // the last active frame
+            bool act[K];
+            int act_bits = 0;
+            float outL[K], outR[K];
+            float* dst = filtered + v.filtered;
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                const int at = base + k;
+                act[k] = at >= lo && at < hi;
+                act_bits |= act[k] ? 1 << k : 0;
+                outL[k] = outR[k] = 0.f;
+                if (act[k]) {
+                    const float* src = dst + uint64_t(f0 + at - first) * uint64_t(v.channels);
+                    outL[k] = src[0];
+                    if (stereo)
+                        outR[k] = src[1];
+                }
+            }
             if (hasFilter) {
-                // Filter inputs of this lane's frame for slot f: cutoff (Hz) and
-                // resonance, or for EQ slots frequency, bandwidth and gain.
-                const auto slotInputs = [&](int f, double& in0, double& in1, double& in2) {
-                    double cents = 0, db = 0, depth = cv.extra[0].depth, eq0 = 0, eq1 = 0, eq2 = 0;
-                    const auto add = [&](int k) {
-                        const int t = mods[v.firstMod + k].target;
-                        if (f == 0 && t == CLAY_MOD_CUTOFF) cents += reader.value(mods, points, k, frame);
-                        else if (f == 0 && t == CLAY_MOD_RESONANCE) db += reader.value(mods, points, k, frame);
-                        else if (f == 0 && t == CLAY_MOD_FILEG_DEPTH) depth += reader.value(mods, points, k, frame);
-                        else if (f == 1 && t == CLAY_MOD_CUTOFF2) cents += reader.value(mods, points, k, frame);
-                        else if (f == 1 && t == CLAY_MOD_RESONANCE2) db += reader.value(mods, points, k, frame);
-                        else if (f >= 2 && t >= CLAY_MOD_EQ1_FREQUENCY && (t - CLAY_MOD_EQ1_FREQUENCY) / 3 == f - 2) {
-                            const int field = (t - CLAY_MOD_EQ1_FREQUENCY) % 3;
-                            const double x = reader.value(mods, points, k, frame);
-                            if (field == 0) eq0 += x;
-                            else if (field == 1) eq1 += x;
-                            else eq2 += x;
-                        }
-                    };
-                    // Unrolled over the cursor slots so the cursors stay in registers.
-    #pragma unroll
-                    for (int k = 0; k < kCursors; ++k)
-                        if (k < v.modCount)
-                            add(k);
-                    for (int k = kCursors; k < v.modCount; ++k)
-                        add(k);
-                    const double eq[3] = {eq0, eq1, eq2};
-                    const clay_voice_filter& flt = cv.filters[f];
-                    if (f < 2) {
-                        if (f == 0 && cv.active[0])
-                            cents += depth * envelopeShapeD(cv.extra[0], frame);
-                        in0 = cents;  // the caller turns it into the cutoff
-                        in1 = double(flt.resonance) + db;
-                        in2 = 0;
-                    } else {
-                        in0 = double(flt.cutoff) + eq[0];
-                        in1 = double(flt.bandwidth) + eq[1];
-                        in2 = double(flt.gain) + eq[2];
-                    }
-                };
-                const bool recompute = moving || !initialized;
+                double ul[K], ur[K];
+#pragma unroll
+                for (int k = 0; k < K; ++k) {
+                    ul[k] = outL[k];
+                    ur[k] = outR[k];
+                }
                 int sec = 0;
-                double ul = xl, ur = xr;
                 for (int f = 0; f < CLAY_FILTER_SLOTS; ++f) {
                     const int slotSections = sections(cold[head].filters[f].type);
                     if (!slotSections)
                         continue;
-                    double c[5];
-                    if (recompute) {
-                        double in0 = 0, in1 = 0, in2 = 0;
-                        if (active)
-                            slotInputs(f, in0, in1, in2);
-                        double cents = 0;
-                        if (f < 2) {
-                            // Cutoff = static cutoff x 2^(cents / 1200). When every active lane has
-                            // the cents of the span's last evaluated frame, the carried cutoff is
-                            // the value the formula gives: skip the FP64 exp2.
-                            cents = in0;
-                            if ((centsValid >> f & 1) && __ballot(active && cents != carry.cents[f]) == 0)
-                                in0 = carry.input[f][0];
-                            else if (active)
-                                in0 = double(cv.filters[f].cutoff) * exp2(cents / 1200);
-                        }
-                        // 2. Targets: recompute where inputs change, copy forward otherwise.
-                        double p0 = previousLane(in0, 0.0), p1 = previousLane(in1, 0.0), p2 = previousLane(in2, 0.0);
-                        bool prevValid = lane > lo;
-                        if (lane == lo) {
-                            p0 = carry.input[f][0], p1 = carry.input[f][1], p2 = carry.input[f][2];
-                            prevValid = carry.have[f] != 0;
-                        }
-                        const bool flag = active && (!prevValid || in0 != p0 || in1 != p1 || in2 != p2);
-                        const bool anyFlag = __ballot(flag) != 0;
-                        double t[5] = {0, 0, 0, 0, 0};
-                        if (!anyFlag) {
-                            for (int i = 0; i < 5; ++i)
-                                t[i] = carry.target[f][i];
-                        } else {
-                            if (flag) {
-                                const CoeffD q = f < 2 ? coefficientsD(cv.filters[f].type, in0, in1, rate, 0, 0)
-                                                       : coefficientsD(cv.filters[f].type, in0, 0, rate, in1, in2);
-                                t[0] = q.b0, t[1] = q.b1, t[2] = q.b2, t[3] = q.a1, t[4] = q.a2;
+                    // Coefficients (b0, b1, b2, a1, a2) of the lane's frame k: t + power[k] d
+                    // (the closed form of unchanged targets, or the carried coefficients with
+                    // d = 0), or, after a target change, staged per frame in LDS.
+                    double t[5], d[5], power[K];
+                    for (int i = 0; i < 5; ++i) {
+                        t[i] = carry.coef[f][i];
+                        d[i] = 0;
+                    }
+#pragma unroll
+                    for (int k = 0; k < K; ++k)
+                        power[k] = 0;
+                    bool staged = false;
+                    if (moving || !initialized) {
+                        int routes = 0;
+                        for (int k = 0; k < kCursors && k < v.modCount; ++k)
+                            routes |= routeOf(f, mods[v.firstMod + k].target) << (3 * k);
+                        routes = uniform(routes);
+                        // 2. Inputs of the lane's frames in order, flags where they differ
+                        // from the previous frame's (the first active frame compares with the
+                        // carried inputs), and the targets of flagged frames, staged. One loop
+                        // iteration per frame slot keeps the input and coefficient code in one
+                        // copy; the last iteration judges slot 0 against the previous lane's
+                        // last frame, which every lane has evaluated by then.
+                        double first3[3] = {0, 0, 0}, prev[3] = {0, 0, 0}, last4[4] = {0, 0, 0, 0};
+                        int flags = 0;
+#pragma unroll 1
+                        for (int k = 0; k <= K; ++k) {
+                            const int slot = k == K ? 0 : k, at = base + slot;
+                            const bool on = (act_bits >> slot) & 1;
+                            double x[3], q[3];
+                            bool valid = at > lo;
+                            if (k < K) {
+                                x[0] = x[1] = x[2] = 0;
+                                if (on)
+                                    slotInputs(f, routes, f0 + at, x[0], x[1], x[2]);
+                                const double cents = x[0];
+                                if (f < 2) {
+                                    // Cutoff = static cutoff x 2^(cents / 1200). When no active lane's
+                                    // cents differ from those behind the carried cutoff (same span),
+                                    // the carried cutoff is the value the formula gives: skip the exp2.
+                                    if ((centsValid >> f & 1) && __ballot(on && cents != carry.cents[f]) == 0)
+                                        x[0] = carry.input[f][0];
+                                    else if (on)
+                                        x[0] = double(cv.filters[f].cutoff) * exp2(cents / 1200);
+                                }
+                                if (slot == lastSlot)
+                                    last4[0] = x[0], last4[1] = x[1], last4[2] = x[2], last4[3] = cents;
+                                for (int i = 0; i < 3; ++i) {
+                                    q[i] = prev[i];
+                                    prev[i] = x[i];
+                                }
+                                if (k == 0) {
+                                    for (int i = 0; i < 3; ++i)
+                                        first3[i] = x[i];
+                                    continue;  // judged in the last iteration
+                                }
+                            } else {
+                                for (int i = 0; i < 3; ++i) {
+                                    x[i] = first3[i];
+                                    q[i] = previousLane(prev[i], 0.0);
+                                }
                             }
-                            // Nearest flagged lane at or before each lane (the later one wins).
-                            int have = flag ? 1 : 0;
+                            if (at == lo) {
+                                for (int i = 0; i < 3; ++i)
+                                    q[i] = carry.input[f][i];
+                                valid = carry.have[f] != 0;
+                            }
+                            const bool flag = on && (!valid || x[0] != q[0] || x[1] != q[1] || x[2] != q[2]);
+                            flags |= flag ? 1 << slot : 0;
+                            if (__ballot(flag) == 0)
+                                continue;
+                            if (flag) {
+                                const CoeffD c = coefficientsD(cv.filters[f].type, x[0], f < 2 ? x[1] : 0, rate,
+                                                               f < 2 ? 0 : x[1], f < 2 ? 0 : x[2]);
+                                carry.stage[0][slot][lane] = c.b0;
+                                carry.stage[1][slot][lane] = c.b1;
+                                carry.stage[2][slot][lane] = c.b2;
+                                carry.stage[3][slot][lane] = c.a1;
+                                carry.stage[4][slot][lane] = c.a2;
+                            }
+                        }
+                        double nt[5], nc[5];  // target and coefficients at the last active frame
+                        if (__ballot(flags != 0) == 0) {
+                            // 3a. Unchanged target t over the pass (a chain not yet initialized
+                            // flags its first frame, so here it is initialized and moving):
+                            // c after n smoothing steps is t + keep^n (c_0 - t), keep^n from the
+                            // exclusive product over the lanes of keep^(active frames of a lane).
+                            for (int i = 0; i < 5; ++i) {
+                                t[i] = nt[i] = carry.target[f][i];
+                                d[i] = carry.coef[f][i] - t[i];
+                            }
+                            double m = 1;
+#pragma unroll
+                            for (int k = 0; k < K; ++k)
+                                if (act[k])
+                                    m *= keep;
+                            scanWave([&](auto st) {
+                                constexpr int C = decltype(st)::ctrl, R = decltype(st)::rowMask;
+                                m *= shiftD<C, R>(m, 1.0);
+                            });
+                            double p = previousLane(m, 1.0), pl = 0;
+#pragma unroll
+                            for (int k = 0; k < K; ++k) {
+                                if (act[k])
+                                    p *= keep;
+                                power[k] = p;
+                                if (k == lastSlot)
+                                    pl = p;
+                            }
+                            pl = laneValue(pl, lastLane);
+                            for (int i = 0; i < 5; ++i)
+                                nc[i] = fma(pl, d[i], t[i]);
+                        } else {
+                            staged = true;
+                            // The nearest flagged frame at or before each frame: the lane's last
+                            // flag, a scan over the lanes, then forward within the lane.
+                            int have = 0;
+                            double lt[5] = {0, 0, 0, 0, 0};
+#pragma unroll
+                            for (int k = 0; k < K; ++k)
+                                if (flags >> k & 1) {
+                                    have = 1;
+                                    for (int i = 0; i < 5; ++i)
+                                        lt[i] = carry.stage[i][k][lane];
+                                }
                             scanWave([&](auto st) {
                                 constexpr int C = decltype(st)::ctrl, R = decltype(st)::rowMask;
                                 const int hp = shiftI<C, R>(have, 0);
                                 double tp[5];
                                 for (int i = 0; i < 5; ++i)
-                                    tp[i] = shiftD<C, R>(t[i], 0.0);
+                                    tp[i] = shiftD<C, R>(lt[i], 0.0);
                                 if (!have && hp) {
                                     for (int i = 0; i < 5; ++i)
-                                        t[i] = tp[i];
+                                        lt[i] = tp[i];
                                     have = 1;
                                 }
                             });
-                            if (!have)
-                                for (int i = 0; i < 5; ++i)
-                                    t[i] = carry.target[f][i];
-                        }
-                        if (!anyFlag && moving && initialized) {
-                            // 3a. Constant target over the chunk: c_n = t + q^k (c - t), q = 1 - a,
-                            // k = frames smoothed so far in this chunk.
-                            const int k = lane - lo + 1;
-                            double qk = 1, qp = 1 - amount;
-                            for (int bit = 1; bit <= 64; bit <<= 1, qp *= qp)
-                                if (k & bit)
-                                    qk *= qp;
-                            for (int i = 0; i < 5; ++i)
-                                c[i] = fma(qk, carry.coef[f][i] - t[i], t[i]);
-                        } else {
-                            // 3b. Smoothing scan: c_n = M_n c_{n-1} + V_n.
-                            // A filter without modulation takes its coefficients once, unsmoothed.
-                            double M, V[5];
-                            if (!active || (!moving && lane != lo)) {
-                                M = 1;
-                                for (int i = 0; i < 5; ++i)
-                                    V[i] = 0;
-                            } else if (!initialized && lane == lo) {
-                                M = 0;
-                                for (int i = 0; i < 5; ++i)
-                                    V[i] = t[i];
-                            } else {
-                                M = 1 - amount;
-                                for (int i = 0; i < 5; ++i)
-                                    V[i] = amount * t[i];
+                            double cur[5];
+                            const int incoming = shiftI<kWaveShr1>(have, 0);
+                            for (int i = 0; i < 5; ++i) {
+                                cur[i] = previousLane(lt[i], 0.0);
+                                if (!incoming)
+                                    cur[i] = carry.target[f][i];
                             }
-                            // (M, V) o (Mp, Vp) = (M Mp, M Vp + V); identity (1, 0).
+#pragma unroll
+                            for (int k = 0; k < K; ++k)
+                                for (int i = 0; i < 5; ++i) {
+                                    if (flags >> k & 1)
+                                        cur[i] = carry.stage[i][k][lane];
+                                    else
+                                        carry.stage[i][k][lane] = cur[i];
+                                }
+                            __builtin_amdgcn_wave_barrier();
+                            for (int i = 0; i < 5; ++i)
+                                nt[i] = carry.stage[i][lastSlot][lastLane];
+                            __builtin_amdgcn_wave_barrier();
+                            // 3b. Smoothing maps per frame: identity where inactive or for a
+                            // filter without modulation after its first frame; (0, t) for the
+                            // first frame of a chain; (keep, amount t) otherwise.
+                            const auto kind = [&](int k) {
+                                const int at = base + k;
+                                if (!act[k] || (!moving && at != lo))
+                                    return 0;
+                                return !initialized && at == lo ? 1 : 2;
+                            };
+                            double M = 1, V[5] = {0, 0, 0, 0, 0};
+#pragma unroll
+                            for (int k = 0; k < K; ++k) {
+                                const int how = kind(k);
+                                if (how == 1) {
+                                    M = 0;
+                                    for (int i = 0; i < 5; ++i)
+                                        V[i] = carry.stage[i][k][lane];
+                                } else if (how == 2) {
+                                    M *= keep;
+                                    for (int i = 0; i < 5; ++i)
+                                        V[i] = fma(keep, V[i], amount * carry.stage[i][k][lane]);
+                                }
+                            }
+                            // (M, V) after (Mp, Vp) = (M Mp, M Vp + V); identity (1, 0).
                             scanWave([&](auto st) {
                                 constexpr int C = decltype(st)::ctrl, R = decltype(st)::rowMask;
                                 const double Mp = shiftD<C, R>(M, 1.0);
@@ -501,100 +658,152 @@ __device__ __forceinline__ void voiceChain(const Span* __restrict__ spans, const
                                     V[i] = fma(M, shiftD<C, R>(V[i], 0.0), V[i]);
                                 M *= Mp;
                             });
+                            const double Me = previousLane(M, 1.0);
                             for (int i = 0; i < 5; ++i)
-                                c[i] = fma(M, carry.coef[f][i], V[i]);
+                                cur[i] = fma(Me, carry.coef[f][i], previousLane(V[i], 0.0));
+#pragma unroll
+                            for (int k = 0; k < K; ++k) {
+                                const int how = kind(k);
+                                for (int i = 0; i < 5; ++i) {
+                                    if (how == 1)
+                                        cur[i] = carry.stage[i][k][lane];
+                                    else if (how == 2)
+                                        cur[i] = fma(keep, cur[i], amount * carry.stage[i][k][lane]);
+                                    carry.stage[i][k][lane] = cur[i];
+                                }
+                            }
+                            __builtin_amdgcn_wave_barrier();
+                            for (int i = 0; i < 5; ++i)
+                                nc[i] = carry.stage[i][lastSlot][lastLane];
                         }
-                        // Carry: values of the last active lane.
-                        const int last = hi - 1;
-                        double nc[5], nt[5], ni[3];
-                        for (int i = 0; i < 5; ++i) {
-                            nc[i] = laneValue(c[i], last);
-                            nt[i] = laneValue(t[i], last);
-                        }
-                        ni[0] = laneValue(in0, last);
-                        ni[1] = laneValue(in1, last);
-                        ni[2] = laneValue(in2, last);
-                        const double nCents = laneValue(cents, last);
+                        // Carry: the values of the last active frame.
+                        double ni[4];
+                        for (int i = 0; i < 4; ++i)
+                            ni[i] = laneValue(last4[i], lastLane);
                         __builtin_amdgcn_wave_barrier();
                         if (lane == 0) {
-                            if (f < 2)
-                                carry.cents[f] = nCents;
                             for (int i = 0; i < 5; ++i) {
                                 carry.coef[f][i] = nc[i];
                                 carry.target[f][i] = nt[i];
                             }
                             for (int i = 0; i < 3; ++i)
                                 carry.input[f][i] = ni[i];
+                            if (f < 2)
+                                carry.cents[f] = ni[3];
                             carry.have[f] = 1;
                         }
                         if (f < 2)
                             centsValid |= 1 << f;
                         __builtin_amdgcn_wave_barrier();
-                    } else {
-                        for (int i = 0; i < 5; ++i)
-                            c[i] = carry.coef[f][i];
                     }
-                    // 4. Biquad sections of this slot: s_n = A_n s_{n-1} + B_n u_n.
-                    for (int k = 0; k < slotSections; ++k, ++sec) {
-                        // A = [[-a1, 1], [-a2, 0]]; inactive lanes are the identity.
-                        double a00 = -c[3], a01 = 1, a10 = -c[4], a11 = 0;
-                        double wl0 = (c[1] - c[3] * c[0]) * ul, wl1 = (c[2] - c[4] * c[0]) * ul;
-                        double wr0 = (c[1] - c[3] * c[0]) * ur, wr1 = (c[2] - c[4] * c[0]) * ur;
-                        if (!active) {
-                            a00 = 1, a01 = 0, a10 = 0, a11 = 1;
-                            wl0 = wl1 = wr0 = wr1 = 0;
-                        }
-                        // (A, w) o (B, p) = (A B, A p + w); identity (I, 0).
-                        scanWave([&](auto st) {
-                            constexpr int C = decltype(st)::ctrl, R = decltype(st)::rowMask;
-                            const double b00 = shiftD<C, R>(a00, 1.0), b01 = shiftD<C, R>(a01, 0.0);
-                            const double b10 = shiftD<C, R>(a10, 0.0), b11 = shiftD<C, R>(a11, 1.0);
-                            const double pl0 = shiftD<C, R>(wl0, 0.0), pl1 = shiftD<C, R>(wl1, 0.0);
-                            wl0 = fma(a00, pl0, fma(a01, pl1, wl0));
-                            wl1 = fma(a10, pl0, fma(a11, pl1, wl1));
-                            if (stereo) {
-                                const double pr0 = shiftD<C, R>(wr0, 0.0), pr1 = shiftD<C, R>(wr1, 0.0);
-                                wr0 = fma(a00, pr0, fma(a01, pr1, wr0));
-                                wr1 = fma(a10, pr0, fma(a11, pr1, wr1));
+                    // 4. Biquad sections of this slot: s -> A s + d u per frame, with
+                    // A = [[-a1, 1], [-a2, 0]] and d = (b1 - a1 b0, b2 - a2 b0).
+                    const auto runSections = [&](auto&& coef) {
+                        for (int kk = 0; kk < slotSections; ++kk, ++sec) {
+                            double p00 = 1, p01 = 0, p10 = 0, p11 = 1, wl0 = 0, wl1 = 0, wr0 = 0, wr1 = 0;
+#pragma unroll
+                            for (int k = 0; k < K; ++k)
+                                if (act[k]) {
+                                    const double b0 = coef(k, 0), a1 = coef(k, 3), a2 = coef(k, 4);
+                                    const double d1 = fma(-a1, b0, coef(k, 1)), d2 = fma(-a2, b0, coef(k, 2));
+                                    const double n00 = fma(-a1, p00, p10), n01 = fma(-a1, p01, p11);
+                                    p10 = -a2 * p00;
+                                    p11 = -a2 * p01;
+                                    p00 = n00;
+                                    p01 = n01;
+                                    const double l0 = fma(d1, ul[k], fma(-a1, wl0, wl1));
+                                    wl1 = fma(d2, ul[k], -a2 * wl0);
+                                    wl0 = l0;
+                                    if (stereo) {
+                                        const double r0 = fma(d1, ur[k], fma(-a1, wr0, wr1));
+                                        wr1 = fma(d2, ur[k], -a2 * wr0);
+                                        wr0 = r0;
+                                    }
+                                }
+                            // (A, w) after (B, p) = (A B, A p + w); identity (I, 0).
+                            scanWave([&](auto st) {
+                                constexpr int C = decltype(st)::ctrl, R = decltype(st)::rowMask;
+                                const double b00 = shiftD<C, R>(p00, 1.0), b01 = shiftD<C, R>(p01, 0.0);
+                                const double b10 = shiftD<C, R>(p10, 0.0), b11 = shiftD<C, R>(p11, 1.0);
+                                const double ql0 = shiftD<C, R>(wl0, 0.0), ql1 = shiftD<C, R>(wl1, 0.0);
+                                wl0 = fma(p00, ql0, fma(p01, ql1, wl0));
+                                wl1 = fma(p10, ql0, fma(p11, ql1, wl1));
+                                if (stereo) {
+                                    const double qr0 = shiftD<C, R>(wr0, 0.0), qr1 = shiftD<C, R>(wr1, 0.0);
+                                    wr0 = fma(p00, qr0, fma(p01, qr1, wr0));
+                                    wr1 = fma(p10, qr0, fma(p11, qr1, wr1));
+                                }
+                                const double n00 = fma(p00, b00, p01 * b10), n01 = fma(p00, b01, p01 * b11);
+                                const double n10 = fma(p10, b00, p11 * b10), n11 = fma(p10, b01, p11 * b11);
+                                p00 = n00, p01 = n01, p10 = n10, p11 = n11;
+                            });
+                            // The state entering the lane: the previous lanes' map applied to the carry.
+                            const double e00 = previousLane(p00, 1.0), e01 = previousLane(p01, 0.0);
+                            const double e10 = previousLane(p10, 0.0), e11 = previousLane(p11, 1.0);
+                            double zl1 = carry.z[sec][0][0], zl2 = carry.z[sec][0][1];
+                            double zr1 = carry.z[sec][1][0], zr2 = carry.z[sec][1][1];
+                            {
+                                const double el0 = previousLane(wl0, 0.0), el1 = previousLane(wl1, 0.0);
+                                const double s1 = fma(e00, zl1, fma(e01, zl2, el0));
+                                const double s2 = fma(e10, zl1, fma(e11, zl2, el1));
+                                zl1 = s1, zl2 = s2;
                             }
-                            const double n00 = fma(a00, b00, a01 * b10), n01 = fma(a00, b01, a01 * b11);
-                            const double n10 = fma(a10, b00, a11 * b10), n11 = fma(a10, b01, a11 * b11);
-                            a00 = n00, a01 = n01, a10 = n10, a11 = n11;
-                        });
-                        const double zl1 = carry.z[sec][0][0], zl2 = carry.z[sec][0][1];
-                        const double zr1 = carry.z[sec][1][0], zr2 = carry.z[sec][1][1];
-                        const double sl1 = fma(a00, zl1, fma(a01, zl2, wl0)), sl2 = fma(a10, zl1, fma(a11, zl2, wl1));
-                        const double sr1 = fma(a00, zr1, fma(a01, zr2, wr0)), sr2 = fma(a10, zr1, fma(a11, zr2, wr1));
-                        // y_n = b0 u_n + z1_{n-1}
-                        double prevL = previousLane(sl1, zl1), prevR = previousLane(sr1, zr1);
-                        if (lane == lo) {
-                            prevL = zl1;
-                            prevR = zr1;
+                            if (stereo) {
+                                const double er0 = previousLane(wr0, 0.0), er1 = previousLane(wr1, 0.0);
+                                const double s1 = fma(e00, zr1, fma(e01, zr2, er0));
+                                const double s2 = fma(e10, zr1, fma(e11, zr2, er1));
+                                zr1 = s1, zr2 = s2;
+                            }
+                            // The lane's frames in order: y = b0 u + z1.
+#pragma unroll
+                            for (int k = 0; k < K; ++k)
+                                if (act[k]) {
+                                    const double b0 = coef(k, 0), a1 = coef(k, 3), a2 = coef(k, 4);
+                                    const double d1 = fma(-a1, b0, coef(k, 1)), d2 = fma(-a2, b0, coef(k, 2));
+                                    const double yl = fma(b0, ul[k], zl1);
+                                    const double l1 = fma(d1, ul[k], fma(-a1, zl1, zl2));
+                                    zl2 = fma(d2, ul[k], -a2 * zl1);
+                                    zl1 = l1;
+                                    ul[k] = yl;
+                                    if (stereo) {
+                                        const double yr = fma(b0, ur[k], zr1);
+                                        const double r1 = fma(d1, ur[k], fma(-a1, zr1, zr2));
+                                        zr2 = fma(d2, ur[k], -a2 * zr1);
+                                        zr1 = r1;
+                                        ur[k] = yr;
+                                    }
+                                }
+                            // Carry: the state after the last active frame.
+                            const double ql1 = laneValue(zl1, lastLane), ql2 = laneValue(zl2, lastLane);
+                            const double qr1 = laneValue(zr1, lastLane), qr2 = laneValue(zr2, lastLane);
+                            __builtin_amdgcn_wave_barrier();
+                            if (lane == 0) {
+                                carry.z[sec][0][0] = ql1, carry.z[sec][0][1] = ql2;
+                                carry.z[sec][1][0] = qr1, carry.z[sec][1][1] = qr2;
+                            }
+                            __builtin_amdgcn_wave_barrier();
                         }
-                        ul = fma(c[0], ul, prevL);
-                        ur = stereo ? fma(c[0], ur, prevR) : 0.0;
-                        const int last = hi - 1;
-                        const double ql1 = laneValue(sl1, last), ql2 = laneValue(sl2, last);
-                        const double qr1 = laneValue(sr1, last), qr2 = laneValue(sr2, last);
-                        __builtin_amdgcn_wave_barrier();
-                        if (lane == 0) {
-                            carry.z[sec][0][0] = ql1, carry.z[sec][0][1] = ql2;
-                            carry.z[sec][1][0] = qr1, carry.z[sec][1][1] = qr2;
-                        }
-                        __builtin_amdgcn_wave_barrier();
-                    }
+                    };
+                    if (staged)
+                        runSections([&](int k, int i) { return carry.stage[i][k][lane]; });
+                    else
+                        runSections([&](int k, int i) { return fma(power[k], d[i], t[i]); });
                 }
                 initialized = true;
-                outL = float(ul);
-                outR = float(ur);
-            }
-            if (active && hasFilter) {
-                dst[0] = outL;
-                if (stereo)
-                    dst[1] = outR;
+#pragma unroll
+                for (int k = 0; k < K; ++k) {
+                    outL[k] = float(ul[k]);
+                    outR[k] = stereo ? float(ur[k]) : 0.f;
+                    if (act[k]) {
+                        float* o = dst + uint64_t(f0 + base + k - first) * uint64_t(v.channels);
+                        o[0] = outL[k];
+                        if (stereo)
+                            o[1] = outR[k];
+                    }
+                }
             }
             if (silenceLevel > 0.f &&
-                silenceStep(track, silenceLevel, f0, hi, active, outL, stereo ? outR : 0.f, frame, v.releaseStart))
+                silencePass(track, silenceLevel, f0, lo, hi, act, outL, outR, base, v.releaseStart))
                 break;
         }
         if (track.retired)

@@ -796,7 +796,7 @@ struct Shape {
     int mixBlocks = 0, chainBlocks = 0, pitchBlocks = 0, audioBlocks = 0;
 };
 
-constexpr int kChainWavesPerBlock = 4;   // render.hip: voice chains per 256-thread filter block
+constexpr int kChainWavesPerBlock = 2;   // render.hip: voice chains per 128-thread filter block
 constexpr int kAudioWavesPerBlock = 16;  // render.hip: voice chains per 1024-thread audio block
 constexpr int kSharedTapsMax = 32;       // render.hip: widest sinc table the audio pass keeps in LDS
 
@@ -1658,7 +1658,7 @@ int renderWindowPersistent(clay_hip* h, const clay_plan_view* plans, size_t coun
                     }
                     Shape fixed;
                     fixed.mixBlocks = shape.second;
-                    fixed.chainBlocks = cus * 2;
+                    fixed.chainBlocks = cus * 6;
                     fixed.pitchBlocks = cus;
                     fixed.audioBlocks = cus;
                     const bool measure = tuner.tuning && h->stats.rounds > 0;
