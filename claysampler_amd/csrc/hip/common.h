@@ -387,15 +387,40 @@ struct Cursor {
         }
         return pointValue(p, frame);
     }
+    // The value at `frame`; clears `steady` unless the track stays constant
+    // (zero slope, no further point) through frame `until`.
+    __device__ __forceinline__ double steadyValue(const clay_control_point* __restrict__ points, int last, int frame,
+                                                  int until, bool& steady)
+    {
+        clay_control_point p = points[at], n = points[min(at + 1, last - 1)];
+        while (at + 1 < last && n.frame <= frame) {
+            ++at;
+            p = n;
+            n = points[min(at + 1, last - 1)];
+        }
+        steady = steady && p.step == 0.f && (at + 1 >= last || n.frame > until);
+        return pointValue(p, frame);
+    }
 };
 
 // Up to kCursors modulation bindings of a span read through cursors; any
 // further ones fall back to the binary search.
 constexpr int kCursors = 4;
 
+// The cursors persist across the spans of a chain: a binding to the track a
+// cursor already follows keeps its position (frames only move forward), so
+// only a new track costs a binary search.
 struct ModReader {
     Cursor track[kCursors], shape[kCursors];
+    int trackKey[kCursors], shapeKey[kCursors];  // first point of the followed track, -1: none
     int first, count;
+
+    __device__ ModReader()
+    {
+#pragma unroll
+        for (int j = 0; j < kCursors; ++j)
+            trackKey[j] = shapeKey[j] = -1;
+    }
 
     __device__ void reset(const Mod* __restrict__ mods, const clay_control_point* __restrict__ points, int firstMod,
                           int modCount, int frame)
@@ -407,10 +432,14 @@ struct ModReader {
             if (j >= count)
                 break;
             const Mod m = mods[first + j];
-            if (m.first >= 0)
+            if (m.first >= 0 && m.first != trackKey[j]) {
                 track[j].reset(points, m.first, m.last, frame);
-            if (m.shapeFirst >= 0)
+                trackKey[j] = m.first;
+            }
+            if (m.shapeFirst >= 0 && m.shapeFirst != shapeKey[j]) {
                 shape[j].reset(points, m.shapeFirst, m.shapeLast, frame);
+                shapeKey[j] = m.shapeFirst;
+            }
         }
     }
 
@@ -435,6 +464,20 @@ struct ModReader {
             }();
             v *= x;
         }
+        return v;
+    }
+
+    // value() of cursor slot j < kCursors at `frame`, clearing `steady` unless
+    // it stays constant through frame `until`.
+    __device__ double steadyValue(const Mod* __restrict__ mods, const clay_control_point* __restrict__ points, int j,
+                                  int frame, int until, bool& steady)
+    {
+        const Mod m = mods[first + j];
+        double v = m.depth;
+        if (m.first >= 0)
+            v = double(m.depth) * track[j].steadyValue(points, m.last, frame, until, steady);
+        if (m.shapeFirst >= 0)
+            v *= shape[j].steadyValue(points, m.shapeLast, frame, until, steady);
         return v;
     }
 };

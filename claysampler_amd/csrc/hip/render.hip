@@ -156,6 +156,7 @@ struct ChainCarry {
     double target[CLAY_FILTER_SLOTS][5]; // last target
     double input[CLAY_FILTER_SLOTS][3];  // inputs of the last target
     double cents[2];                     // cutoff offsets (cents) behind input[0..1][0]
+    double cutoffOf[2];                  // the static cutoff behind them (NaN: not known)
     int have[CLAY_FILTER_SLOTS];
     double stage[5][kLaneFrames][64];    // per-frame targets, then coefficients, after a target change
 };
@@ -362,8 +363,10 @@ __device__ __forceinline__ void voiceChain(const Span* __restrict__ spans, const
                     carry.z[k][c][1] = st.z2[f][s][c];
                 }
         }
+        carry.cutoffOf[0] = carry.cutoffOf[1] = __builtin_nan("");
     }
     __builtin_amdgcn_wave_barrier();
+    ModReader reader;
     for (int s = head; s >= 0; s = spans[s].next) {
         const Span& v = spans[s];
         if (v.end <= segFirst || v.begin >= segEnd)
@@ -380,10 +383,7 @@ __device__ __forceinline__ void voiceChain(const Span* __restrict__ spans, const
                       t >= CLAY_MOD_EQ1_FREQUENCY;
         }
         const int base = K * lane;  // the lane's first frame within a pass
-        ModReader reader;
         reader.reset(mods, points, v.firstMod, v.modCount, max(first, (first & ~(kPassFrames - 1)) + base));
-        // Bit f: carry.cents[f] holds the cents behind carry.input[f][0], computed in this span.
-        int centsValid = 0;
         // Where modulation target t feeds filter slot f: 0 nowhere, 1 cutoff
         // (cents), 2 resonance (dB), 3 filter-envelope depth, 4..6 EQ frequency,
         // bandwidth and gain.
@@ -401,7 +401,8 @@ __device__ __forceinline__ void voiceChain(const Span* __restrict__ spans, const
         // into Hz by the caller) and resonance, or for EQ slots frequency,
         // bandwidth and gain. `routes` holds routeOf for the cursor slots, 3
         // bits each, so the per-frame code branches on one uniform word.
-        const auto slotInputs = [&](int f, int routes, int frame, double& in0, double& in1, double& in2) {
+        const auto slotInputs = [&](int f, int routes, int frame, auto&& value, double& in0, double& in1,
+                                    double& in2) {
             double cents = 0, db = 0, depth = cv.extra[0].depth, eq0 = 0, eq1 = 0, eq2 = 0;
             const auto add = [&](int route, double x) {
                 switch (route) {
@@ -419,7 +420,7 @@ __device__ __forceinline__ void voiceChain(const Span* __restrict__ spans, const
             for (int k = 0; k < kCursors; ++k) {
                 const int route = routes >> (3 * k) & 7;
                 if (route)
-                    add(route, reader.value(mods, points, k, frame));
+                    add(route, value(k));
             }
             for (int k = kCursors; k < v.modCount; ++k) {
                 const int route = routeOf(f, mods[v.firstMod + k].target);
@@ -488,6 +489,71 @@ __device__ __forceinline__ void voiceChain(const Span* __restrict__ spans, const
                         for (int k = 0; k < kCursors && k < v.modCount; ++k)
                             routes |= routeOf(f, mods[v.firstMod + k].target) << (3 * k);
                         routes = uniform(routes);
+                        double nt[5], nc[5];  // target and coefficients at the last active frame
+                        // 3a. Unchanged target t over the pass (a chain not yet initialized
+                        // flags its first frame, so here it is initialized and moving): c after
+                        // n smoothing steps is t + keep^n (c_0 - t), keep^n from the exclusive
+                        // product over the lanes of keep^(active frames of a lane).
+                        const auto closedForm = [&] {
+                            for (int i = 0; i < 5; ++i) {
+                                t[i] = nt[i] = carry.target[f][i];
+                                d[i] = carry.coef[f][i] - t[i];
+                            }
+                            double m = 1;
+#pragma unroll
+                            for (int k = 0; k < K; ++k)
+                                if (act[k])
+                                    m *= keep;
+                            scanWave([&](auto st) {
+                                constexpr int C = decltype(st)::ctrl, R = decltype(st)::rowMask;
+                                m *= shiftD<C, R>(m, 1.0);
+                            });
+                            double p = previousLane(m, 1.0), pl = 0;
+#pragma unroll
+                            for (int k = 0; k < K; ++k) {
+                                if (act[k])
+                                    p *= keep;
+                                power[k] = p;
+                                if (k == lastSlot)
+                                    pl = p;
+                            }
+                            pl = laneValue(pl, lastLane);
+                            for (int i = 0; i < 5; ++i)
+                                nc[i] = fma(pl, d[i], t[i]);
+                        };
+                        // Steady pass: when every routed track stays constant over each lane's
+                        // active frames and the first of them has the carried inputs, no frame
+                        // changes targets: one input evaluation per lane instead of one per frame.
+                        bool quick = false;
+                        if (initialized && !cv.active[0] && v.modCount <= kCursors && carry.have[f]) {
+                            const int fa = max(base, lo), la = min(base + K, hi) - 1;
+                            bool same = true;
+                            if (fa <= la) {
+                                bool steady = true;
+                                double x[3] = {0, 0, 0};
+                                slotInputs(
+                                    f, routes, f0 + fa,
+                                    [&](int j) { return reader.steadyValue(mods, points, j, f0 + fa, f0 + la, steady); },
+                                    x[0], x[1], x[2]);
+                                if (f < 2) {
+                                    if (carry.cutoffOf[f] == double(cv.filters[f].cutoff) && x[0] == carry.cents[f])
+                                        x[0] = carry.input[f][0];
+                                    else
+                                        steady = false;
+                                }
+                                same = steady && x[0] == carry.input[f][0] && x[1] == carry.input[f][1] &&
+                                       x[2] == carry.input[f][2];
+                            }
+                            quick = __ballot(!same) == 0;
+                        }
+                        if (quick) {
+                            closedForm();
+                            __builtin_amdgcn_wave_barrier();
+                            if (lane == 0)
+                                for (int i = 0; i < 5; ++i)
+                                    carry.coef[f][i] = nc[i];
+                            __builtin_amdgcn_wave_barrier();
+                        } else {
                         // 2. Inputs of the lane's frames in order, flags where they differ
                         // from the previous frame's (the first active frame compares with the
                         // carried inputs), and the targets of flagged frames, staged. One loop
@@ -505,13 +571,16 @@ __device__ __forceinline__ void voiceChain(const Span* __restrict__ spans, const
                             if (k < K) {
                                 x[0] = x[1] = x[2] = 0;
                                 if (on)
-                                    slotInputs(f, routes, f0 + at, x[0], x[1], x[2]);
+                                    slotInputs(
+                                        f, routes, f0 + at, [&](int j) { return reader.value(mods, points, j, f0 + at); },
+                                        x[0], x[1], x[2]);
                                 const double cents = x[0];
                                 if (f < 2) {
                                     // Cutoff = static cutoff x 2^(cents / 1200). When no active lane's
                                     // cents differ from those behind the carried cutoff (same span),
                                     // the carried cutoff is the value the formula gives: skip the exp2.
-                                    if ((centsValid >> f & 1) && __ballot(on && cents != carry.cents[f]) == 0)
+                                    if (carry.cutoffOf[f] == double(cv.filters[f].cutoff) &&
+                                        __ballot(on && cents != carry.cents[f]) == 0)
                                         x[0] = carry.input[f][0];
                                     else if (on)
                                         x[0] = double(cv.filters[f].cutoff) * exp2(cents / 1200);
@@ -552,37 +621,8 @@ __device__ __forceinline__ void voiceChain(const Span* __restrict__ spans, const
                                 carry.stage[4][slot][lane] = c.a2;
                             }
                         }
-                        double nt[5], nc[5];  // target and coefficients at the last active frame
                         if (__ballot(flags != 0) == 0) {
-                            // 3a. Unchanged target t over the pass (a chain not yet initialized
-                            // flags its first frame, so here it is initialized and moving):
-                            // c after n smoothing steps is t + keep^n (c_0 - t), keep^n from the
-                            // exclusive product over the lanes of keep^(active frames of a lane).
-                            for (int i = 0; i < 5; ++i) {
-                                t[i] = nt[i] = carry.target[f][i];
-                                d[i] = carry.coef[f][i] - t[i];
-                            }
-                            double m = 1;
-#pragma unroll
-                            for (int k = 0; k < K; ++k)
-                                if (act[k])
-                                    m *= keep;
-                            scanWave([&](auto st) {
-                                constexpr int C = decltype(st)::ctrl, R = decltype(st)::rowMask;
-                                m *= shiftD<C, R>(m, 1.0);
-                            });
-                            double p = previousLane(m, 1.0), pl = 0;
-#pragma unroll
-                            for (int k = 0; k < K; ++k) {
-                                if (act[k])
-                                    p *= keep;
-                                power[k] = p;
-                                if (k == lastSlot)
-                                    pl = p;
-                            }
-                            pl = laneValue(pl, lastLane);
-                            for (int i = 0; i < 5; ++i)
-                                nc[i] = fma(pl, d[i], t[i]);
+                            closedForm();
                         } else {
                             staged = true;
                             // The nearest flagged frame at or before each frame: the lane's last
@@ -688,13 +728,14 @@ __device__ __forceinline__ void voiceChain(const Span* __restrict__ spans, const
                             }
                             for (int i = 0; i < 3; ++i)
                                 carry.input[f][i] = ni[i];
-                            if (f < 2)
+                            if (f < 2) {
                                 carry.cents[f] = ni[3];
+                                carry.cutoffOf[f] = cv.filters[f].cutoff;
+                            }
                             carry.have[f] = 1;
                         }
-                        if (f < 2)
-                            centsValid |= 1 << f;
                         __builtin_amdgcn_wave_barrier();
+                        }
                     }
                     // 4. Biquad sections of this slot: s -> A s + d u per frame, with
                     // A = [[-a1, 1], [-a2, 0]] and d = (b1 - a1 b0, b2 - a2 b0).
