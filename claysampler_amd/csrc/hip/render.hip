@@ -404,16 +404,16 @@ __device__ __forceinline__ void voiceChain(const Span* __restrict__ spans, const
         const auto slotInputs = [&](int f, int routes, int frame, auto&& value, double& in0, double& in1,
                                     double& in2) {
             double cents = 0, db = 0, depth = cv.extra[0].depth, eq0 = 0, eq1 = 0, eq2 = 0;
+            // Every accumulator adds its share, 0 unless routed (choosing one of
+            // them by reference kept them in a stack array). Adding +0 changes no
+            // value: the sums start at +0 or a finite depth and never become -0.
             const auto add = [&](int route, double x) {
-                switch (route) {
-                case 1: cents += x; break;
-                case 2: db += x; break;
-                case 3: depth += x; break;
-                case 4: eq0 += x; break;
-                case 5: eq1 += x; break;
-                case 6: eq2 += x; break;
-                default: break;
-                }
+                cents += route == 1 ? x : 0.0;
+                db += route == 2 ? x : 0.0;
+                depth += route == 3 ? x : 0.0;
+                eq0 += route == 4 ? x : 0.0;
+                eq1 += route == 5 ? x : 0.0;
+                eq2 += route == 6 ? x : 0.0;
             };
             // Unrolled over the cursor slots so the cursors stay in registers.
 #pragma unroll
