@@ -138,13 +138,24 @@ __device__ __forceinline__ float2 interpolateAt(const int16_t* __restrict__ bank
     if (centre >= first && centre < v.sourceEnd - T / 2) {
         const int16_t* __restrict__ src = bank + v.sample + uint64_t(centre - first) * uint64_t(v.channels);
         if (v.channels == 1) {
+            // The window as dword-aligned loads (the bank allocation is padded for
+            // the one dword past its end), realigned by 16 bits on an odd start.
+            const uintptr_t addr = reinterpret_cast<uintptr_t>(src);
+            const uint32_t* __restrict__ words = reinterpret_cast<const uint32_t*>(addr & ~uintptr_t(3));
+            const uint32_t shift = (addr & 2) ? 16u : 0u;
+            uint32_t dw[T / 2 + 1];
+#pragma unroll
+            for (int i = 0; i < T / 2 + 1; ++i)
+                dw[i] = words[i];
 #pragma unroll
             for (int q = 0; q < T / 4; ++q) {
                 const float4 w = w4[q];
-                a0 = fmaf(w.x, float(src[4 * q + 0]), a0);
-                a1 = fmaf(w.y, float(src[4 * q + 1]), a1);
-                a0 = fmaf(w.z, float(src[4 * q + 2]), a0);
-                a1 = fmaf(w.w, float(src[4 * q + 3]), a1);
+                const uint32_t p0 = __builtin_amdgcn_alignbit(dw[2 * q + 1], dw[2 * q], shift);
+                const uint32_t p1 = __builtin_amdgcn_alignbit(dw[2 * q + 2], dw[2 * q + 1], shift);
+                a0 = fmaf(w.x, float(int16_t(p0 & 0xffffu)), a0);
+                a1 = fmaf(w.y, float(int16_t(p0 >> 16)), a1);
+                a0 = fmaf(w.z, float(int16_t(p1 & 0xffffu)), a0);
+                a1 = fmaf(w.w, float(int16_t(p1 >> 16)), a1);
             }
             const float m = (a0 + a1) * scale;
             return make_float2(m, m);
