@@ -30,26 +30,31 @@ extern "C" __global__ void clayScanStereo(const Job* __restrict__ jobs, int coun
         stats[block] = {power, peak, 0};
 }
 
-// Natural endpoint, silence-beyond-plan rule and peak repair, one lane per job
-// (src/cuda/tail.cu:83-145).
-extern "C" __global__ void claySelectNative(const Job* __restrict__ jobs, int count, const float2* __restrict__ mix,
-                                            const StereoStats* __restrict__ stats, clay_tail_result* __restrict__ results)
+// Natural endpoint, silence-beyond-plan rule and peak repair, one wave per job
+// (src/cuda/tail.cu:83-145). The reference walks the 1024-frame blocks in
+// order and stops at the first non-finite block or the first block that ends
+// the tail; here the lanes test the blocks in parallel, the first of each kind
+// is a wave minimum, and the peak is the maximum over the blocks the walk
+// would have visited.
+extern "C" __global__ void __launch_bounds__(64) claySelectNative(const Job* __restrict__ jobs, int count,
+                                                                  const float2* __restrict__ mix,
+                                                                  const StereoStats* __restrict__ stats,
+                                                                  clay_tail_result* __restrict__ results)
 {
-    const int index = blockIdx.x * blockDim.x + threadIdx.x;
+    const int index = int(blockIdx.x), lane = int(threadIdx.x);
     if (index >= count)
         return;
     const Job j = jobs[index];
-    clay_tail_result r{};
-    float peak = 0;
-    int end = 0;
-    for (int first = 0; first < j.frames; first += kTailBlock) {
-        const StereoStats st = stats[j.blockFirst + first / kTailBlock];
+    const int blocks = (j.frames + kTailBlock - 1) / kTailBlock;
+    constexpr int kNone = 0x7fffffff;
+    int firstBad = kNone, firstEnd = kNone;
+    for (int b = lane; b < blocks; b += 64) {
+        const StereoStats st = stats[j.blockFirst + b];
         if (!isfinite(st.peak)) {
-            r.status = CLAY_TAIL_NONFINITE;
-            results[index] = r;
-            return;
+            firstBad = b;
+            break;  // later blocks of this lane come after it
         }
-        peak = fmaxf(peak, st.peak);
+        const int first = b * kTailBlock;
         const int blockEnd = min(j.limit, first + kTailBlock);
         if (blockEnd < j.musicEnd)
             continue;
@@ -65,10 +70,27 @@ extern "C" __global__ void claySelectNative(const Job* __restrict__ jobs, int co
             }
         }
         if (power <= threshold) {
-            end = blockEnd;
+            firstEnd = b;
             break;
         }
     }
+    firstBad = waveReduce(firstBad, Min{});
+    firstEnd = waveReduce(firstEnd, Min{});
+    clay_tail_result r{};
+    if (firstBad < firstEnd) {
+        r.status = CLAY_TAIL_NONFINITE;
+        if (lane == 0)
+            results[index] = r;
+        return;
+    }
+    const int visited = firstEnd == kNone ? blocks : firstEnd + 1;
+    float peak = 0;
+    for (int b = lane; b < visited; b += 64)
+        peak = fmaxf(peak, stats[j.blockFirst + b].peak);
+    peak = waveReduce(peak, Max{});
+    if (lane)
+        return;
+    int end = firstEnd == kNone ? 0 : min(j.limit, firstEnd * kTailBlock + kTailBlock);
     if (!end) {
         const int silent = (j.frames + kTailBlock - 1) / kTailBlock * kTailBlock;
         if (silent < j.limit)
@@ -121,30 +143,36 @@ extern "C" __global__ void clayQuantizeMono(const Job* __restrict__ jobs, int co
         stats[block] = {power, peak, 0};
 }
 
-// Weak-tail trim, one lane per job (src/cuda/tail.cu:183-213).
-extern "C" __global__ void claySelectTrim(const Job* __restrict__ jobs, int count, const MonoStats* __restrict__ stats,
-                                          clay_tail_result* __restrict__ results, TailConfig cfg)
+// Weak-tail trim, one wave per job (src/cuda/tail.cu:183-213): the loudest
+// window and the last window above the threshold as wave reductions.
+extern "C" __global__ void __launch_bounds__(64) claySelectTrim(const Job* __restrict__ jobs, int count,
+                                                                const MonoStats* __restrict__ stats,
+                                                                clay_tail_result* __restrict__ results, TailConfig cfg)
 {
-    const int index = blockIdx.x * blockDim.x + threadIdx.x;
+    const int index = int(blockIdx.x), lane = int(threadIdx.x);
     if (index >= count || results[index].status != CLAY_TAIL_OK)
         return;
-    clay_tail_result& r = results[index];
+    const clay_tail_result r0 = results[index];
     const MonoStats* w = stats + jobs[index].windowFirst;
-    const int windows = (r.data_frames + cfg.window - 1) / cfg.window;
+    const int windows = (r0.data_frames + cfg.window - 1) / cfg.window;
     double maxPower = 0;
-    for (int i = 0; i < windows; ++i)
+    for (int i = lane; i < windows; i += 64)
         maxPower = fmax(maxPower, w[i].power);
+    maxPower = waveReduce(maxPower, Max{});
     const double threshold = fmin(cfg.ceiling, fmax(cfg.floor, sqrt(maxPower / cfg.window) * cfg.relative));
-    for (int i = windows - 1; i >= 0; --i) {
-        if (sqrt(w[i].power / cfg.window) > threshold || w[i].peak > threshold * 4) {
-            const int lastSound = min(r.frames, (i + 1) * cfg.window);
-            if (r.frames - lastSound >= cfg.quietFrames) {
-                r.frames = lastSound + cfg.padding;
-                r.data_frames = min(r.frames, r.data_frames);
-                r.fade = min(r.frames, cfg.fade);
-            }
-            break;
-        }
+    int last = -1;
+    for (int i = lane; i < windows; i += 64)
+        if (sqrt(w[i].power / cfg.window) > threshold || w[i].peak > threshold * 4)
+            last = i;
+    last = waveReduce(last, Max{});
+    if (lane || last < 0)
+        return;
+    clay_tail_result& r = results[index];
+    const int lastSound = min(r.frames, (last + 1) * cfg.window);
+    if (r.frames - lastSound >= cfg.quietFrames) {
+        r.frames = lastSound + cfg.padding;
+        r.data_frames = min(r.frames, r.data_frames);
+        r.fade = min(r.frames, cfg.fade);
     }
 }
 

@@ -1006,8 +1006,8 @@ void finishBatch(clay_hip* h, clay_hip_batch& b)
             });
         });
         h->launch("claySelectNative", [&] {
-            hipLaunchKernelGGL(claySelectNative, dim3(ceilDiv(count, lanes)), dim3(lanes), 0, s, b.dJobs, count, mix2,
-                               b.dStereo, b.dResults);
+            hipLaunchKernelGGL(claySelectNative, dim3(count), dim3(64), 0, s, b.dJobs, count, mix2, b.dStereo,
+                               b.dResults);
         });
         chunked(b.windows, [&](int first, int n) {
             h->launch("clayQuantizeMono", [&] {
@@ -1016,8 +1016,8 @@ void finishBatch(clay_hip* h, clay_hip_batch& b)
             });
         });
         h->launch("claySelectTrim", [&] {
-            hipLaunchKernelGGL(claySelectTrim, dim3(ceilDiv(count, lanes)), dim3(lanes), 0, s, b.dJobs, count, b.dWindows,
-                               b.dResults, h->tail);
+            hipLaunchKernelGGL(claySelectTrim, dim3(count), dim3(64), 0, s, b.dJobs, count, b.dWindows, b.dResults,
+                               h->tail);
         });
     }
     h->launch("clayLayoutPcm", [&] {
