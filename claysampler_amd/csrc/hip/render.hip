@@ -988,53 +988,108 @@ clayVoiceChains(const Span* __restrict__ spans, const SpanCold* __restrict__ col
                carries[wave], lane);
 }
 
-// Per output frame: the dry voices of the tile rendered inline, times the job
-// gain, plus the filtered voices from scratch (pan applied for mono). Voices
-// are summed in span order, so every execution form yields the same bits
-// (src/cuda/render.cu:458-523). Natural writes float2, orig/custom short2.
+// A tile voice as the mix loop reads it, staged in LDS once per tile so that
+// the per-voice loop reads no span record or voice state from memory.
+struct MixVoice {
+    int span;            // span index (a dry voice renders from the full record)
+    int begin, stop;     // frames the voice contributes: [begin, end), or up to a retired chain's silence
+    int wet;             // 1: filtered scratch; 0: dry, rendered inline
+    int channels, firstMod, modCount, panMods;  // panMods: 1 when a modulation binding targets pan
+    int64_t base;        // scratch index of frame f: base + f * channels
+    float left, right, pan, gain;
+};
+constexpr int kMixVoices = 64;  // voices staged per batch (more run in further batches)
+
+// Per output frame (one per thread of the tile's block): the dry voices of the
+// tile rendered inline, times the job gain, plus the filtered voices from
+// scratch (pan applied for mono). Voices are summed in span order, so every
+// execution form yields the same bits (src/cuda/render.cu:458-523). Natural
+// writes float2, orig/custom short2. Every thread of the block calls this
+// (block barriers stage the voice records).
 template<int T, bool FloatOut>
-__device__ __forceinline__ void mixFrame(const int16_t* __restrict__ bank, const float* __restrict__ sinc,
-                                         const Span* __restrict__ spans, const Job* __restrict__ jobs,
-                                         const int* __restrict__ voices, const clay_control_point* __restrict__ points,
-                                         const Mod* __restrict__ mods, const float* __restrict__ filtered,
-                                         const VoiceState* __restrict__ states, void* __restrict__ out, int segFirst,
-                                         int segEnd, const Tile& tile, int lane)
+__device__ __forceinline__ void mixTile(const int16_t* __restrict__ bank, const float* __restrict__ sinc,
+                                        const Span* __restrict__ spans, const Job* __restrict__ jobs,
+                                        const int* __restrict__ voices, const clay_control_point* __restrict__ points,
+                                        const Mod* __restrict__ mods, const float* __restrict__ filtered,
+                                        const VoiceState* __restrict__ states, void* __restrict__ out, int segFirst,
+                                        int segEnd, const Tile& tile, MixVoice* staged)
 {
+    const int lane = int(threadIdx.x);
     const int frame = tile.frame + lane;
-    if (lane >= tile.frames || frame < segFirst || frame >= segEnd)
-        return;
+    const bool on = lane < tile.frames && frame >= segFirst && frame < segEnd;
     const Job& j = jobs[uniform(tile.job)];
-    const float pan = controlAt(points, j.controls[2], j.controls[3], frame);
+    const float pan = on ? controlAt(points, j.controls[2], j.controls[3], frame) : 0.f;
     float left = 0, right = 0, wetLeft = 0, wetRight = 0;
-    for (int i = tile.first; i < tile.first + tile.count; ++i) {
-        const Span& v = spans[uniform(voices[i])];
-        if (frame < v.begin || frame >= v.end)
-            continue;
-        if (v.filterTrack >= 0) {
-            const VoiceState& st = states[v.filterTrack];
-            if (st.retired && frame >= st.silence)
-                continue;  // retired under the voice-silence policy
-            const float* src = filtered + v.filtered + uint64_t(frame - max(v.begin, segFirst)) * uint64_t(v.channels);
-            float2 s = make_float2(src[0], v.channels == 1 ? src[0] : src[1]);
-            if (v.channels == 1) {
-                float offset = pan;
-                for (int k = v.firstMod; k < v.firstMod + v.modCount; ++k) {
-                    const Mod m = mods[k];
-                    if (m.target == CLAY_MOD_PAN)
-                        offset += modulationAt(m, points, frame);
-                }
-                const float2 g = panGains(v, offset);
-                s.x *= g.x;
-                s.y *= g.y;
+    for (int b0 = tile.first; b0 < tile.first + tile.count; b0 += kMixVoices) {
+        const int n = min(kMixVoices, tile.first + tile.count - b0);
+        __syncthreads();  // the previous batch has been read
+        if (lane < n) {
+            const int s = voices[b0 + lane];
+            const Span& v = spans[s];
+            MixVoice m;
+            m.span = s;
+            m.begin = v.begin;
+            m.stop = v.end;
+            m.wet = v.filterTrack >= 0;
+            if (m.wet) {
+                const VoiceState& st = states[v.filterTrack];
+                if (st.retired)
+                    m.stop = min(m.stop, st.silence);  // retired under the voice-silence policy
             }
-            wetLeft += s.x;
-            wetRight += s.y;
-        } else {
-            const float2 s = voiceAudio<T>(bank, GlobalTaps<T>{sinc}, v, points, mods, pan, frame, false, -1.0);
-            left += s.x;
-            right += s.y;
+            m.channels = v.channels;
+            m.firstMod = v.firstMod;
+            m.modCount = v.modCount;
+            int panMods = 0;
+            for (int k = v.firstMod; k < v.firstMod + v.modCount; ++k)
+                panMods |= mods[k].target == CLAY_MOD_PAN;
+            m.panMods = panMods;
+            m.base = int64_t(v.filtered) - int64_t(max(v.begin, segFirst)) * v.channels;
+            m.left = v.left;
+            m.right = v.right;
+            m.pan = v.pan;
+            m.gain = v.gain;
+            staged[lane] = m;
+        }
+        __syncthreads();
+        if (!on)
+            continue;
+        for (int i = 0; i < n; ++i) {
+            const MixVoice& m = staged[i];
+            if (frame < m.begin || frame >= m.stop)
+                continue;
+            if (m.wet) {
+                const float* src = filtered + (m.base + int64_t(frame) * m.channels);
+                float2 s = make_float2(src[0], m.channels == 1 ? src[0] : src[1]);
+                if (m.channels == 1) {
+                    float offset = pan;
+                    if (m.panMods)
+                        for (int k = m.firstMod; k < m.firstMod + m.modCount; ++k) {
+                            const Mod md = mods[k];
+                            if (md.target == CLAY_MOD_PAN)
+                                offset += modulationAt(md, points, frame);
+                        }
+                    // panGains (src/cuda/render.cu:130-136)
+                    float gl = m.left, gr = m.right;
+                    if (offset != 0) {
+                        const float p = fminf(1.f, fmaxf(-1.f, m.pan + offset));
+                        gl = m.gain * cosf((1 + p) * kQuarterPi);
+                        gr = m.gain * cosf((1 - p) * kQuarterPi);
+                    }
+                    s.x *= gl;
+                    s.y *= gr;
+                }
+                wetLeft += s.x;
+                wetRight += s.y;
+            } else {
+                const Span& v = spans[uniform(m.span)];
+                const float2 s = voiceAudio<T>(bank, GlobalTaps<T>{sinc}, v, points, mods, pan, frame, false, -1.0);
+                left += s.x;
+                right += s.y;
+            }
         }
     }
+    if (!on)
+        return;
     const float amp = jobGain(points, j, frame);
     const float2 mixed = make_float2(fmaf(left, amp, wetLeft), fmaf(right, amp, wetRight));
     const uint64_t at = j.mix + uint64_t(frame);
@@ -1052,14 +1107,15 @@ template<int T, bool FloatOut>
 __global__ void __launch_bounds__(kTileFrames) __attribute__((amdgpu_waves_per_eu(kMixOccupancy)))
 clayRoundMixTiles(const int16_t* __restrict__ bank, const float* __restrict__ sinc, UnitsPtr units, RoundPtr round)
 {
+    __shared__ MixVoice staged[kMixVoices];
     const RoundHeader header = *round;
     for (int item = int(blockIdx.x); item < header.tiles; item += int(gridDim.x)) {
         const int unit = unitOf(header, item, [&](int i) { return units[i].tileBase; });
         const auto& u = constant(units)[unit];
         const Tile tile = global(u.tiles)[u.tileFirst + item - u.tileBase];
-        mixFrame<T, FloatOut>(bank, sinc, global(u.spans), global(u.jobs), global(u.voices), global(u.points),
-                              global(u.mods), global(u.filtered), global(u.states), global(u.mix), u.segFirst,
-                              u.segEnd, tile, int(threadIdx.x));
+        mixTile<T, FloatOut>(bank, sinc, global(u.spans), global(u.jobs), global(u.voices), global(u.points),
+                             global(u.mods), global(u.filtered), global(u.states), global(u.mix), u.segFirst, u.segEnd,
+                             tile, staged);
     }
 }
 
@@ -1071,9 +1127,10 @@ __global__ void __launch_bounds__(kTileFrames) __attribute__((amdgpu_waves_per_e
     const clay_control_point* __restrict__ points, const Mod* __restrict__ mods, const float* __restrict__ filtered,
     const VoiceState* __restrict__ states, void* __restrict__ out, int segFirst, int segEnd, int firstTile)
 {
+    __shared__ MixVoice staged[kMixVoices];
     const Tile tile = tiles[firstTile + int(blockIdx.x)];
-    mixFrame<T, FloatOut>(bank, sinc, spans, jobs, voices, points, mods, filtered, states, out, segFirst, segEnd,
-                          tile, int(threadIdx.x));
+    mixTile<T, FloatOut>(bank, sinc, spans, jobs, voices, points, mods, filtered, states, out, segFirst, segEnd, tile,
+                         staged);
 }
 
 #define CLAY_INSTANTIATE_TAPS(T)                                                                                       \
