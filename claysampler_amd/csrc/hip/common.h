@@ -98,22 +98,22 @@ template<int T>
 struct SharedTaps {
     static constexpr int kPitch = T + 4;
     static constexpr int kRows = kPhases / 2 + 1;
-    const float* table;
+    const float4* table;  // 16-byte aligned rows of kPitch floats (ds_read_b128)
     struct Row {
         const float4* w;
         bool mirror;
+        // One 16-byte read at the selected index, then the element order (two
+        // loads under a condition became per-element loads of selected addresses).
         __device__ __forceinline__ float4 operator[](int q) const
         {
-            if (!mirror)
-                return w[q];
-            const float4 r = w[T / 4 - 1 - q];
-            return make_float4(r.w, r.z, r.y, r.x);
+            const float4 r = w[mirror ? T / 4 - 1 - q : q];
+            return mirror ? make_float4(r.w, r.z, r.y, r.x) : r;
         }
     };
     __device__ __forceinline__ Row row(int phase) const
     {
         const bool mirror = phase > kPhases / 2;
-        return {reinterpret_cast<const float4*>(table + (mirror ? kPhases - phase : phase) * kPitch), mirror};
+        return {table + (mirror ? kPhases - phase : phase) * (kPitch / 4), mirror};
     }
 };
 

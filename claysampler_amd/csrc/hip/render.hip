@@ -895,7 +895,7 @@ __device__ __forceinline__ void withAudioTaps(const float* __restrict__ sinc, Bo
             table[row * (S::kPitch / 4) + q] = reinterpret_cast<const float4*>(sinc + size_t(row) * T)[q];
         }
         __syncthreads();
-        body(S{reinterpret_cast<const float*>(table)});
+        body(S{table});
     } else {
         body(GlobalTaps<T>{sinc});
     }
