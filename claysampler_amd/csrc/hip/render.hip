@@ -166,18 +166,29 @@ struct ChainCarry {
 constexpr int kRowShr1 = 0x111, kRowShr2 = 0x112, kRowShr4 = 0x114, kRowShr8 = 0x118;
 constexpr int kRowBcast15 = 0x142, kRowBcast31 = 0x143, kWaveShr1 = 0x138;
 
+// One 32-bit half. With every row enabled and a zero fill, the bound control
+// writes 0 where the source lane is missing, so no fill value has to be moved
+// into the destination first (the fills are constants at every call).
+template<int Ctrl, int RowMask>
+__device__ __forceinline__ int shiftHalf(int x, int fill)
+{
+    if (RowMask == 0xf && fill == 0)
+        return __builtin_amdgcn_mov_dpp(x, Ctrl, 0xf, 0xf, true);
+    return __builtin_amdgcn_update_dpp(fill, x, Ctrl, RowMask, 0xf, false);
+}
+
 template<int Ctrl, int RowMask = 0xf>
 __device__ __forceinline__ double shiftD(double x, double fill)
 {
-    const int lo = __builtin_amdgcn_update_dpp(__double2loint(fill), __double2loint(x), Ctrl, RowMask, 0xf, false);
-    const int hi = __builtin_amdgcn_update_dpp(__double2hiint(fill), __double2hiint(x), Ctrl, RowMask, 0xf, false);
+    const int lo = shiftHalf<Ctrl, RowMask>(__double2loint(x), __double2loint(fill));
+    const int hi = shiftHalf<Ctrl, RowMask>(__double2hiint(x), __double2hiint(fill));
     return __hiloint2double(hi, lo);
 }
 
 template<int Ctrl, int RowMask = 0xf>
 __device__ __forceinline__ int shiftI(int x, int fill)
 {
-    return __builtin_amdgcn_update_dpp(fill, x, Ctrl, RowMask, 0xf, false);
+    return shiftHalf<Ctrl, RowMask>(x, fill);
 }
 
 template<int C, int M>
