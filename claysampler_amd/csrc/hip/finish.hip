@@ -201,10 +201,13 @@ extern "C" __global__ void clayLayoutPcm(const Job* __restrict__ jobs, int count
 }
 
 // Fade, pack, peak, audible extent and clip count per 1024-frame block
-// (src/cuda/tail.cu:240-297). `channels` is 1 (natural) or 2.
-extern "C" __global__ void clayFinalizePcm(const Job* __restrict__ jobs, int count, const int16_t* __restrict__ input,
-                                           int16_t* __restrict__ output, clay_tail_result* __restrict__ results,
-                                           int channels, int countClipped, int firstBlock)
+// (src/cuda/tail.cu:240-297), one wave per block: the reductions are wave
+// shuffles, with no LDS round or block barrier. `channels` is 1 (natural) or 2.
+extern "C" __global__ void __launch_bounds__(64) clayFinalizePcm(const Job* __restrict__ jobs, int count,
+                                                                 const int16_t* __restrict__ input,
+                                                                 int16_t* __restrict__ output,
+                                                                 clay_tail_result* __restrict__ results, int channels,
+                                                                 int countClipped, int firstBlock)
 {
     const int block = firstBlock + blockIdx.x;
     const int index = ownerOf(count, block, [&](int i) { return jobs[i].blockFirst; });
@@ -237,11 +240,11 @@ extern "C" __global__ void clayFinalizePcm(const Job* __restrict__ jobs, int cou
             audibleLast = f;
         }
     }
-    peak = blockReduce(peak, 0, Max{});
-    audibleFirst = blockReduce(audibleFirst, frames, Min{});
-    audibleLast = blockReduce(audibleLast, -1, Max{});
+    peak = waveReduce(peak, Max{});
+    audibleFirst = waveReduce(audibleFirst, Min{});
+    audibleLast = waveReduce(audibleLast, Max{});
     if (countClipped) {
-        clipped = blockReduce(clipped, 0, Add{});
+        clipped = waveReduce(clipped, Add{});
         if (threadIdx.x == 0 && clipped)
             atomicAdd(&results[index].audio.clipped, clipped);
     }
