@@ -117,13 +117,15 @@ def cpu_baseline(sfz, mids, taps, jobs, threads):
                       f"{threads} threads, {dt:.1f} s"}
 
 
-def window_pass(policy, taps, form, batch_jobs, plans, bank, checksums=False, flags=0):
+def window_pass(policy, taps, form, batch_jobs, plans, bank, checksums=False, flags=0, statuses=None, **renderer):
     """One clay_hip_render_window over every plan. Returns (wall s, per-job
     checksums or None, stats). The publish callback only releases the PCM
-    (and optionally checksums it), so the time is the render core's."""
+    (and optionally checksums it), so the time is the render core's. Extra
+    keyword arguments go to ca.Renderer; `statuses`, a dict, receives each
+    job's tail status."""
     import ctypes as C
 
-    r = ca.Renderer(policy=policy, taps=taps, form=form, batch_jobs=batch_jobs, flags=flags)
+    r = ca.Renderer(policy=policy, taps=taps, form=form, batch_jobs=batch_jobs, flags=flags, **renderer)
     r.upload_bank(*bank)
     sums = {} if checksums else None
     published = [0]
@@ -138,6 +140,9 @@ def window_pass(policy, taps, form, batch_jobs, plans, bank, checksums=False, fl
                                                     C.POINTER(C.c_int16)), shape=(n,)) if n else np.zeros(0, np.int16)
                 sums[first + k] = (res.status, res.frames, res.data_frames, int(view.astype(np.int64).sum()),
                                    int((view.astype(np.int64) * np.arange(n, dtype=np.int64) % 65521).sum()))
+        if statuses is not None:
+            for k in range(b.result_count):
+                statuses[first + k] = abi.TAIL_STATUS[b.results[k].status]
         published[0] += max(1, b.result_count)
         if b.pcm_handle:
             r.lib.clay_hip_release_pcm(r.handle, b.pcm_handle)
@@ -152,6 +157,70 @@ def window_pass(policy, taps, form, batch_jobs, plans, bank, checksums=False, fl
     r.close()
     assert published[0] == len(plans)
     return wall, sums, stats
+
+
+def c5_main(args):
+    """Configuration 5: the dense stress workload (96 kHz, 64 taps, 2 GB bank,
+    volume groups of 8) through render_window, persistent form; with --forms
+    also the plain form and the PCM16 identity check."""
+    jobs = args.jobs or 32
+    seconds = args.seconds if args.seconds is not None else 240.0
+    t0 = time.perf_counter()
+    sfz, mids = synth.config5(os.path.join(args.synth_dir, "c5"), jobs, seconds)
+    log(f"inputs ready in {time.perf_counter() - t0:.1f} s")
+    t0 = time.perf_counter()
+    rate = 96000
+    session = ca.Session(rate=rate, measure=True)
+    for j, m in enumerate(mids[:jobs]):
+        session.add_job(sfz, m, group=j // 8 + 1)
+    plans = session.plans()
+    bank = session.bank()
+    frames = sum(p.frames for p in plans)
+    voice_frames = 0
+    for p in plans:
+        v = np.ctypeslib.as_array(p.spans, shape=(p.span_count,))
+        voice_frames += int((v["end"] - v["begin"]).astype(np.int64).sum())
+    log(f"planned {jobs} jobs, {sum(p.span_count for p in plans)} spans, {bank[1] * 2 / 1e9:.2f} GB bank "
+        f"in {time.perf_counter() - t0:.1f} s")
+    kw = dict(rate=rate, group_minimum=8)
+    warm = (abi.PlanView * min(2, len(plans)))(*plans[:2])
+    window_pass(abi.POLICY_NATURAL, 64, abi.FORM_PERSISTENT, args.batch_jobs, warm, bank, **kw)
+    log("warmup done")
+    statuses = {}
+    rolled, got, stats = window_pass(abi.POLICY_NATURAL, 64, abi.FORM_PERSISTENT, args.batch_jobs, plans, bank,
+                                     checksums=args.forms, statuses=statuses, **kw)
+    log(f"persistent: {rolled:.3f} s, {stats.rounds} rounds, segment {stats.segment_frames}")
+    line = {
+        "metric": "rendered audio-sec/sec and voice-samples/sec on 1 GPU; % of HBM roofline",
+        "value": frames / rate / rolled,
+        "unit": "audio-s/s",
+        "n_gpus": 1,
+        "steps": 1,
+        "warmup": 1,
+        "ms_per_step": rolled * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"config 5: {jobs} jobs x {seconds:g} s at 96 kHz, 64-tap sinc, CC64 pedal, 4 s "
+                               f"releases, {bank[1] * 2 / 1e9:.2f} GB bank, volume groups of 8, persistent form, "
+                               "whole window incl. finishing, group matching and PCM download",
+                   "jobs": jobs, "sinc_taps": 64, "tail_policy": "natural", "rate": rate, "form": "persistent"},
+        "audio_seconds": frames / rate,
+        "voice_samples_per_s": voice_frames / rolled,
+        "mean_voices": voice_frames / max(1, frames),
+        "statuses": sorted(set(statuses.values())),
+        "persistent": {"rounds": stats.rounds, "segment_frames": stats.segment_frames,
+                       "mix_blocks": stats.mix_blocks, "graph_launches": stats.graph_launches},
+    }
+    if args.forms:
+        plain, want, _ = window_pass(abi.POLICY_NATURAL, 64, abi.FORM_PLAIN, args.batch_jobs, plans, bank,
+                                     checksums=True, **kw)
+        line["plain_value"] = frames / rate / plain
+        line["identical_pcm16"] = want == got
+        log(f"plain: {plain:.3f} s, identical PCM16: {want == got}")
+    print(json.dumps(line), flush=True)
 
 
 def forms_main(args, jobs, sfz, mids, taps, plans, bank):
@@ -322,9 +391,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4"])
+    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4", "c5"])
     ap.add_argument("--jobs", type=int, default=None)
-    ap.add_argument("--seconds", type=float, default=60.0)
+    ap.add_argument("--seconds", type=float, default=None, help="job length (configuration 2: 60, 5: 240)")
     ap.add_argument("--synth-dir", default=os.environ.get("CLAY_SYNTH_DIR", "/tmp/clay_synth"))
     ap.add_argument("--cpu-jobs", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
@@ -341,6 +410,11 @@ def main():
     if args.config == "c4":
         predictor_main(args)
         return
+    if args.config == "c5":
+        c5_main(args)
+        return
+    if args.seconds is None:
+        args.seconds = 60.0
     default_jobs = {"c1": 1, "c2": 64, "c3": 512}[args.config]
     jobs = args.jobs or default_jobs
 

@@ -15,6 +15,16 @@ Choices the configurations leave open are fixed here:
       walks at 20 Hz (bend step U{-1024..1024} clipped to +-8191, CC1 step U{-6..6}).
   C3  C1 instrument; job length lognormal(ln 30 s, 1) clipped to [5, 300] s;
       polyphony P ~ U{4..64} from overlapping notes of U[0.5, 2.0] s at rate P/1.25 s.
+  C5  keys 21..108 x 4 velocity layers (1-31, 32-63, 64-95, 96-127) = 352
+      regions of 2,840,000 frames at 44.1 kHz (a 2.0 GB int16 bank). Each region
+      repeats one second of 8 harmonics (1/n, random phase, harmonics >= Nyquist
+      dropped) under a slow exp decay (tau 20 s), + -40 dB noise over the whole
+      region; no loop; volume -30 dB (hundreds of voices sum without clipping);
+      ampeg attack 5 ms, decay 2 s, sustain 40 %, release 4 s.
+      MIDI 240 s rendered at 96 kHz: Poisson onsets 24/s, key U{21..108},
+      velocity U{1..127}, duration U[0.2, 2.0] s, at most 256 sounding notes
+      (held, or sustained by the pedal); CC64 down for U[2, 6] s then up for
+      U[0.5, 1.5] s in turn. Jobs form volume groups of 8; 64 taps.
 """
 
 import math
@@ -229,5 +239,89 @@ def config3(directory, jobs=512, max_seconds=300.0):
         path = os.path.join(directory, f"c3_{j}.mid")
         if not os.path.exists(path):
             c3_midi(path, 3000 + j, seconds, polyphony)
+        mids.append(path)
+    return sfz, mids
+
+
+# ----------------------------------------------------------------- config 5
+
+C5_LAYERS = ((1, 31, 0.25), (32, 63, 0.4), (64, 95, 0.6), (96, 127, 0.85))
+
+
+def c5_instrument(directory, seed=5, frames=2_840_000, keys=(21, 108)):
+    """The dense stress instrument; `frames` and `keys` shrink it for tests.
+    One second of each region's tone is synthesised and repeated (phase jumps
+    at the seams are harmless for a stress bank), so the 2 GB bank is written
+    in seconds."""
+    os.makedirs(directory, exist_ok=True)
+    n = np.arange(RATE) / RATE
+    decay = np.exp(-np.arange(frames) / (20.0 * RATE))
+    lines = ["<group> loop_mode=no_loop volume=-30 ampeg_attack=0.005 ampeg_decay=2 ampeg_sustain=40 "
+             "ampeg_release=4\n"]
+    for key in range(keys[0], keys[1] + 1):
+        f0 = 440.0 * 2 ** ((key - 69) / 12)
+        for layer, (lo, hi, level) in enumerate(C5_LAYERS):
+            name = f"k{key}_v{layer}_{frames}.wav"
+            path = os.path.join(directory, name)
+            if not os.path.exists(path):
+                rng = SplitMix64(seed * 1000003 + key * 4 + layer)
+                second = np.zeros(RATE)
+                for h in range(1, 9):
+                    if f0 * h >= RATE / 2:
+                        break
+                    second += np.sin(2 * np.pi * f0 * h * n + 2 * np.pi * rng.uniform()) / h
+                second *= level / max(1e-9, np.max(np.abs(second)))
+                x = np.resize(second, frames) * decay
+                x += 0.01 * level * np.random.Generator(np.random.PCG64(rng.next())).uniform(-1.0, 1.0, frames)
+                write_wav(path, np.clip(np.rint(x * 32767), -32768, 32767))
+            lines.append(f"<region> sample={name} lokey={key} hikey={key} pitch_keycenter={key} lovel={lo} hivel={hi}\n")
+    sfz = os.path.join(directory, f"c5_{frames}_{keys[0]}_{keys[1]}.sfz")
+    with open(sfz, "w") as f:
+        f.writelines(lines)
+    return sfz
+
+
+def c5_midi(path, seed, seconds=240.0, rate_hz=24.0, max_sounding=256, keys=(21, 108)):
+    rng = SplitMix64(seed)
+    events = []
+    # Pedal: down for U[2, 6] s, then up for U[0.5, 1.5] s, in turn.
+    pedal, t, down = [], 0.0, True
+    while t < seconds:
+        length = 2.0 + 4.0 * rng.uniform() if down else 0.5 + rng.uniform()
+        pedal.append((t, min(seconds, t + length), down))
+        events.append((seconds_to_ticks(t), 0, 0xB0, 64, 127 if down else 0))
+        t += length
+        down = not down
+
+    def released(end):
+        """When a note released at `end` stops sustaining (the next pedal-up)."""
+        for a, b, d in pedal:
+            if d and a <= end < b:
+                return b
+        return end
+
+    sounding = []  # frames (seconds) when each note stops sounding
+    t = rng.exponential(rate_hz)
+    while t < seconds:
+        sounding = [e for e in sounding if e > t]
+        key, vel = rng.integer(keys[0], keys[1]), rng.integer(1, 127)
+        end = min(seconds, t + 0.2 + 1.8 * rng.uniform())
+        if len(sounding) < max_sounding and seconds_to_ticks(end) > seconds_to_ticks(t):
+            events.append((seconds_to_ticks(t), 2, 0x90, key, vel))
+            events.append((seconds_to_ticks(end), 1, 0x80, key, 0))
+            sounding.append(released(end) + 4.0)
+        t += rng.exponential(rate_hz)
+    write_midi(path, events)
+    return path
+
+
+def config5(directory, jobs=32, seconds=240.0, frames=2_840_000, keys=(21, 108)):
+    """(sfz, [midi]) for configuration 5; jobs j belong to volume group j // 8 + 1."""
+    sfz = c5_instrument(directory, frames=frames, keys=keys)
+    mids = []
+    for j in range(jobs):
+        path = os.path.join(directory, f"c5_{j}_{int(seconds)}_{keys[0]}_{keys[1]}.mid")
+        if not os.path.exists(path):
+            c5_midi(path, 5000 + j, seconds, keys=keys)
         mids.append(path)
     return sfz, mids
