@@ -229,13 +229,25 @@ __device__ __forceinline__ float2 interpolate(const int16_t* __restrict__ bank, 
     return s;
 }
 
+// The pan-law cosines of the last clamped pan position a lane evaluated:
+// consecutive voices of one frame usually share their pan, so the two cosf
+// calls run once per position instead of once per voice-frame.
+struct PanCache {
+    float p = __builtin_nanf(""), a = 0.f, b = 0.f;
+};
+
 // src/cuda/render.cu:130-136
-__device__ __forceinline__ float2 panGains(const Span& v, float offset)
+__device__ __forceinline__ float2 panGains(const Span& v, float offset, PanCache& cache)
 {
     if (offset == 0)
         return make_float2(v.left, v.right);
     const float p = fminf(1.f, fmaxf(-1.f, v.pan + offset));
-    return make_float2(v.gain * cosf((1 + p) * kQuarterPi), v.gain * cosf((1 - p) * kQuarterPi));
+    if (p != cache.p) {
+        cache.p = p;
+        cache.a = cosf((1 + p) * kQuarterPi);
+        cache.b = cosf((1 - p) * kQuarterPi);
+    }
+    return make_float2(v.gain * cache.a, v.gain * cache.b);
 }
 
 // Envelope, modulation, resampling, pan and stereo matrix of one voice frame
@@ -244,7 +256,7 @@ template<int T, class Taps>
 __device__ __forceinline__ float2 voiceAudio(const int16_t* __restrict__ bank, const Taps& taps, const Span& v,
                                              const clay_control_point* __restrict__ points,
                                              const Mod* __restrict__ mods, float pan, int frame, bool filterInput,
-                                             double phase)
+                                             double phase, PanCache& panCache)
 {
     float level = envelopeSpan(v, frame);
     if (level <= 0)
@@ -269,7 +281,7 @@ __device__ __forceinline__ float2 voiceAudio(const int16_t* __restrict__ bank, c
     const float2 s = interpolate<T>(bank, taps, v, frame, phase);
     if (filterInput && v.channels == 1)
         return make_float2(s.x * level, s.x * level);
-    const float2 g = panGains(v, pan);
+    const float2 g = panGains(v, pan, panCache);
     const float l = s.x * g.x, r = s.y * g.y;
     if (v.stereoMatrix) {
         float direct = v.widthDirect, cross = v.widthCross;

@@ -256,7 +256,8 @@ __device__ __forceinline__ void chainAudio(const int16_t* __restrict__ bank, con
             const int local = frame - first;
             const double ph = v.variablePitch ? phase[v.phase + uint64_t(local)] : -1.0;
             const float pan = controlAt(points, j.controls[2], j.controls[3], frame);
-            const float2 a = voiceAudio<T>(bank, taps, v, points, mods, pan, frame, true, ph);
+            PanCache panCache;  // unused: mono filter inputs are not panned here
+            const float2 a = voiceAudio<T>(bank, taps, v, points, mods, pan, frame, true, ph, panCache);
             const float g = jobGain(points, j, frame);
             float* dst = filtered + v.filtered + uint64_t(local) * uint64_t(v.channels);
             dst[0] = a.x * g;
@@ -1031,6 +1032,7 @@ __device__ __forceinline__ void mixTile(const int16_t* __restrict__ bank, const 
     const Job& j = jobs[uniform(tile.job)];
     const float pan = on ? controlAt(points, j.controls[2], j.controls[3], frame) : 0.f;
     float left = 0, right = 0, wetLeft = 0, wetRight = 0;
+    PanCache panCache;
     for (int b0 = tile.first; b0 < tile.first + tile.count; b0 += kMixVoices) {
         const int n = min(kMixVoices, tile.first + tile.count - b0);
         __syncthreads();  // the previous batch has been read
@@ -1083,8 +1085,13 @@ __device__ __forceinline__ void mixTile(const int16_t* __restrict__ bank, const 
                     float gl = m.left, gr = m.right;
                     if (offset != 0) {
                         const float p = fminf(1.f, fmaxf(-1.f, m.pan + offset));
-                        gl = m.gain * cosf((1 + p) * kQuarterPi);
-                        gr = m.gain * cosf((1 - p) * kQuarterPi);
+                        if (p != panCache.p) {
+                            panCache.p = p;
+                            panCache.a = cosf((1 + p) * kQuarterPi);
+                            panCache.b = cosf((1 - p) * kQuarterPi);
+                        }
+                        gl = m.gain * panCache.a;
+                        gr = m.gain * panCache.b;
                     }
                     s.x *= gl;
                     s.y *= gr;
@@ -1093,7 +1100,8 @@ __device__ __forceinline__ void mixTile(const int16_t* __restrict__ bank, const 
                 wetRight += s.y;
             } else {
                 const Span& v = spans[uniform(m.span)];
-                const float2 s = voiceAudio<T>(bank, GlobalTaps<T>{sinc}, v, points, mods, pan, frame, false, -1.0);
+                const float2 s =
+                    voiceAudio<T>(bank, GlobalTaps<T>{sinc}, v, points, mods, pan, frame, false, -1.0, panCache);
                 left += s.x;
                 right += s.y;
             }
