@@ -896,10 +896,15 @@ __device__ __forceinline__ void voiceChain(const Span* __restrict__ spans, const
 // half table staged in this block's LDS (148 KB at 32 taps, so one block per
 // CU), which turns the per-lane gathers of random 128-byte rows from L2 into
 // LDS reads; for wider tables the global table.
-template<int T, class Body>
-__device__ __forceinline__ void withAudioTaps(const float* __restrict__ sinc, Body&& body)
+template<int MaxShared>
+struct SharedUpTo {
+};
+
+template<int T, class Body, int MaxShared = kSharedTapsMax>
+__device__ __forceinline__ void withAudioTaps(const float* __restrict__ sinc, Body&& body,
+                                              SharedUpTo<MaxShared> = {})
 {
-    if constexpr (T <= kSharedTapsMax) {
+    if constexpr (T <= MaxShared) {
         using S = SharedTaps<T>;
         __shared__ float4 table[S::kRows * S::kPitch / 4];
         for (int i = int(threadIdx.x); i < S::kRows * (T / 4); i += int(blockDim.x)) {
@@ -1018,15 +1023,17 @@ constexpr int kMixVoices = 64;  // voices staged per batch (more run in further 
 // execution form yields the same bits (src/cuda/render.cu:458-523). Natural
 // writes float2, orig/custom short2. Every thread of the block calls this
 // (block barriers stage the voice records).
-template<int T, bool FloatOut>
-__device__ __forceinline__ void mixTile(const int16_t* __restrict__ bank, const float* __restrict__ sinc,
+// `lane` is the thread's frame within the tile and `sync` the barrier of the
+// threads that share `staged` (the block, or one wave).
+template<int T, bool FloatOut, class Taps, class Sync>
+__device__ __forceinline__ void mixTile(const int16_t* __restrict__ bank, const Taps& taps,
                                         const Span* __restrict__ spans, const Job* __restrict__ jobs,
                                         const int* __restrict__ voices, const clay_control_point* __restrict__ points,
                                         const Mod* __restrict__ mods, const float* __restrict__ filtered,
                                         const VoiceState* __restrict__ states, void* __restrict__ out, int segFirst,
-                                        int segEnd, const Tile& tile, MixVoice* staged)
+                                        int segEnd, const Tile& tile, MixVoice* staged, int lane, int stager,
+                                        Sync&& sync)
 {
-    const int lane = int(threadIdx.x);
     const int frame = tile.frame + lane;
     const bool on = lane < tile.frames && frame >= segFirst && frame < segEnd;
     const Job& j = jobs[uniform(tile.job)];
@@ -1035,9 +1042,9 @@ __device__ __forceinline__ void mixTile(const int16_t* __restrict__ bank, const 
     PanCache panCache;
     for (int b0 = tile.first; b0 < tile.first + tile.count; b0 += kMixVoices) {
         const int n = min(kMixVoices, tile.first + tile.count - b0);
-        __syncthreads();  // the previous batch has been read
-        if (lane < n) {
-            const int s = voices[b0 + lane];
+        sync();  // the previous batch has been read
+        if (stager < n) {
+            const int s = voices[b0 + stager];
             const Span& v = spans[s];
             MixVoice m;
             m.span = s;
@@ -1061,9 +1068,9 @@ __device__ __forceinline__ void mixTile(const int16_t* __restrict__ bank, const 
             m.right = v.right;
             m.pan = v.pan;
             m.gain = v.gain;
-            staged[lane] = m;
+            staged[stager] = m;
         }
-        __syncthreads();
+        sync();
         if (!on)
             continue;
         for (int i = 0; i < n; ++i) {
@@ -1100,8 +1107,7 @@ __device__ __forceinline__ void mixTile(const int16_t* __restrict__ bank, const 
                 wetRight += s.y;
             } else {
                 const Span& v = spans[uniform(m.span)];
-                const float2 s =
-                    voiceAudio<T>(bank, GlobalTaps<T>{sinc}, v, points, mods, pan, frame, false, -1.0, panCache);
+                const float2 s = voiceAudio<T>(bank, taps, v, points, mods, pan, frame, false, -1.0, panCache);
                 left += s.x;
                 right += s.y;
             }
@@ -1132,9 +1138,10 @@ clayRoundMixTiles(const int16_t* __restrict__ bank, const float* __restrict__ si
         const int unit = unitOf(header, item, [&](int i) { return units[i].tileBase; });
         const auto& u = constant(units)[unit];
         const Tile tile = global(u.tiles)[u.tileFirst + item - u.tileBase];
-        mixTile<T, FloatOut>(bank, sinc, global(u.spans), global(u.jobs), global(u.voices), global(u.points),
-                             global(u.mods), global(u.filtered), global(u.states), global(u.mix), u.segFirst, u.segEnd,
-                             tile, staged);
+        mixTile<T, FloatOut>(bank, GlobalTaps<T>{sinc}, global(u.spans), global(u.jobs), global(u.voices),
+                             global(u.points), global(u.mods), global(u.filtered), global(u.states), global(u.mix),
+                             u.segFirst, u.segEnd, tile, staged, int(threadIdx.x), int(threadIdx.x),
+                             [] { __syncthreads(); });
     }
 }
 
@@ -1148,8 +1155,50 @@ __global__ void __launch_bounds__(kTileFrames) __attribute__((amdgpu_waves_per_e
 {
     __shared__ MixVoice staged[kMixVoices];
     const Tile tile = tiles[firstTile + int(blockIdx.x)];
-    mixTile<T, FloatOut>(bank, sinc, spans, jobs, voices, points, mods, filtered, states, out, segFirst, segEnd, tile,
-                         staged);
+    mixTile<T, FloatOut>(bank, GlobalTaps<T>{sinc}, spans, jobs, voices, points, mods, filtered, states, out, segFirst,
+                         segEnd, tile, staged, int(threadIdx.x), int(threadIdx.x), [] { __syncthreads(); });
+}
+
+// Persistent-form mix with the sinc half table in LDS, for 16 taps (82 KB of
+// table next to 57 KB of voice staging): one 1024-thread block per CU, each
+// wave rendering a quarter tile (64 frames) at a time from voices it stages
+// itself, grid-stride, so the dry voices' FIR reads its weights from LDS
+// instead of gathering random rows from L2, with no block barrier after the
+// table load. The plain form keeps the 256-thread kernel: at one block per CU
+// (4 waves per SIMD) the dry FIR's bank gathers are less well hidden than at
+// the 8 waves per SIMD of that kernel, which a short window can afford to fill.
+constexpr int kMixSharedTaps = 16;
+constexpr int kMixWavesShared = 16;
+
+// Wave-level barrier for a wave's own staging area (LDS operations of one wave
+// complete in order; this keeps the compiler from moving them across).
+__device__ __forceinline__ void waveSync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template<int T, bool FloatOut>
+__global__ void __launch_bounds__(kMixWavesShared * 64) clayRoundMixTilesShared(
+    const int16_t* __restrict__ bank, const float* __restrict__ sinc, UnitsPtr units, RoundPtr round)
+{
+    __shared__ MixVoice staged[kMixWavesShared][kMixVoices];
+    const RoundHeader header = *round;
+    withAudioTaps<T>(sinc, [&](const auto& taps) {
+        const int wave = uniform(int(threadIdx.x) >> 6), lane = int(threadIdx.x) & 63;
+        const int quarters = header.tiles * (kTileFrames / 64);
+        for (int item = int(blockIdx.x) * kMixWavesShared + wave; item < quarters;
+             item += int(gridDim.x) * kMixWavesShared) {
+            const int t = item / (kTileFrames / 64), quarter = item % (kTileFrames / 64);
+            const int unit = unitOf(header, t, [&](int i) { return units[i].tileBase; });
+            const auto& u = constant(units)[unit];
+            const Tile tile = global(u.tiles)[u.tileFirst + t - u.tileBase];
+            mixTile<T, FloatOut>(bank, taps, global(u.spans), global(u.jobs), global(u.voices), global(u.points),
+                                 global(u.mods), global(u.filtered), global(u.states), global(u.mix), u.segFirst,
+                                 u.segEnd, tile, staged[wave], 64 * quarter + lane, lane, waveSync);
+        }
+    }, SharedUpTo<kMixSharedTaps>{});
 }
 
 #define CLAY_INSTANTIATE_TAPS(T)                                                                                       \
@@ -1167,5 +1216,7 @@ __global__ void __launch_bounds__(kTileFrames) __attribute__((amdgpu_waves_per_e
                                                     const float*, const VoiceState*, void*, int, int, int);
 CLAY_INSTANTIATE_TAPS(16)
 CLAY_INSTANTIATE_TAPS(32)
+template __global__ void clayRoundMixTilesShared<16, true>(const int16_t*, const float*, UnitsPtr, RoundPtr);
+template __global__ void clayRoundMixTilesShared<16, false>(const int16_t*, const float*, UnitsPtr, RoundPtr);
 CLAY_INSTANTIATE_TAPS(48)
 CLAY_INSTANTIATE_TAPS(64)

@@ -56,6 +56,9 @@ __global__ void clayRoundChainAudio(const int16_t*, const float*, const UnitView
 __global__ void clayRoundVoiceChains(const UnitView*, const RoundHeader*);
 template<int T, bool FloatOut>
 __global__ void clayRoundMixTiles(const int16_t*, const float*, const UnitView*, const RoundHeader*);
+// The same with the sinc half table in LDS (16 taps): one 1024-thread block per CU, a quarter tile per wave.
+template<int T, bool FloatOut>
+__global__ void clayRoundMixTilesShared(const int16_t*, const float*, const UnitView*, const RoundHeader*);
 // Finishing kernels (finish.hip).
 extern "C" __global__ void clayScanStereo(const Job*, int, const float2*, StereoStats*, int);
 extern "C" __global__ void claySelectNative(const Job*, int, const float2*, const StereoStats*, clay_tail_result*);
@@ -799,6 +802,8 @@ struct Shape {
 constexpr int kChainWavesPerBlock = 2;   // render.hip: voice chains per 128-thread filter block
 constexpr int kAudioWavesPerBlock = 16;  // render.hip: voice chains per 1024-thread audio block
 constexpr int kSharedTapsMax = 32;       // render.hip: widest sinc table the audio pass keeps in LDS
+constexpr int kMixSharedTaps = 16;       // render.hip: sinc width of the mix with the table in LDS
+constexpr int kMixWavesShared = 16;      // render.hip: waves (64-frame quarter tiles) per block of that mix
 
 // Persistent form: the round kernels with the fixed grids of `shape`.
 template<int T>
@@ -828,6 +833,16 @@ void launchRound(clay_hip* h, const Shape& shape, bool timed)
                            round);
     });
     run("clayRoundMixTiles", [&] {
+        if constexpr (T == kMixSharedTaps) {
+            const int blocks = std::max(1, h->props.multiProcessorCount);
+            if (h->natural)
+                hipLaunchKernelGGL((clayRoundMixTilesShared<T, true>), dim3(blocks), dim3(64 * kMixWavesShared),
+                                   0, s, h->bank, h->sinc, units, round);
+            else
+                hipLaunchKernelGGL((clayRoundMixTilesShared<T, false>), dim3(blocks),
+                                   dim3(64 * kMixWavesShared), 0, s, h->bank, h->sinc, units, round);
+            return;
+        }
         if (h->natural)
             hipLaunchKernelGGL((clayRoundMixTiles<T, true>), dim3(shape.mixBlocks), dim3(kTileFrames), 0, s, h->bank,
                                h->sinc, units, round);
