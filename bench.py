@@ -395,7 +395,8 @@ def main():
     ap.add_argument("--jobs", type=int, default=None)
     ap.add_argument("--seconds", type=float, default=None, help="job length (configuration 2: 60, 5: 240)")
     ap.add_argument("--synth-dir", default=os.environ.get("CLAY_SYNTH_DIR", "/tmp/clay_synth"))
-    ap.add_argument("--cpu-jobs", type=int, default=4)
+    ap.add_argument("--cpu-jobs", type=int, default=8)
+    ap.add_argument("--cpu-threads", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--forms", action="store_true",
                     help="plain vs persistent form through render_window (configuration 3)")
@@ -492,7 +493,8 @@ def main():
     batch.close()
     renderer.close()
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(sfz, mids, taps, min(args.cpu_jobs, jobs), min(args.cpu_jobs, jobs))
+        line["cpu_baseline"] = cpu_baseline(sfz, mids, taps, min(args.cpu_jobs, jobs),
+                                            min(args.cpu_threads, args.cpu_jobs, jobs))
     print(json.dumps(line), flush=True)
 
 
