@@ -99,21 +99,29 @@ struct SharedTaps {
     static constexpr int kPitch = T + 4;
     static constexpr int kRows = kPhases / 2 + 1;
     const float4* table;  // 16-byte aligned rows of kPitch floats (ds_read_b128)
+    // A lane walks its row forwards, or a mirrored row backwards from its last
+    // 16-byte group, and reverses the elements of each group with v_perm_b32
+    // (a byte select between two registers): branch-free, where a conditional
+    // element order compiled to a divergent branch per group.
     struct Row {
-        const float4* w;
-        bool mirror;
-        // One 16-byte read at the selected index, then the element order (two
-        // loads under a condition became per-element loads of selected addresses).
+        const float4* w;  // group 0 in this lane's tap order
+        int dir;          // +1, or -1 for a mirrored row
+        uint32_t sel;     // v_perm_b32 selector: 0x03020100 keeps an element, 0x07060504 takes its mirror
+        __device__ __forceinline__ float pick(float mirrored, float kept) const
+        {
+            return __uint_as_float(__builtin_amdgcn_perm(__float_as_uint(mirrored), __float_as_uint(kept), sel));
+        }
         __device__ __forceinline__ float4 operator[](int q) const
         {
-            const float4 r = w[mirror ? T / 4 - 1 - q : q];
-            return mirror ? make_float4(r.w, r.z, r.y, r.x) : r;
+            const float4 r = w[q * dir];
+            return make_float4(pick(r.w, r.x), pick(r.z, r.y), pick(r.y, r.z), pick(r.x, r.w));
         }
     };
     __device__ __forceinline__ Row row(int phase) const
     {
         const bool mirror = phase > kPhases / 2;
-        return {table + (mirror ? kPhases - phase : phase) * (kPitch / 4), mirror};
+        const float4* start = table + (mirror ? kPhases - phase : phase) * (kPitch / 4);
+        return {mirror ? start + (T / 4 - 1) : start, mirror ? -1 : 1, mirror ? 0x07060504u : 0x03020100u};
     }
 };
 
@@ -140,9 +148,11 @@ __device__ __forceinline__ float2 interpolateAt(const int16_t* __restrict__ bank
         if (v.channels == 1) {
             // The window as dword-aligned loads (the bank allocation is padded for
             // the one dword past its end), realigned by 16 bits on an odd start.
-            const uintptr_t addr = reinterpret_cast<uintptr_t>(src);
-            const uint32_t* __restrict__ words = reinterpret_cast<const uint32_t*>(addr & ~uintptr_t(3));
-            const uint32_t shift = (addr & 2) ? 16u : 0u;
+            // (Stepping back from `src` keeps the pointer's global address
+            // space, so these stay global loads rather than flat ones.)
+            const int odd = int(reinterpret_cast<uintptr_t>(src) >> 1) & 1;
+            const uint32_t* __restrict__ words = reinterpret_cast<const uint32_t*>(src - odd);
+            const uint32_t shift = odd ? 16u : 0u;
             uint32_t dw[T / 2 + 1];
 #pragma unroll
             for (int i = 0; i < T / 2 + 1; ++i)
@@ -189,15 +199,22 @@ __device__ __forceinline__ float2 interpolateAt(const int16_t* __restrict__ bank
 // Source position, generators, loop wrap and loop crossfade (src/cuda/render.cu:63-100).
 // The position arithmetic uses explicitly rounded double operations so that
 // the phase index equals the one of a plain IEEE double evaluation.
+// The denominator of a bent span's travel, expm1(slope), nonzero for a
+// nonzero slope: the same for every frame of the span, so the audio pass
+// evaluates it once per span and passes it as `bend`; a zero `bend` (the mix,
+// short of registers) evaluates it per frame.
+__device__ __forceinline__ double bendDenominator(const Span& v) { return v.pitchSlope != 0 ? expm1(v.pitchSlope) : 1.0; }
+
 template<int T, class Taps>
 __device__ __forceinline__ float2 interpolate(const int16_t* __restrict__ bank, const Taps& taps, const Span& v,
-                                              int frame, double pos)
+                                              int frame, double pos, double bend)
 {
     if (pos < 0) {
         const int elapsed = frame - v.begin;
         double distance = elapsed;
         if (v.pitchSlope != 0)
-            distance = __ddiv_rn(expm1(__dmul_rn(v.pitchSlope, double(elapsed))), expm1(v.pitchSlope));
+            distance = __ddiv_rn(expm1(__dmul_rn(v.pitchSlope, double(elapsed))),
+                                 bend != 0 ? bend : expm1(v.pitchSlope));
         pos = __dadd_rn(v.position, __dmul_rn(distance, v.step));
     }
     if (v.generator != CLAY_GEN_NONE) {
@@ -256,13 +273,14 @@ template<int T, class Taps>
 __device__ __forceinline__ float2 voiceAudio(const int16_t* __restrict__ bank, const Taps& taps, const Span& v,
                                              const clay_control_point* __restrict__ points,
                                              const Mod* __restrict__ mods, float pan, int frame, bool filterInput,
-                                             double phase, PanCache& panCache)
+                                             double phase, PanCache& panCache, double bend = 0.0,
+                                             bool audioMods = true)
 {
     float level = envelopeSpan(v, frame);
     if (level <= 0)
         return make_float2(0.f, 0.f);
     float db = 0, widthOffset = 0, positionOffset = 0;
-    for (int j = v.firstMod; j < v.firstMod + v.modCount; ++j) {
+    for (int j = v.firstMod; audioMods && j < v.firstMod + v.modCount; ++j) {
         const Mod m = mods[j];
         if (m.target > CLAY_MOD_POSITION)
             continue;
@@ -278,7 +296,7 @@ __device__ __forceinline__ float2 voiceAudio(const int16_t* __restrict__ bank, c
     }
     if (db != 0)
         level *= exp2f(db * .166096404744368f);
-    const float2 s = interpolate<T>(bank, taps, v, frame, phase);
+    const float2 s = interpolate<T>(bank, taps, v, frame, phase, bend);
     if (filterInput && v.channels == 1)
         return make_float2(s.x * level, s.x * level);
     const float2 g = panGains(v, pan, panCache);

@@ -252,12 +252,19 @@ __device__ __forceinline__ void chainAudio(const int16_t* __restrict__ bank, con
             continue;
         const Job& j = jobs[v.job];
         const int first = max(v.begin, segFirst), stop = min(v.end, segEnd);
+        // Whether any modulation reaches the voice audio (filter targets do
+        // not), so that frames skip the scan of the span's modulations.
+        bool audioMods = false;
+        for (int k = v.firstMod; k < v.firstMod + v.modCount; ++k)
+            audioMods = audioMods || mods[k].target <= CLAY_MOD_POSITION;
+        const double bend = bendDenominator(v);
         for (int frame = first + lane; frame < stop; frame += 64) {
             const int local = frame - first;
             const double ph = v.variablePitch ? phase[v.phase + uint64_t(local)] : -1.0;
-            const float pan = controlAt(points, j.controls[2], j.controls[3], frame);
+            // A mono filter input is not panned: only stereo voices read the pan.
+            const float pan = v.channels == 1 ? 0.f : controlAt(points, j.controls[2], j.controls[3], frame);
             PanCache panCache;  // unused: mono filter inputs are not panned here
-            const float2 a = voiceAudio<T>(bank, taps, v, points, mods, pan, frame, true, ph, panCache);
+            const float2 a = voiceAudio<T>(bank, taps, v, points, mods, pan, frame, true, ph, panCache, bend, audioMods);
             const float g = jobGain(points, j, frame);
             float* dst = filtered + v.filtered + uint64_t(local) * uint64_t(v.channels);
             dst[0] = a.x * g;
