@@ -546,11 +546,30 @@ struct CoeffD {
 };
 
 // src/cuda/render.cu:278-310 in double.
-__device__ inline CoeffD coefficientsD(int type, double cutoff, double resonance, int rate, double bandwidth,
+// The resonance factor 10^(-dB / 20) of the two-pole filters (dB clamped to
+// +-60), kept apart so that callers reuse it while the resonance repeats.
+__device__ __forceinline__ double resonanceGain(double resonance)
+{
+    return exp10(-fmin(60.0, fmax(-60.0, resonance)) / 20);
+}
+
+// 1 / x for x in [1, 2]: the hardware reciprocal refined by two Newton steps
+// (within an ulp of the quotient; the coefficients tolerate far more).
+__device__ __forceinline__ double reciprocalD(double x)
+{
+    double r = __builtin_amdgcn_rcp(x);
+    r = fma(fma(-x, r, 1.0), r, r);
+    return fma(fma(-x, r, 1.0), r, r);
+}
+
+// `gainOfResonance` is resonanceGain(resonance) for the two-pole types. The
+// angular frequency multiplies by 2 pi / rate (loop-invariant for callers)
+// instead of dividing by the rate.
+__device__ inline CoeffD coefficientsD(int type, double cutoff, double gainOfResonance, int rate, double bandwidth,
                                        double gain)
 {
     constexpr double pi = 3.14159265358979323846;
-    const double w = 2 * pi * fmin(fmin(20000.0, rate * .499), fmax(1.0, cutoff)) / rate;
+    const double w = fmin(fmin(20000.0, rate * .499), fmax(1.0, cutoff)) * (2 * pi / rate);
     if (type == CLAY_FILTER_LOW1 || type == CLAY_FILTER_HIGH1) {
         const double p = exp(-w);
         if (type == CLAY_FILTER_LOW1)
@@ -568,8 +587,8 @@ __device__ inline CoeffD coefficientsD(int type, double cutoff, double resonance
         const double mid = -2 * c * inv;
         return {(1 + alpha * a) * inv, mid, (1 - alpha * a) * inv, mid, (1 - alpha / a) * inv};
     }
-    const double alpha = .5 * s * exp10(-fmin(60.0, fmax(-60.0, resonance)) / 20);
-    const double inv = 1 / (1 + alpha);
+    const double alpha = .5 * s * gainOfResonance;
+    const double inv = reciprocalD(1 + alpha);
     const double a1 = -2 * c * inv, a2 = (1 - alpha) * inv;
     if (type == CLAY_FILTER_BAND2 || type == CLAY_FILTER_BAND4 || type == CLAY_FILTER_BAND6)
         return {alpha * inv, 0, -alpha * inv, a1, a2};

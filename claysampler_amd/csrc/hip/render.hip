@@ -581,6 +581,8 @@ __device__ __forceinline__ void voiceChain(const Span* __restrict__ spans, const
                         // last frame, which every lane has evaluated by then.
                         double first3[3] = {0, 0, 0}, prev[3] = {0, 0, 0}, last4[4] = {0, 0, 0, 0};
                         int flags = 0;
+                        // The resonance factor of the last flagged frame's resonance.
+                        double resonanceOf = __builtin_nan(""), resonanceFactor = 0;
 #pragma unroll 1
                         for (int k = 0; k <= K; ++k) {
                             const int slot = k == K ? 0 : k, at = base + slot;
@@ -602,7 +604,7 @@ __device__ __forceinline__ void voiceChain(const Span* __restrict__ spans, const
                                         __ballot(on && cents != carry.cents[f]) == 0)
                                         x[0] = carry.input[f][0];
                                     else if (on)
-                                        x[0] = double(cv.filters[f].cutoff) * exp2(cents / 1200);
+                                        x[0] = double(cv.filters[f].cutoff) * exp2(cents * (1.0 / 1200));
                                 }
                                 if (slot == lastSlot)
                                     last4[0] = x[0], last4[1] = x[1], last4[2] = x[2], last4[3] = cents;
@@ -631,7 +633,11 @@ __device__ __forceinline__ void voiceChain(const Span* __restrict__ spans, const
                             if (__ballot(flag) == 0)
                                 continue;
                             if (flag) {
-                                const CoeffD c = coefficientsD(cv.filters[f].type, x[0], f < 2 ? x[1] : 0, rate,
+                                if (f < 2 && x[1] != resonanceOf) {
+                                    resonanceOf = x[1];
+                                    resonanceFactor = resonanceGain(x[1]);
+                                }
+                                const CoeffD c = coefficientsD(cv.filters[f].type, x[0], resonanceFactor, rate,
                                                                f < 2 ? 0 : x[1], f < 2 ? 0 : x[2]);
                                 carry.stage[0][slot][lane] = c.b0;
                                 carry.stage[1][slot][lane] = c.b1;
