@@ -345,6 +345,10 @@ __device__ __forceinline__ bool silencePass(SilenceTrack& t, float level, int f0
 //   5. the filtered frames written back over the audio for the mix kernel.
 // Composing four frames per lane in registers leaves one wave scan per 256
 // frames. State carries across passes in LDS and across segments in VoiceState.
+// `Stereo` is the channel count of the chain's spans (all of one voice, so of
+// one sample), fixed at compile time so that mono chains hold no registers
+// for a right channel.
+template<bool Stereo>
 __device__ __forceinline__ void voiceChain(const Span* __restrict__ spans, const SpanCold* __restrict__ cold,
                                            const clay_control_point* __restrict__ points, const Mod* __restrict__ mods,
                                            float* __restrict__ filtered, VoiceState* __restrict__ states,
@@ -392,7 +396,7 @@ __device__ __forceinline__ void voiceChain(const Span* __restrict__ spans, const
             continue;
         const SpanCold& cv = cold[s];
         const int first = max(v.begin, segFirst), stop = min(v.end, segEnd);
-        const bool stereo = v.channels == 2;
+        constexpr bool stereo = Stereo;
         const int rate = cv.sampleRate;
         const double amount = 1 - exp(-1000.0 / rate), keep = 1 - amount;
         bool moving = cv.active[0] != 0;
@@ -996,9 +1000,16 @@ clayRoundVoiceChains(UnitsPtr units, RoundPtr round)
     for (int item = int(blockIdx.x) * kChainWaves + wave; item < header.chains; item += int(gridDim.x) * kChainWaves) {
         const int unit = unitOf(header, item, [&](int i) { return units[i].chainBase; });
         const auto& u = constant(units)[unit];
-        voiceChain(global(u.spans), global(u.cold), global(u.points), global(u.mods), global(u.filtered),
-                   global(u.states), u.segFirst, u.segEnd, uniform(global(u.chainHeads)[item - u.chainBase]),
-                   header.silence, carries[wave], lane);
+        const int head = uniform(global(u.chainHeads)[item - u.chainBase]);
+        const auto run = [&](auto stereo) {
+            voiceChain<decltype(stereo)::value>(global(u.spans), global(u.cold), global(u.points), global(u.mods),
+                                                global(u.filtered), global(u.states), u.segFirst, u.segEnd, head,
+                                                header.silence, carries[wave], lane);
+        };
+        if (global(u.spans)[head].channels == 2)
+            run(std::true_type{});
+        else
+            run(std::false_type{});
     }
 }
 
@@ -1014,8 +1025,13 @@ clayVoiceChains(const Span* __restrict__ spans, const SpanCold* __restrict__ col
     const int item = int(blockIdx.x) * kChainWaves + wave;
     if (item >= count)
         return;  // uniform per wave; no block barriers below
-    voiceChain(spans, cold, points, mods, filtered, states, segFirst, segEnd, uniform(heads[item]), silenceLevel,
-               carries[wave], lane);
+    const int head = uniform(heads[item]);
+    if (spans[head].channels == 2)
+        voiceChain<true>(spans, cold, points, mods, filtered, states, segFirst, segEnd, head, silenceLevel,
+                         carries[wave], lane);
+    else
+        voiceChain<false>(spans, cold, points, mods, filtered, states, segFirst, segEnd, head, silenceLevel,
+                          carries[wave], lane);
 }
 
 // A tile voice as the mix loop reads it, staged in LDS once per tile so that
