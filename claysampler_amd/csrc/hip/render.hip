@@ -347,8 +347,9 @@ __device__ __forceinline__ bool silencePass(SilenceTrack& t, float level, int f0
 // frames. State carries across passes in LDS and across segments in VoiceState.
 // `Stereo` is the channel count of the chain's spans (all of one voice, so of
 // one sample), fixed at compile time so that mono chains hold no registers
-// for a right channel.
-template<bool Stereo>
+// for a right channel. `Single`: the chain's only filter is one biquad
+// section in slot 0 (the common case), so the slot and section loops vanish.
+template<bool Stereo, bool Single>
 __device__ __forceinline__ void voiceChain(const Span* __restrict__ spans, const SpanCold* __restrict__ cold,
                                            const clay_control_point* __restrict__ points, const Mod* __restrict__ mods,
                                            float* __restrict__ filtered, VoiceState* __restrict__ states,
@@ -491,8 +492,8 @@ __device__ __forceinline__ void voiceChain(const Span* __restrict__ spans, const
                     ur[k] = outR[k];
                 }
                 int sec = 0;
-                for (int f = 0; f < CLAY_FILTER_SLOTS; ++f) {
-                    const int slotSections = sections(cold[head].filters[f].type);
+                for (int f = 0; f < (Single ? 1 : CLAY_FILTER_SLOTS); ++f) {
+                    const int slotSections = Single ? 1 : sections(cold[head].filters[f].type);
                     if (!slotSections)
                         continue;
                     // Coefficients (b0, b1, b2, a1, a2) of the lane's frame k: t + power[k] d
@@ -989,6 +990,28 @@ __global__ void __launch_bounds__(64 * kAudioWaves) clayChainAudio(
     });
 }
 
+// Calls body(stereo, single) with the compile-time variant of the chain that
+// starts at span `head` (the filter slots are the head span's, as voiceChain
+// reads them).
+template<class Body>
+__device__ __forceinline__ void chainVariant(const Span* __restrict__ spans, const SpanCold* __restrict__ cold, int head,
+                                             Body&& body)
+{
+    bool single = sections(cold[head].filters[0].type) == 1;
+    for (int f = 1; f < CLAY_FILTER_SLOTS; ++f)
+        single = single && sections(cold[head].filters[f].type) == 0;
+    if (spans[head].channels == 2) {
+        if (single)
+            body(std::true_type{}, std::true_type{});
+        else
+            body(std::true_type{}, std::false_type{});
+    } else if (single) {
+        body(std::false_type{}, std::true_type{});
+    } else {
+        body(std::false_type{}, std::false_type{});
+    }
+}
+
 // Filter pass: one wave per voice chain of the round, grid-stride; 2 chains per block.
 __global__ void __launch_bounds__(64 * kChainWaves) __attribute__((amdgpu_waves_per_eu(kChainOccupancy)))
 clayRoundVoiceChains(UnitsPtr units, RoundPtr round)
@@ -1001,15 +1024,11 @@ clayRoundVoiceChains(UnitsPtr units, RoundPtr round)
         const int unit = unitOf(header, item, [&](int i) { return units[i].chainBase; });
         const auto& u = constant(units)[unit];
         const int head = uniform(global(u.chainHeads)[item - u.chainBase]);
-        const auto run = [&](auto stereo) {
-            voiceChain<decltype(stereo)::value>(global(u.spans), global(u.cold), global(u.points), global(u.mods),
-                                                global(u.filtered), global(u.states), u.segFirst, u.segEnd, head,
-                                                header.silence, carries[wave], lane);
-        };
-        if (global(u.spans)[head].channels == 2)
-            run(std::true_type{});
-        else
-            run(std::false_type{});
+        chainVariant(global(u.spans), global(u.cold), head, [&](auto stereo, auto single) {
+            voiceChain<decltype(stereo)::value, decltype(single)::value>(
+                global(u.spans), global(u.cold), global(u.points), global(u.mods), global(u.filtered),
+                global(u.states), u.segFirst, u.segEnd, head, header.silence, carries[wave], lane);
+        });
     }
 }
 
@@ -1026,12 +1045,11 @@ clayVoiceChains(const Span* __restrict__ spans, const SpanCold* __restrict__ col
     if (item >= count)
         return;  // uniform per wave; no block barriers below
     const int head = uniform(heads[item]);
-    if (spans[head].channels == 2)
-        voiceChain<true>(spans, cold, points, mods, filtered, states, segFirst, segEnd, head, silenceLevel,
-                         carries[wave], lane);
-    else
-        voiceChain<false>(spans, cold, points, mods, filtered, states, segFirst, segEnd, head, silenceLevel,
-                          carries[wave], lane);
+    chainVariant(spans, cold, head, [&](auto stereo, auto single) {
+        voiceChain<decltype(stereo)::value, decltype(single)::value>(spans, cold, points, mods, filtered, states,
+                                                                     segFirst, segEnd, head, silenceLevel,
+                                                                     carries[wave], lane);
+    });
 }
 
 // A tile voice as the mix loop reads it, staged in LDS once per tile so that
