@@ -128,7 +128,8 @@ struct SharedTaps {
 // Windowed-sinc tap sum at the nearest of 2048 phases (src/cuda/render.cu:30-61).
 // Accumulating the raw int16 values and scaling once by 2^-15 gives the same
 // float result as weighting the pre-scaled float bank of the reference.
-template<int T, class Taps>
+// `Plain`: the caller knows the voice is mono (see voiceAudio).
+template<int T, bool Plain = false, class Taps>
 __device__ __forceinline__ float2 interpolateAt(const int16_t* __restrict__ bank, const Taps& taps, const Span& v,
                                                 double pos)
 {
@@ -145,7 +146,7 @@ __device__ __forceinline__ float2 interpolateAt(const int16_t* __restrict__ bank
     float a0 = 0.f, a1 = 0.f, b0 = 0.f, b1 = 0.f;
     if (centre >= first && centre < v.sourceEnd - T / 2) {
         const int16_t* __restrict__ src = bank + v.sample + uint64_t(centre - first) * uint64_t(v.channels);
-        if (v.channels == 1) {
+        if (Plain || v.channels == 1) {
             // The window as dword-aligned loads (the bank allocation is padded for
             // the one dword past its end), realigned by 16 bits on an odd start.
             // (Stepping back from `src` keeps the pointer's global address
@@ -205,11 +206,11 @@ __device__ __forceinline__ float2 interpolateAt(const int16_t* __restrict__ bank
 // short of registers) evaluates it per frame.
 __device__ __forceinline__ double bendDenominator(const Span& v) { return v.pitchSlope != 0 ? expm1(v.pitchSlope) : 1.0; }
 
-template<int T, class Taps>
+template<int T, bool Plain = false, class Taps>
 __device__ __forceinline__ float2 interpolate(const int16_t* __restrict__ bank, const Taps& taps, const Span& v,
                                               int frame, double pos, double bend)
 {
-    if (pos < 0) {
+    if (Plain || pos < 0) {
         const int elapsed = frame - v.begin;
         double distance = elapsed;
         if (v.pitchSlope != 0)
@@ -217,7 +218,7 @@ __device__ __forceinline__ float2 interpolate(const int16_t* __restrict__ bank, 
                                  bend != 0 ? bend : expm1(v.pitchSlope));
         pos = __dadd_rn(v.position, __dmul_rn(distance, v.step));
     }
-    if (v.generator != CLAY_GEN_NONE) {
+    if (!Plain && v.generator != CLAY_GEN_NONE) {
         if (v.generator == CLAY_GEN_NOISE) {
             unsigned bits = v.noiseSeed + unsigned(frame - v.noiseStart);
             bits = (bits ^ (bits >> 16)) * 0x7feb352du;
@@ -234,10 +235,10 @@ __device__ __forceinline__ float2 interpolate(const int16_t* __restrict__ bank, 
         pos = __dadd_rn(double(v.loopFirst), fmod(__dsub_rn(pos, double(v.loopFirst)), double(period)));
     if (pos >= v.sourceEnd)
         return make_float2(0.f, 0.f);
-    float2 s = interpolateAt<T>(bank, taps, v, pos);
+    float2 s = interpolateAt<T, Plain>(bank, taps, v, pos);
     const int fadeStart = v.loopEnd - v.loopCrossfade;
     if (v.loopEnd > 0 && v.loopCrossfade > 0 && pos >= fadeStart) {
-        const float2 entry = interpolateAt<T>(bank, taps, v, __dsub_rn(pos, double(period)));
+        const float2 entry = interpolateAt<T, Plain>(bank, taps, v, __dsub_rn(pos, double(period)));
         const float phase = float((pos - fadeStart) / v.loopCrossfade);
         const float blend = .5f - .5f * cosf(3.14159265358979323846f * phase);
         s.x = fmaf(blend, entry.x - s.x, s.x);
@@ -268,8 +269,10 @@ __device__ __forceinline__ float2 panGains(const Span& v, float offset, PanCache
 }
 
 // Envelope, modulation, resampling, pan and stereo matrix of one voice frame
-// (src/cuda/render.cu:138-187).
-template<int T, class Taps>
+// (src/cuda/render.cu:138-187). `Plain` compiles the common voice only: mono,
+// sampled (no generator), its position from the span (no pitch chain) and no
+// modulation that reaches the audio; the caller checks this per span.
+template<int T, bool Plain = false, class Taps>
 __device__ __forceinline__ float2 voiceAudio(const int16_t* __restrict__ bank, const Taps& taps, const Span& v,
                                              const clay_control_point* __restrict__ points,
                                              const Mod* __restrict__ mods, float pan, int frame, bool filterInput,
@@ -280,7 +283,7 @@ __device__ __forceinline__ float2 voiceAudio(const int16_t* __restrict__ bank, c
     if (level <= 0)
         return make_float2(0.f, 0.f);
     float db = 0, widthOffset = 0, positionOffset = 0;
-    for (int j = v.firstMod; audioMods && j < v.firstMod + v.modCount; ++j) {
+    for (int j = v.firstMod; !Plain && audioMods && j < v.firstMod + v.modCount; ++j) {
         const Mod m = mods[j];
         if (m.target > CLAY_MOD_POSITION)
             continue;
@@ -296,8 +299,8 @@ __device__ __forceinline__ float2 voiceAudio(const int16_t* __restrict__ bank, c
     }
     if (db != 0)
         level *= exp2f(db * .166096404744368f);
-    const float2 s = interpolate<T>(bank, taps, v, frame, phase, bend);
-    if (filterInput && v.channels == 1)
+    const float2 s = interpolate<T, Plain>(bank, taps, v, frame, phase, bend);
+    if (filterInput && (Plain || v.channels == 1))
         return make_float2(s.x * level, s.x * level);
     const float2 g = panGains(v, pan, panCache);
     const float l = s.x * g.x, r = s.y * g.y;

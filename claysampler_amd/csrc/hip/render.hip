@@ -258,19 +258,30 @@ __device__ __forceinline__ void chainAudio(const int16_t* __restrict__ bank, con
         for (int k = v.firstMod; k < v.firstMod + v.modCount; ++k)
             audioMods = audioMods || mods[k].target <= CLAY_MOD_POSITION;
         const double bend = bendDenominator(v);
-        for (int frame = first + lane; frame < stop; frame += 64) {
-            const int local = frame - first;
-            const double ph = v.variablePitch ? phase[v.phase + uint64_t(local)] : -1.0;
-            // A mono filter input is not panned: only stereo voices read the pan.
-            const float pan = v.channels == 1 ? 0.f : controlAt(points, j.controls[2], j.controls[3], frame);
-            PanCache panCache;  // unused: mono filter inputs are not panned here
-            const float2 a = voiceAudio<T>(bank, taps, v, points, mods, pan, frame, true, ph, panCache, bend, audioMods);
-            const float g = jobGain(points, j, frame);
-            float* dst = filtered + v.filtered + uint64_t(local) * uint64_t(v.channels);
-            dst[0] = a.x * g;
-            if (v.channels == 2)
-                dst[1] = a.y * g;
-        }
+        // The frame loop compiled twice: for the common voice (voiceAudio's
+        // `Plain`), and for any other.
+        const auto frames = [&](auto plainTag) {
+            constexpr bool plain = decltype(plainTag)::value;
+            for (int frame = first + lane; frame < stop; frame += 64) {
+                const int local = frame - first;
+                const double ph = !plain && v.variablePitch ? phase[v.phase + uint64_t(local)] : -1.0;
+                // A mono filter input is not panned: only stereo voices read the pan.
+                const float pan =
+                    plain || v.channels == 1 ? 0.f : controlAt(points, j.controls[2], j.controls[3], frame);
+                PanCache panCache;  // unused: mono filter inputs are not panned here
+                const float2 a =
+                    voiceAudio<T, plain>(bank, taps, v, points, mods, pan, frame, true, ph, panCache, bend, audioMods);
+                const float g = jobGain(points, j, frame);
+                float* dst = filtered + v.filtered + uint64_t(local) * uint64_t(plain ? 1 : v.channels);
+                dst[0] = a.x * g;
+                if (!plain && v.channels == 2)
+                    dst[1] = a.y * g;
+            }
+        };
+        if (v.channels == 1 && v.generator == CLAY_GEN_NONE && !v.variablePitch && !audioMods)
+            frames(std::true_type{});
+        else
+            frames(std::false_type{});
     }
 }
 
