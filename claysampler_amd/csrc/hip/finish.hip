@@ -201,58 +201,95 @@ extern "C" __global__ void clayLayoutPcm(const Job* __restrict__ jobs, int count
 }
 
 // Fade, pack, peak, audible extent and clip count per 1024-frame block
-// (src/cuda/tail.cu:240-297), one wave per block: the reductions are wave
-// shuffles, with no LDS round or block barrier. `channels` is 1 (natural) or 2.
+// (src/cuda/tail.cu:240-297). One wave takes kFinalizeBlocks consecutive
+// blocks (`blocks` in all from `firstBlock`), so the owner search, the job and
+// result loads and the atomics are paid once per run of blocks of one job; the
+// reductions are wave shuffles, with no LDS round or block barrier. `channels`
+// is 1 (natural) or 2. In place (orig/custom) the output offset of a job equals
+// its mix offset, so every frame is read and written at one address.
 extern "C" __global__ void __launch_bounds__(64) clayFinalizePcm(const Job* __restrict__ jobs, int count,
                                                                  const int16_t* __restrict__ input,
                                                                  int16_t* __restrict__ output,
                                                                  clay_tail_result* __restrict__ results, int channels,
-                                                                 int countClipped, int firstBlock)
+                                                                 int countClipped, int firstBlock, int blocks)
 {
-    const int block = firstBlock + blockIdx.x;
-    const int index = ownerOf(count, block, [&](int i) { return jobs[i].blockFirst; });
-    const Job j = jobs[index];
-    const uint64_t outAt = results[index].offset;
-    const int frames = results[index].frames, fade = results[index].fade;
-    const int first = (block - j.blockFirst) * kTailBlock;
-    const int end = min(first + kTailBlock, results[index].data_frames);
-    if (first >= end)
+    const int begin = firstBlock + int(blockIdx.x) * kFinalizeBlocks;
+    const int stopBlock = min(begin + kFinalizeBlocks, firstBlock + blocks);
+    if (begin >= stopBlock)
         return;
-    int peak = 0, audibleFirst = frames, audibleLast = -1, clipped = 0;
-    const int fadeStart = frames - fade;
+    int index = ownerOf(count, begin, [&](int i) { return jobs[i].blockFirst; });
+    int peak = 0, audibleFirst = 0x7fffffff, audibleLast = -1, clipped = 0;
+    // The run's reductions into the job's result, once per job.
+    const auto flush = [&] {
+        peak = waveReduce(peak, Max{});
+        audibleFirst = waveReduce(audibleFirst, Min{});
+        audibleLast = waveReduce(audibleLast, Max{});
+        if (countClipped) {
+            clipped = waveReduce(clipped, Add{});
+            if (threadIdx.x == 0 && clipped)
+                atomicAdd(&results[index].audio.clipped, clipped);
+        }
+        if (threadIdx.x == 0 && audibleLast >= 0) {
+            atomicMax(&results[index].peak, peak);
+            atomicMin(&results[index].first, audibleFirst);
+            atomicMax(&results[index].last, audibleLast);
+        }
+        peak = 0, audibleFirst = 0x7fffffff, audibleLast = -1, clipped = 0;
+    };
     constexpr double pi = 3.14159265358979323846;
-    for (int f = first + int(threadIdx.x); f < end; f += blockDim.x) {
-        int framePeak = 0;
-        for (int c = 0; c < channels; ++c) {
-            int v = input[(j.mix + uint64_t(f)) * uint64_t(channels) + uint64_t(c)];
-            if (f >= fadeStart) {
-                const int i = f - fadeStart;
-                const double gain = fade == 1 ? 0 : .5 * (1 + cos(pi * i / (fade - 1)));
-                v = int16_t(__double2int_rn(v * gain));
+    constexpr int kPer = kTailBlock / 64;
+    for (int block = begin; block < stopBlock; ++block) {
+        if (index + 1 < count && jobs[index + 1].blockFirst <= block) {
+            flush();
+            do
+                ++index;
+            while (index + 1 < count && jobs[index + 1].blockFirst <= block);
+        }
+        const Job j = jobs[index];
+        const uint64_t outAt = results[index].offset;
+        const int frames = results[index].frames, fade = results[index].fade;
+        const int first = (block - j.blockFirst) * kTailBlock;
+        const int end = min(first + kTailBlock, results[index].data_frames);
+        const int fadeStart = frames - fade;
+        // Every load of the lane's frames is issued before any is used (one
+        // 128-byte wave load at a time left the kernel bound by latency).
+        int16_t in[kPer][2];
+#pragma unroll
+        for (int i = 0; i < kPer; ++i) {
+            const int f = first + int(threadIdx.x) + 64 * i;
+#pragma unroll
+            for (int c = 0; c < 2; ++c)
+                in[i][c] =
+                    f < end && c < channels ? input[(j.mix + uint64_t(f)) * uint64_t(channels) + uint64_t(c)] : 0;
+        }
+#pragma unroll
+        for (int i = 0; i < kPer; ++i) {
+            const int f = first + int(threadIdx.x) + 64 * i;
+            if (f >= end)
+                break;
+            int framePeak = 0;
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
+                if (c >= channels)
+                    break;
+                int v = in[i][c];
+                if (f >= fadeStart) {
+                    const int k = f - fadeStart;
+                    const double gain = fade == 1 ? 0 : .5 * (1 + cos(pi * k / (fade - 1)));
+                    v = int16_t(__double2int_rn(v * gain));
+                }
+                output[(outAt + uint64_t(f)) * uint64_t(channels) + uint64_t(c)] = int16_t(v);
+                framePeak = max(framePeak, abs(v));
+                clipped += abs(v) >= 32767;
             }
-            output[(outAt + uint64_t(f)) * uint64_t(channels) + uint64_t(c)] = int16_t(v);
-            framePeak = max(framePeak, abs(v));
-            clipped += abs(v) >= 32767;
-        }
-        peak = max(peak, framePeak);
-        if (framePeak > 3) {
-            audibleFirst = min(audibleFirst, f);
-            audibleLast = f;
+            peak = max(peak, framePeak);
+            if (framePeak > 3) {
+                audibleFirst = min(audibleFirst, f);
+                audibleLast = f;
+            }
         }
     }
-    peak = waveReduce(peak, Max{});
-    audibleFirst = waveReduce(audibleFirst, Min{});
-    audibleLast = waveReduce(audibleLast, Max{});
-    if (countClipped) {
-        clipped = waveReduce(clipped, Add{});
-        if (threadIdx.x == 0 && clipped)
-            atomicAdd(&results[index].audio.clipped, clipped);
-    }
-    if (threadIdx.x == 0 && audibleLast >= 0) {
-        atomicMax(&results[index].peak, peak);
-        atomicMin(&results[index].first, audibleFirst);
-        atomicMax(&results[index].last, audibleLast);
-    }
+    flush();
 }
 
 // ====================================================================== quality

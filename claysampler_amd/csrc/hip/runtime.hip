@@ -66,7 +66,8 @@ extern "C" __global__ void clayQuantizeMono(const Job*, int, const float2*, cons
                                             MonoStats*, int, int);
 extern "C" __global__ void claySelectTrim(const Job*, int, const MonoStats*, clay_tail_result*, TailConfig);
 extern "C" __global__ void clayLayoutPcm(const Job*, int, clay_tail_result*, int);
-extern "C" __global__ void clayFinalizePcm(const Job*, int, const int16_t*, int16_t*, clay_tail_result*, int, int, int);
+extern "C" __global__ void clayFinalizePcm(const Job*, int, const int16_t*, int16_t*, clay_tail_result*, int, int, int,
+                                            int);
 extern "C" __global__ void clayMeasureBody(const AudioJobDesc*, int, const int16_t*, const clay_tail_result*,
                                            unsigned long long*, clay_audio_config, int, int);
 extern "C" __global__ void clayMeasureOnsets(const AudioJobDesc*, int, const int*, const int16_t*,
@@ -1043,8 +1044,8 @@ void finishBatch(clay_hip* h, clay_hip_batch& b)
     int16_t* packed = static_cast<int16_t*>(b.dMix);
     chunked(b.blocks, [&](int first, int n) {
         h->launch("clayFinalizePcm", [&] {
-            hipLaunchKernelGGL(clayFinalizePcm, dim3(n), dim3(64), 0, s, b.dJobs, count, finalIn, packed, b.dResults, ch,
-                               h->cfg.audio.enabled ? 1 : 0, first);
+            hipLaunchKernelGGL(clayFinalizePcm, dim3(ceilDiv(n, kFinalizeBlocks)), dim3(64), 0, s, b.dJobs, count,
+                               finalIn, packed, b.dResults, ch, h->cfg.audio.enabled ? 1 : 0, first, n);
         });
     });
     if (b.assess) {
