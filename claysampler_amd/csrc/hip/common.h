@@ -454,15 +454,16 @@ constexpr int kCursors = 4;
 // The cursors persist across the spans of a chain: a binding to the track a
 // cursor already follows keeps its position (frames only move forward), so
 // only a new track costs a binary search.
-struct ModReader {
-    Cursor track[kCursors], shape[kCursors];
-    int trackKey[kCursors], shapeKey[kCursors];  // first point of the followed track, -1: none
+template<int N>
+struct ModReaderN {
+    Cursor track[N], shape[N];
+    int trackKey[N], shapeKey[N];  // first point of the followed track, -1: none
     int first, count;
 
-    __device__ ModReader()
+    __device__ ModReaderN()
     {
 #pragma unroll
-        for (int j = 0; j < kCursors; ++j)
+        for (int j = 0; j < N; ++j)
             trackKey[j] = shapeKey[j] = -1;
     }
 
@@ -472,7 +473,7 @@ struct ModReader {
         first = firstMod;
         count = modCount;
 #pragma unroll
-        for (int j = 0; j < kCursors; ++j) {
+        for (int j = 0; j < N; ++j) {
             if (j >= count)
                 break;
             const Mod m = mods[first + j];
@@ -493,7 +494,7 @@ struct ModReader {
         const Mod m = mods[first + j];
         double v = m.depth;
         if (m.first >= 0) {
-            const double x = j < kCursors ? track[j].value(points, m.last, frame) : [&] {
+            const double x = j < N ? track[j].value(points, m.last, frame) : [&] {
                 Cursor c;
                 c.reset(points, m.first, m.last, frame);
                 return pointValue(points[c.at], frame);
@@ -501,7 +502,7 @@ struct ModReader {
             v = double(m.depth) * x;
         }
         if (m.shapeFirst >= 0) {
-            const double x = j < kCursors ? shape[j].value(points, m.shapeLast, frame) : [&] {
+            const double x = j < N ? shape[j].value(points, m.shapeLast, frame) : [&] {
                 Cursor c;
                 c.reset(points, m.shapeFirst, m.shapeLast, frame);
                 return pointValue(points[c.at], frame);
@@ -511,7 +512,7 @@ struct ModReader {
         return v;
     }
 
-    // value() of cursor slot j < kCursors at `frame`, clearing `steady` unless
+    // value() of cursor slot j < N at `frame`, clearing `steady` unless
     // it stays constant through frame `until`.
     __device__ double steadyValue(const Mod* __restrict__ mods, const clay_control_point* __restrict__ points, int j,
                                   int frame, int until, bool& steady)
@@ -525,6 +526,7 @@ struct ModReader {
         return v;
     }
 };
+using ModReader = ModReaderN<kCursors>;
 
 __device__ __forceinline__ double envelopeShapeD(const clay_envelope_shape& v, int frame)
 {

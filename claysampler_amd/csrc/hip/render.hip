@@ -360,7 +360,9 @@ __device__ __forceinline__ bool silencePass(SilenceTrack& t, float level, int f0
 // one sample), fixed at compile time so that mono chains hold no registers
 // for a right channel. `Single`: the chain's only filter is one biquad
 // section in slot 0 (the common case), so the slot and section loops vanish.
-template<bool Stereo, bool Single>
+// `Cursors`: modulation bindings per span that follow their tracks with
+// cursors (any further one searches its track per frame).
+template<bool Stereo, bool Single, int Cursors>
 __device__ __forceinline__ void voiceChain(const Span* __restrict__ spans, const SpanCold* __restrict__ cold,
                                            const clay_control_point* __restrict__ points, const Mod* __restrict__ mods,
                                            float* __restrict__ filtered, VoiceState* __restrict__ states,
@@ -401,7 +403,7 @@ __device__ __forceinline__ void voiceChain(const Span* __restrict__ spans, const
         carry.cutoffOf[0] = carry.cutoffOf[1] = __builtin_nan("");
     }
     __builtin_amdgcn_wave_barrier();
-    ModReader reader;
+    ModReaderN<Cursors> reader;
     for (int s = head; s >= 0; s = spans[s].next) {
         const Span& v = spans[s];
         if (v.end <= segFirst || v.begin >= segEnd)
@@ -452,12 +454,12 @@ __device__ __forceinline__ void voiceChain(const Span* __restrict__ spans, const
             };
             // Unrolled over the cursor slots so the cursors stay in registers.
 #pragma unroll
-            for (int k = 0; k < kCursors; ++k) {
+            for (int k = 0; k < Cursors; ++k) {
                 const int route = routes >> (3 * k) & 7;
                 if (route)
                     add(route, value(k));
             }
-            for (int k = kCursors; k < v.modCount; ++k) {
+            for (int k = Cursors; k < v.modCount; ++k) {
                 const int route = routeOf(f, mods[v.firstMod + k].target);
                 if (route)
                     add(route, reader.value(mods, points, k, frame));
@@ -521,7 +523,7 @@ __device__ __forceinline__ void voiceChain(const Span* __restrict__ spans, const
                     bool staged = false;
                     if (moving || !initialized) {
                         int routes = 0;
-                        for (int k = 0; k < kCursors && k < v.modCount; ++k)
+                        for (int k = 0; k < Cursors && k < v.modCount; ++k)
                             routes |= routeOf(f, mods[v.firstMod + k].target) << (3 * k);
                         routes = uniform(routes);
                         double nt[5], nc[5];  // target and coefficients at the last active frame
@@ -560,7 +562,7 @@ __device__ __forceinline__ void voiceChain(const Span* __restrict__ spans, const
                         // active frames and the first of them has the carried inputs, no frame
                         // changes targets: one input evaluation per lane instead of one per frame.
                         bool quick = false;
-                        if (initialized && !cv.active[0] && v.modCount <= kCursors && carry.have[f]) {
+                        if (initialized && !cv.active[0] && v.modCount <= Cursors && carry.have[f]) {
                             const int fa = max(base, lo), la = min(base + K, hi) - 1;
                             bool same = true;
                             if (fa <= la) {
@@ -1001,7 +1003,7 @@ __global__ void __launch_bounds__(64 * kAudioWaves) clayChainAudio(
     });
 }
 
-// Calls body(stereo, single) with the compile-time variant of the chain that
+// Calls body(stereo, single, cursors) with the compile-time variant of the chain that
 // starts at span `head` (the filter slots are the head span's, as voiceChain
 // reads them).
 template<class Body>
@@ -1011,15 +1013,21 @@ __device__ __forceinline__ void chainVariant(const Span* __restrict__ spans, con
     bool single = sections(cold[head].filters[0].type) == 1;
     for (int f = 1; f < CLAY_FILTER_SLOTS; ++f)
         single = single && sections(cold[head].filters[f].type) == 0;
+    using One = std::integral_constant<int, 1>;
+    using All = std::integral_constant<int, kCursors>;
     if (spans[head].channels == 2) {
         if (single)
-            body(std::true_type{}, std::true_type{});
+            body(std::true_type{}, std::true_type{}, All{});
         else
-            body(std::true_type{}, std::false_type{});
+            body(std::true_type{}, std::false_type{}, All{});
     } else if (single) {
-        body(std::false_type{}, std::true_type{});
+        // One binding (a single cutoff or resonance control) is the common voice.
+        if (spans[head].modCount <= 1)
+            body(std::false_type{}, std::true_type{}, One{});
+        else
+            body(std::false_type{}, std::true_type{}, All{});
     } else {
-        body(std::false_type{}, std::false_type{});
+        body(std::false_type{}, std::false_type{}, All{});
     }
 }
 
@@ -1035,8 +1043,8 @@ clayRoundVoiceChains(UnitsPtr units, RoundPtr round)
         const int unit = unitOf(header, item, [&](int i) { return units[i].chainBase; });
         const auto& u = constant(units)[unit];
         const int head = uniform(global(u.chainHeads)[item - u.chainBase]);
-        chainVariant(global(u.spans), global(u.cold), head, [&](auto stereo, auto single) {
-            voiceChain<decltype(stereo)::value, decltype(single)::value>(
+        chainVariant(global(u.spans), global(u.cold), head, [&](auto stereo, auto single, auto cursors) {
+            voiceChain<decltype(stereo)::value, decltype(single)::value, decltype(cursors)::value>(
                 global(u.spans), global(u.cold), global(u.points), global(u.mods), global(u.filtered),
                 global(u.states), u.segFirst, u.segEnd, head, header.silence, carries[wave], lane);
         });
@@ -1056,10 +1064,9 @@ clayVoiceChains(const Span* __restrict__ spans, const SpanCold* __restrict__ col
     if (item >= count)
         return;  // uniform per wave; no block barriers below
     const int head = uniform(heads[item]);
-    chainVariant(spans, cold, head, [&](auto stereo, auto single) {
-        voiceChain<decltype(stereo)::value, decltype(single)::value>(spans, cold, points, mods, filtered, states,
-                                                                     segFirst, segEnd, head, silenceLevel,
-                                                                     carries[wave], lane);
+    chainVariant(spans, cold, head, [&](auto stereo, auto single, auto cursors) {
+        voiceChain<decltype(stereo)::value, decltype(single)::value, decltype(cursors)::value>(
+            spans, cold, points, mods, filtered, states, segFirst, segEnd, head, silenceLevel, carries[wave], lane);
     });
 }
 
