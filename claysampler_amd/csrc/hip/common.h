@@ -447,6 +447,42 @@ struct Cursor {
     }
 };
 
+// A Cursor that keeps its current and next point in registers: a frame in the
+// same segment as the previous one loads nothing (readers of one binding can
+// afford the registers; the loads were a serial latency per frame).
+struct CachedCursor {
+    int at;
+    clay_control_point p, n;
+    __device__ void reset(const clay_control_point* __restrict__ points, int first, int end, int frame)
+    {
+        Cursor c;
+        c.reset(points, first, end, frame);
+        at = c.at;
+        p = points[at];
+        n = points[min(at + 1, end - 1)];
+    }
+    __device__ __forceinline__ void advance(const clay_control_point* __restrict__ points, int last, int frame)
+    {
+        while (at + 1 < last && n.frame <= frame) {
+            ++at;
+            p = n;
+            n = points[min(at + 1, last - 1)];
+        }
+    }
+    __device__ __forceinline__ double value(const clay_control_point* __restrict__ points, int last, int frame)
+    {
+        advance(points, last, frame);
+        return pointValue(p, frame);
+    }
+    __device__ __forceinline__ double steadyValue(const clay_control_point* __restrict__ points, int last, int frame,
+                                                  int until, bool& steady)
+    {
+        advance(points, last, frame);
+        steady = steady && p.step == 0.f && (at + 1 >= last || n.frame > until);
+        return pointValue(p, frame);
+    }
+};
+
 // Up to kCursors modulation bindings of a span read through cursors; any
 // further ones fall back to the binary search.
 constexpr int kCursors = 4;
@@ -456,7 +492,8 @@ constexpr int kCursors = 4;
 // only a new track costs a binary search.
 template<int N>
 struct ModReaderN {
-    Cursor track[N], shape[N];
+    using C = std::conditional_t<N == 1, CachedCursor, Cursor>;
+    C track[N], shape[N];
     int trackKey[N], shapeKey[N];  // first point of the followed track, -1: none
     int first, count;
 
