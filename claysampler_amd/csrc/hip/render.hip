@@ -669,43 +669,47 @@ __device__ __forceinline__ void voiceChain(const Span* __restrict__ spans, const
                         } else {
                             staged = true;
                             // The nearest flagged frame at or before each frame: the lane's last
-                            // flag, a scan over the lanes, then forward within the lane.
-                            int have = 0;
-                            double lt[5] = {0, 0, 0, 0, 0};
-#pragma unroll
-                            for (int k = 0; k < K; ++k)
-                                if (flags >> k & 1) {
-                                    have = 1;
-                                    for (int i = 0; i < 5; ++i)
-                                        lt[i] = carry.stage[i][k][lane];
-                                }
-                            scanWave([&](auto st) {
-                                constexpr int C = decltype(st)::ctrl, R = decltype(st)::rowMask;
-                                const int hp = shiftI<C, R>(have, 0);
-                                double tp[5];
-                                for (int i = 0; i < 5; ++i)
-                                    tp[i] = shiftD<C, R>(lt[i], 0.0);
-                                if (!have && hp) {
-                                    for (int i = 0; i < 5; ++i)
-                                        lt[i] = tp[i];
-                                    have = 1;
-                                }
-                            });
+                            // flag, a scan over the lanes, then forward within the lane. When
+                            // every active frame is flagged (a control ramp), each frame has its
+                            // own target staged already.
                             double cur[5];
-                            const int incoming = shiftI<kWaveShr1>(have, 0);
-                            for (int i = 0; i < 5; ++i) {
-                                cur[i] = previousLane(lt[i], 0.0);
-                                if (!incoming)
-                                    cur[i] = carry.target[f][i];
-                            }
+                            if (__ballot((flags & act_bits) != act_bits) != 0) {
+                                int have = 0;
+                                double lt[5] = {0, 0, 0, 0, 0};
 #pragma unroll
-                            for (int k = 0; k < K; ++k)
+                                for (int k = 0; k < K; ++k)
+                                    if (flags >> k & 1) {
+                                        have = 1;
+                                        for (int i = 0; i < 5; ++i)
+                                            lt[i] = carry.stage[i][k][lane];
+                                    }
+                                scanWave([&](auto st) {
+                                    constexpr int C = decltype(st)::ctrl, R = decltype(st)::rowMask;
+                                    const int hp = shiftI<C, R>(have, 0);
+                                    double tp[5];
+                                    for (int i = 0; i < 5; ++i)
+                                        tp[i] = shiftD<C, R>(lt[i], 0.0);
+                                    if (!have && hp) {
+                                        for (int i = 0; i < 5; ++i)
+                                            lt[i] = tp[i];
+                                        have = 1;
+                                    }
+                                });
+                                const int incoming = shiftI<kWaveShr1>(have, 0);
                                 for (int i = 0; i < 5; ++i) {
-                                    if (flags >> k & 1)
-                                        cur[i] = carry.stage[i][k][lane];
-                                    else
-                                        carry.stage[i][k][lane] = cur[i];
+                                    cur[i] = previousLane(lt[i], 0.0);
+                                    if (!incoming)
+                                        cur[i] = carry.target[f][i];
                                 }
+#pragma unroll
+                                for (int k = 0; k < K; ++k)
+                                    for (int i = 0; i < 5; ++i) {
+                                        if (flags >> k & 1)
+                                            cur[i] = carry.stage[i][k][lane];
+                                        else
+                                            carry.stage[i][k][lane] = cur[i];
+                                    }
+                            }
                             __builtin_amdgcn_wave_barrier();
                             for (int i = 0; i < 5; ++i)
                                 nt[i] = carry.stage[i][lastSlot][lastLane];
