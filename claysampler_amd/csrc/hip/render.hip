@@ -145,9 +145,13 @@ constexpr int kChainOccupancy = 3;
 constexpr int kAudioWaves = 16;
 constexpr int kSharedTapsMax = 32;
 // The mix kernels: 8 waves per SIMD (64 VGPRs) keep the wet path's scratch
-// reads in flight (7.2 ms vs 12.9 ms at the unconstrained 5 on configuration
-// 2) at no cost to the dry path (configuration 3 unchanged).
+// reads in flight (7.2 ms vs 12.9 ms at the unconstrained 5 on a 32-tap
+// workload of filtered voices) at no cost to 16-tap dry voices. A 48- or
+// 64-tap dry FIR spills heavily in 64 VGPRs: those mixes run at 5 waves per
+// SIMD (93 VGPRs, no spills), which took a 64-tap workload of dry voices from
+// 11.1 s to 7.0 s.
 constexpr int kMixOccupancy = 8;
+constexpr int mixOccupancy(int taps) { return taps >= 48 ? 5 : kMixOccupancy; }
 
 // Per-wave carry between 64-frame chunks, in LDS (broadcast reads).
 struct ChainCarry {
@@ -1198,7 +1202,7 @@ __device__ __forceinline__ void mixTile(const int16_t* __restrict__ bank, const 
 
 // One block per tile of the round (256 frames, one per thread), grid-stride.
 template<int T, bool FloatOut>
-__global__ void __launch_bounds__(kTileFrames) __attribute__((amdgpu_waves_per_eu(kMixOccupancy)))
+__global__ void __launch_bounds__(kTileFrames) __attribute__((amdgpu_waves_per_eu(mixOccupancy(T))))
 clayRoundMixTiles(const int16_t* __restrict__ bank, const float* __restrict__ sinc, UnitsPtr units, RoundPtr round)
 {
     __shared__ MixVoice staged[kMixVoices];
@@ -1216,7 +1220,7 @@ clayRoundMixTiles(const int16_t* __restrict__ bank, const float* __restrict__ si
 
 // Plain form: one unit, its pointers as kernel arguments, one block per tile.
 template<int T, bool FloatOut>
-__global__ void __launch_bounds__(kTileFrames) __attribute__((amdgpu_waves_per_eu(kMixOccupancy))) clayMixTiles(
+__global__ void __launch_bounds__(kTileFrames) __attribute__((amdgpu_waves_per_eu(mixOccupancy(T)))) clayMixTiles(
     const int16_t* __restrict__ bank, const float* __restrict__ sinc, const Span* __restrict__ spans,
     const Job* __restrict__ jobs, const Tile* __restrict__ tiles, const int* __restrict__ voices,
     const clay_control_point* __restrict__ points, const Mod* __restrict__ mods, const float* __restrict__ filtered,
