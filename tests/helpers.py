@@ -79,6 +79,44 @@ def group_case(root):
     return presets, melody
 
 
+def chain_shapes_case(root):
+    """One voice chain of each compiled filter-chain shape: stereo with one
+    two-pole section, mono with two sections (lpf_4p), mono two-pole with one
+    binding (cutoff CC1) and with two (cutoff and resonance CC1), and a
+    high-pass; CC1 moves in steps and ramps, and notes overlap and bend."""
+    n = np.arange(RATE)
+    left = np.rint(8000 * np.sin(2 * np.pi * 220 * n / RATE) + 2000 * np.sin(2 * np.pi * 3100 * n / RATE))
+    right = np.rint(6000 * np.sin(2 * np.pi * 331 * n / RATE + 0.5) + 1500 * np.sin(2 * np.pi * 4700 * n / RATE))
+    wav(root / "shape_st.wav", np.stack([left, right], 1).reshape(-1), channels=2)
+    mono = np.rint(9000 * np.sin(2 * np.pi * 180 * n / RATE) + 2500 * np.sin(2 * np.pi * 2900 * n / RATE))
+    wav(root / "shape_mono.wav", mono)
+    sfz = text(root / "shapes.sfz", (
+        "<region> sample=shape_st.wav lokey=40 hikey=49 pitch_keycenter=45 fil_type=lpf_2p cutoff=400 "
+        "cutoff_oncc1=4000 resonance=4 loop_mode=loop_continuous loop_start=100 loop_end=40000 ampeg_release=0.2\n"
+        "<region> sample=shape_mono.wav lokey=50 hikey=59 pitch_keycenter=55 fil_type=lpf_4p cutoff=700 "
+        "cutoff_oncc1=3000 resonance=2 ampeg_release=0.15\n"
+        "<region> sample=shape_mono.wav lokey=60 hikey=69 pitch_keycenter=64 fil_type=lpf_2p cutoff=250 "
+        "cutoff_oncc1=5000 resonance=6 loop_mode=loop_continuous loop_start=0 loop_end=44000 ampeg_release=0.3\n"
+        "<region> sample=shape_mono.wav lokey=70 hikey=79 pitch_keycenter=74 fil_type=lpf_2p cutoff=900 "
+        "cutoff_oncc1=2400 resonance_oncc1=9 ampeg_release=0.1\n"
+        "<region> sample=shape_mono.wav lokey=80 hikey=89 pitch_keycenter=84 fil_type=hpf_2p cutoff=1500 "
+        "cutoff_oncc1=-1200 resonance=3 ampeg_release=0.1\n"))
+    events = []
+    rng = synth.SplitMix64(91)
+    tick = 0
+    for _ in range(20):
+        key, vel = rng.integer(40, 89), rng.integer(40, 127)
+        dur = rng.integer(150, 900)
+        events.append((tick, 2, 0x90, key, vel))
+        events.append((tick + dur, 1, 0x80, key, 0))
+        events.append((tick + dur // 3, 0, 0xB0, 1, rng.integer(0, 127)))
+        events.append((tick + dur // 2, 0, 0xE0, rng.integer(0, 127), rng.integer(0, 127)))
+        tick += rng.integer(80, 300)
+    mid = str(root / "shapes.mid")
+    synth.write_midi(mid, events)
+    return sfz, mid
+
+
 def rich_case(root):
     """Stereo sample with pan/width/position, loop crossfade, generators,
     velocity-modulated volume, a multi-section filter and a pitch-CC voice."""

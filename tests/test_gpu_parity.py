@@ -10,8 +10,8 @@ import pytest
 import claysampler_amd as ca
 from claysampler_amd import abi, synth
 
-from helpers import (assert_parity, gpu_render, group_case, oracle_render, pedal_case, rich_case, segment_case,
-                     tail_case)
+from helpers import (assert_parity, chain_shapes_case, gpu_render, group_case, oracle_render, pedal_case, rich_case,
+                     segment_case, tail_case)
 
 pytestmark = pytest.mark.gpu
 
@@ -88,6 +88,16 @@ def test_segment_case(tmp_path, policy):
 
 def test_rich_features(tmp_path):
     sfz, mid = rich_case(tmp_path)
+    for policy in (abi.POLICY_NATURAL, abi.POLICY_ORIG):
+        s = ca.Session(policy=policy)
+        s.add_job(sfz, mid)
+        assert_parity(*both(s, policy, 32))
+
+
+def test_filter_chain_shapes(tmp_path):
+    """Every compiled filter-chain shape (stereo, multi-section, one or more
+    bindings, high-pass) against the oracle, in both output layouts."""
+    sfz, mid = chain_shapes_case(tmp_path)
     for policy in (abi.POLICY_NATURAL, abi.POLICY_ORIG):
         s = ca.Session(policy=policy)
         s.add_job(sfz, mid)
