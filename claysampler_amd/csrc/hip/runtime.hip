@@ -594,8 +594,12 @@ void flatten(clay_hip* h, const clay_plan_view* plans, size_t count, clay_hip_ba
     std::vector<std::pair<int64_t, int>> order;
     for (int head : b.chainHeads) {
         int64_t frames = 0;
-        for (int s = head; s >= 0; s = b.spans[size_t(s)].next)
+        for (int s = head; s >= 0; s = b.spans[size_t(s)].next) {
             frames += b.spans[size_t(s)].end - b.spans[size_t(s)].begin;
+            // The filter pass compiles a chain for its head span's channel count.
+            if (b.spans[size_t(s)].channels != b.spans[size_t(head)].channels)
+                throw Failure(CLAY_E_INVALID, "a voice chain mixes mono and stereo spans");
+        }
         order.emplace_back(-frames, head);
     }
     std::stable_sort(order.begin(), order.end());
