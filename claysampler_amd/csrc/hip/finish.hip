@@ -251,41 +251,59 @@ extern "C" __global__ void __launch_bounds__(64) clayFinalizePcm(const Job* __re
         const int first = (block - j.blockFirst) * kTailBlock;
         const int end = min(first + kTailBlock, results[index].data_frames);
         const int fadeStart = frames - fade;
-        // Every load of the lane's frames is issued before any is used (one
-        // 128-byte wave load at a time left the kernel bound by latency).
-        int16_t in[kPer][2];
-#pragma unroll
-        for (int i = 0; i < kPer; ++i) {
-            const int f = first + int(threadIdx.x) + 64 * i;
-#pragma unroll
-            for (int c = 0; c < 2; ++c)
-                in[i][c] =
-                    f < end && c < channels ? input[(j.mix + uint64_t(f)) * uint64_t(channels) + uint64_t(c)] : 0;
-        }
-#pragma unroll
-        for (int i = 0; i < kPer; ++i) {
-            const int f = first + int(threadIdx.x) + 64 * i;
-            if (f >= end)
-                break;
-            int framePeak = 0;
-#pragma unroll
-            for (int c = 0; c < 2; ++c) {
-                if (c >= channels)
-                    break;
-                int v = in[i][c];
-                if (f >= fadeStart) {
-                    const int k = f - fadeStart;
-                    const double gain = fade == 1 ? 0 : .5 * (1 + cos(pi * k / (fade - 1)));
-                    v = int16_t(__double2int_rn(v * gain));
-                }
-                output[(outAt + uint64_t(f)) * uint64_t(channels) + uint64_t(c)] = int16_t(v);
-                framePeak = max(framePeak, abs(v));
-                clipped += abs(v) >= 32767;
-            }
+        // Blocks before the fade: every load of the lane's frames is issued
+        // before any is used (one 128-byte wave load at a time left the kernel
+        // bound by latency), unrolled. Blocks that reach the fade take a plain
+        // loop, so the FP64 cosine is compiled once rather than per frame slot
+        // (unrolled, it made the kernel too large for the instruction cache).
+        const auto frame = [&](int f, int c, int v) {
+            output[(outAt + uint64_t(f)) * uint64_t(channels) + uint64_t(c)] = int16_t(v);
+            clipped += abs(v) >= 32767;
+            return abs(v);
+        };
+        const auto audible = [&](int f, int framePeak) {
             peak = max(peak, framePeak);
             if (framePeak > 3) {
                 audibleFirst = min(audibleFirst, f);
                 audibleLast = f;
+            }
+        };
+        if (end <= fadeStart) {
+            int16_t in[kPer][2];
+#pragma unroll
+            for (int i = 0; i < kPer; ++i) {
+                const int f = first + int(threadIdx.x) + 64 * i;
+#pragma unroll
+                for (int c = 0; c < 2; ++c)
+                    in[i][c] =
+                        f < end && c < channels ? input[(j.mix + uint64_t(f)) * uint64_t(channels) + uint64_t(c)] : 0;
+            }
+#pragma unroll
+            for (int i = 0; i < kPer; ++i) {
+                const int f = first + int(threadIdx.x) + 64 * i;
+                if (f >= end)
+                    break;
+                int framePeak = 0;
+#pragma unroll
+                for (int c = 0; c < 2; ++c)
+                    if (c < channels)
+                        framePeak = max(framePeak, frame(f, c, in[i][c]));
+                audible(f, framePeak);
+            }
+        } else {
+#pragma unroll 1
+            for (int f = first + int(threadIdx.x); f < end; f += 64) {
+                int framePeak = 0;
+                for (int c = 0; c < channels; ++c) {
+                    int v = input[(j.mix + uint64_t(f)) * uint64_t(channels) + uint64_t(c)];
+                    if (f >= fadeStart) {
+                        const int k = f - fadeStart;
+                        const double gain = fade == 1 ? 0 : .5 * (1 + cos(pi * k / (fade - 1)));
+                        v = int16_t(__double2int_rn(v * gain));
+                    }
+                    framePeak = max(framePeak, frame(f, c, v));
+                }
+                audible(f, framePeak);
             }
         }
     }
