@@ -23,6 +23,7 @@
 #include "device_types.h"
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -32,6 +33,7 @@
 #include <mutex>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <vector>
 
 using namespace clay_dev;
@@ -238,6 +240,10 @@ struct clay_hip {
         const void* bank;
     };
     std::vector<Graph> graphs;
+    // Pinned image of a plain batch's records (grown on demand): the copy runs
+    // at DMA speed instead of through a pageable bounce.
+    char* staging = nullptr;
+    size_t stagingBytes = 0;
     // Pinned staging of persistent-form uploads: a ring whose regions are
     // reused once the copy that read them has completed.
     struct Ring {
@@ -434,7 +440,34 @@ void validatePlan(const clay_hip* h, const clay_plan_view& p, size_t index)
     }
 }
 
-// Flatten plans into device records (renderer.cpp:378-462 semantics).
+// Runs f(i) for i in [0, count) on up to 16 host threads (the plans of a batch
+// are independent once their record offsets are known).
+template<class F>
+void parallelFor(size_t count, F&& f)
+{
+    const size_t threads = std::min<size_t>({count, 16, std::max(1u, std::thread::hardware_concurrency())});
+    if (threads <= 1) {
+        for (size_t i = 0; i < count; ++i)
+            f(i);
+        return;
+    }
+    std::atomic<size_t> next{0};
+    std::vector<std::thread> pool;
+    pool.reserve(threads);
+    for (size_t t = 0; t < threads; ++t)
+        pool.emplace_back([&] {
+            for (size_t i; (i = next.fetch_add(1)) < count;)
+                f(i);
+        });
+    for (auto& thread : pool)
+        thread.join();
+}
+
+// Flatten plans into device records (renderer.cpp:378-462 semantics). Three
+// passes: per plan (parallel) validation and the sizes that depend on its
+// spans; record offsets by prefix sums (serial); per plan (parallel) the
+// records at those offsets. Records land exactly where a serial append would
+// put them.
 void flatten(clay_hip* h, const clay_plan_view* plans, size_t count, clay_hip_batch& b)
 {
     const bool assess = h->cfg.audio.enabled || h->cfg.group_minimum;
@@ -444,151 +477,229 @@ void flatten(clay_hip* h, const clay_plan_view* plans, size_t count, clay_hip_ba
         const int chunk = 2 * h->cfg.audio.rate, span = 5 * h->cfg.audio.window;
         return (frames / chunk) * int(ceilDiv(uint64_t(chunk), uint64_t(span))) + int(ceilDiv(uint64_t(frames % chunk), uint64_t(span)));
     };
-    for (size_t i = 0; i < count; ++i) {
+    // 1. Validation and span-dependent sizes.
+    struct Sizes {
+        int voices = 0;
+        uint64_t filtered = 0, pitch = 0;
+        std::string error;
+    };
+    std::vector<Sizes> sizes(count);
+    parallelFor(count, [&](size_t i) {
         const auto& p = plans[i];
-        validatePlan(h, p, i);
-        const int jobIndex = int(b.jobs.size());
-        // Controls and modulation bindings with absolute ranges.
-        const int pointBase = int(b.points.size());
-        b.points.insert(b.points.end(), p.points, p.points + p.track_offsets[p.track_count]);
-        const auto trackFirst = [&](int t) { return pointBase + p.track_offsets[t]; };
-        const int modBase = int(b.mods.size());
-        for (int m = 0; m < p.modulation_count; ++m) {
-            const auto& src = p.modulation[m];
-            Mod d{-1, 0, src.target, src.depth, -1, 0};
-            if (src.track >= 0) {
-                d.first = trackFirst(src.track);
-                d.last = trackFirst(src.track + 1);
-            }
-            if (src.shape_track >= 0) {
-                d.shapeFirst = trackFirst(src.shape_track);
-                d.shapeLast = trackFirst(src.shape_track + 1);
-            }
-            b.mods.push_back(d);
+        Sizes& z = sizes[i];
+        try {
+            validatePlan(h, p, i);
+        } catch (const Failure& f) {
+            z.error = f.what();
+            return;
         }
-        Job j{};
-        j.mix = b.frames;
-        j.frames = p.frames;
-        for (int t = 0; t < 4; ++t)
-            j.controls[t] = trackFirst(t);
-        j.musicEnd = p.music_end;
-        j.limit = p.limit;
-        j.fade = p.fade_frames;
-        j.blockFirst = b.blocks;
-        j.windowFirst = b.windows;
-        const int body = h->cfg.audio.enabled || p.volume_group ? p.body_frames : 0;
-        j.bodyFrames = body;
-        j.bodyFirst = b.bodyBlocks;
-        j.onsetFirst = int(b.onsets.size());
-        j.onsetCount = assess && h->cfg.audio.enabled ? p.onset_count : 0;
-        b.jobs.push_back(j);
-        if (assess) {
-            b.audioJobs.push_back({body, b.bodyBlocks, j.onsetFirst, j.onsetCount});
-            if (h->cfg.audio.enabled)
-                b.onsets.insert(b.onsets.end(), p.onsets, p.onsets + p.onset_count);
-            b.bodyBlocks += bodyWindows(body);
-        }
-        b.blocks += int(ceilDiv(uint64_t(p.frames), kTailBlock));
-        b.windows += h->natural ? int(ceilDiv(uint64_t(p.frames), uint64_t(window))) : 0;
-        // Tiles and their voice lists (difference array over tile indices).
-        const int tileCount = int(ceilDiv(uint64_t(p.frames), kTileFrames));
-        std::vector<int> perTile(size_t(tileCount) + 1, 0);
-        for (int s = 0; s < p.span_count; ++s) {
-            ++perTile[size_t(p.spans[s].begin / kTileFrames)];
-            --perTile[size_t((p.spans[s].end - 1) / kTileFrames + 1)];
-        }
-        for (int t = 1; t <= tileCount; ++t)
-            perTile[size_t(t)] += perTile[size_t(t - 1)];
-        std::vector<int> cursor(static_cast<size_t>(tileCount), 0);
-        int at = int(b.voices.size());
-        for (int t = 0; t < tileCount; ++t) {
-            const int frame = t * kTileFrames;
-            b.tiles.push_back({jobIndex, frame, std::min(kTileFrames, p.frames - frame), at, perTile[size_t(t)]});
-            cursor[size_t(t)] = at;
-            at += perTile[size_t(t)];
-        }
-        b.voices.resize(size_t(at));
-        // Spans, voice chains, scratch offsets.
-        std::vector<int> chainTail(size_t(p.filter_tracks), -1);
-        const int chainBase = b.chains;
-        b.chains += p.filter_tracks;
         for (int s = 0; s < p.span_count; ++s) {
             const auto& v = p.spans[s];
-            const int index = int(b.spans.size());
-            for (int t = v.begin / kTileFrames; t <= (v.end - 1) / kTileFrames; ++t)
-                b.voices[size_t(cursor[size_t(t)]++)] = index;
-            Span d{};
-            d.sample = v.sample;
-            d.position = v.position;
-            d.step = v.step;
-            d.pitchSlope = v.pitch_slope;
-            d.begin = v.begin;
-            d.end = v.end;
-            d.sourceEnd = v.source_end;
-            d.loopFirst = v.loop_first;
-            d.loopEnd = v.loop_end;
-            d.loopCrossfade = v.loop_crossfade;
-            d.channels = v.channels;
-            d.generator = v.generator;
-            d.stereoMatrix = v.stereo_matrix;
-            d.variablePitch = v.variable_pitch;
-            d.noiseSeed = v.noise_seed;
-            d.noiseStart = v.noise_start;
-            d.firstMod = v.first_modulation + modBase;
-            d.modCount = v.modulation_count;
-            d.filterTrack = v.filter_track >= 0 ? chainBase + v.filter_track : -1;
-            d.job = jobIndex;
-            d.next = -1;
-            d.attackStart = v.attack_start;
-            d.holdStart = v.hold_start;
-            d.decayStart = v.decay_start;
-            d.releaseStart = v.release_start;
-            d.fadeStart = v.fade_start;
-            d.start = v.start;
-            d.attackStep = v.attack_step;
-            d.decayRate = v.decay_rate;
-            d.sustain = v.sustain;
-            d.releaseValue = v.release_value;
-            d.releaseRate = v.release_rate;
-            d.fadeValue = v.fade_value;
-            d.fadeStep = v.fade_step;
-            d.gain = v.gain;
-            d.pan = v.pan;
-            d.left = v.left;
-            d.right = v.right;
-            d.widthDirect = v.width_direct;
-            d.widthCross = v.width_cross;
-            d.positionLeft = v.position_left;
-            d.positionRight = v.position_right;
-            d.width = v.width;
-            d.stereoPosition = v.stereo_position;
-            SpanCold c{};
-            std::memcpy(c.filters, v.filters, sizeof c.filters);
-            std::memcpy(c.extra, v.extra_envelope, sizeof c.extra);
-            c.active[0] = v.active_envelope[0];
-            c.active[1] = v.active_envelope[1];
-            c.sampleRate = v.sample_rate;
+            z.voices += (v.end - 1) / kTileFrames - v.begin / kTileFrames + 1;
             if (v.filter_track >= 0) {
-                int& tail = chainTail[size_t(v.filter_track)];
-                if (tail < 0) {
-                    if (v.variable_pitch)
-                        b.pitchHeads.push_back(index);
-                    b.chainHeads.push_back(index);
-                } else {
-                    b.spans[size_t(tail)].next = index;
-                }
-                tail = index;
-                d.filtered = b.filteredSamples;
-                b.filteredSamples += uint64_t(v.end - v.begin) * uint64_t(v.channels);
-                if (v.variable_pitch) {
-                    d.phase = b.pitchFrames;
-                    b.pitchFrames += uint64_t(v.end - v.begin);
-                }
+                z.filtered += uint64_t(v.end - v.begin) * uint64_t(v.channels);
+                if (v.variable_pitch)
+                    z.pitch += uint64_t(v.end - v.begin);
             }
-            b.spans.push_back(d);
-            b.cold.push_back(c);
         }
-        b.frames += uint64_t(p.frames);
+    });
+    for (const auto& z : sizes)
+        if (!z.error.empty())
+            throw Failure(CLAY_E_INVALID, z.error);
+    // 2. Offsets of each plan's records.
+    struct Base {
+        int point, mod, tile, voice, span, chain, onset, block, window, body;
+        uint64_t frame, filtered, pitch;
+    };
+    std::vector<Base> bases(count);
+    Base at{};
+    for (size_t i = 0; i < count; ++i) {
+        const auto& p = plans[i];
+        bases[i] = at;
+        at.point += p.track_offsets[p.track_count];
+        at.mod += p.modulation_count;
+        at.tile += int(ceilDiv(uint64_t(p.frames), kTileFrames));
+        at.voice += sizes[i].voices;
+        at.span += p.span_count;
+        at.chain += p.filter_tracks;
+        at.onset += assess && h->cfg.audio.enabled ? p.onset_count : 0;
+        at.block += int(ceilDiv(uint64_t(p.frames), kTailBlock));
+        at.window += h->natural ? int(ceilDiv(uint64_t(p.frames), uint64_t(window))) : 0;
+        if (assess)
+            at.body += bodyWindows(h->cfg.audio.enabled || p.volume_group ? p.body_frames : 0);
+        at.frame += uint64_t(p.frames);
+        at.filtered += sizes[i].filtered;
+        at.pitch += sizes[i].pitch;
+    }
+    b.points.resize(size_t(at.point));
+    b.mods.resize(size_t(at.mod));
+    b.jobs.resize(count);
+    b.tiles.resize(size_t(at.tile));
+    b.voices.resize(size_t(at.voice));
+    b.spans.resize(size_t(at.span));
+    b.cold.resize(size_t(at.span));
+    if (assess)
+        b.audioJobs.resize(count);
+    b.onsets.resize(size_t(at.onset));
+    b.chains = at.chain;
+    b.blocks = at.block;
+    b.windows = at.window;
+    b.bodyBlocks = at.body;
+    b.frames = at.frame;
+    b.filteredSamples = at.filtered;
+    b.pitchFrames = at.pitch;
+    // 3. Each plan's records.
+    std::vector<std::vector<int>> chainHeads(count), pitchHeads(count);
+    std::vector<std::exception_ptr> failed(count);
+    parallelFor(count, [&](size_t i) {
+        try {
+            const auto& p = plans[i];
+            const Base& o = bases[i];
+            const int jobIndex = int(i);
+            // Controls and modulation bindings with absolute ranges.
+            std::copy(p.points, p.points + p.track_offsets[p.track_count], b.points.begin() + o.point);
+            const auto trackFirst = [&](int t) { return o.point + p.track_offsets[t]; };
+            for (int m = 0; m < p.modulation_count; ++m) {
+                const auto& src = p.modulation[m];
+                Mod d{-1, 0, src.target, src.depth, -1, 0};
+                if (src.track >= 0) {
+                    d.first = trackFirst(src.track);
+                    d.last = trackFirst(src.track + 1);
+                }
+                if (src.shape_track >= 0) {
+                    d.shapeFirst = trackFirst(src.shape_track);
+                    d.shapeLast = trackFirst(src.shape_track + 1);
+                }
+                b.mods[size_t(o.mod + m)] = d;
+            }
+            Job j{};
+            j.mix = o.frame;
+            j.frames = p.frames;
+            for (int t = 0; t < 4; ++t)
+                j.controls[t] = trackFirst(t);
+            j.musicEnd = p.music_end;
+            j.limit = p.limit;
+            j.fade = p.fade_frames;
+            j.blockFirst = o.block;
+            j.windowFirst = o.window;
+            const int body = h->cfg.audio.enabled || p.volume_group ? p.body_frames : 0;
+            j.bodyFrames = body;
+            j.bodyFirst = o.body;
+            j.onsetFirst = o.onset;
+            j.onsetCount = assess && h->cfg.audio.enabled ? p.onset_count : 0;
+            b.jobs[i] = j;
+            if (assess) {
+                b.audioJobs[i] = {body, o.body, j.onsetFirst, j.onsetCount};
+                if (h->cfg.audio.enabled)
+                    std::copy(p.onsets, p.onsets + p.onset_count, b.onsets.begin() + o.onset);
+            }
+            // Tiles and their voice lists (difference array over tile indices).
+            const int tileCount = int(ceilDiv(uint64_t(p.frames), kTileFrames));
+            std::vector<int> perTile(size_t(tileCount) + 1, 0);
+            for (int s = 0; s < p.span_count; ++s) {
+                ++perTile[size_t(p.spans[s].begin / kTileFrames)];
+                --perTile[size_t((p.spans[s].end - 1) / kTileFrames + 1)];
+            }
+            for (int t = 1; t <= tileCount; ++t)
+                perTile[size_t(t)] += perTile[size_t(t - 1)];
+            std::vector<int> cursor(static_cast<size_t>(tileCount), 0);
+            int voiceAt = o.voice;
+            for (int t = 0; t < tileCount; ++t) {
+                const int frame = t * kTileFrames;
+                b.tiles[size_t(o.tile + t)] = {jobIndex, frame, std::min(kTileFrames, p.frames - frame), voiceAt,
+                                               perTile[size_t(t)]};
+                cursor[size_t(t)] = voiceAt;
+                voiceAt += perTile[size_t(t)];
+            }
+            // Spans, voice chains, scratch offsets.
+            std::vector<int> chainTail(size_t(p.filter_tracks), -1);
+            uint64_t filtered = o.filtered, pitch = o.pitch;
+            for (int s = 0; s < p.span_count; ++s) {
+                const auto& v = p.spans[s];
+                const int index = o.span + s;
+                for (int t = v.begin / kTileFrames; t <= (v.end - 1) / kTileFrames; ++t)
+                    b.voices[size_t(cursor[size_t(t)]++)] = index;
+                Span d{};
+                d.sample = v.sample;
+                d.position = v.position;
+                d.step = v.step;
+                d.pitchSlope = v.pitch_slope;
+                d.begin = v.begin;
+                d.end = v.end;
+                d.sourceEnd = v.source_end;
+                d.loopFirst = v.loop_first;
+                d.loopEnd = v.loop_end;
+                d.loopCrossfade = v.loop_crossfade;
+                d.channels = v.channels;
+                d.generator = v.generator;
+                d.stereoMatrix = v.stereo_matrix;
+                d.variablePitch = v.variable_pitch;
+                d.noiseSeed = v.noise_seed;
+                d.noiseStart = v.noise_start;
+                d.firstMod = v.first_modulation + o.mod;
+                d.modCount = v.modulation_count;
+                d.filterTrack = v.filter_track >= 0 ? o.chain + v.filter_track : -1;
+                d.job = jobIndex;
+                d.next = -1;
+                d.attackStart = v.attack_start;
+                d.holdStart = v.hold_start;
+                d.decayStart = v.decay_start;
+                d.releaseStart = v.release_start;
+                d.fadeStart = v.fade_start;
+                d.start = v.start;
+                d.attackStep = v.attack_step;
+                d.decayRate = v.decay_rate;
+                d.sustain = v.sustain;
+                d.releaseValue = v.release_value;
+                d.releaseRate = v.release_rate;
+                d.fadeValue = v.fade_value;
+                d.fadeStep = v.fade_step;
+                d.gain = v.gain;
+                d.pan = v.pan;
+                d.left = v.left;
+                d.right = v.right;
+                d.widthDirect = v.width_direct;
+                d.widthCross = v.width_cross;
+                d.positionLeft = v.position_left;
+                d.positionRight = v.position_right;
+                d.width = v.width;
+                d.stereoPosition = v.stereo_position;
+                SpanCold c{};
+                std::memcpy(c.filters, v.filters, sizeof c.filters);
+                std::memcpy(c.extra, v.extra_envelope, sizeof c.extra);
+                c.active[0] = v.active_envelope[0];
+                c.active[1] = v.active_envelope[1];
+                c.sampleRate = v.sample_rate;
+                if (v.filter_track >= 0) {
+                    int& tail = chainTail[size_t(v.filter_track)];
+                    if (tail < 0) {
+                        if (v.variable_pitch)
+                            pitchHeads[i].push_back(index);
+                        chainHeads[i].push_back(index);
+                    } else {
+                        b.spans[size_t(tail)].next = index;
+                    }
+                    tail = index;
+                    d.filtered = filtered;
+                    filtered += uint64_t(v.end - v.begin) * uint64_t(v.channels);
+                    if (v.variable_pitch) {
+                        d.phase = pitch;
+                        pitch += uint64_t(v.end - v.begin);
+                    }
+                }
+                b.spans[size_t(index)] = d;
+                b.cold[size_t(index)] = c;
+            }
+        } catch (...) {
+            failed[i] = std::current_exception();
+        }
+    });
+    for (const auto& e : failed)
+        if (e)
+            std::rethrow_exception(e);
+    for (size_t i = 0; i < count; ++i) {
+        b.chainHeads.insert(b.chainHeads.end(), chainHeads[i].begin(), chainHeads[i].end());
+        b.pitchHeads.insert(b.pitchHeads.end(), pitchHeads[i].begin(), pitchHeads[i].end());
     }
     // Longest voice chains first: each is one wave's serial work.
     std::vector<std::pair<int64_t, int>> order;
@@ -680,17 +791,33 @@ void upload(clay_hip* h, clay_hip_batch& b, bool streamOrdered = false)
     auto* base = static_cast<char*>(b.arena);
     // Host image of the records: pinned ring (persistent form, the copy stays
     // asynchronous) or a pageable buffer (plain form, which waits anyway).
-    std::vector<char> pageable;
     char* image = nullptr;
     if (streamOrdered) {
         image = h->ring.take(recordBytes);
     } else {
-        pageable.resize(recordBytes);
-        image = pageable.data();
+        if (h->stagingBytes < recordBytes) {
+            if (h->staging)
+                (void)hipHostFree(h->staging);
+            h->staging = nullptr;
+            h->stagingBytes = 0;
+            check(hipHostMalloc(reinterpret_cast<void**>(&h->staging), std::max<size_t>(recordBytes, 1)),
+                  "record staging");
+            h->stagingBytes = recordBytes;
+        }
+        image = h->staging;
     }
+    // Large arrays are copied into the image by several host threads.
     const auto put = [&](size_t offset, const auto& v) {
-        if (!v.empty())
-            std::memcpy(image + offset, v.data(), bytesOf(v));
+        const size_t bytes = bytesOf(v);
+        const auto* src = reinterpret_cast<const char*>(v.data());
+        constexpr size_t kPiece = size_t(8) << 20;
+        if (bytes > 2 * kPiece)
+            parallelFor(ceilDiv(bytes, kPiece), [&](size_t k) {
+                const size_t at = k * kPiece;
+                std::memcpy(image + offset + at, src + at, std::min(kPiece, bytes - at));
+            });
+        else if (bytes)
+            std::memcpy(image + offset, src, bytes);
         return base + offset;
     };
     b.dSpans = reinterpret_cast<Span*>(put(oSpans, b.spans));
@@ -1241,6 +1368,8 @@ int clay_hip_destroy(clay_hip* h)
     for (auto& g : h->graphs)
         (void)hipGraphExecDestroy(g.exec);
     h->ring.release();
+    if (h->staging)
+        (void)hipHostFree(h->staging);
     for (auto& p : h->pinnedSizes)
         (void)hipHostFree(p.first);
     for (auto& stage : h->stages) {
