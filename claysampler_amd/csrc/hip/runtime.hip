@@ -548,8 +548,16 @@ void flatten(clay_hip* h, const clay_plan_view* plans, size_t count, clay_hip_ba
     b.frames = at.frame;
     b.filteredSamples = at.filtered;
     b.pitchFrames = at.pitch;
-    // 3. Each plan's records.
-    std::vector<std::vector<int>> chainHeads(count), pitchHeads(count);
+    // 3. Each plan's records, and each voice chain's head, frame count, first
+    // and last frame (in head order), so that sorting them walks no span list.
+    struct ChainInfo {
+        int head;
+        int64_t frames;
+        int begin, end;
+    };
+    std::vector<std::vector<ChainInfo>> chainInfo(count);
+    std::vector<std::vector<int>> pitchHeads(count);
+    std::vector<char> mixedChannels(count, 0);
     std::vector<std::exception_ptr> failed(count);
     parallelFor(count, [&](size_t i) {
         try {
@@ -612,7 +620,7 @@ void flatten(clay_hip* h, const clay_plan_view* plans, size_t count, clay_hip_ba
                 voiceAt += perTile[size_t(t)];
             }
             // Spans, voice chains, scratch offsets.
-            std::vector<int> chainTail(size_t(p.filter_tracks), -1);
+            std::vector<int> chainTail(size_t(p.filter_tracks), -1), chainSlot(size_t(p.filter_tracks), -1);
             uint64_t filtered = o.filtered, pitch = o.pitch;
             for (int s = 0; s < p.span_count; ++s) {
                 const auto& v = p.spans[s];
@@ -672,13 +680,21 @@ void flatten(clay_hip* h, const clay_plan_view* plans, size_t count, clay_hip_ba
                 c.sampleRate = v.sample_rate;
                 if (v.filter_track >= 0) {
                     int& tail = chainTail[size_t(v.filter_track)];
+                    int& slot = chainSlot[size_t(v.filter_track)];
                     if (tail < 0) {
                         if (v.variable_pitch)
                             pitchHeads[i].push_back(index);
-                        chainHeads[i].push_back(index);
+                        slot = int(chainInfo[i].size());
+                        chainInfo[i].push_back({index, 0, v.begin, 0});
                     } else {
                         b.spans[size_t(tail)].next = index;
+                        // The filter pass compiles a chain for its head span's channel count.
+                        if (v.channels != p.spans[s - (index - chainInfo[i][size_t(slot)].head)].channels)
+                            mixedChannels[i] = 1;
                     }
+                    ChainInfo& info = chainInfo[i][size_t(slot)];
+                    info.frames += v.end - v.begin;
+                    info.end = std::max(info.end, v.end);
                     tail = index;
                     d.filtered = filtered;
                     filtered += uint64_t(v.end - v.begin) * uint64_t(v.channels);
@@ -697,32 +713,29 @@ void flatten(clay_hip* h, const clay_plan_view* plans, size_t count, clay_hip_ba
     for (const auto& e : failed)
         if (e)
             std::rethrow_exception(e);
+    for (size_t i = 0; i < count; ++i)
+        if (mixedChannels[i])
+            throw Failure(CLAY_E_INVALID, "a voice chain mixes mono and stereo spans");
+    std::vector<ChainInfo> chains;
     for (size_t i = 0; i < count; ++i) {
-        b.chainHeads.insert(b.chainHeads.end(), chainHeads[i].begin(), chainHeads[i].end());
+        chains.insert(chains.end(), chainInfo[i].begin(), chainInfo[i].end());
         b.pitchHeads.insert(b.pitchHeads.end(), pitchHeads[i].begin(), pitchHeads[i].end());
     }
     // Longest voice chains first: each is one wave's serial work.
     std::vector<std::pair<int64_t, int>> order;
-    for (int head : b.chainHeads) {
-        int64_t frames = 0;
-        for (int s = head; s >= 0; s = b.spans[size_t(s)].next) {
-            frames += b.spans[size_t(s)].end - b.spans[size_t(s)].begin;
-            // The filter pass compiles a chain for its head span's channel count.
-            if (b.spans[size_t(s)].channels != b.spans[size_t(head)].channels)
-                throw Failure(CLAY_E_INVALID, "a voice chain mixes mono and stereo spans");
-        }
-        order.emplace_back(-frames, head);
-    }
+    order.reserve(chains.size());
+    for (size_t c = 0; c < chains.size(); ++c)
+        order.emplace_back(-chains[c].frames, int(c));
     std::stable_sort(order.begin(), order.end());
+    b.chainHeads.resize(chains.size());
     for (size_t i = 0; i < order.size(); ++i)
-        b.chainHeads[i] = order[i].second;
+        b.chainHeads[i] = chains[size_t(order[i].second)].head;
     // Windowed rounds: chains by first frame, with their extents.
     std::vector<std::pair<int2, int>> byStart;
-    for (int head : b.chainHeads) {
-        int end = 0;
-        for (int s = head; s >= 0; s = b.spans[size_t(s)].next)
-            end = std::max(end, b.spans[size_t(s)].end);
-        byStart.push_back({make_int2(b.spans[size_t(head)].begin, end), head});
+    byStart.reserve(chains.size());
+    for (size_t i = 0; i < order.size(); ++i) {
+        const ChainInfo& c = chains[size_t(order[i].second)];
+        byStart.push_back({make_int2(c.begin, c.end), c.head});
     }
     std::stable_sort(byStart.begin(), byStart.end(), [](const auto& x, const auto& y) { return x.first.x < y.first.x; });
     for (const auto& c : byStart) {
