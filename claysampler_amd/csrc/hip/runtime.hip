@@ -467,9 +467,17 @@ void parallelFor(size_t count, F&& f)
 // passes: per plan (parallel) validation and the sizes that depend on its
 // spans; record offsets by prefix sums (serial); per plan (parallel) the
 // records at those offsets. Records land exactly where a serial append would
-// put them.
-void flatten(clay_hip* h, const clay_plan_view* plans, size_t count, clay_hip_batch& b)
+// put them. `threaded` false keeps it on the calling thread (callers that
+// flatten several batches in parallel already).
+void flatten(clay_hip* h, const clay_plan_view* plans, size_t count, clay_hip_batch& b, bool threaded = true)
 {
+    const auto forEachPlan = [&](auto&& f) {
+        if (threaded)
+            parallelFor(count, f);
+        else
+            for (size_t i = 0; i < count; ++i)
+                f(i);
+    };
     const bool assess = h->cfg.audio.enabled || h->cfg.group_minimum;
     b.assess = assess;
     const int window = h->tail.window;
@@ -484,7 +492,7 @@ void flatten(clay_hip* h, const clay_plan_view* plans, size_t count, clay_hip_ba
         std::string error;
     };
     std::vector<Sizes> sizes(count);
-    parallelFor(count, [&](size_t i) {
+    forEachPlan([&](size_t i) {
         const auto& p = plans[i];
         Sizes& z = sizes[i];
         try {
@@ -559,7 +567,7 @@ void flatten(clay_hip* h, const clay_plan_view* plans, size_t count, clay_hip_ba
     std::vector<std::vector<int>> pitchHeads(count);
     std::vector<char> mixedChannels(count, 0);
     std::vector<std::exception_ptr> failed(count);
-    parallelFor(count, [&](size_t i) {
+    forEachPlan([&](size_t i) {
         try {
             const auto& p = plans[i];
             const Base& o = bases[i];
@@ -1755,55 +1763,89 @@ int renderWindowPersistent(clay_hip* h, const clay_plan_view* plans, size_t coun
         try {
             for (;;) {
                 // 1. Admission: fill free places from the queue, refilling it when empty.
-                while (resident.size() < h->cfg.batch_jobs) {
-                    if (next == queue.size()) {
-                        if (exhausted)
-                            break;
-                        const clay_plan_view* more = nullptr;
-                        size_t n = 0;
-                        const int wait = resident.empty() && completing.empty();
-                        if (!refill(user, wait, &more, &n) || !n) {
-                            if (wait)
-                                exhausted = true;
+                // The records of the units admitted together are flattened by
+                // several host threads; their uploads follow in queue order.
+                for (bool admitting = true; admitting && resident.size() < h->cfg.batch_jobs;) {
+                    std::vector<std::pair<size_t, size_t>> pending;  // [first, end) of each unit
+                    size_t scan = next;
+                    while (resident.size() + pending.size() < h->cfg.batch_jobs) {
+                        if (scan == queue.size()) {
+                            if (exhausted || !pending.empty())
+                                break;
+                            const clay_plan_view* more = nullptr;
+                            size_t n = 0;
+                            const int wait = resident.empty() && completing.empty();
+                            if (!refill(user, wait, &more, &n) || !n) {
+                                if (wait)
+                                    exhausted = true;
+                                break;
+                            }
+                            queue.insert(queue.end(), more, more + n);
+                        }
+                        size_t end = scan + 1;
+                        if (h->cfg.group_minimum)
+                            while (end < queue.size() && queue[scan].volume_group &&
+                                   queue[end].volume_group == queue[scan].volume_group)
+                                ++end;
+                        pending.emplace_back(scan, end);
+                        scan = end;
+                    }
+                    if (pending.empty())
+                        break;
+                    std::vector<std::unique_ptr<Unit>> made(pending.size());
+                    std::vector<int> flattened(pending.size(), CLAY_OK);
+                    std::vector<std::string> message(pending.size());
+                    for (size_t k = 0; k < pending.size(); ++k) {
+                        made[k] = std::make_unique<Unit>();
+                        made[k]->first = pending[k].first;
+                        made[k]->jobs = pending[k].second - pending[k].first;
+                        made[k]->b = new clay_hip_batch();
+                    }
+                    parallelFor(pending.size(), [&](size_t k) {
+                        try {
+                            flatten(h, queue.data() + pending[k].first, made[k]->jobs, *made[k]->b, false);
+                        } catch (const Failure& f) {
+                            flattened[k] = f.code;
+                            message[k] = f.what();
+                        } catch (const std::exception& e) {
+                            flattened[k] = CLAY_E_INVALID;
+                            message[k] = e.what();
+                        }
+                    });
+                    for (size_t k = 0; k < pending.size(); ++k) {
+                        auto& u = made[k];
+                        const size_t end = pending[k].second;
+                        int st = flattened[k];
+                        if (st == CLAY_OK)
+                            st = guarded(h, [&] { upload(h, *u->b, true); });
+                        else
+                            h->error = message[k];
+                        if (st == CLAY_E_MEMORY && (!resident.empty() || !completing.empty())) {
+                            for (size_t r = k; r < pending.size(); ++r)
+                                drop(*made[r]);  // retry once memory is returned
+                            admitting = false;
                             break;
                         }
-                        queue.insert(queue.end(), more, more + n);
-                    }
-                    size_t end = next + 1;
-                    if (h->cfg.group_minimum)
-                        while (end < queue.size() && queue[next].volume_group &&
-                               queue[end].volume_group == queue[next].volume_group)
-                            ++end;
-                    auto u = std::make_unique<Unit>();
-                    u->first = next;
-                    u->jobs = end - next;
-                    u->b = new clay_hip_batch();
-                    int st = guarded(h, [&] {
-                        flatten(h, queue.data() + next, end - next, *u->b);
-                        upload(h, *u->b, true);
-                    });
-                    if (st == CLAY_E_MEMORY && (!resident.empty() || !completing.empty())) {
-                        drop(*u);  // retry once memory is returned
-                        break;
-                    }
-                    if (st == CLAY_E_MEMORY) {
-                        drop(*u);
-                        clay_render_batch rejected{};
-                        rejected.failed_scratch_bytes = 1;
-                        if (publish(user, &rejected, next))
-                            throw Failure(CLAY_E_CALLBACK, "publish aborted the window");
+                        if (st == CLAY_E_MEMORY) {
+                            drop(*u);
+                            clay_render_batch rejected{};
+                            rejected.failed_scratch_bytes = 1;
+                            if (publish(user, &rejected, next))
+                                throw Failure(CLAY_E_CALLBACK, "publish aborted the window");
+                            next = end;
+                            continue;
+                        }
+                        if (st != CLAY_OK) {
+                            for (size_t r = k; r < pending.size(); ++r)
+                                drop(*made[r]);
+                            throw Failure(st, h->error);
+                        }
+                        u->begin = h->event();
+                        check(hipEventRecord(u->begin, h->stream), "hipEventRecord");
+                        ++h->stats.admissions;
                         next = end;
-                        continue;
+                        resident.push_back(std::move(u));
                     }
-                    if (st != CLAY_OK) {
-                        drop(*u);
-                        throw Failure(st, h->error);
-                    }
-                    u->begin = h->event();
-                    check(hipEventRecord(u->begin, h->stream), "hipEventRecord");
-                    ++h->stats.admissions;
-                    next = end;
-                    resident.push_back(std::move(u));
                 }
                 if (resident.empty() && completing.empty() && (exhausted || next == queue.size()))
                     break;
