@@ -28,6 +28,7 @@
 #include <cmath>
 #include <cstring>
 #include <deque>
+#include <functional>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -1587,9 +1588,45 @@ namespace {
 int renderWindowPlain(clay_hip* h, const clay_plan_view* plans, size_t count, clay_publish_fn publish,
                       clay_refill_fn refill, void* user)
 {
+    // The next batch's records are flattened on a worker thread while the
+    // current batch renders (flattening only reads the plans and the renderer's
+    // configuration); its upload waits for its turn. A range that changes (a
+    // halved batch) or a failed flatten falls back to the plain preparation.
+    struct Ahead {
+        size_t first = 0, end = 0;
+        clay_hip_batch* b = nullptr;
+        std::thread worker;
+        bool ok = false;
+        ~Ahead() { finish(); }
+        void finish()
+        {
+            if (worker.joinable())
+                worker.join();
+        }
+    } ahead;
+    const auto dropAhead = [&] {
+        ahead.finish();
+        if (ahead.b) {
+            release(h, *ahead.b);
+            delete ahead.b;
+            ahead.b = nullptr;
+        }
+    };
     return guarded(h, [&] {
         std::vector<clay_plan_view> queue(plans, plans + count);
         size_t first = 0;
+        // A batch holds up to batch_jobs jobs and never splits a volume group.
+        const auto batchEnd = [&](size_t from) {
+            size_t end = std::min(queue.size(), from + size_t(h->cfg.batch_jobs));
+            if (h->cfg.group_minimum)
+                while (end < queue.size() && queue[end].volume_group && queue[end].volume_group == queue[end - 1].volume_group)
+                    ++end;
+            return end;
+        };
+        struct Cleanup {
+            std::function<void()> f;
+            ~Cleanup() { f(); }
+        } cleanup{dropAhead};
         for (;;) {
             if (first == queue.size()) {
                 const clay_plan_view* more = nullptr;
@@ -1598,14 +1635,25 @@ int renderWindowPlain(clay_hip* h, const clay_plan_view* plans, size_t count, cl
                     break;
                 queue.insert(queue.end(), more, more + n);
             }
-            // A batch holds up to batch_jobs jobs and never splits a volume group.
-            size_t end = std::min(queue.size(), first + size_t(h->cfg.batch_jobs));
-            if (h->cfg.group_minimum)
-                while (end < queue.size() && queue[end].volume_group && queue[end].volume_group == queue[end - 1].volume_group)
-                    ++end;
+            size_t end = batchEnd(first);
             clay_hip_batch* b = nullptr;
             int status = CLAY_OK;
-            for (;;) {
+            ahead.finish();
+            if (ahead.b && ahead.ok && ahead.first == first && ahead.end == end) {
+                b = ahead.b;
+                ahead.b = nullptr;
+                status = guarded(h, [&] {
+                    check(hipSetDevice(h->device), "hipSetDevice");
+                    upload(h, *b);
+                });
+                if (status != CLAY_OK) {
+                    release(h, *b);
+                    delete b;
+                    b = nullptr;
+                }
+            }
+            dropAhead();
+            for (; !b;) {
                 status = clay_hip_batch_prepare(h, queue.data() + first, end - first, &b);
                 if (status != CLAY_E_MEMORY || end - first == 1)
                     break;
@@ -1622,6 +1670,23 @@ int renderWindowPlain(clay_hip* h, const clay_plan_view* plans, size_t count, cl
             }
             if (status != CLAY_OK)
                 throw Failure(status, h->error);
+            // Flatten the next batch while this one renders.
+            if (end < queue.size()) {
+                ahead.first = end;
+                ahead.end = batchEnd(end);
+                ahead.b = new clay_hip_batch();
+                ahead.ok = false;
+                const clay_plan_view* next = queue.data() + ahead.first;
+                const size_t n = ahead.end - ahead.first;
+                ahead.worker = std::thread([h, next, n, &ahead] {
+                    try {
+                        flatten(h, next, n, *ahead.b);
+                        ahead.ok = true;
+                    } catch (...) {
+                        ahead.ok = false;
+                    }
+                });
+            }
             double seconds = 0;
             status = clay_hip_batch_run(h, b, &seconds);
             std::vector<clay_tail_result> results(end - first);
