@@ -604,6 +604,38 @@ __device__ __forceinline__ double reciprocalD(double x)
     return fma(fma(-x, r, 1.0), r, r);
 }
 
+// sin and cos of an angle w in [0, pi] (a filter's angular frequency below
+// Nyquist): w = k pi/2 + r with k = 0, 1 or 2 and |r| <= pi/4 (pi/2 in two
+// parts, so k times the leading part is exact), then the Taylor series of sin
+// and cos in r to degree 17 and 18, whose truncation error is below 1e-19.
+__device__ __forceinline__ void sincosQuadrants(double w, double& s, double& c)
+{
+    constexpr double halfPiHi = 1.5707963267948966, halfPiLo = 6.123233995736766e-17;
+    const double k = rint(w * 0.63661977236758134);  // w / (pi/2), rounded
+    const double r = fma(-k, halfPiLo, fma(-k, halfPiHi, w)), r2 = r * r;
+    double sp = 1.0 / 355687428096000.0;  // 1/17!
+    sp = fma(sp, r2, -1.0 / 1307674368000.0);
+    sp = fma(sp, r2, 1.0 / 6227020800.0);
+    sp = fma(sp, r2, -1.0 / 39916800.0);
+    sp = fma(sp, r2, 1.0 / 362880.0);
+    sp = fma(sp, r2, -1.0 / 5040.0);
+    sp = fma(sp, r2, 1.0 / 120.0);
+    sp = fma(sp, r2, -1.0 / 6.0);
+    const double sr = fma(r * r2, sp, r);
+    double cp = -1.0 / 6402373705728000.0;  // -1/18!
+    cp = fma(cp, r2, 1.0 / 20922789888000.0);
+    cp = fma(cp, r2, -1.0 / 87178291200.0);
+    cp = fma(cp, r2, 1.0 / 479001600.0);
+    cp = fma(cp, r2, -1.0 / 3628800.0);
+    cp = fma(cp, r2, 1.0 / 40320.0);
+    cp = fma(cp, r2, -1.0 / 720.0);
+    cp = fma(cp, r2, 1.0 / 24.0);
+    cp = fma(cp, r2, -0.5);
+    const double cr = fma(r2, cp, 1.0);
+    s = k == 1 ? cr : k == 2 ? -sr : sr;
+    c = k == 1 ? -sr : k == 2 ? -cr : cr;
+}
+
 // `gainOfResonance` is resonanceGain(resonance) for the two-pole types. The
 // angular frequency multiplies by 2 pi / rate (loop-invariant for callers)
 // instead of dividing by the rate.
@@ -620,7 +652,7 @@ __device__ inline CoeffD coefficientsD(int type, double cutoff, double gainOfRes
         return {h, -h, 0, -p, 0};
     }
     double s, c;
-    sincos(w, &s, &c);
+    sincosQuadrants(w, s, c);
     if (type == CLAY_FILTER_PEAK) {
         const double a = exp10(fmin(60.0, fmax(-96.0, gain)) / 40);
         const double width = fmin(12.0, fmax(.01, bandwidth));
