@@ -81,6 +81,10 @@ struct GlobalTaps {
     struct Row {
         const float4* __restrict__ w;
         __device__ __forceinline__ float4 operator[](int q) const { return w[q]; }
+        // The row as stored, and the sample pair at mono position pair q: every
+        // row is stored, so no lane reads a mirror.
+        __device__ __forceinline__ float4 stored(int q) const { return w[q]; }
+        __device__ __forceinline__ uint32_t pair(uint32_t, uint32_t kept) const { return kept; }
     };
     __device__ __forceinline__ Row row(int phase) const
     {
@@ -99,37 +103,57 @@ struct SharedTaps {
     static constexpr int kPitch = T + 4;
     static constexpr int kRows = kPhases / 2 + 1;
     const float4* table;  // 16-byte aligned rows of kPitch floats (ds_read_b128)
-    // A lane walks its row forwards, or a mirrored row backwards from its last
-    // 16-byte group, and reverses the elements of each group with v_perm_b32
-    // (a byte select between two registers): branch-free, where a conditional
-    // element order compiled to a divergent branch per group.
+    // Weights in tap order (stereo and edge frames): a lane walks its row
+    // forwards, or a mirrored row backwards from its last 16-byte group, and
+    // reverses the elements of each group with v_perm_b32 (a byte select
+    // between two registers): branch-free, where a conditional element order
+    // compiled to a divergent branch per group. Mono frames read the stored
+    // row forwards and reverse the samples instead, one v_perm_b32 per pair of
+    // int16 samples rather than one per weight.
     struct Row {
-        const float4* w;  // group 0 in this lane's tap order
-        int dir;          // +1, or -1 for a mirrored row
-        uint32_t sel;     // v_perm_b32 selector: 0x03020100 keeps an element, 0x07060504 takes its mirror
+        const float4* w;     // group 0 in this lane's tap order
+        int dir;             // +1, or -1 for a mirrored row
+        uint32_t sel;        // v_perm_b32 selector: 0x03020100 keeps an element, 0x07060504 takes its mirror
+        const float4* base;  // the stored row (this lane's phase, or its mirror)
+        bool mirror;
         __device__ __forceinline__ float pick(float mirrored, float kept) const
         {
             return __uint_as_float(__builtin_amdgcn_perm(__float_as_uint(mirrored), __float_as_uint(kept), sel));
         }
+        // Group q of the weights in this lane's tap order.
         __device__ __forceinline__ float4 operator[](int q) const
         {
             const float4 r = w[q * dir];
             return make_float4(pick(r.w, r.x), pick(r.z, r.y), pick(r.y, r.z), pick(r.x, r.w));
+        }
+        // Group q of the row as stored. A mirrored lane pairs stored weight i
+        // with sample T - 1 - i instead: the same products.
+        __device__ __forceinline__ float4 stored(int q) const { return base[q]; }
+        // The two int16 samples at mono positions 2q, 2q + 1 from the aligned
+        // pairs `kept` (samples 2q, 2q + 1) and `mirrored` (samples T - 2 - 2q,
+        // T - 1 - 2q), swapped into reverse order for a mirrored lane.
+        __device__ __forceinline__ uint32_t pair(uint32_t mirrored, uint32_t kept) const
+        {
+            return __builtin_amdgcn_perm(mirrored, kept, mirror ? 0x05040706u : 0x03020100u);
         }
     };
     __device__ __forceinline__ Row row(int phase) const
     {
         const bool mirror = phase > kPhases / 2;
         const float4* start = table + (mirror ? kPhases - phase : phase) * (kPitch / 4);
-        return {mirror ? start + (T / 4 - 1) : start, mirror ? -1 : 1, mirror ? 0x07060504u : 0x03020100u};
+        return {mirror ? start + (T / 4 - 1) : start, mirror ? -1 : 1, mirror ? 0x07060504u : 0x03020100u, start, mirror};
     }
 };
 
 // Windowed-sinc tap sum at the nearest of 2048 phases (src/cuda/render.cu:30-61).
 // Accumulating the raw int16 values and scaling once by 2^-15 gives the same
 // float result as weighting the pre-scaled float bank of the reference.
-// `Plain`: the caller knows the voice is mono (see voiceAudio).
-template<int T, bool Plain = false, class Taps>
+// `Plain`: the caller knows the voice is mono (see voiceAudio). `Pairs`: mono
+// frames sum in the order that lets a mirrored LDS row reverse sample pairs
+// instead of weights (the audio pass); the mix keeps the tap order, whose
+// global-table form issues its weight and sample loads together. Either
+// order gives the same bits with either table, so both execution forms agree.
+template<int T, bool Plain = false, bool Pairs = false, class Taps>
 __device__ __forceinline__ float2 interpolateAt(const int16_t* __restrict__ bank, const Taps& taps, const Span& v,
                                                 double pos)
 {
@@ -158,15 +182,48 @@ __device__ __forceinline__ float2 interpolateAt(const int16_t* __restrict__ bank
 #pragma unroll
             for (int i = 0; i < T / 2 + 1; ++i)
                 dw[i] = words[i];
+            if constexpr (!Pairs) {
 #pragma unroll
-            for (int q = 0; q < T / 4; ++q) {
-                const float4 w = w4[q];
-                const uint32_t p0 = __builtin_amdgcn_alignbit(dw[2 * q + 1], dw[2 * q], shift);
-                const uint32_t p1 = __builtin_amdgcn_alignbit(dw[2 * q + 2], dw[2 * q + 1], shift);
-                a0 = fmaf(w.x, float(int16_t(p0 & 0xffffu)), a0);
-                a1 = fmaf(w.y, float(int16_t(p0 >> 16)), a1);
-                a0 = fmaf(w.z, float(int16_t(p1 & 0xffffu)), a0);
-                a1 = fmaf(w.w, float(int16_t(p1 >> 16)), a1);
+                for (int q = 0; q < T / 4; ++q) {
+                    const float4 w = w4[q];
+                    const uint32_t p0 = __builtin_amdgcn_alignbit(dw[2 * q + 1], dw[2 * q], shift);
+                    const uint32_t p1 = __builtin_amdgcn_alignbit(dw[2 * q + 2], dw[2 * q + 1], shift);
+                    a0 = fmaf(w.x, float(int16_t(p0 & 0xffffu)), a0);
+                    a1 = fmaf(w.y, float(int16_t(p0 >> 16)), a1);
+                    a0 = fmaf(w.z, float(int16_t(p1 & 0xffffu)), a0);
+                    a1 = fmaf(w.w, float(int16_t(p1 >> 16)), a1);
+                }
+                const float m = (a0 + a1) * scale;
+                return make_float2(m, m);
+            }
+            uint32_t p[T / 2], s[T / 2];
+#pragma unroll
+            for (int i = 0; i < T / 2; ++i)
+                p[i] = __builtin_amdgcn_alignbit(dw[i + 1], dw[i], shift);  // samples 2i, 2i + 1
+#pragma unroll
+            for (int i = 0; i < T / 2; ++i)
+                s[i] = w4.pair(p[T / 2 - 1 - i], p[i]);  // the samples at positions 2i, 2i + 1
+            // Weights as stored, positions 0 .. T/2 - 1 ascending into a0 and
+            // T - 1 .. T/2 descending into a1. A mirrored row holds the weights of
+            // tap T - 1 - i at position i, so its a0 and a1 sum the terms that a
+            // lane with the row unmirrored sums in a1 and a0, in the same order:
+            // the result does not depend on which of the two rows a lane reads.
+            const auto sample = [&](int i) {
+                const uint32_t x = s[i >> 1];
+                return float(int16_t(i & 1 ? x >> 16 : x & 0xffffu));
+            };
+#pragma unroll
+            for (int q = 0; q < T / 8; ++q) {
+                const float4 lw = w4.stored(q), hw = w4.stored(T / 4 - 1 - q);
+                const int i = 4 * q, j = T - 1 - 4 * q;
+                a0 = fmaf(lw.x, sample(i), a0);
+                a1 = fmaf(hw.w, sample(j), a1);
+                a0 = fmaf(lw.y, sample(i + 1), a0);
+                a1 = fmaf(hw.z, sample(j - 1), a1);
+                a0 = fmaf(lw.z, sample(i + 2), a0);
+                a1 = fmaf(hw.y, sample(j - 2), a1);
+                a0 = fmaf(lw.w, sample(i + 3), a0);
+                a1 = fmaf(hw.x, sample(j - 3), a1);
             }
             const float m = (a0 + a1) * scale;
             return make_float2(m, m);
@@ -206,7 +263,7 @@ __device__ __forceinline__ float2 interpolateAt(const int16_t* __restrict__ bank
 // short of registers) evaluates it per frame.
 __device__ __forceinline__ double bendDenominator(const Span& v) { return v.pitchSlope != 0 ? expm1(v.pitchSlope) : 1.0; }
 
-template<int T, bool Plain = false, class Taps>
+template<int T, bool Plain = false, bool Pairs = false, class Taps>
 __device__ __forceinline__ float2 interpolate(const int16_t* __restrict__ bank, const Taps& taps, const Span& v,
                                               int frame, double pos, double bend)
 {
@@ -235,10 +292,10 @@ __device__ __forceinline__ float2 interpolate(const int16_t* __restrict__ bank, 
         pos = __dadd_rn(double(v.loopFirst), fmod(__dsub_rn(pos, double(v.loopFirst)), double(period)));
     if (pos >= v.sourceEnd)
         return make_float2(0.f, 0.f);
-    float2 s = interpolateAt<T, Plain>(bank, taps, v, pos);
+    float2 s = interpolateAt<T, Plain, Pairs>(bank, taps, v, pos);
     const int fadeStart = v.loopEnd - v.loopCrossfade;
     if (v.loopEnd > 0 && v.loopCrossfade > 0 && pos >= fadeStart) {
-        const float2 entry = interpolateAt<T, Plain>(bank, taps, v, __dsub_rn(pos, double(period)));
+        const float2 entry = interpolateAt<T, Plain, Pairs>(bank, taps, v, __dsub_rn(pos, double(period)));
         const float phase = float((pos - fadeStart) / v.loopCrossfade);
         const float blend = .5f - .5f * cosf(3.14159265358979323846f * phase);
         s.x = fmaf(blend, entry.x - s.x, s.x);
@@ -272,7 +329,7 @@ __device__ __forceinline__ float2 panGains(const Span& v, float offset, PanCache
 // (src/cuda/render.cu:138-187). `Plain` compiles the common voice only: mono,
 // sampled (no generator), its position from the span (no pitch chain) and no
 // modulation that reaches the audio; the caller checks this per span.
-template<int T, bool Plain = false, class Taps>
+template<int T, bool Plain = false, bool Pairs = false, class Taps>
 __device__ __forceinline__ float2 voiceAudio(const int16_t* __restrict__ bank, const Taps& taps, const Span& v,
                                              const clay_control_point* __restrict__ points,
                                              const Mod* __restrict__ mods, float pan, int frame, bool filterInput,
@@ -299,7 +356,7 @@ __device__ __forceinline__ float2 voiceAudio(const int16_t* __restrict__ bank, c
     }
     if (db != 0)
         level *= exp2f(db * .166096404744368f);
-    const float2 s = interpolate<T, Plain>(bank, taps, v, frame, phase, bend);
+    const float2 s = interpolate<T, Plain, Pairs>(bank, taps, v, frame, phase, bend);
     if (filterInput && (Plain || v.channels == 1))
         return make_float2(s.x * level, s.x * level);
     const float2 g = panGains(v, pan, panCache);

@@ -274,7 +274,7 @@ __device__ __forceinline__ void chainAudio(const int16_t* __restrict__ bank, con
                     plain || v.channels == 1 ? 0.f : controlAt(points, j.controls[2], j.controls[3], frame);
                 PanCache panCache;  // unused: mono filter inputs are not panned here
                 const float2 a =
-                    voiceAudio<T, plain>(bank, taps, v, points, mods, pan, frame, true, ph, panCache, bend, audioMods);
+                    voiceAudio<T, plain, true>(bank, taps, v, points, mods, pan, frame, true, ph, panCache, bend, audioMods);
                 const float g = jobGain(points, j, frame);
                 float* dst = filtered + v.filtered + uint64_t(local) * uint64_t(plain ? 1 : v.channels);
                 dst[0] = a.x * g;
