@@ -35,7 +35,7 @@ __device__ __forceinline__ float envelopeSpan(const Span& v, int frame)
 // ---------------------------------------------------------------- controls
 // Last point at or before `frame` (the first point when none), then linear
 // (src/cuda/render.cu:107-118).
-__device__ __forceinline__ float controlAt(const clay_control_point* __restrict__ points, int lo, int hi, int frame)
+__device__ __forceinline__ int controlSegment(const clay_control_point* __restrict__ points, int lo, int hi, int frame)
 {
     while (lo + 1 < hi) {
         const int mid = lo + (hi - lo) / 2;
@@ -44,8 +44,17 @@ __device__ __forceinline__ float controlAt(const clay_control_point* __restrict_
         else
             hi = mid;
     }
-    const clay_control_point p = points[lo];
+    return lo;
+}
+
+__device__ __forceinline__ float pointAt(const clay_control_point& p, int frame)
+{
     return fmaf(p.step, float(frame - p.frame), p.value);
+}
+
+__device__ __forceinline__ float controlAt(const clay_control_point* __restrict__ points, int lo, int hi, int frame)
+{
+    return pointAt(points[controlSegment(points, lo, hi, frame)], frame);
 }
 
 __device__ __forceinline__ float modulationAt(const Mod& m, const clay_control_point* __restrict__ points, int frame)

@@ -262,6 +262,14 @@ __device__ __forceinline__ void chainAudio(const int16_t* __restrict__ bank, con
         for (int k = v.firstMod; k < v.firstMod + v.modCount; ++k)
             audioMods = audioMods || mods[k].target <= CLAY_MOD_POSITION;
         const double bend = bendDenominator(v);
+        // The job gain's tracks (volume and expression curves) usually stay in
+        // one segment over the span's frames: then every frame evaluates the
+        // same two points, read once here instead of searched per frame.
+        const int gainA = uniform(controlSegment(points, j.controls[0], j.controls[1], first));
+        const int gainB = uniform(controlSegment(points, j.controls[1], j.controls[2], first));
+        const bool fixedGain = gainA == controlSegment(points, j.controls[0], j.controls[1], stop - 1) &&
+                               gainB == controlSegment(points, j.controls[1], j.controls[2], stop - 1);
+        const clay_control_point pointA = points[gainA], pointB = points[gainB];
         // The frame loop compiled twice: for the common voice (voiceAudio's
         // `Plain`), and for any other.
         const auto frames = [&](auto plainTag) {
@@ -275,7 +283,7 @@ __device__ __forceinline__ void chainAudio(const int16_t* __restrict__ bank, con
                 PanCache panCache;  // unused: mono filter inputs are not panned here
                 const float2 a =
                     voiceAudio<T, plain, true>(bank, taps, v, points, mods, pan, frame, true, ph, panCache, bend, audioMods);
-                const float g = jobGain(points, j, frame);
+                const float g = fixedGain ? pointAt(pointA, frame) * pointAt(pointB, frame) : jobGain(points, j, frame);
                 float* dst = filtered + v.filtered + uint64_t(local) * uint64_t(plain ? 1 : v.channels);
                 dst[0] = a.x * g;
                 if (!plain && v.channels == 2)

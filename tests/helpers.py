@@ -117,6 +117,29 @@ def chain_shapes_case(root):
     return sfz, mid
 
 
+def gain_case(root):
+    """Volume (CC7) and expression (CC11) moving under held notes, so that the
+    job gain changes inside voice spans, for filtered mono and stereo voices
+    and a dry one; the controllers also step between notes."""
+    sfz, _ = chain_shapes_case(root)
+    text(root / "gain.sfz", open(sfz).read() + "<region> sample=shape_mono.wav lokey=90 hikey=99 pitch_keycenter=94 "
+         "ampeg_release=0.2\n")
+    events = []
+    rng = synth.SplitMix64(123)
+    tick = 0
+    for _ in range(16):
+        key, vel = rng.integer(40, 99), rng.integer(50, 127)
+        dur = rng.integer(300, 1200)
+        events.append((tick, 2, 0x90, key, vel))
+        events.append((tick + dur, 1, 0x80, key, 0))
+        for k in range(1, 4):
+            events.append((tick + k * dur // 4, 0, 0xB0, 7 if k % 2 else 11, rng.integer(20, 127)))
+        tick += rng.integer(100, 500)
+    mid = str(root / "gain.mid")
+    synth.write_midi(mid, events)
+    return str(root / "gain.sfz"), mid
+
+
 def rich_case(root):
     """Stereo sample with pan/width/position, loop crossfade, generators,
     velocity-modulated volume, a multi-section filter and a pitch-CC voice."""

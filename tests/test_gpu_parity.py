@@ -10,8 +10,8 @@ import pytest
 import claysampler_amd as ca
 from claysampler_amd import abi, synth
 
-from helpers import (assert_parity, chain_shapes_case, gpu_render, group_case, oracle_render, pedal_case, rich_case,
-                     segment_case, tail_case)
+from helpers import (assert_parity, chain_shapes_case, gain_case, gpu_render, group_case, oracle_render, pedal_case,
+                     rich_case, segment_case, tail_case)
 
 pytestmark = pytest.mark.gpu
 
@@ -98,6 +98,16 @@ def test_filter_chain_shapes(tmp_path):
     """Every compiled filter-chain shape (stereo, multi-section, one or more
     bindings, high-pass) against the oracle, in both output layouts."""
     sfz, mid = chain_shapes_case(tmp_path)
+    for policy in (abi.POLICY_NATURAL, abi.POLICY_ORIG):
+        s = ca.Session(policy=policy)
+        s.add_job(sfz, mid)
+        assert_parity(*both(s, policy, 32))
+
+
+def test_job_gain_automation(tmp_path):
+    """Volume and expression changing inside voice spans (the job gain read
+    per frame) and between them (one point per span), wet and dry voices."""
+    sfz, mid = gain_case(tmp_path)
     for policy in (abi.POLICY_NATURAL, abi.POLICY_ORIG):
         s = ca.Session(policy=policy)
         s.add_job(sfz, mid)
