@@ -245,6 +245,11 @@ struct clay_hip {
     // at DMA speed instead of through a pageable bounce.
     char* staging = nullptr;
     size_t stagingBytes = 0;
+    // Recorded behind the last copy out of `staging`: the image is rewritten or
+    // freed only after it (a caller may prepare a batch while another's
+    // upload is still in flight).
+    hipEvent_t stagingCopied = nullptr;
+    bool stagingPending = false;
     // Pinned staging of persistent-form uploads: a ring whose regions are
     // reused once the copy that read them has completed.
     struct Ring {
@@ -812,11 +817,16 @@ void upload(clay_hip* h, clay_hip_batch& b, bool streamOrdered = false)
     b.arenaBytes = a.bytes;
     auto* base = static_cast<char*>(b.arena);
     // Host image of the records: pinned ring (persistent form, the copy stays
-    // asynchronous) or a pageable buffer (plain form, which waits anyway).
+    // asynchronous) or the renderer's one pinned image, grown as needed (plain
+    // form, whose batches upload one at a time).
     char* image = nullptr;
     if (streamOrdered) {
         image = h->ring.take(recordBytes);
     } else {
+        if (h->stagingPending) {
+            check(hipEventSynchronize(h->stagingCopied), "record staging");
+            h->stagingPending = false;
+        }
         if (h->stagingBytes < recordBytes) {
             if (h->staging)
                 (void)hipHostFree(h->staging);
@@ -856,8 +866,14 @@ void upload(clay_hip* h, clay_hip_batch& b, bool streamOrdered = false)
     b.dOnsets = reinterpret_cast<int*>(put(oOnsets, b.onsets));
     b.dGroups = reinterpret_cast<int2*>(put(oGroups, b.groups));
     check(hipMemcpyAsync(base, image, recordBytes, hipMemcpyHostToDevice, h->stream), "batch upload");
-    if (streamOrdered)
+    if (streamOrdered) {
         h->ring.commit(h->stream);
+    } else {
+        if (!h->stagingCopied)
+            check(hipEventCreateWithFlags(&h->stagingCopied, hipEventDisableTiming), "hipEventCreate");
+        check(hipEventRecord(h->stagingCopied, h->stream), "hipEventRecord");
+        h->stagingPending = true;
+    }
     b.dStates = reinterpret_cast<VoiceState*>(base + oStates);
     b.dMix = base + oMix;
     b.dFiltered = reinterpret_cast<float*>(base + oFiltered);
@@ -1392,6 +1408,8 @@ int clay_hip_destroy(clay_hip* h)
     h->ring.release();
     if (h->staging)
         (void)hipHostFree(h->staging);
+    if (h->stagingCopied)
+        (void)hipEventDestroy(h->stagingCopied);
     for (auto& p : h->pinnedSizes)
         (void)hipHostFree(p.first);
     for (auto& stage : h->stages) {

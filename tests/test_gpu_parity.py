@@ -142,3 +142,15 @@ def test_batching_is_bit_exact(synth_root):
     assert len(pieces) == 3
     joined = np.concatenate([p for _, p in pieces])
     assert np.array_equal(joined, whole[1])
+    # Every batch prepared before any runs: each upload after the first
+    # rewrites the renderer's pinned record image while the copy before it
+    # may still be in flight.
+    plans = s.plans()
+    batches = [r.prepare((abi.PlanView * 1)(plans[j])) for j in range(3)]
+    ahead = []
+    for b in batches:
+        b.run()
+        ahead.append(b.results()[1])
+        b.close()
+    r.close()
+    assert np.array_equal(np.concatenate(ahead), whole[1])
