@@ -1098,6 +1098,7 @@ struct MixVoice {
     float left, right, pan, gain;
 };
 constexpr int kMixVoices = 64;  // voices staged per batch (more run in further batches)
+constexpr int kWetAhead = 4;    // scratch frames in flight per thread in the wet-only mix
 
 // Per output frame (one per thread of the tile's block): the dry voices of the
 // tile rendered inline, times the job gain, plus the filtered voices from
@@ -1106,8 +1107,10 @@ constexpr int kMixVoices = 64;  // voices staged per batch (more run in further 
 // writes float2, orig/custom short2. Every thread of the block calls this
 // (block barriers stage the voice records).
 // `lane` is the thread's frame within the tile and `sync` the barrier of the
-// threads that share `staged` (the block, or one wave).
-template<int T, bool FloatOut, class Taps, class Sync>
+// threads that share `staged` (the block, or one wave). `WetOnly`: the unit
+// has no dry voice, so the inline FIR is compiled out and kWetAhead voices'
+// scratch frames are read before any of them is summed (same sums, same order).
+template<int T, bool FloatOut, bool WetOnly = false, class Taps, class Sync>
 __device__ __forceinline__ void mixTile(const int16_t* __restrict__ bank, const Taps& taps,
                                         const Span* __restrict__ spans, const Job* __restrict__ jobs,
                                         const int* __restrict__ voices, const clay_control_point* __restrict__ points,
@@ -1155,38 +1158,61 @@ __device__ __forceinline__ void mixTile(const int16_t* __restrict__ bank, const 
         sync();
         if (!on)
             continue;
+        // A wet voice's scratch frame, panned when mono (render.cu:458-523).
+        const auto wetFrame = [&](const MixVoice& m, float2 s) {
+            if (m.channels == 1) {
+                float offset = pan;
+                if (m.panMods)
+                    for (int k = m.firstMod; k < m.firstMod + m.modCount; ++k) {
+                        const Mod md = mods[k];
+                        if (md.target == CLAY_MOD_PAN)
+                            offset += modulationAt(md, points, frame);
+                    }
+                // panGains (src/cuda/render.cu:130-136)
+                float gl = m.left, gr = m.right;
+                if (offset != 0) {
+                    const float p = fminf(1.f, fmaxf(-1.f, m.pan + offset));
+                    if (p != panCache.p) {
+                        panCache.p = p;
+                        panCache.a = cosf((1 + p) * kQuarterPi);
+                        panCache.b = cosf((1 - p) * kQuarterPi);
+                    }
+                    gl = m.gain * panCache.a;
+                    gr = m.gain * panCache.b;
+                }
+                s.x *= gl;
+                s.y *= gr;
+            }
+            wetLeft += s.x;
+            wetRight += s.y;
+        };
+        const auto read = [&](const MixVoice& m) {
+            const float* src = filtered + (m.base + int64_t(frame) * m.channels);
+            return make_float2(src[0], m.channels == 1 ? src[0] : src[1]);
+        };
+        if constexpr (WetOnly) {
+            for (int i = 0; i < n; i += kWetAhead) {
+                bool in[kWetAhead];
+                float2 got[kWetAhead];
+#pragma unroll
+                for (int k = 0; k < kWetAhead; ++k) {
+                    const MixVoice& m = staged[min(i + k, n - 1)];
+                    in[k] = i + k < n && frame >= m.begin && frame < m.stop;
+                    got[k] = in[k] ? read(m) : make_float2(0.f, 0.f);
+                }
+#pragma unroll
+                for (int k = 0; k < kWetAhead; ++k)
+                    if (in[k])
+                        wetFrame(staged[i + k], got[k]);
+            }
+            continue;
+        }
         for (int i = 0; i < n; ++i) {
             const MixVoice& m = staged[i];
             if (frame < m.begin || frame >= m.stop)
                 continue;
             if (m.wet) {
-                const float* src = filtered + (m.base + int64_t(frame) * m.channels);
-                float2 s = make_float2(src[0], m.channels == 1 ? src[0] : src[1]);
-                if (m.channels == 1) {
-                    float offset = pan;
-                    if (m.panMods)
-                        for (int k = m.firstMod; k < m.firstMod + m.modCount; ++k) {
-                            const Mod md = mods[k];
-                            if (md.target == CLAY_MOD_PAN)
-                                offset += modulationAt(md, points, frame);
-                        }
-                    // panGains (src/cuda/render.cu:130-136)
-                    float gl = m.left, gr = m.right;
-                    if (offset != 0) {
-                        const float p = fminf(1.f, fmaxf(-1.f, m.pan + offset));
-                        if (p != panCache.p) {
-                            panCache.p = p;
-                            panCache.a = cosf((1 + p) * kQuarterPi);
-                            panCache.b = cosf((1 - p) * kQuarterPi);
-                        }
-                        gl = m.gain * panCache.a;
-                        gr = m.gain * panCache.b;
-                    }
-                    s.x *= gl;
-                    s.y *= gr;
-                }
-                wetLeft += s.x;
-                wetRight += s.y;
+                wetFrame(m, read(m));
             } else {
                 const Span& v = spans[uniform(m.span)];
                 const float2 s = voiceAudio<T>(bank, taps, v, points, mods, pan, frame, false, -1.0, panCache);
@@ -1228,7 +1254,7 @@ clayRoundMixTiles(const int16_t* __restrict__ bank, const float* __restrict__ si
 }
 
 // Plain form: one unit, its pointers as kernel arguments, one block per tile.
-template<int T, bool FloatOut>
+template<int T, bool FloatOut, bool WetOnly>
 __global__ void __launch_bounds__(kTileFrames) __attribute__((amdgpu_waves_per_eu(mixOccupancy(T)))) clayMixTiles(
     const int16_t* __restrict__ bank, const float* __restrict__ sinc, const Span* __restrict__ spans,
     const Job* __restrict__ jobs, const Tile* __restrict__ tiles, const int* __restrict__ voices,
@@ -1237,7 +1263,7 @@ __global__ void __launch_bounds__(kTileFrames) __attribute__((amdgpu_waves_per_e
 {
     __shared__ MixVoice staged[kMixVoices];
     const Tile tile = tiles[firstTile + int(blockIdx.x)];
-    mixTile<T, FloatOut>(bank, GlobalTaps<T>{sinc}, spans, jobs, voices, points, mods, filtered, states, out, segFirst,
+    mixTile<T, FloatOut, WetOnly>(bank, GlobalTaps<T>{sinc}, spans, jobs, voices, points, mods, filtered, states, out, segFirst,
                          segEnd, tile, staged, int(threadIdx.x), int(threadIdx.x), [] { __syncthreads(); });
 }
 
@@ -1283,6 +1309,11 @@ __global__ void __launch_bounds__(kMixWavesShared * 64) clayRoundMixTilesShared(
     }, SharedUpTo<kMixSharedTaps>{});
 }
 
+#define CLAY_INSTANTIATE_MIX(T, FloatOut, WetOnly)                                                                     \
+    template __global__ void clayMixTiles<T, FloatOut, WetOnly>(const int16_t*, const float*, const Span*, const Job*, \
+                                                                const Tile*, const int*, const clay_control_point*,    \
+                                                                const Mod*, const float*, const VoiceState*, void*,    \
+                                                                int, int, int);
 #define CLAY_INSTANTIATE_TAPS(T)                                                                                       \
     template __global__ void clayRoundChainAudio<T>(const int16_t*, const float*, UnitsPtr, RoundPtr);                 \
     template __global__ void clayRoundMixTiles<T, true>(const int16_t*, const float*, UnitsPtr, RoundPtr);             \
@@ -1290,12 +1321,10 @@ __global__ void __launch_bounds__(kMixWavesShared * 64) clayRoundMixTilesShared(
     template __global__ void clayChainAudio<T>(const int16_t*, const float*, const Span*, const Job*, const int*,      \
                                                const clay_control_point*, const Mod*, const double*, float*,           \
                                                const VoiceState*, int, int, int);                                      \
-    template __global__ void clayMixTiles<T, true>(const int16_t*, const float*, const Span*, const Job*, const Tile*, \
-                                                   const int*, const clay_control_point*, const Mod*, const float*,     \
-                                                   const VoiceState*, void*, int, int, int);                            \
-    template __global__ void clayMixTiles<T, false>(const int16_t*, const float*, const Span*, const Job*,             \
-                                                    const Tile*, const int*, const clay_control_point*, const Mod*,     \
-                                                    const float*, const VoiceState*, void*, int, int, int);
+    CLAY_INSTANTIATE_MIX(T, true, false)                                                                                \
+    CLAY_INSTANTIATE_MIX(T, false, false)                                                                               \
+    CLAY_INSTANTIATE_MIX(T, true, true)                                                                                 \
+    CLAY_INSTANTIATE_MIX(T, false, true)
 CLAY_INSTANTIATE_TAPS(16)
 CLAY_INSTANTIATE_TAPS(32)
 template __global__ void clayRoundMixTilesShared<16, true>(const int16_t*, const float*, UnitsPtr, RoundPtr);

@@ -48,7 +48,7 @@ __global__ void clayChainAudio(const int16_t*, const float*, const Span*, const 
                                int, int);
 __global__ void clayVoiceChains(const Span*, const SpanCold*, const int*, const clay_control_point*, const Mod*, float*,
                                 VoiceState*, int, int, int, float);
-template<int T, bool FloatOut>
+template<int T, bool FloatOut, bool WetOnly>
 __global__ void clayMixTiles(const int16_t*, const float*, const Span*, const Job*, const Tile*, const int*,
                              const clay_control_point*, const Mod*, const float*, const VoiceState*, void*, int, int,
                              int);
@@ -167,6 +167,7 @@ struct clay_hip_batch {
     uint64_t frames = 0, filteredSamples = 0, pitchFrames = 0;
     int blocks = 0, windows = 0, bodyBlocks = 0, chains = 0;
     bool assess = false;
+    bool allWet = false;  // every span renders through a voice chain (no dry voice in the mix)
     // Device arena and views into it.
     void* arena = nullptr;
     size_t arenaBytes = 0;
@@ -571,7 +572,7 @@ void flatten(clay_hip* h, const clay_plan_view* plans, size_t count, clay_hip_ba
     };
     std::vector<std::vector<ChainInfo>> chainInfo(count);
     std::vector<std::vector<int>> pitchHeads(count);
-    std::vector<char> mixedChannels(count, 0);
+    std::vector<char> mixedChannels(count, 0), dry(count, 0);
     std::vector<std::exception_ptr> failed(count);
     forEachPlan([&](size_t i) {
         try {
@@ -692,6 +693,8 @@ void flatten(clay_hip* h, const clay_plan_view* plans, size_t count, clay_hip_ba
                 c.active[0] = v.active_envelope[0];
                 c.active[1] = v.active_envelope[1];
                 c.sampleRate = v.sample_rate;
+                if (v.filter_track < 0)
+                    dry[i] = 1;
                 if (v.filter_track >= 0) {
                     int& tail = chainTail[size_t(v.filter_track)];
                     int& slot = chainSlot[size_t(v.filter_track)];
@@ -730,6 +733,7 @@ void flatten(clay_hip* h, const clay_plan_view* plans, size_t count, clay_hip_ba
     for (size_t i = 0; i < count; ++i)
         if (mixedChannels[i])
             throw Failure(CLAY_E_INVALID, "a voice chain mixes mono and stereo spans");
+    b.allWet = std::find(dry.begin(), dry.end(), 1) == dry.end();
     std::vector<ChainInfo> chains;
     for (size_t i = 0; i < count; ++i) {
         chains.insert(chains.end(), chainInfo[i].begin(), chainInfo[i].end());
@@ -1053,14 +1057,15 @@ void launchWindow(clay_hip* h, const Window& w)
     }
     chunked(v.tileCount, [&](int first, int n) {
         h->launch("clayMixTiles", [&] {
+            const auto mix = [&](auto kernel) {
+                hipLaunchKernelGGL(kernel, dim3(n), dim3(kTileFrames), 0, s, h->bank, h->sinc, b.dSpans, b.dJobs,
+                                   b.dTiles, b.dVoices, b.dPoints, b.dMods, b.dFiltered, b.dStates, b.dMix, w.first,
+                                   w.end, v.tileFirst + first);
+            };
             if (h->natural)
-                hipLaunchKernelGGL((clayMixTiles<T, true>), dim3(n), dim3(kTileFrames), 0, s, h->bank, h->sinc, b.dSpans,
-                                   b.dJobs, b.dTiles, b.dVoices, b.dPoints, b.dMods, b.dFiltered, b.dStates, b.dMix,
-                                   w.first, w.end, v.tileFirst + first);
+                b.allWet ? mix(clayMixTiles<T, true, true>) : mix(clayMixTiles<T, true, false>);
             else
-                hipLaunchKernelGGL((clayMixTiles<T, false>), dim3(n), dim3(kTileFrames), 0, s, h->bank, h->sinc,
-                                   b.dSpans, b.dJobs, b.dTiles, b.dVoices, b.dPoints, b.dMods, b.dFiltered, b.dStates,
-                                   b.dMix, w.first, w.end, v.tileFirst + first);
+                b.allWet ? mix(clayMixTiles<T, false, true>) : mix(clayMixTiles<T, false, false>);
         });
     });
 }
