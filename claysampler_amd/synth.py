@@ -13,8 +13,10 @@ Choices the configurations leave open are fixed here:
       MIDI 60 s: Poisson onsets 8/s, key U{36..96}, velocity U{20..127},
       duration U[0.1, 1.5] s, at most 16 held notes; pitch bend and CC1 random
       walks at 20 Hz (bend step U{-1024..1024} clipped to +-8191, CC1 step U{-6..6}).
-  C3  C1 instrument; job length lognormal(ln 30 s, 1) clipped to [5, 300] s;
-      polyphony P ~ U{4..64} from overlapping notes of U[0.5, 2.0] s at rate P/1.25 s.
+  C3  C1 instrument with volume=-18 (up to 64 overlapping sines at amplitude
+      0.5 would otherwise exceed full scale and end in the `peak` status); job
+      length lognormal(ln 30 s, 1) clipped to [5, 300] s; polyphony
+      P ~ U{4..64} from overlapping notes of U[0.5, 2.0] s at rate P/1.25 s.
   C5  keys 21..108 x 4 velocity layers (1-31, 32-63, 64-95, 96-127) = 352
       regions of 2,840,000 frames at 44.1 kHz (a 2.0 GB int16 bank). Each region
       repeats one second of 8 harmonics (1/n, random phase, harmonics >= Nyquist
@@ -104,13 +106,13 @@ C1_SFZ = ("<region> sample=sine440.wav lokey=0 hikey=127 pitch_keycenter=69 loop
           "ampeg_attack=0.005 ampeg_decay=0.1 ampeg_sustain=70 ampeg_release=0.2\n")
 
 
-def c1_instrument(directory):
+def c1_instrument(directory, volume_db=None):
     os.makedirs(directory, exist_ok=True)
     n = np.arange(RATE)
     write_wav(os.path.join(directory, "sine440.wav"), np.rint(16384 * np.sin(2 * np.pi * 440 * n / RATE)))
-    sfz = os.path.join(directory, "c1.sfz")
+    sfz = os.path.join(directory, "c1.sfz" if volume_db is None else "c3.sfz")
     with open(sfz, "w") as f:
-        f.write(C1_SFZ)
+        f.write(C1_SFZ if volume_db is None else C1_SFZ.replace("<region> ", f"<region> volume={volume_db:g} "))
     return sfz
 
 
@@ -230,7 +232,7 @@ def c3_midi(path, seed, seconds, polyphony):
 
 
 def config3(directory, jobs=512, max_seconds=300.0):
-    sfz = c1_instrument(directory)
+    sfz = c1_instrument(directory, volume_db=-18)
     mids = []
     rng = SplitMix64(3)
     for j in range(jobs):

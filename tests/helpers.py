@@ -214,3 +214,104 @@ def assert_parity(got, want, lsb=1, extent=True):
             assert abs(g.peak - w.peak) <= lsb
             assert abs(g.first - w.first) <= 64 and abs(g.last - w.last) <= 64
     return gres, gpcm
+
+
+def quality_cases(root):
+    """One scenario per quality / group verdict (src/cuda/quality.cu:107-209):
+    name -> (policy, jobs [(sfz, midi, group)], audio config, group minimum,
+    expected status of each job)."""
+    presets, melody = group_case(root)
+    wav(root / "zero.wav", [0] * RATE)
+    presets["zero"] = text(root / "zero.sfz", "<region> sample=zero.wav key=60 loop_mode=one_shot amp_veltrack=0\n")
+    wav(root / "full.wav", [32767] * RATE)
+    presets["hot"] = text(root / "hot.sfz",
+                          "<region> sample=full.wav key=60 loop_mode=one_shot amp_veltrack=0 volume=20\n")
+    gap = smf(root / "gap2.mid", bytes.fromhex("00 90 3c 64 60 80 3c 00 96 40 90 3c 64 60 80 3c 00 00 ff 2f 00"))
+    notes = bytearray()
+    for k, vel in enumerate((100, 40, 127, 70, 90, 20)):
+        notes += bytes([0x00, 0x90, 0x3C, vel, 0x81, 0x70, 0x80, 0x3C, 0x00])
+        notes += bytes([0x60 if k % 2 else 0x20, 0xB0, 0x07, 127 - 10 * k])
+    notes += bytes.fromhex("00 ff 2f 00")
+    phrase = smf(root / "phrase.mid", bytes(notes))
+    base = abi.default_audio_config(RATE, True)
+
+    def audio(**fields):
+        a = abi.AudioConfig.from_buffer_copy(base)
+        for k, v in fields.items():
+            setattr(a, k, v)
+        return a
+
+    return {
+        "ok": (abi.POLICY_NATURAL, [(presets["soft"], phrase, 0)], base, 0, ["ok"]),
+        "quietPerformance": (abi.POLICY_NATURAL, [(presets["quiet"], melody, 0)], base, 0, ["quietPerformance"]),
+        "excessiveSilence": (abi.POLICY_NATURAL, [(presets["soft"], gap, 0)], audio(max_silent_frames=1.0 * RATE), 0,
+                             ["excessiveSilence"]),
+        "silentAudio": (abi.POLICY_NATURAL, [(presets["tiny"], melody, 0)],
+                        audio(min_median=-200.0, min_quartile=-200.0, silence_power=0.0, max_silent_fraction=1.0),
+                        0, ["silentAudio"]),
+        "clipping": (abi.POLICY_ORIG, [(presets["hot"], melody, 0)], base, 0, ["clipping"]),
+        "audioDuration": (abi.POLICY_NATURAL, [(presets["soft"], melody, 0)], audio(max_frames=RATE // 5), 0,
+                          ["audioDuration"]),
+        "silentBody": (abi.POLICY_NATURAL, [(presets["zero"], melody, 1), (presets["soft"], melody, 1),
+                                            (presets["loud"], phrase, 2), (presets["soft"], phrase, 2)],
+                       abi.default_audio_config(RATE, False), 2, ["silentBody", "incompleteGroup", "ok", "ok"]),
+        "orig_metrics": (abi.POLICY_ORIG, [(presets["loud"], phrase, 0), (presets["soft"], gap, 0)], base, 0,
+                         ["ok", "ok"]),
+    }
+
+
+def quality_session(case, rate=RATE):
+    policy, jobs, _, group_minimum, _ = case
+    s = ca.Session(policy=policy, measure=True, keep_onsets=True, rate=rate)
+    for sfz, mid, group in jobs:
+        s.add_job(sfz, mid, group)
+    return s
+
+
+def quality_measures(pcm, channels, frames, body, onsets, a):
+    """Quality measures of one job's PCM, restated with numpy."""
+    x = pcm.reshape(-1, channels).astype(np.int64)
+    mono = 2 * x[:, 0] if channels == 1 else x[:, 0] + x[:, 1]
+    power = mono * mono
+    end = min(frames, body)
+    span = 5 * a.window
+    chunk = 2 * a.rate
+    # Body windows: consecutive 250 ms windows restarting at every 2 s chunk.
+    starts = [c + w for c in range(0, end, chunk) for w in range(0, chunk, span) if c + w < end]
+    total, silent, run, longest = 0, 0, 0, 0
+    for i, first in enumerate(starts):
+        nxt = min(end, starts[i + 1] if i + 1 < len(starts) else first + min(span, chunk - (first % chunk)))
+        stop = min(min(body, len(power)), nxt)
+        wp = int(power[first:stop].sum())
+        total += wp
+        if wp < (nxt - first) * a.silence_power:
+            silent += nxt - first
+            run += nxt - first
+            longest = max(longest, run)
+        else:
+            run = 0
+    body_rms = math.sqrt(total / body) * 0.5 if body else 0.0
+    levels = []
+    for o in onsets:
+        best = max(int(power[o + w * a.window:min(o + (w + 1) * a.window, len(power))].sum()) for w in range(5))
+        levels.append(max(-120.0, 10 * math.log10(best / (a.window * 2.0 ** 32))) if best else -120.0)
+    median = float(np.percentile(levels, 50)) if levels else -120.0
+    quartile = float(np.percentile(levels, 25)) if levels else -120.0
+    clipped = int((np.abs(x) >= 32767).sum())
+    return dict(body_rms=body_rms, median=median, quartile=quartile, silent_fraction=silent / max(1, end),
+                longest_silence=longest, clipped=clipped, clipped_fraction=clipped / (max(1, frames) * channels),
+                levels=levels)
+
+
+def quality_verdict(m, frames, peak, a):
+    if frames > a.max_frames:
+        return "audioDuration"
+    if not m["levels"] or m["median"] < a.min_median or m["quartile"] < a.min_quartile:
+        return "quietPerformance"
+    if m["silent_fraction"] > a.max_silent_fraction or m["longest_silence"] > a.max_silent_frames:
+        return "excessiveSilence"
+    if peak < a.min_peak:
+        return "silentAudio"
+    if m["clipped_fraction"] > a.max_clipped_fraction:
+        return "clipping"
+    return "ok"
