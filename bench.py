@@ -81,6 +81,7 @@ def kernel_bytes(plans, taps, natural):
         "clayVoiceChains": 8 * wet_samples,
         # dry source windows + wet reads + the mix write + span records
         "clayMixTiles": src["dry"] + 4 * wet_samples + mix_bytes + spans * span_bytes,
+        "clayMixWaves": src["dry"] + 4 * wet_samples + mix_bytes + spans * span_bytes,
     }, voice_frames, frames
 
 
@@ -452,7 +453,7 @@ def main():
     os.environ["CLAY_KERNEL_TIMING"] = "0"
     bytes_per, voice_frames, frames = kernel_bytes(plans, taps, True)
     kernels = {}
-    for name in ("clayChainAudio", "clayVoiceChains", "clayMixTiles", "clayScanStereo", "clayQuantizeMono", "clayFinalizePcm",
+    for name in ("clayChainAudio", "clayVoiceChains", "clayMixTiles", "clayMixWaves", "clayScanStereo", "clayQuantizeMono", "clayFinalizePcm",
                  "clayPitchChains", "claySelectNative", "claySelectTrim"):
         ms, n = batch.kernel_time(name)
         if n:

@@ -740,3 +740,49 @@ __device__ inline CoeffD coefficientsD(int type, double cutoff, double gainOfRes
 }
 
 } // namespace
+
+// ----------------------------------------------------------------- round descriptors
+// The round descriptors are read through the constant address space: the
+// compiler then takes the pointers stored in them for global memory (global
+// and, for uniform addresses of read-only records, scalar loads instead of
+// flat loads). The per-item functions take them as __restrict__ parameters,
+// which is what licenses the scalar loads next to the stores of one kernel.
+#define CLAY_CONST __attribute__((address_space(4)))
+using UnitsPtr = const UnitView* __restrict__;
+using RoundPtr = const RoundHeader* __restrict__;
+
+__device__ __forceinline__ const CLAY_CONST UnitView* constant(const UnitView* p)
+{
+    return (const CLAY_CONST UnitView*)p;
+}
+
+template<class P>
+__device__ __forceinline__ P* global(P* p)
+{
+    return p;
+}
+
+// Resident unit of work item `item` given the exclusive prefix sums of the
+// round (units are few; the search runs on scalar registers).
+template<class F>
+__device__ __forceinline__ int unitOf(const RoundHeader& header, int item, F baseOf)
+{
+    int lo = 0, hi = header.units;
+    while (lo + 1 < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (baseOf(mid) <= item)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+// Wave-level barrier for a wave's own staging area (LDS operations of one wave
+// complete in order; this keeps the compiler from moving them across).
+__device__ __forceinline__ void waveSync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}

@@ -39,6 +39,11 @@ struct Span {
     float start, attackStep, decayRate, sustain, releaseValue, releaseRate, fadeValue, fadeStep;
     float gain, pan, left, right;
     float widthDirect, widthCross, positionLeft, positionRight, width, stereoPosition;
+    // 1: a mono sampled dry voice whose audio no modulation reaches (no
+    // generator, variable pitch or stereo matrix): the mix renders it with the
+    // integer FIR of the packed int16 weight table.
+    int32_t plain;
+    int32_t pad;
 };
 
 // Cold span record: filter slots and the filter/pitch envelopes.

@@ -1,0 +1,461 @@
+// Output mix of the render core for gfx950 (src/cuda/render.cu:21-187, :458-523).
+//
+// One wave renders one 256-frame tile of one job: lane l holds the frames
+// l, l + 64, l + 128 and l + 192 of the tile, and the wave walks the tile's
+// voices in span order, so the per-voice work (the span record in scalar
+// registers, the voice's kind, its envelope constants) is paid once per 256
+// frames instead of once per 64. Sixteen waves share a 1024-thread block, one
+// block per CU, grid-stride over the tiles of an XCD's share of the window.
+//
+// Plain voices (mono, sampled, dry, no modulation of the audio) take an
+// integer FIR: the sinc weights are stored as packed int16 pairs,
+//     w = hi * 2^-14 + lo * 2^-24,   hi = rint(w 2^14),  lo = rint((w - hi 2^-14) 2^24),
+// exact to 2^-25, and the int16 sample pairs of the bank are dotted with them
+// by v_dot2c_i32_i16, two taps per instruction, in exact int32 sums
+// (|sum hi s| < 1.6 * 2^29 and |sum lo s| <= T 2^9 2^15 <= 2^30 for T <= 64).
+// A 16-tap FIR is 16 dot instructions where the float FIR takes 16 converts
+// and 16 FMAs. For 16 taps the weight table lives in LDS as a mirrored half
+// table (82 KB). The envelope of a plain voice uses one exp2 per frame.
+// Other dry voices use the float FIR of voiceAudio, and filtered voices are
+// read from the chain scratch; every execution form inlines these same
+// functions, so their PCM is bit-identical.
+#include "common.h"
+
+namespace {
+
+constexpr int kMixWaves = 16;                  // tiles in flight per 1024-thread block
+constexpr int kLaneFrames = kTileFrames / 64;  // frames per lane of a tile
+constexpr int kXcds = 8;                       // MI355X: 8 XCDs, blocks dispatched round-robin over them
+constexpr int kXcdChunk = 64;                  // consecutive tiles (16384 frames of a job) kept on one XCD
+constexpr int kIntSharedTaps = 16;             // widest integer table held in LDS
+
+// Integer weights of the full table in global memory: (kPhases + 1) rows of T
+// dwords, the T/2 hi pairs then the T/2 lo pairs.
+template<int T>
+struct IntTapsGlobal {
+    static constexpr bool kMirrored = false;
+    const uint32_t* __restrict__ table;
+    struct Row {
+        const uint32_t* __restrict__ w;
+        uint32_t sel;
+        bool mirror;
+    };
+    __device__ __forceinline__ Row row(int phase) const { return {table + size_t(phase) * T, 0x03020100u, false}; }
+};
+
+// The same weights as a half table in LDS: rows 0..kPhases/2 at a pitch of
+// T + 4 dwords (rows of different lanes start on different banks); a phase
+// above kPhases/2 reads row kPhases - phase, whose taps run backwards, and
+// reverses its sample pairs instead (one v_perm_b32 per pair).
+template<int T>
+struct IntTapsShared {
+    static constexpr bool kMirrored = true;
+    static constexpr int kPitch = T + 4;
+    static constexpr int kRows = kPhases / 2 + 1;
+    const uint32_t* table;
+    struct Row {
+        const uint32_t* w;
+        uint32_t sel;  // 0x03020100 keeps pair i; 0x05040706 takes pair T/2-1-i with its halves swapped
+        bool mirror;
+    };
+    __device__ __forceinline__ Row row(int phase) const
+    {
+        const bool mirror = phase > kPhases / 2;
+        return {table + (mirror ? kPhases - phase : phase) * kPitch, mirror ? 0x05040706u : 0x03020100u, mirror};
+    }
+};
+
+__device__ __forceinline__ int dot2(uint32_t a, uint32_t b, int c)
+{
+    typedef short v2s __attribute__((ext_vector_type(2)));
+    return __builtin_amdgcn_sdot2(__builtin_bit_cast(v2s, a), __builtin_bit_cast(v2s, b), c, false);
+}
+
+// Source value from the two integer sums: (hi + lo 2^-10) 2^-14, scaled by 2^-15.
+__device__ __forceinline__ float fromSums(int hi, int lo) { return fmaf(float(lo), 0x1p-10f, float(hi)) * 0x1p-29f; }
+
+// FIR over the sample pairs p[i] = (s[2i], s[2i+1]) of the window (low half first).
+template<int T, class Taps>
+__device__ __forceinline__ float firPairs(const typename Taps::Row& row, const uint32_t (&p)[T / 2])
+{
+    int hi = 0, lo = 0;
+#pragma unroll
+    for (int g = 0; g < T / 8; ++g) {
+        const uint4 h = reinterpret_cast<const uint4*>(row.w)[g];
+        const uint4 l = reinterpret_cast<const uint4*>(row.w)[T / 8 + g];
+        const uint32_t hv[4] = {h.x, h.y, h.z, h.w}, lv[4] = {l.x, l.y, l.z, l.w};
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int i = 4 * g + k;
+            const uint32_t q = Taps::kMirrored ? __builtin_amdgcn_perm(p[T / 2 - 1 - i], p[i], row.sel) : p[i];
+            hi = dot2(q, hv[k], hi);
+            lo = dot2(q, lv[k], lo);
+        }
+    }
+    return fromSums(hi, lo);
+}
+
+// The same sums one tap at a time, zero outside [0, sourceEnd): windows at the
+// edges of the source.
+template<int T, class Taps>
+__device__ __forceinline__ float firEdge(const int16_t* __restrict__ bank, uint64_t sample, int sourceEnd, int centre,
+                         const typename Taps::Row& row)
+{
+    int hi = 0, lo = 0;
+#pragma unroll 1
+    for (int t = 0; t < T; ++t) {
+        const int i = centre - (T / 2 - 1) + t;
+        const int s = i >= 0 && i < sourceEnd ? int(bank[sample + uint64_t(i)]) : 0;
+        const int stored = row.mirror ? T - 1 - t : t;
+        const uint32_t h = row.w[stored >> 1], l = row.w[T / 2 + (stored >> 1)];
+        const int wh = int(int16_t(stored & 1 ? h >> 16 : h & 0xffffu));
+        const int wl = int(int16_t(stored & 1 ? l >> 16 : l & 0xffffu));
+        hi += wh * s;
+        lo += wl * s;
+    }
+    return fromSums(hi, lo);
+}
+
+// Source value of a plain voice at a position (src/cuda/render.cu:21-61): the
+// exact sample at an integer position, else the FIR at the nearest of 2048
+// phases. `active`: the lane renders this frame; inactive lanes load from a
+// valid dummy address and take no edge path.
+template<int T, class Taps>
+__device__ __forceinline__ float sourceAt(const int16_t* __restrict__ bank, const Taps& taps, const Span& v, double pos,
+                                          bool active, const uint32_t* __restrict__ dummy)
+{
+    const double floorPos = floor(pos);
+    const int centre = int(floorPos);
+    const int phase = __double2int_rn((pos - floorPos) * kPhases);
+    const auto row = taps.row(min(max(phase, 0), kPhases));
+    const bool interior = centre >= T / 2 - 1 && centre < v.sourceEnd - T / 2;
+    const uint64_t first = v.sample + uint64_t(int64_t(centre - (T / 2 - 1)));
+    // Dword-aligned window (the bank is padded for the dword past its end),
+    // realigned by 16 bits on an odd start.
+    const int odd = interior ? int(first & 1) : 0;
+    const uint32_t* __restrict__ words = interior ? reinterpret_cast<const uint32_t*>(bank + (first - uint64_t(odd)))
+                                                  : dummy;
+    uint32_t dw[T / 2 + 1];
+#pragma unroll
+    for (int i = 0; i < T / 2 + 1; ++i)
+        dw[i] = words[i];
+    uint32_t p[T / 2];
+#pragma unroll
+    for (int i = 0; i < T / 2; ++i)
+        p[i] = __builtin_amdgcn_alignbit(dw[i + 1], dw[i], odd ? 16u : 0u);
+    float s = firPairs<T, Taps>(row, p);
+    if (pos == floorPos) {
+        // Exact sample (render.cu:33-34): window tap T/2 - 1 when interior.
+        const uint32_t pair = p[(T / 2 - 1) >> 1];
+        const int x = interior ? int(int16_t(((T / 2 - 1) & 1) ? pair >> 16 : pair & 0xffffu))
+                               : (centre >= 0 && centre < v.sourceEnd ? int(bank[v.sample + uint64_t(centre)]) : 0);
+        s = float(x) * 0x1p-15f;
+    } else if (active && !interior) {
+        s = firEdge<T, Taps>(bank, v.sample, v.sourceEnd, centre, row);
+    }
+    return s;
+}
+
+// Source position of a bent span at `frame` (src/cuda/render.cu:63-70), `bend` = expm1(pitchSlope).
+__device__ __forceinline__ double bentPosition(const Span& v, int frame, double bend)
+{
+    const int elapsed = frame - v.begin;
+    const double distance = __ddiv_rn(expm1(__dmul_rn(v.pitchSlope, double(elapsed))), bend);
+    return __dadd_rn(v.position, __dmul_rn(distance, v.step));
+}
+
+// Loop wrap (src/cuda/render.cu:85-87).
+__device__ __forceinline__ double wrapLoop(double pos, int loopFirst, int period)
+{
+    return __dadd_rn(double(loopFirst), fmod(__dsub_rn(pos, double(loopFirst)), double(period)));
+}
+
+template<int T, class Taps>
+__device__ __forceinline__ float crossfadeEntry(const int16_t* __restrict__ bank, const Taps& taps, const Span& v,
+                                             double pos, bool active, const uint32_t* __restrict__ dummy)
+{
+    return sourceAt<T>(bank, taps, v, pos, active, dummy);
+}
+
+// Position, loop wrap and loop crossfade of a plain voice at `frame`
+// (src/cuda/render.cu:63-100 without generators).
+template<int T, class Taps>
+__device__ __forceinline__ float plainSource(const int16_t* __restrict__ bank, const Taps& taps, const Span& v,
+                                             int frame, double bend, bool active, const uint32_t* __restrict__ dummy)
+{
+    double pos = v.pitchSlope != 0 ? bentPosition(v, frame, bend)
+                                   : __dadd_rn(v.position, __dmul_rn(double(frame - v.begin), v.step));
+    const int period = v.loopEnd - v.loopFirst;
+    if (v.loopEnd > 0 && pos >= v.loopEnd)
+        pos = wrapLoop(pos, v.loopFirst, period);
+    if (pos >= v.sourceEnd)
+        return 0.f;
+    float s = sourceAt<T>(bank, taps, v, pos, active, dummy);
+    const int fadeStart = v.loopEnd - v.loopCrossfade;
+    if (v.loopEnd > 0 && v.loopCrossfade > 0 && pos >= fadeStart) {
+        const float entry = crossfadeEntry<T>(bank, taps, v, __dsub_rn(pos, double(period)), active, dummy);
+        const float phase = float((pos - fadeStart) / v.loopCrossfade);
+        const float blend = .5f - .5f * cosf(3.14159265358979323846f * phase);
+        s = fmaf(blend, entry - s, s);
+    }
+    return s;
+}
+
+// Amplitude envelope of a plain voice (src/cuda/render.cu:6-19) with one
+// exp2 per frame: the decay and release exponentials share the evaluation,
+// their rates given in log2 units.
+struct EnvelopeRates {
+    float decay2, release2;
+};
+
+__device__ __forceinline__ float envelopePlain(const Span& v, const EnvelopeRates& r, int frame)
+{
+    const bool release = v.releaseStart >= 0 && frame >= v.releaseStart;
+    const int origin = release ? v.releaseStart : v.decayStart;
+    const float e = __builtin_amdgcn_exp2f(float(frame - origin + 1) * (release ? r.release2 : r.decay2));
+    if (release)
+        return frame >= v.fadeStart ? fmaxf(0.f, v.fadeValue - float(frame - v.fadeStart + 1) * v.fadeStep)
+                                    : v.releaseValue * e;
+    if (frame < v.attackStart)
+        return v.start;
+    if (frame < v.holdStart)
+        return fminf(1.f, v.start + float(frame - v.attackStart + 1) * v.attackStep);
+    if (frame < v.decayStart)
+        return 1.f;
+    return v.decayRate == 0 ? v.sustain : fmaxf(v.sustain, e);
+}
+
+// Pan gains of a mono voice for a pan offset (src/cuda/render.cu:130-136),
+// the two cosines cached per lane while the clamped position repeats.
+__device__ __forceinline__ float2 monoGains(float left, float right, float gain, float pan, float offset,
+                                           PanCache& cache)
+{
+    if (offset == 0)
+        return make_float2(left, right);
+    const float p = fminf(1.f, fmaxf(-1.f, pan + offset));
+    if (p != cache.p) {
+        cache.p = p;
+        cache.a = cosf((1 + p) * kQuarterPi);
+        cache.b = cosf((1 - p) * kQuarterPi);
+    }
+    return make_float2(gain * cache.a, gain * cache.b);
+}
+
+// One tile by one wave. Every lane of the wave calls this; nothing in it
+// synchronises across waves.
+template<int T, bool FloatOut, bool Simple, class Taps>
+__device__ __forceinline__ void mixTileWave(const int16_t* __restrict__ bank, const Taps& taps,
+                                            const float* __restrict__ sinc, const uint32_t* __restrict__ dummy,
+                                            const Span* __restrict__ spans, const Job* __restrict__ jobs,
+                                            const int* __restrict__ voices,
+                                            const clay_control_point* __restrict__ points,
+                                            const Mod* __restrict__ mods, const float* __restrict__ filtered,
+                                            const VoiceState* __restrict__ states, void* __restrict__ out,
+                                            int segFirst, int segEnd, const Tile& tile, int lane)
+{
+    const Job& j = jobs[uniform(tile.job)];
+    int frame[kLaneFrames];
+    bool on[kLaneFrames];
+    float pan[kLaneFrames], left[kLaneFrames], right[kLaneFrames], wetLeft[kLaneFrames], wetRight[kLaneFrames];
+#pragma unroll
+    for (int k = 0; k < kLaneFrames; ++k) {
+        frame[k] = tile.frame + lane + 64 * k;
+        on[k] = lane + 64 * k < tile.frames && frame[k] >= segFirst && frame[k] < segEnd;
+        pan[k] = on[k] ? controlAt(points, j.controls[2], j.controls[3], frame[k]) : 0.f;
+        left[k] = right[k] = wetLeft[k] = wetRight[k] = 0.f;
+    }
+    PanCache panCache;
+    const int end = tile.first + tile.count;
+    for (int b0 = tile.first; b0 < end; b0 += 64) {
+        // Span indices of the next 64 voices, one per lane, read out uniformly.
+        const int held = b0 + lane < end ? voices[b0 + lane] : 0;
+        const int n = min(64, end - b0);
+        for (int i = 0; i < n; ++i) {
+            const int s = __builtin_amdgcn_readlane(held, i);
+            const Span& v = spans[s];
+            if (v.filterTrack >= 0) {
+                // A filtered voice from the chain scratch, panned when mono (render.cu:458-523).
+                int stop = v.end;
+                const VoiceState& st = states[v.filterTrack];
+                if (st.retired)
+                    stop = min(stop, st.silence);  // retired under the voice-silence policy
+                const int64_t base = int64_t(v.filtered) - int64_t(max(v.begin, segFirst)) * v.channels;
+#pragma unroll
+                for (int k = 0; k < kLaneFrames; ++k) {
+                    if (!on[k] || frame[k] < v.begin || frame[k] >= stop)
+                        continue;
+                    const float* src = filtered + (base + int64_t(frame[k]) * v.channels);
+                    float2 x = make_float2(src[0], v.channels == 1 ? src[0] : src[1]);
+                    if (v.channels == 1) {
+                        float offset = pan[k];
+                        for (int m = v.firstMod; m < v.firstMod + v.modCount; ++m) {
+                            const Mod md = mods[m];
+                            if (md.target == CLAY_MOD_PAN)
+                                offset += modulationAt(md, points, frame[k]);
+                        }
+                        const float2 g = monoGains(v.left, v.right, v.gain, v.pan, offset, panCache);
+                        x.x *= g.x;
+                        x.y *= g.y;
+                    }
+                    wetLeft[k] += x.x;
+                    wetRight[k] += x.y;
+                }
+            } else if (Simple || v.plain) {
+                // A plain voice; in a Simple launch every dry voice is plain
+                // and straight (no loop, no bend), checked by the host.
+                const EnvelopeRates rates{v.decayRate * 1.44269504088896341f, v.releaseRate * 1.44269504088896341f};
+                const double bend = !Simple && v.pitchSlope != 0 ? expm1(v.pitchSlope) : 1.0;
+#pragma unroll
+                for (int k = 0; k < kLaneFrames; ++k) {
+                    const bool act = on[k] && frame[k] >= v.begin && frame[k] < v.end;
+                    const float level = envelopePlain(v, rates, frame[k]);
+                    float src;
+                    if constexpr (Simple) {
+                        const double pos = __dadd_rn(v.position, __dmul_rn(double(frame[k] - v.begin), v.step));
+                        src = pos >= v.sourceEnd ? 0.f : sourceAt<T>(bank, taps, v, pos, act, dummy);
+                    } else {
+                        src = plainSource<T>(bank, taps, v, frame[k], bend, act, dummy);
+                    }
+                    if (act && level > 0) {
+                        const float2 g = monoGains(v.left, v.right, v.gain, v.pan, pan[k], panCache);
+                        left[k] += src * g.x * level;
+                        right[k] += src * g.y * level;
+                    }
+                }
+            } else {
+                // Any other dry voice: the float FIR of voiceAudio, one frame at a time.
+#pragma unroll
+                for (int k = 0; k < kLaneFrames; ++k) {
+                    if (!on[k] || frame[k] < v.begin || frame[k] >= v.end)
+                        continue;
+                    const float2 x = voiceAudio<T>(bank, GlobalTaps<T>{sinc}, v, points, mods, pan[k], frame[k], false,
+                                                   -1.0, panCache);
+                    left[k] += x.x;
+                    right[k] += x.y;
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < kLaneFrames; ++k) {
+        if (!on[k])
+            continue;
+        const float amp = jobGain(points, j, frame[k]);
+        const float2 mixed = make_float2(fmaf(left[k], amp, wetLeft[k]), fmaf(right[k], amp, wetRight[k]));
+        const uint64_t at = j.mix + uint64_t(frame[k]);
+        if (FloatOut) {
+            static_cast<float2*>(out)[at] = mixed;
+        } else {
+            const short l = short(__float2int_rn(fminf(32767.f, fmaxf(-32768.f, mixed.x * 32768.f))));
+            const short r = short(__float2int_rn(fminf(32767.f, fmaxf(-32768.f, mixed.y * 32768.f))));
+            static_cast<short2*>(out)[at] = make_short2(l, r);
+        }
+    }
+}
+
+// Runs body(taps) with the integer weights: for T <= kIntSharedTaps the half
+// table staged in this block's LDS, else the global table.
+template<int T, class Body>
+__device__ __forceinline__ void withIntTaps(const uint32_t* __restrict__ table, Body&& body)
+{
+    if constexpr (T <= kIntSharedTaps) {
+        using S = IntTapsShared<T>;
+        __shared__ uint4 shared[S::kRows * S::kPitch / 4];
+        for (int i = int(threadIdx.x); i < S::kRows * (T / 4); i += int(blockDim.x)) {
+            const int row = i / (T / 4), q = i % (T / 4);
+            shared[row * (S::kPitch / 4) + q] = reinterpret_cast<const uint4*>(table + size_t(row) * T)[q];
+        }
+        __syncthreads();
+        body(S{reinterpret_cast<const uint32_t*>(shared)});
+    } else {
+        body(IntTapsGlobal<T>{table});
+    }
+}
+
+// Tile n of this wave's XCD share: tiles go to XCDs in chunks of kXcdChunk
+// consecutive tiles, round-robin, so the consecutive tiles of a job (the same
+// voices, the same source windows) meet in one L2.
+struct XcdWalk {
+    int xcd, wave, waves;
+    __device__ XcdWalk(int waveInBlock)
+    {
+        const int perXcd = max(1, int(gridDim.x) / kXcds);
+        xcd = int(blockIdx.x) % kXcds;
+        wave = (int(blockIdx.x) / kXcds) * kMixWaves + waveInBlock;
+        waves = perXcd * kMixWaves;
+        if (int(blockIdx.x) >= perXcd * kXcds)
+            wave = -1;  // a block past the last full group of kXcds renders nothing
+    }
+    __device__ __forceinline__ int tile(int n) const { return ((n / kXcdChunk) * kXcds + xcd) * kXcdChunk + n % kXcdChunk; }
+};
+
+} // namespace
+
+// Plain form: one unit, pointers as kernel arguments; tiles [firstTile, firstTile + tileCount).
+template<int T, bool FloatOut, bool Simple>
+__global__ void __launch_bounds__(64 * kMixWaves) clayMixWaves(
+    const int16_t* __restrict__ bank, const uint32_t* __restrict__ intTable, const float* __restrict__ sinc,
+    const Span* __restrict__ spans, const Job* __restrict__ jobs, const Tile* __restrict__ tiles,
+    const int* __restrict__ voices, const clay_control_point* __restrict__ points, const Mod* __restrict__ mods,
+    const float* __restrict__ filtered, const VoiceState* __restrict__ states, void* __restrict__ out, int segFirst,
+    int segEnd, int firstTile, int tileCount)
+{
+    const int lane = int(threadIdx.x) & 63;
+    const XcdWalk walk(uniform(int(threadIdx.x) >> 6));
+    withIntTaps<T>(intTable, [&](const auto& taps) {
+        if (walk.wave < 0)
+            return;
+        for (int n = walk.wave;; n += walk.waves) {
+            const int t = walk.tile(n);
+            if (t >= tileCount)
+                break;
+            const Tile tile = tiles[firstTile + t];
+            mixTileWave<T, FloatOut, Simple>(bank, taps, sinc, intTable, spans, jobs, voices, points, mods, filtered, states,
+                                     out, segFirst, segEnd, tile, lane);
+        }
+    });
+}
+
+// Persistent form: the tiles of every unit of the round.
+template<int T, bool FloatOut, bool Simple>
+__global__ void __launch_bounds__(64 * kMixWaves) clayRoundMixWaves(const int16_t* __restrict__ bank,
+                                                                    const uint32_t* __restrict__ intTable,
+                                                                    const float* __restrict__ sinc, UnitsPtr units,
+                                                                    RoundPtr round)
+{
+    const int lane = int(threadIdx.x) & 63;
+    const XcdWalk walk(uniform(int(threadIdx.x) >> 6));
+    const RoundHeader header = *round;
+    withIntTaps<T>(intTable, [&](const auto& taps) {
+        if (walk.wave < 0)
+            return;
+        for (int n = walk.wave;; n += walk.waves) {
+            const int item = walk.tile(n);
+            if (item >= header.tiles)
+                break;
+            const int unit = unitOf(header, item, [&](int i) { return units[i].tileBase; });
+            const auto& u = constant(units)[unit];
+            const Tile tile = global(u.tiles)[u.tileFirst + item - u.tileBase];
+            mixTileWave<T, FloatOut, Simple>(bank, taps, sinc, intTable, global(u.spans), global(u.jobs), global(u.voices),
+                                     global(u.points), global(u.mods), global(u.filtered), global(u.states),
+                                     global(u.mix), u.segFirst, u.segEnd, tile, lane);
+        }
+    });
+}
+
+#define CLAY_INSTANTIATE_MIX_WAVES(T, FloatOut, Simple)                                                                \
+    template __global__ void clayMixWaves<T, FloatOut, Simple>(const int16_t*, const uint32_t*, const float*,         \
+                                                               const Span*, const Job*, const Tile*, const int*,      \
+                                                               const clay_control_point*, const Mod*, const float*,   \
+                                                               const VoiceState*, void*, int, int, int, int);          \
+    template __global__ void clayRoundMixWaves<T, FloatOut, Simple>(const int16_t*, const uint32_t*, const float*,    \
+                                                                    UnitsPtr, RoundPtr);
+#define CLAY_INSTANTIATE_MIX_WAVES_TAPS(T)                                                                             \
+    CLAY_INSTANTIATE_MIX_WAVES(T, true, true)                                                                          \
+    CLAY_INSTANTIATE_MIX_WAVES(T, false, true)                                                                         \
+    CLAY_INSTANTIATE_MIX_WAVES(T, true, false)                                                                         \
+    CLAY_INSTANTIATE_MIX_WAVES(T, false, false)
+CLAY_INSTANTIATE_MIX_WAVES_TAPS(16)
+CLAY_INSTANTIATE_MIX_WAVES_TAPS(32)
+CLAY_INSTANTIATE_MIX_WAVES_TAPS(48)
+CLAY_INSTANTIATE_MIX_WAVES_TAPS(64)

@@ -15,42 +15,6 @@
 
 // ====================================================================== render
 
-// The round descriptors are read through the constant address space: the
-// compiler then takes the pointers stored in them for global memory (global
-// and, for uniform addresses of read-only records, scalar loads instead of
-// flat loads). The per-item functions take them as __restrict__ parameters,
-// which is what licenses the scalar loads next to the stores of one kernel.
-#define CLAY_CONST __attribute__((address_space(4)))
-using UnitsPtr = const UnitView* __restrict__;
-using RoundPtr = const RoundHeader* __restrict__;
-
-__device__ __forceinline__ const CLAY_CONST UnitView* constant(const UnitView* p)
-{
-    return (const CLAY_CONST UnitView*)p;
-}
-
-template<class P>
-__device__ __forceinline__ P* global(P* p)
-{
-    return p;
-}
-
-// Resident unit of work item `item` given the exclusive prefix sums of the
-// round (units are few; the search runs on scalar registers).
-template<class F>
-__device__ __forceinline__ int unitOf(const RoundHeader& header, int item, F baseOf)
-{
-    int lo = 0, hi = header.units;
-    while (lo + 1 < hi) {
-        const int mid = (lo + hi) >> 1;
-        if (baseOf(mid) <= item)
-            lo = mid;
-        else
-            hi = mid;
-    }
-    return lo;
-}
-
 // Variable-pitch recurrence of one voice chain: per-frame source phase
 // (src/cuda/render.cu:222-266), in FP64.
 __device__ void pitchChain(const Span* __restrict__ spans, const SpanCold* __restrict__ cold,
@@ -1235,24 +1199,6 @@ __device__ __forceinline__ void mixTile(const int16_t* __restrict__ bank, const 
     }
 }
 
-// One block per tile of the round (256 frames, one per thread), grid-stride.
-template<int T, bool FloatOut>
-__global__ void __launch_bounds__(kTileFrames) __attribute__((amdgpu_waves_per_eu(mixOccupancy(T))))
-clayRoundMixTiles(const int16_t* __restrict__ bank, const float* __restrict__ sinc, UnitsPtr units, RoundPtr round)
-{
-    __shared__ MixVoice staged[kMixVoices];
-    const RoundHeader header = *round;
-    for (int item = int(blockIdx.x); item < header.tiles; item += int(gridDim.x)) {
-        const int unit = unitOf(header, item, [&](int i) { return units[i].tileBase; });
-        const auto& u = constant(units)[unit];
-        const Tile tile = global(u.tiles)[u.tileFirst + item - u.tileBase];
-        mixTile<T, FloatOut>(bank, GlobalTaps<T>{sinc}, global(u.spans), global(u.jobs), global(u.voices),
-                             global(u.points), global(u.mods), global(u.filtered), global(u.states), global(u.mix),
-                             u.segFirst, u.segEnd, tile, staged, int(threadIdx.x), int(threadIdx.x),
-                             [] { __syncthreads(); });
-    }
-}
-
 // Plain form: one unit, its pointers as kernel arguments, one block per tile.
 template<int T, bool FloatOut, bool WetOnly>
 __global__ void __launch_bounds__(kTileFrames) __attribute__((amdgpu_waves_per_eu(mixOccupancy(T)))) clayMixTiles(
@@ -1267,48 +1213,6 @@ __global__ void __launch_bounds__(kTileFrames) __attribute__((amdgpu_waves_per_e
                          segEnd, tile, staged, int(threadIdx.x), int(threadIdx.x), [] { __syncthreads(); });
 }
 
-// Persistent-form mix with the sinc half table in LDS, for 16 taps (82 KB of
-// table next to 57 KB of voice staging): one 1024-thread block per CU, each
-// wave rendering a quarter tile (64 frames) at a time from voices it stages
-// itself, grid-stride, so the dry voices' FIR reads its weights from LDS
-// instead of gathering random rows from L2, with no block barrier after the
-// table load. The plain form keeps the 256-thread kernel: at one block per CU
-// (4 waves per SIMD) the dry FIR's bank gathers are less well hidden than at
-// the 8 waves per SIMD of that kernel, which a short window can afford to fill.
-constexpr int kMixSharedTaps = 16;
-constexpr int kMixWavesShared = 16;
-
-// Wave-level barrier for a wave's own staging area (LDS operations of one wave
-// complete in order; this keeps the compiler from moving them across).
-__device__ __forceinline__ void waveSync()
-{
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-template<int T, bool FloatOut>
-__global__ void __launch_bounds__(kMixWavesShared * 64) clayRoundMixTilesShared(
-    const int16_t* __restrict__ bank, const float* __restrict__ sinc, UnitsPtr units, RoundPtr round)
-{
-    __shared__ MixVoice staged[kMixWavesShared][kMixVoices];
-    const RoundHeader header = *round;
-    withAudioTaps<T>(sinc, [&](const auto& taps) {
-        const int wave = uniform(int(threadIdx.x) >> 6), lane = int(threadIdx.x) & 63;
-        const int quarters = header.tiles * (kTileFrames / 64);
-        for (int item = int(blockIdx.x) * kMixWavesShared + wave; item < quarters;
-             item += int(gridDim.x) * kMixWavesShared) {
-            const int t = item / (kTileFrames / 64), quarter = item % (kTileFrames / 64);
-            const int unit = unitOf(header, t, [&](int i) { return units[i].tileBase; });
-            const auto& u = constant(units)[unit];
-            const Tile tile = global(u.tiles)[u.tileFirst + t - u.tileBase];
-            mixTile<T, FloatOut>(bank, taps, global(u.spans), global(u.jobs), global(u.voices), global(u.points),
-                                 global(u.mods), global(u.filtered), global(u.states), global(u.mix), u.segFirst,
-                                 u.segEnd, tile, staged[wave], 64 * quarter + lane, lane, waveSync);
-        }
-    }, SharedUpTo<kMixSharedTaps>{});
-}
-
 #define CLAY_INSTANTIATE_MIX(T, FloatOut, WetOnly)                                                                     \
     template __global__ void clayMixTiles<T, FloatOut, WetOnly>(const int16_t*, const float*, const Span*, const Job*, \
                                                                 const Tile*, const int*, const clay_control_point*,    \
@@ -1316,18 +1220,12 @@ __global__ void __launch_bounds__(kMixWavesShared * 64) clayRoundMixTilesShared(
                                                                 int, int, int);
 #define CLAY_INSTANTIATE_TAPS(T)                                                                                       \
     template __global__ void clayRoundChainAudio<T>(const int16_t*, const float*, UnitsPtr, RoundPtr);                 \
-    template __global__ void clayRoundMixTiles<T, true>(const int16_t*, const float*, UnitsPtr, RoundPtr);             \
-    template __global__ void clayRoundMixTiles<T, false>(const int16_t*, const float*, UnitsPtr, RoundPtr);            \
     template __global__ void clayChainAudio<T>(const int16_t*, const float*, const Span*, const Job*, const int*,      \
                                                const clay_control_point*, const Mod*, const double*, float*,           \
                                                const VoiceState*, int, int, int);                                      \
-    CLAY_INSTANTIATE_MIX(T, true, false)                                                                                \
-    CLAY_INSTANTIATE_MIX(T, false, false)                                                                               \
     CLAY_INSTANTIATE_MIX(T, true, true)                                                                                 \
     CLAY_INSTANTIATE_MIX(T, false, true)
 CLAY_INSTANTIATE_TAPS(16)
 CLAY_INSTANTIATE_TAPS(32)
-template __global__ void clayRoundMixTilesShared<16, true>(const int16_t*, const float*, UnitsPtr, RoundPtr);
-template __global__ void clayRoundMixTilesShared<16, false>(const int16_t*, const float*, UnitsPtr, RoundPtr);
 CLAY_INSTANTIATE_TAPS(48)
 CLAY_INSTANTIATE_TAPS(64)

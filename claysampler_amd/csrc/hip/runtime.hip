@@ -57,11 +57,13 @@ extern "C" __global__ void clayRoundPitchChains(const UnitView*, const RoundHead
 template<int T>
 __global__ void clayRoundChainAudio(const int16_t*, const float*, const UnitView*, const RoundHeader*);
 __global__ void clayRoundVoiceChains(const UnitView*, const RoundHeader*);
-template<int T, bool FloatOut>
-__global__ void clayRoundMixTiles(const int16_t*, const float*, const UnitView*, const RoundHeader*);
-// The same with the sinc half table in LDS (16 taps): one 1024-thread block per CU, a quarter tile per wave.
-template<int T, bool FloatOut>
-__global__ void clayRoundMixTilesShared(const int16_t*, const float*, const UnitView*, const RoundHeader*);
+// Mix (mix.hip): one wave per tile, integer FIR for plain voices; plain and persistent forms.
+template<int T, bool FloatOut, bool Simple>
+__global__ void clayMixWaves(const int16_t*, const uint32_t*, const float*, const Span*, const Job*, const Tile*,
+                             const int*, const clay_control_point*, const Mod*, const float*, const VoiceState*, void*,
+                             int, int, int, int);
+template<int T, bool FloatOut, bool Simple>
+__global__ void clayRoundMixWaves(const int16_t*, const uint32_t*, const float*, const UnitView*, const RoundHeader*);
 // Finishing kernels (finish.hip).
 extern "C" __global__ void clayScanStereo(const Job*, int, const float2*, StereoStats*, int);
 extern "C" __global__ void claySelectNative(const Job*, int, const float2*, const StereoStats*, clay_tail_result*);
@@ -142,6 +144,32 @@ std::vector<float> sincTable(int taps)
     return t;
 }
 
+// The phase-major table as the mix's integer weights: w = hi 2^-14 + lo 2^-24
+// with hi = rint(w 2^14) and lo = rint((w - hi 2^-14) 2^24), exact to 2^-25.
+// Row p holds T dwords: the pairs (hi[2j], hi[2j+1]) for j < T/2 (low half
+// first), then the pairs of lo. Mirror-symmetric like the float table, since
+// hi and lo are functions of w alone.
+std::vector<uint32_t> intSincTable(const std::vector<float>& phaseMajor, int taps)
+{
+    std::vector<uint32_t> out(phaseMajor.size());
+    for (int p = 0; p <= kPhases; ++p)
+        for (int j = 0; j < taps / 2; ++j) {
+            uint32_t hi = 0, lo = 0;
+            for (int k = 0; k < 2; ++k) {
+                const double w = phaseMajor[size_t(p) * size_t(taps) + size_t(2 * j + k)];
+                const long h = std::lrint(w * 16384.0);
+                const long l = std::lrint((w - double(h) / 16384.0) * 16777216.0);
+                if (h < -32768 || h > 32767 || l < -32768 || l > 32767)
+                    throw Failure(CLAY_E_DEVICE, "sinc weight outside the integer table's range");
+                hi |= uint32_t(uint16_t(int16_t(h))) << (16 * k);
+                lo |= uint32_t(uint16_t(int16_t(l))) << (16 * k);
+            }
+            out[size_t(p) * size_t(taps) + size_t(j)] = hi;
+            out[size_t(p) * size_t(taps) + size_t(taps / 2 + j)] = lo;
+        }
+    return out;
+}
+
 struct Timer {
     std::string name;
     hipEvent_t begin, end;
@@ -168,6 +196,7 @@ struct clay_hip_batch {
     int blocks = 0, windows = 0, bodyBlocks = 0, chains = 0;
     bool assess = false;
     bool allWet = false;  // every span renders through a voice chain (no dry voice in the mix)
+    bool simpleMix = false;  // every dry voice is plain with no loop and no bend (the mix's Simple variant)
     // Device arena and views into it.
     void* arena = nullptr;
     size_t arenaBytes = 0;
@@ -205,6 +234,7 @@ struct clay_hip {
     hipStream_t stream = nullptr;
     hipDeviceProp_t props{};
     float* sinc = nullptr;       // phase-major taps x (phases+1)
+    uint32_t* sincInt = nullptr; // the same weights as packed int16 hi/lo pairs (intSincTable)
     int16_t* bank = nullptr;
     uint64_t bankCount = 0, bankCapacity = 0;
     std::string error;
@@ -239,6 +269,7 @@ struct clay_hip {
     struct Graph {
         hipGraphExec_t exec;
         int mixBlocks, chainBlocks;
+        bool simpleMix;
         const void* bank;
     };
     std::vector<Graph> graphs;
@@ -572,7 +603,7 @@ void flatten(clay_hip* h, const clay_plan_view* plans, size_t count, clay_hip_ba
     };
     std::vector<std::vector<ChainInfo>> chainInfo(count);
     std::vector<std::vector<int>> pitchHeads(count);
-    std::vector<char> mixedChannels(count, 0), dry(count, 0);
+    std::vector<char> mixedChannels(count, 0), dry(count, 0), complexDry(count, 0);
     std::vector<std::exception_ptr> failed(count);
     forEachPlan([&](size_t i) {
         try {
@@ -687,6 +718,11 @@ void flatten(clay_hip* h, const clay_plan_view* plans, size_t count, clay_hip_ba
                 d.positionRight = v.position_right;
                 d.width = v.width;
                 d.stereoPosition = v.stereo_position;
+                bool audioMods = false;
+                for (int k = 0; k < v.modulation_count; ++k)
+                    audioMods = audioMods || p.modulation[v.first_modulation + k].target <= CLAY_MOD_POSITION;
+                d.plain = v.filter_track < 0 && v.channels == 1 && v.generator == CLAY_GEN_NONE && !v.variable_pitch &&
+                          !v.stereo_matrix && !audioMods;
                 SpanCold c{};
                 std::memcpy(c.filters, v.filters, sizeof c.filters);
                 std::memcpy(c.extra, v.extra_envelope, sizeof c.extra);
@@ -695,6 +731,8 @@ void flatten(clay_hip* h, const clay_plan_view* plans, size_t count, clay_hip_ba
                 c.sampleRate = v.sample_rate;
                 if (v.filter_track < 0)
                     dry[i] = 1;
+                if (v.filter_track < 0 && (!d.plain || v.loop_end > 0 || v.pitch_slope != 0))
+                    complexDry[i] = 1;
                 if (v.filter_track >= 0) {
                     int& tail = chainTail[size_t(v.filter_track)];
                     int& slot = chainSlot[size_t(v.filter_track)];
@@ -734,6 +772,7 @@ void flatten(clay_hip* h, const clay_plan_view* plans, size_t count, clay_hip_ba
         if (mixedChannels[i])
             throw Failure(CLAY_E_INVALID, "a voice chain mixes mono and stereo spans");
     b.allWet = std::find(dry.begin(), dry.end(), 1) == dry.end();
+    b.simpleMix = std::find(complexDry.begin(), complexDry.end(), 1) == complexDry.end();
     std::vector<ChainInfo> chains;
     for (size_t i = 0; i < count; ++i) {
         chains.insert(chains.end(), chainInfo[i].begin(), chainInfo[i].end());
@@ -971,13 +1010,23 @@ UnitView viewOf(const Window& w)
 // Launch shape of a round: grids of the three render kernels (0 = sized to the work).
 struct Shape {
     int mixBlocks = 0, chainBlocks = 0, pitchBlocks = 0, audioBlocks = 0;
+    bool simpleMix = false;  // every resident unit's dry voices are plain and straight
 };
 
 constexpr int kChainWavesPerBlock = 2;   // render.hip: voice chains per 128-thread filter block
 constexpr int kAudioWavesPerBlock = 16;  // render.hip: voice chains per 1024-thread audio block
 constexpr int kSharedTapsMax = 32;       // render.hip: widest sinc table the audio pass keeps in LDS
-constexpr int kMixSharedTaps = 16;       // render.hip: sinc width of the mix with the table in LDS
-constexpr int kMixWavesShared = 16;      // render.hip: waves (64-frame quarter tiles) per block of that mix
+constexpr int kMixWaves = 16;            // mix.hip: tiles (one per wave) in flight per 1024-thread block
+constexpr int kXcds = 8;                 // mix.hip: XCDs the mix blocks are spread over
+
+// Grid of the wave mix for `tiles` tiles: one 1024-thread block per CU at
+// most, a multiple of the XCD count (the kernel maps blocks to XCDs).
+int mixGrid(clay_hip* h, int tiles)
+{
+    const int cus = std::max(kXcds, h->props.multiProcessorCount / kXcds * kXcds);
+    const int wanted = int(std::min<int64_t>(cus, (int64_t(tiles) + kMixWaves - 1) / kMixWaves));
+    return std::max(kXcds, (wanted + kXcds - 1) / kXcds * kXcds);
+}
 
 // Persistent form: the round kernels with the fixed grids of `shape`.
 template<int T>
@@ -1006,23 +1055,16 @@ void launchRound(clay_hip* h, const Shape& shape, bool timed)
         hipLaunchKernelGGL(clayRoundVoiceChains, dim3(shape.chainBlocks), dim3(64 * kChainWavesPerBlock), 0, s, units,
                            round);
     });
-    run("clayRoundMixTiles", [&] {
-        if constexpr (T == kMixSharedTaps) {
-            const int blocks = std::max(1, h->props.multiProcessorCount);
-            if (h->natural)
-                hipLaunchKernelGGL((clayRoundMixTilesShared<T, true>), dim3(blocks), dim3(64 * kMixWavesShared),
-                                   0, s, h->bank, h->sinc, units, round);
-            else
-                hipLaunchKernelGGL((clayRoundMixTilesShared<T, false>), dim3(blocks),
-                                   dim3(64 * kMixWavesShared), 0, s, h->bank, h->sinc, units, round);
-            return;
-        }
-        if (h->natural)
-            hipLaunchKernelGGL((clayRoundMixTiles<T, true>), dim3(shape.mixBlocks), dim3(kTileFrames), 0, s, h->bank,
-                               h->sinc, units, round);
+    run("clayRoundMixWaves", [&] {
+        const int blocks = mixGrid(h, 1 << 30);
+        const auto mix = [&](auto kernel) {
+            hipLaunchKernelGGL(kernel, dim3(blocks), dim3(64 * kMixWaves), 0, s, h->bank, h->sincInt, h->sinc, units,
+                               round);
+        };
+        if (shape.simpleMix)
+            h->natural ? mix(clayRoundMixWaves<T, true, true>) : mix(clayRoundMixWaves<T, false, true>);
         else
-            hipLaunchKernelGGL((clayRoundMixTiles<T, false>), dim3(shape.mixBlocks), dim3(kTileFrames), 0, s, h->bank,
-                               h->sinc, units, round);
+            h->natural ? mix(clayRoundMixWaves<T, true, false>) : mix(clayRoundMixWaves<T, false, false>);
     });
 }
 
@@ -1055,19 +1097,33 @@ void launchWindow(clay_hip* h, const Window& w)
                                b.dFiltered, b.dStates, w.first, w.end, chainCount, h->silenceLevel);
         });
     }
-    chunked(v.tileCount, [&](int first, int n) {
-        h->launch("clayMixTiles", [&] {
-            const auto mix = [&](auto kernel) {
-                hipLaunchKernelGGL(kernel, dim3(n), dim3(kTileFrames), 0, s, h->bank, h->sinc, b.dSpans, b.dJobs,
-                                   b.dTiles, b.dVoices, b.dPoints, b.dMods, b.dFiltered, b.dStates, b.dMix, w.first,
-                                   w.end, v.tileFirst + first);
-            };
-            if (h->natural)
-                b.allWet ? mix(clayMixTiles<T, true, true>) : mix(clayMixTiles<T, true, false>);
-            else
-                b.allWet ? mix(clayMixTiles<T, false, true>) : mix(clayMixTiles<T, false, false>);
+    if (b.allWet) {
+        // No dry voice: the scratch-only mix, four voices' frames in flight per thread.
+        chunked(v.tileCount, [&](int first, int n) {
+            h->launch("clayMixTiles", [&] {
+                const auto mix = [&](auto kernel) {
+                    hipLaunchKernelGGL(kernel, dim3(n), dim3(kTileFrames), 0, s, h->bank, h->sinc, b.dSpans, b.dJobs,
+                                       b.dTiles, b.dVoices, b.dPoints, b.dMods, b.dFiltered, b.dStates, b.dMix,
+                                       w.first, w.end, v.tileFirst + first);
+                };
+                h->natural ? mix(clayMixTiles<T, true, true>) : mix(clayMixTiles<T, false, true>);
+            });
         });
-    });
+        return;
+    }
+    if (v.tileCount)
+        h->launch("clayMixWaves", [&] {
+            const int blocks = mixGrid(h, v.tileCount);
+            const auto mix = [&](auto kernel) {
+                hipLaunchKernelGGL(kernel, dim3(blocks), dim3(64 * kMixWaves), 0, s, h->bank, h->sincInt, h->sinc,
+                                   b.dSpans, b.dJobs, b.dTiles, b.dVoices, b.dPoints, b.dMods, b.dFiltered, b.dStates,
+                                   b.dMix, w.first, w.end, v.tileFirst, v.tileCount);
+            };
+            if (b.simpleMix)
+                h->natural ? mix(clayMixWaves<T, true, true>) : mix(clayMixWaves<T, false, true>);
+            else
+                h->natural ? mix(clayMixWaves<T, true, false>) : mix(clayMixWaves<T, false, false>);
+        });
 }
 
 void launchRoundTaps(clay_hip* h, const Shape& shape, bool timed)
@@ -1134,7 +1190,8 @@ void renderRoundFixed(clay_hip* h, const std::vector<Window>& windows, const Sha
     }
     const clay_hip::Graph* found = nullptr;
     for (const auto& g : h->graphs)
-        if (g.mixBlocks == shape.mixBlocks && g.chainBlocks == shape.chainBlocks && g.bank == h->bank)
+        if (g.mixBlocks == shape.mixBlocks && g.chainBlocks == shape.chainBlocks && g.simpleMix == shape.simpleMix &&
+            g.bank == h->bank)
             found = &g;
     if (!found) {
         hipGraph_t graph = nullptr;
@@ -1152,7 +1209,7 @@ void renderRoundFixed(clay_hip* h, const std::vector<Window>& windows, const Sha
         const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
         (void)hipGraphDestroy(graph);
         check(e, "hipGraphInstantiate");
-        h->graphs.push_back({exec, shape.mixBlocks, shape.chainBlocks, h->bank});
+        h->graphs.push_back({exec, shape.mixBlocks, shape.chainBlocks, shape.simpleMix, h->bank});
         found = &h->graphs.back();
     }
     if (h->timing) {
@@ -1369,6 +1426,9 @@ int clay_hip_create(const clay_hip_config* config, clay_hip** out)
                 phaseMajor[size_t(p) * size_t(taps) + size_t(t)] = tapMajor[size_t(t) * (kPhases + 1) + size_t(p)];
         check(hipMalloc(&h->sinc, bytesOf(phaseMajor)), "sinc table");
         check(hipMemcpy(h->sinc, phaseMajor.data(), bytesOf(phaseMajor), hipMemcpyHostToDevice), "sinc upload");
+        const auto packed = intSincTable(phaseMajor, taps);
+        check(hipMalloc(&h->sincInt, bytesOf(packed)), "integer sinc table");
+        check(hipMemcpy(h->sincInt, packed.data(), bytesOf(packed), hipMemcpyHostToDevice), "integer sinc upload");
         // Round descriptors: header + one view per resident unit.
         h->unitCapacity = size_t(std::min<uint64_t>(h->cfg.batch_jobs, 1u << 16)) + 1;
         const size_t roundBytes = 256 + h->unitCapacity * sizeof(UnitView);
@@ -1429,6 +1489,8 @@ int clay_hip_destroy(clay_hip* h)
         (void)hipFree(h->bank);
     if (h->sinc)
         (void)hipFree(h->sinc);
+    if (h->sincInt)
+        (void)hipFree(h->sincInt);
     if (h->stream)
         (void)hipStreamDestroy(h->stream);
     delete h;
@@ -1769,15 +1831,11 @@ struct RoundTuner {
         tuning = !(h->cfg.flags & CLAY_FLAG_NO_TUNE);
         if (!tuning)
             return;
-        for (int seg : {1 << 16, 1 << 17, 1 << 18}) {
-            if (fixedSegment > 0 && seg != fixedSegment)
-                continue;
-            for (int blocks : {cus * 8, cus * 32})
-                candidates.emplace_back(seg, blocks);
-        }
+        for (int seg : {1 << 16, 1 << 17, 1 << 18})
+            if (fixedSegment <= 0 || seg == fixedSegment)
+                candidates.emplace_back(seg, mixBlocks);
         if (fixedSegment > 0 && candidates.empty())
-            for (int blocks : {cus * 8, cus * 32})
-                candidates.emplace_back(fixedSegment, blocks);
+            candidates.emplace_back(fixedSegment, mixBlocks);
         cost.assign(candidates.size(), 0);
         trials.assign(candidates.size(), 0);
     }
@@ -1942,7 +2000,9 @@ int renderWindowPersistent(clay_hip* h, const clay_plan_view* plans, size_t coun
                     const auto shape = tuner.pick();
                     windows.clear();
                     uint64_t frames = 0;
+                    bool simpleMix = true;
                     for (auto& u : resident) {
+                        simpleMix = simpleMix && u->b->simpleMix;
                         const int first = u->cursor, end = std::min(u->b->lastFrame, first + shape.first);
                         int hi = 0;
                         windowChains(*u->b, first, end, u->chainLo, hi);
@@ -1955,6 +2015,7 @@ int renderWindowPersistent(clay_hip* h, const clay_plan_view* plans, size_t coun
                     fixed.chainBlocks = cus * 6;
                     fixed.pitchBlocks = cus;
                     fixed.audioBlocks = cus;
+                    fixed.simpleMix = simpleMix;
                     const bool measure = tuner.tuning && h->stats.rounds > 0;
                     if (measure)
                         check(hipEventRecord(roundBegin, h->stream), "hipEventRecord");

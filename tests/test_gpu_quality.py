@@ -36,7 +36,7 @@ def render_both(case, taps=32):
     r.close()
     want = oracle.render(oracle.config(policy=policy, taps=taps, audio=audio, group_minimum=group_minimum), bp, bc,
                          plans)
-    return plans, got, want
+    return s, plans, got, want  # the session owns the memory the plans point into
 
 
 @pytest.mark.parametrize("name", ["ok", "quietPerformance", "excessiveSilence", "silentAudio", "clipping",
@@ -45,7 +45,7 @@ def test_quality_verdicts_match_the_oracle(tmp_path, name):
     case = quality_cases(tmp_path)[name]
     policy, _, a, group_minimum, want_status = case
     channels = 1 if policy == abi.POLICY_NATURAL else 2
-    plans, (gres, gpcm), (wres, wpcm) = render_both(case)
+    session, plans, (gres, gpcm), (wres, wpcm) = render_both(case)
     print(f"case {name}: {[abi.TAIL_STATUS[r.status] for r in gres]}", flush=True)
     assert [abi.TAIL_STATUS[r.status] for r in gres] == want_status
     lsb = 1 + max([int(np.ceil(r.audio.gain)) for r in wres] + [0])

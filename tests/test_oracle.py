@@ -185,6 +185,16 @@ def test_mono_average_rounds_ties_to_even():
         assert mono(l, r) == round((l + r) / 2)
 
 
+def test_mono_average_turns_two_channel_flips_into_two_steps():
+    """When both channels of a frame sit at a .5 tie, rounding each the other
+    way moves their sum by 2, and the ties-to-even average then moves by 2:
+    the natural policy's mono PCM can differ by 2 where the float mix differs
+    by far less than 1 LSB (the GPU parity check accepts exactly these)."""
+    mono = oracle.lib().clay_oracle_mono
+    assert mono(-4258, -4311) == -4284 and mono(-4259, -4312) == -4286
+    assert mono(10, 11) == 10 and mono(11, 12) == 12
+
+
 # ------------------------------------------------------------------ known answers of the reference tests
 
 def session_render(policy, jobs, taps=48, **kw):
