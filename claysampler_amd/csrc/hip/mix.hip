@@ -156,6 +156,56 @@ __device__ __forceinline__ float sourceAt(const int16_t* __restrict__ bank, cons
     return s;
 }
 
+// Per-voice constants of a straight plain voice (no loop, no bend): the
+// voice's samples as a base pointer with 32-bit offsets, and the last window
+// start that stays inside the source.
+struct Straight {
+    const int16_t* __restrict__ base;  // bank + sample - parity: dword aligned
+    int parity;                        // sample & 1
+    int last;                          // last interior window start (centre - (T/2 - 1)), -1 for none
+};
+
+// Source value of a straight plain voice at `pos` (src/cuda/render.cu:21-61),
+// as sourceAt but with every lane loading a window clamped into the source
+// (the bank is padded by 2 x CLAY_MAX_SINC_TAPS samples), so no lane takes a
+// different address path; edges and integer positions outside the interior
+// take the rare branch.
+template<int T, class Taps>
+__device__ __forceinline__ float straightSource(const int16_t* __restrict__ bank, const Taps& taps, const Span& v,
+                                                const Straight& sv, double pos, bool active)
+{
+    const double floorPos = floor(pos);
+    const int centre = int(floorPos);
+    const double frac = pos - floorPos;
+    const int phase = min(max(__double2int_rn(frac * kPhases), 0), kPhases);
+    const auto row = taps.row(phase);
+    const int start = centre - (T / 2 - 1);
+    const bool interior = sv.last >= 0 && uint32_t(start) <= uint32_t(sv.last);
+    const int at = min(max(start, 0), max(sv.last, 0)) + sv.parity;  // window start from `base`
+    const int odd = at & 1;
+    const uint32_t* __restrict__ words = reinterpret_cast<const uint32_t*>(sv.base + uint32_t(at & ~1));
+    uint32_t dw[T / 2 + 1];
+#pragma unroll
+    for (int i = 0; i < T / 2 + 1; ++i)
+        dw[i] = words[i];
+    const uint32_t shift = uint32_t(odd) << 4;
+    uint32_t p[T / 2];
+#pragma unroll
+    for (int i = 0; i < T / 2; ++i)
+        p[i] = __builtin_amdgcn_alignbit(dw[i + 1], dw[i], shift);
+    float s = firPairs<T, Taps>(row, p);
+    // Exact sample at an integer position (render.cu:33-34): window tap T/2 - 1.
+    const uint32_t pair = p[(T / 2 - 1) >> 1];
+    const int exact = int(int16_t(((T / 2 - 1) & 1) ? pair >> 16 : pair & 0xffffu));
+    s = frac == 0 ? float(exact) * 0x1p-15f : s;
+    if (active && !interior && centre < v.sourceEnd) {
+        // Window past either end of the source.
+        s = frac == 0 ? (centre >= 0 ? float(bank[v.sample + uint64_t(centre)]) * 0x1p-15f : 0.f)
+                      : firEdge<T, Taps>(bank, v.sample, v.sourceEnd, centre, row);
+    }
+    return centre < v.sourceEnd ? s : 0.f;
+}
+
 // Source position of a bent span at `frame` (src/cuda/render.cu:63-70), `bend` = expm1(pitchSlope).
 __device__ __forceinline__ double bentPosition(const Span& v, int frame, double bend)
 {
@@ -201,28 +251,48 @@ __device__ __forceinline__ float plainSource(const int16_t* __restrict__ bank, c
     return s;
 }
 
-// Amplitude envelope of a plain voice (src/cuda/render.cu:6-19) with one
-// exp2 per frame: the decay and release exponentials share the evaluation,
-// their rates given in log2 units.
-struct EnvelopeRates {
-    float decay2, release2;
+// Amplitude envelope of a plain voice (src/cuda/render.cu:6-19). Its six
+// stages share one formula with stage parameters:
+//     n = frame - origin + 1,
+//     level = isExp ? max(lo, scale 2^(rate n)) : min(hi, max(lo, base + slope n)),
+// rates in log2 units (one v_exp_f32). Before the attack: base = start;
+// attack: start + n attackStep below 1; hold: 1; decay: max(sustain,
+// 2^(rate n)), or sustain for a zero rate; release: releaseValue 2^(rate n);
+// fade: max(0, fadeValue - n fadeStep). A release overrides the earlier stages.
+struct EnvStage {
+    int origin;
+    float slope, base, rate, scale, lo, hi;
+    bool isExp;
 };
 
-__device__ __forceinline__ float envelopePlain(const Span& v, const EnvelopeRates& r, int frame)
+__device__ __forceinline__ int envStageIndex(const Span& v, int frame)
 {
-    const bool release = v.releaseStart >= 0 && frame >= v.releaseStart;
-    const int origin = release ? v.releaseStart : v.decayStart;
-    const float e = __builtin_amdgcn_exp2f(float(frame - origin + 1) * (release ? r.release2 : r.decay2));
-    if (release)
-        return frame >= v.fadeStart ? fmaxf(0.f, v.fadeValue - float(frame - v.fadeStart + 1) * v.fadeStep)
-                                    : v.releaseValue * e;
-    if (frame < v.attackStart)
-        return v.start;
-    if (frame < v.holdStart)
-        return fminf(1.f, v.start + float(frame - v.attackStart + 1) * v.attackStep);
-    if (frame < v.decayStart)
-        return 1.f;
-    return v.decayRate == 0 ? v.sustain : fmaxf(v.sustain, e);
+    if (v.releaseStart >= 0 && frame >= v.releaseStart)
+        return frame >= v.fadeStart ? 5 : 4;
+    return frame < v.attackStart ? 0 : frame < v.holdStart ? 1 : frame < v.decayStart ? 2 : 3;
+}
+
+// Parameters of stage `k` (selects, no branches: `k` may differ per lane).
+__device__ __forceinline__ EnvStage envStage(const Span& v, float decay2, float release2, int k)
+{
+    constexpr float inf = __builtin_huge_valf();
+    EnvStage s;
+    s.isExp = k == 4 || (k == 3 && v.decayRate != 0);
+    s.origin = k == 1 ? v.attackStart : k == 3 ? v.decayStart : k == 4 ? v.releaseStart : v.fadeStart;
+    s.slope = k == 1 ? v.attackStep : k == 5 ? -v.fadeStep : 0.f;
+    s.base = k <= 1 ? v.start : k == 2 ? 1.f : k == 3 ? v.sustain : v.fadeValue;
+    s.rate = k == 4 ? release2 : decay2;
+    s.scale = k == 4 ? v.releaseValue : 1.f;
+    s.lo = k == 3 ? v.sustain : k == 5 ? 0.f : -inf;
+    s.hi = k == 1 ? 1.f : inf;
+    return s;
+}
+
+__device__ __forceinline__ float envLevel(const EnvStage& s, int frame)
+{
+    const float n = float(frame - s.origin + 1);
+    return s.isExp ? fmaxf(s.lo, s.scale * __builtin_amdgcn_exp2f(n * s.rate))
+                   : fminf(s.hi, fmaxf(s.lo, fmaf(n, s.slope, s.base)));
 }
 
 // Pan gains of a mono voice for a pan offset (src/cuda/render.cu:130-136),
@@ -241,6 +311,48 @@ __device__ __forceinline__ float2 monoGains(float left, float right, float gain,
     return make_float2(gain * cache.a, gain * cache.b);
 }
 
+// Accumulates a plain voice's frames into the lane's dry sums: the envelope
+// stage and the pan gains are taken once per voice when they hold over the
+// whole tile (the usual case), per frame otherwise.
+template<class Source>
+__device__ __forceinline__ void plainFrames(const Span& v, const int (&frame)[kLaneFrames],
+                                            const bool (&on)[kLaneFrames], const float (&pan)[kLaneFrames],
+                                            bool panUniform, int first, int last, float (&left)[kLaneFrames],
+                                            float (&right)[kLaneFrames], PanCache& panCache, Source&& source)
+{
+    const float decay2 = v.decayRate * 1.44269504088896341f, release2 = v.releaseRate * 1.44269504088896341f;
+    // The voice's frames in this tile are [first, last]; one stage over all of them?
+    const int k0 = envStageIndex(v, first), k1 = envStageIndex(v, last);
+    const bool stageUniform = k0 == k1;
+    const EnvStage uniformStage = envStage(v, decay2, release2, k0);
+    const bool constLevel = stageUniform && !uniformStage.isExp && uniformStage.slope == 0;
+    const float level0 = fminf(uniformStage.hi, fmaxf(uniformStage.lo, uniformStage.base));
+    float2 g0 = make_float2(v.left, v.right);
+    if (panUniform)
+        g0 = monoGains(v.left, v.right, v.gain, v.pan, pan[0], panCache);
+    const float2 lg0 = make_float2(g0.x * level0, g0.y * level0);
+#pragma unroll
+    for (int k = 0; k < kLaneFrames; ++k) {
+        const bool act = on[k] && frame[k] >= v.begin && frame[k] < v.end;
+        const float src = source(frame[k], act);
+        float2 lg;
+        if (constLevel && panUniform) {
+            lg = lg0;
+        } else {
+            const float level = constLevel ? level0
+                                : stageUniform ? envLevel(uniformStage, frame[k])
+                                               : envLevel(envStage(v, decay2, release2, envStageIndex(v, frame[k])),
+                                                          frame[k]);
+            const float2 g = panUniform ? g0 : monoGains(v.left, v.right, v.gain, v.pan, pan[k], panCache);
+            lg = make_float2(g.x * level, g.y * level);
+        }
+        if (act) {
+            left[k] = fmaf(src, lg.x, left[k]);
+            right[k] = fmaf(src, lg.y, right[k]);
+        }
+    }
+}
+
 // One tile by one wave. Every lane of the wave calls this; nothing in it
 // synchronises across waves.
 template<int T, bool FloatOut, bool Simple, class Taps>
@@ -254,6 +366,11 @@ __device__ __forceinline__ void mixTileWave(const int16_t* __restrict__ bank, co
                                             int segFirst, int segEnd, const Tile& tile, int lane)
 {
     const Job& j = jobs[uniform(tile.job)];
+    // The pan offset (CC10 track) is one value over the tile when its segment
+    // is flat and ends after the tile.
+    const int panAt = controlSegment(points, j.controls[2], j.controls[3], tile.frame);
+    const bool panUniform = points[panAt].step == 0 &&
+                            (panAt + 1 >= j.controls[3] || points[panAt + 1].frame >= tile.frame + tile.frames);
     int frame[kLaneFrames];
     bool on[kLaneFrames];
     float pan[kLaneFrames], left[kLaneFrames], right[kLaneFrames], wetLeft[kLaneFrames], wetRight[kLaneFrames];
@@ -261,7 +378,8 @@ __device__ __forceinline__ void mixTileWave(const int16_t* __restrict__ bank, co
     for (int k = 0; k < kLaneFrames; ++k) {
         frame[k] = tile.frame + lane + 64 * k;
         on[k] = lane + 64 * k < tile.frames && frame[k] >= segFirst && frame[k] < segEnd;
-        pan[k] = on[k] ? controlAt(points, j.controls[2], j.controls[3], frame[k]) : 0.f;
+        pan[k] = panUniform ? pointAt(points[panAt], tile.frame)
+                            : on[k] ? controlAt(points, j.controls[2], j.controls[3], frame[k]) : 0.f;
         left[k] = right[k] = wetLeft[k] = wetRight[k] = 0.f;
     }
     PanCache panCache;
@@ -273,6 +391,7 @@ __device__ __forceinline__ void mixTileWave(const int16_t* __restrict__ bank, co
         for (int i = 0; i < n; ++i) {
             const int s = __builtin_amdgcn_readlane(held, i);
             const Span& v = spans[s];
+            const int first = max(v.begin, tile.frame), last = min(v.end, tile.frame + tile.frames) - 1;
             if (v.filterTrack >= 0) {
                 // A filtered voice from the chain scratch, panned when mono (render.cu:458-523).
                 int stop = v.end;
@@ -300,28 +419,20 @@ __device__ __forceinline__ void mixTileWave(const int16_t* __restrict__ bank, co
                     wetLeft[k] += x.x;
                     wetRight[k] += x.y;
                 }
-            } else if (Simple || v.plain) {
-                // A plain voice; in a Simple launch every dry voice is plain
-                // and straight (no loop, no bend), checked by the host.
-                const EnvelopeRates rates{v.decayRate * 1.44269504088896341f, v.releaseRate * 1.44269504088896341f};
-                const double bend = !Simple && v.pitchSlope != 0 ? expm1(v.pitchSlope) : 1.0;
-#pragma unroll
-                for (int k = 0; k < kLaneFrames; ++k) {
-                    const bool act = on[k] && frame[k] >= v.begin && frame[k] < v.end;
-                    const float level = envelopePlain(v, rates, frame[k]);
-                    float src;
-                    if constexpr (Simple) {
-                        const double pos = __dadd_rn(v.position, __dmul_rn(double(frame[k] - v.begin), v.step));
-                        src = pos >= v.sourceEnd ? 0.f : sourceAt<T>(bank, taps, v, pos, act, dummy);
-                    } else {
-                        src = plainSource<T>(bank, taps, v, frame[k], bend, act, dummy);
-                    }
-                    if (act && level > 0) {
-                        const float2 g = monoGains(v.left, v.right, v.gain, v.pan, pan[k], panCache);
-                        left[k] += src * g.x * level;
-                        right[k] += src * g.y * level;
-                    }
-                }
+            } else if (Simple || (v.plain && v.loopEnd <= 0 && v.pitchSlope == 0)) {
+                // A straight plain voice; in a Simple launch every dry voice is
+                // one (checked by the host).
+                const Straight sv{bank + (v.sample - (v.sample & 1)), int(v.sample & 1), v.sourceEnd - T};
+                plainFrames(v, frame, on, pan, panUniform, first, last, left, right, panCache, [&](int f, bool act) {
+                    const double pos = __dadd_rn(v.position, __dmul_rn(double(f - v.begin), v.step));
+                    return straightSource<T>(bank, taps, v, sv, pos, act);
+                });
+            } else if (v.plain) {
+                // A plain voice with a loop or a bend.
+                const double bend = v.pitchSlope != 0 ? expm1(v.pitchSlope) : 1.0;
+                plainFrames(v, frame, on, pan, panUniform, first, last, left, right, panCache, [&](int f, bool act) {
+                    return plainSource<T>(bank, taps, v, f, bend, act, dummy);
+                });
             } else {
                 // Any other dry voice: the float FIR of voiceAudio, one frame at a time.
 #pragma unroll

@@ -1519,9 +1519,10 @@ int clay_hip_upload_bank(clay_hip* h, const int16_t* pcm, uint64_t count, int fr
             const uint64_t slack = std::min<uint64_t>(count / 2, 64ull << 20);
             const uint64_t capacity = std::max<uint64_t>(count + slack, 1);
             int16_t* grown = nullptr;
-            // 16 samples of padding: the FIR's dword-aligned window loads may read
-            // one dword past the last sample.
-            check(hipMalloc(&grown, (capacity + 16) * sizeof(int16_t)), "sample bank");
+            // Padding: the FIR's dword-aligned window loads may read one dword past
+            // the last sample, and the mix clamps the window of a source shorter
+            // than the taps to the source's start (up to taps + 2 samples read).
+            check(hipMalloc(&grown, (capacity + 2 * CLAY_MAX_SINC_TAPS) * sizeof(int16_t)), "sample bank");
             if (h->bankCount)
                 check(hipMemcpyAsync(grown, h->bank, h->bankCount * sizeof(int16_t), hipMemcpyDeviceToDevice, h->stream),
                       "sample bank growth");
