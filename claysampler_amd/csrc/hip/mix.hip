@@ -14,8 +14,8 @@
 // by v_dot2c_i32_i16, two taps per instruction, in exact int32 sums
 // (|sum hi s| < 1.6 * 2^29 and |sum lo s| <= T 2^9 2^15 <= 2^30 for T <= 64).
 // A 16-tap FIR is 16 dot instructions where the float FIR takes 16 converts
-// and 16 FMAs. For 16 taps the weight table lives in LDS as a mirrored half
-// table (82 KB). The envelope of a plain voice uses one exp2 per frame.
+// and 16 FMAs. For 16 taps the weight table lives in LDS as an XOR-swizzled
+// table (128 KB). The envelope of a plain voice uses one exp2 per frame.
 // Other dry voices use the float FIR of voiceAudio, and filtered voices are
 // read from the chain scratch; every execution form inlines these same
 // functions, so their PCM is bit-identical.
@@ -33,36 +33,34 @@ constexpr int kIntSharedTaps = 16;             // widest integer table held in L
 // dwords, the T/2 hi pairs then the T/2 lo pairs.
 template<int T>
 struct IntTapsGlobal {
-    static constexpr bool kMirrored = false;
     const uint32_t* __restrict__ table;
     struct Row {
         const uint32_t* __restrict__ w;
-        uint32_t sel;
-        bool mirror;
+        __device__ __forceinline__ uint4 quad(int q) const { return reinterpret_cast<const uint4*>(w)[q]; }
+        __device__ __forceinline__ uint32_t word(int i) const { return w[i]; }
     };
-    __device__ __forceinline__ Row row(int phase) const { return {table + size_t(phase) * T, 0x03020100u, false}; }
+    __device__ __forceinline__ Row row(int phase) const { return {table + size_t(phase) * T}; }
 };
 
-// The same weights as a half table in LDS: rows 0..kPhases/2 at a pitch of
-// T + 4 dwords (rows of different lanes start on different banks); a phase
-// above kPhases/2 reads row kPhases - phase, whose taps run backwards, and
-// reverses its sample pairs instead (one v_perm_b32 per pair).
+// The same full table in LDS (128 KB at 16 taps), its 16-byte quads stored in
+// XOR-swizzled order (quad q of row r at q ^ (r & 3)): rows are 64 bytes, so
+// unswizzled the first quad of every row would fall on one of two bank
+// groups; swizzled, the rows that the lanes of a wave read start on eight.
 template<int T>
 struct IntTapsShared {
-    static constexpr bool kMirrored = true;
-    static constexpr int kPitch = T + 4;
-    static constexpr int kRows = kPhases / 2 + 1;
-    const uint32_t* table;
+    static constexpr int kQuads = T / 4;
+    static_assert(kQuads == 4, "the LDS table is laid out for 16 taps");
+    const uint4* table;
     struct Row {
-        const uint32_t* w;
-        uint32_t sel;  // 0x03020100 keeps pair i; 0x05040706 takes pair T/2-1-i with its halves swapped
-        bool mirror;
+        const uint4* w;
+        int x;  // r & 3
+        __device__ __forceinline__ uint4 quad(int q) const { return w[q ^ x]; }
+        __device__ __forceinline__ uint32_t word(int i) const
+        {
+            return reinterpret_cast<const uint32_t*>(w)[(((i >> 2) ^ x) << 2) | (i & 3)];
+        }
     };
-    __device__ __forceinline__ Row row(int phase) const
-    {
-        const bool mirror = phase > kPhases / 2;
-        return {table + (mirror ? kPhases - phase : phase) * kPitch, mirror ? 0x05040706u : 0x03020100u, mirror};
-    }
+    __device__ __forceinline__ Row row(int phase) const { return {table + phase * kQuads, phase & 3}; }
 };
 
 __device__ __forceinline__ int dot2(uint32_t a, uint32_t b, int c)
@@ -81,15 +79,12 @@ __device__ __forceinline__ float firPairs(const typename Taps::Row& row, const u
     int hi = 0, lo = 0;
 #pragma unroll
     for (int g = 0; g < T / 8; ++g) {
-        const uint4 h = reinterpret_cast<const uint4*>(row.w)[g];
-        const uint4 l = reinterpret_cast<const uint4*>(row.w)[T / 8 + g];
+        const uint4 h = row.quad(g), l = row.quad(T / 8 + g);
         const uint32_t hv[4] = {h.x, h.y, h.z, h.w}, lv[4] = {l.x, l.y, l.z, l.w};
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-            const int i = 4 * g + k;
-            const uint32_t q = Taps::kMirrored ? __builtin_amdgcn_perm(p[T / 2 - 1 - i], p[i], row.sel) : p[i];
-            hi = dot2(q, hv[k], hi);
-            lo = dot2(q, lv[k], lo);
+            hi = dot2(p[4 * g + k], hv[k], hi);
+            lo = dot2(p[4 * g + k], lv[k], lo);
         }
     }
     return fromSums(hi, lo);
@@ -99,17 +94,16 @@ __device__ __forceinline__ float firPairs(const typename Taps::Row& row, const u
 // edges of the source.
 template<int T, class Taps>
 __device__ __forceinline__ float firEdge(const int16_t* __restrict__ bank, uint64_t sample, int sourceEnd, int centre,
-                         const typename Taps::Row& row)
+                                         const typename Taps::Row& row)
 {
     int hi = 0, lo = 0;
 #pragma unroll 1
     for (int t = 0; t < T; ++t) {
         const int i = centre - (T / 2 - 1) + t;
         const int s = i >= 0 && i < sourceEnd ? int(bank[sample + uint64_t(i)]) : 0;
-        const int stored = row.mirror ? T - 1 - t : t;
-        const uint32_t h = row.w[stored >> 1], l = row.w[T / 2 + (stored >> 1)];
-        const int wh = int(int16_t(stored & 1 ? h >> 16 : h & 0xffffu));
-        const int wl = int(int16_t(stored & 1 ? l >> 16 : l & 0xffffu));
+        const uint32_t h = row.word(t >> 1), l = row.word(T / 2 + (t >> 1));
+        const int wh = int(int16_t(t & 1 ? h >> 16 : h & 0xffffu));
+        const int wl = int(int16_t(t & 1 ? l >> 16 : l & 0xffffu));
         hi += wh * s;
         lo += wl * s;
     }
@@ -161,7 +155,9 @@ __device__ __forceinline__ float sourceAt(const int16_t* __restrict__ bank, cons
 // start that stays inside the source.
 struct Straight {
     const int16_t* __restrict__ base;  // bank + sample - parity: dword aligned
+    uint64_t sample;                   // bank offset of frame 0
     int parity;                        // sample & 1
+    int sourceEnd;                     // source frames
     int last;                          // last interior window start (centre - (T/2 - 1)), -1 for none
 };
 
@@ -169,41 +165,63 @@ struct Straight {
 // as sourceAt but with every lane loading a window clamped into the source
 // (the bank is padded by 2 x CLAY_MAX_SINC_TAPS samples), so no lane takes a
 // different address path; edges and integer positions outside the interior
-// take the rare branch.
-template<int T, class Taps>
-__device__ __forceinline__ float straightSource(const int16_t* __restrict__ bank, const Taps& taps, const Span& v,
-                                                const Straight& sv, double pos, bool active)
+// take the rare branch. In two halves, so that a caller can issue the loads of
+// several frames before it waits for any of them.
+template<int T>
+struct Window {
+    uint32_t dw[T / 2 + 1];  // the window's dwords, from the dword-aligned start
+    double frac;             // pos - floor(pos)
+    int centre, phase;
+    uint32_t shift;          // 16 when the window starts on the high half of its first dword
+    bool interior;
+};
+
+template<int T>
+__device__ __forceinline__ Window<T> fetchWindow(const Straight& sv, double pos)
 {
+    Window<T> w;
     const double floorPos = floor(pos);
-    const int centre = int(floorPos);
-    const double frac = pos - floorPos;
-    const int phase = min(max(__double2int_rn(frac * kPhases), 0), kPhases);
-    const auto row = taps.row(phase);
-    const int start = centre - (T / 2 - 1);
-    const bool interior = sv.last >= 0 && uint32_t(start) <= uint32_t(sv.last);
+    w.centre = int(floorPos);
+    w.frac = pos - floorPos;
+    w.phase = __double2int_rn(w.frac * kPhases);  // 0..kPhases: 0 <= frac < 1
+    const int start = w.centre - (T / 2 - 1);
+    w.interior = sv.last >= 0 && uint32_t(start) <= uint32_t(sv.last);
     const int at = min(max(start, 0), max(sv.last, 0)) + sv.parity;  // window start from `base`
-    const int odd = at & 1;
+    w.shift = uint32_t(at & 1) << 4;
     const uint32_t* __restrict__ words = reinterpret_cast<const uint32_t*>(sv.base + uint32_t(at & ~1));
-    uint32_t dw[T / 2 + 1];
 #pragma unroll
     for (int i = 0; i < T / 2 + 1; ++i)
-        dw[i] = words[i];
-    const uint32_t shift = uint32_t(odd) << 4;
+        w.dw[i] = words[i];
+    return w;
+}
+
+template<int T, class Taps>
+__device__ __forceinline__ float finishWindow(const int16_t* __restrict__ bank, const Taps& taps, const Straight& sv,
+                                              const Window<T>& w, bool active)
+{
+    const auto row = taps.row(w.phase);
     uint32_t p[T / 2];
 #pragma unroll
     for (int i = 0; i < T / 2; ++i)
-        p[i] = __builtin_amdgcn_alignbit(dw[i + 1], dw[i], shift);
+        p[i] = __builtin_amdgcn_alignbit(w.dw[i + 1], w.dw[i], w.shift);
     float s = firPairs<T, Taps>(row, p);
     // Exact sample at an integer position (render.cu:33-34): window tap T/2 - 1.
     const uint32_t pair = p[(T / 2 - 1) >> 1];
     const int exact = int(int16_t(((T / 2 - 1) & 1) ? pair >> 16 : pair & 0xffffu));
-    s = frac == 0 ? float(exact) * 0x1p-15f : s;
-    if (active && !interior && centre < v.sourceEnd) {
+    s = w.frac == 0 ? float(exact) * 0x1p-15f : s;
+    if (active && !w.interior && w.centre < sv.sourceEnd) {
         // Window past either end of the source.
-        s = frac == 0 ? (centre >= 0 ? float(bank[v.sample + uint64_t(centre)]) * 0x1p-15f : 0.f)
-                      : firEdge<T, Taps>(bank, v.sample, v.sourceEnd, centre, row);
+        s = w.frac == 0 ? (w.centre >= 0 ? float(bank[sv.sample + uint64_t(w.centre)]) * 0x1p-15f : 0.f)
+                        : firEdge<T, Taps>(bank, sv.sample, sv.sourceEnd, w.centre, row);
     }
-    return centre < v.sourceEnd ? s : 0.f;
+    return w.centre < sv.sourceEnd ? s : 0.f;
+}
+
+template<int T, class Taps>
+__device__ __forceinline__ float straightSource(const int16_t* __restrict__ bank, const Taps& taps, const Straight& sv,
+                                                double pos, bool active)
+{
+    return finishWindow<T>(bank, taps, sv, fetchWindow<T>(sv, pos), active);
 }
 
 // Source position of a bent span at `frame` (src/cuda/render.cu:63-70), `bend` = expm1(pitchSlope).
@@ -265,7 +283,20 @@ struct EnvStage {
     bool isExp;
 };
 
-__device__ __forceinline__ int envStageIndex(const Span& v, int frame)
+// The envelope fields of a span, read unconditionally (so the scalar loads
+// are issued together rather than one behind each branch that needs them).
+struct EnvFields {
+    int attackStart, holdStart, decayStart, releaseStart, fadeStart;
+    float start, attackStep, decayRate, sustain, releaseValue, releaseRate, fadeValue, fadeStep;
+};
+
+__device__ __forceinline__ EnvFields envFields(const Span& v)
+{
+    return {v.attackStart, v.holdStart, v.decayStart, v.releaseStart, v.fadeStart, v.start, v.attackStep,
+            v.decayRate, v.sustain, v.releaseValue, v.releaseRate, v.fadeValue, v.fadeStep};
+}
+
+__device__ __forceinline__ int envStageIndex(const EnvFields& v, int frame)
 {
     if (v.releaseStart >= 0 && frame >= v.releaseStart)
         return frame >= v.fadeStart ? 5 : 4;
@@ -273,7 +304,7 @@ __device__ __forceinline__ int envStageIndex(const Span& v, int frame)
 }
 
 // Parameters of stage `k` (selects, no branches: `k` may differ per lane).
-__device__ __forceinline__ EnvStage envStage(const Span& v, float decay2, float release2, int k)
+__device__ __forceinline__ EnvStage envStage(const EnvFields& v, float decay2, float release2, int k)
 {
     constexpr float inf = __builtin_huge_valf();
     EnvStage s;
@@ -314,27 +345,53 @@ __device__ __forceinline__ float2 monoGains(float left, float right, float gain,
 // Accumulates a plain voice's frames into the lane's dry sums: the envelope
 // stage and the pan gains are taken once per voice when they hold over the
 // whole tile (the usual case), per frame otherwise.
-template<class Source>
-__device__ __forceinline__ void plainFrames(const Span& v, const int (&frame)[kLaneFrames],
+// `fetch(frame)` starts a frame's source value (its loads) and `finish(started,
+// active)` completes it; `Ahead` frames of the lane are fetched before the
+// first of them is finished, so their loads are in flight together (as many
+// as the registers allow: four windows at 16 taps).
+template<int Ahead, class Fetch, class Finish>
+__device__ __forceinline__ void plainFrames(const Span& span, const int (&frame)[kLaneFrames],
                                             const bool (&on)[kLaneFrames], const float (&pan)[kLaneFrames],
-                                            bool panUniform, int first, int last, float (&left)[kLaneFrames],
-                                            float (&right)[kLaneFrames], PanCache& panCache, Source&& source)
+                                            bool panUniform, int tileFirst, int tileEnd, float (&left)[kLaneFrames],
+                                            float (&right)[kLaneFrames], PanCache& panCache, Fetch&& fetch,
+                                            Finish&& finish)
 {
+    const EnvFields v = envFields(span);
+    const int begin = span.begin, end = span.end;
+    const float gl = span.left, gr = span.right, gain = span.gain, voicePan = span.pan;
     const float decay2 = v.decayRate * 1.44269504088896341f, release2 = v.releaseRate * 1.44269504088896341f;
     // The voice's frames in this tile are [first, last]; one stage over all of them?
+    const int first = max(begin, tileFirst), last = min(end, tileEnd) - 1;
     const int k0 = envStageIndex(v, first), k1 = envStageIndex(v, last);
     const bool stageUniform = k0 == k1;
-    const EnvStage uniformStage = envStage(v, decay2, release2, k0);
+    EnvStage uniformStage = envStage(v, decay2, release2, k0);
+    // A decay that has fallen below the sustain level at the tile's first frame
+    // stays below it (the exponential only falls), so the level is the sustain
+    // level itself; the margin covers the last-place error of v_exp_f32.
+    if (stageUniform && k0 == 3 && uniformStage.isExp && v.decayRate < 0 &&
+        envLevel(uniformStage, first) == uniformStage.lo &&
+        __builtin_amdgcn_exp2f(float(first - uniformStage.origin + 1) * uniformStage.rate) <
+            uniformStage.lo * (1.f - 0x1p-20f)) {
+        uniformStage.isExp = false;
+        uniformStage.slope = 0.f;
+        uniformStage.base = uniformStage.lo;
+    }
     const bool constLevel = stageUniform && !uniformStage.isExp && uniformStage.slope == 0;
     const float level0 = fminf(uniformStage.hi, fmaxf(uniformStage.lo, uniformStage.base));
-    float2 g0 = make_float2(v.left, v.right);
+    float2 g0 = make_float2(gl, gr);
     if (panUniform)
-        g0 = monoGains(v.left, v.right, v.gain, v.pan, pan[0], panCache);
+        g0 = monoGains(gl, gr, gain, voicePan, pan[0], panCache);
     const float2 lg0 = make_float2(g0.x * level0, g0.y * level0);
+    decltype(fetch(0)) started[kLaneFrames];
 #pragma unroll
     for (int k = 0; k < kLaneFrames; ++k) {
-        const bool act = on[k] && frame[k] >= v.begin && frame[k] < v.end;
-        const float src = source(frame[k], act);
+        if (k % Ahead == 0) {
+#pragma unroll
+            for (int a = k; a < k + Ahead; ++a)
+                started[a] = fetch(frame[a]);
+        }
+        const bool act = on[k] && frame[k] >= begin && frame[k] < end;
+        const float src = finish(started[k], act);
         float2 lg;
         if (constLevel && panUniform) {
             lg = lg0;
@@ -343,7 +400,7 @@ __device__ __forceinline__ void plainFrames(const Span& v, const int (&frame)[kL
                                 : stageUniform ? envLevel(uniformStage, frame[k])
                                                : envLevel(envStage(v, decay2, release2, envStageIndex(v, frame[k])),
                                                           frame[k]);
-            const float2 g = panUniform ? g0 : monoGains(v.left, v.right, v.gain, v.pan, pan[k], panCache);
+            const float2 g = panUniform ? g0 : monoGains(gl, gr, gain, voicePan, pan[k], panCache);
             lg = make_float2(g.x * level, g.y * level);
         }
         if (act) {
@@ -391,7 +448,6 @@ __device__ __forceinline__ void mixTileWave(const int16_t* __restrict__ bank, co
         for (int i = 0; i < n; ++i) {
             const int s = __builtin_amdgcn_readlane(held, i);
             const Span& v = spans[s];
-            const int first = max(v.begin, tile.frame), last = min(v.end, tile.frame + tile.frames) - 1;
             if (v.filterTrack >= 0) {
                 // A filtered voice from the chain scratch, panned when mono (render.cu:458-523).
                 int stop = v.end;
@@ -422,17 +478,21 @@ __device__ __forceinline__ void mixTileWave(const int16_t* __restrict__ bank, co
             } else if (Simple || (v.plain && v.loopEnd <= 0 && v.pitchSlope == 0)) {
                 // A straight plain voice; in a Simple launch every dry voice is
                 // one (checked by the host).
-                const Straight sv{bank + (v.sample - (v.sample & 1)), int(v.sample & 1), v.sourceEnd - T};
-                plainFrames(v, frame, on, pan, panUniform, first, last, left, right, panCache, [&](int f, bool act) {
-                    const double pos = __dadd_rn(v.position, __dmul_rn(double(f - v.begin), v.step));
-                    return straightSource<T>(bank, taps, v, sv, pos, act);
-                });
+                const uint64_t sample = v.sample;
+                const int sourceEnd = v.sourceEnd;
+                const Straight sv{bank + (sample - (sample & 1)), sample, int(sample & 1), sourceEnd, sourceEnd - T};
+                const double position = v.position, step = v.step;
+                const int begin = v.begin;
+                plainFrames<T <= 16 ? kLaneFrames : T <= 32 ? 2 : 1>(
+                    v, frame, on, pan, panUniform, tile.frame, tile.frame + tile.frames, left, right, panCache,
+                    [&](int f) { return fetchWindow<T>(sv, __dadd_rn(position, __dmul_rn(double(f - begin), step))); },
+                    [&](const Window<T>& w, bool act) { return finishWindow<T>(bank, taps, sv, w, act); });
             } else if (v.plain) {
                 // A plain voice with a loop or a bend.
                 const double bend = v.pitchSlope != 0 ? expm1(v.pitchSlope) : 1.0;
-                plainFrames(v, frame, on, pan, panUniform, first, last, left, right, panCache, [&](int f, bool act) {
-                    return plainSource<T>(bank, taps, v, f, bend, act, dummy);
-                });
+                plainFrames<1>(v, frame, on, pan, panUniform, tile.frame, tile.frame + tile.frames, left, right, panCache,
+                            [&](int f) { return f; },
+                            [&](int f, bool act) { return plainSource<T>(bank, taps, v, f, bend, act, dummy); });
             } else {
                 // Any other dry voice: the float FIR of voiceAudio, one frame at a time.
 #pragma unroll
@@ -464,20 +524,21 @@ __device__ __forceinline__ void mixTileWave(const int16_t* __restrict__ bank, co
     }
 }
 
-// Runs body(taps) with the integer weights: for T <= kIntSharedTaps the half
-// table staged in this block's LDS, else the global table.
+// Runs body(taps) with the integer weights: for T <= kIntSharedTaps the full
+// table staged in this block's LDS (one block per CU), else the global table.
 template<int T, class Body>
 __device__ __forceinline__ void withIntTaps(const uint32_t* __restrict__ table, Body&& body)
 {
     if constexpr (T <= kIntSharedTaps) {
         using S = IntTapsShared<T>;
-        __shared__ uint4 shared[S::kRows * S::kPitch / 4];
-        for (int i = int(threadIdx.x); i < S::kRows * (T / 4); i += int(blockDim.x)) {
-            const int row = i / (T / 4), q = i % (T / 4);
-            shared[row * (S::kPitch / 4) + q] = reinterpret_cast<const uint4*>(table + size_t(row) * T)[q];
+        __shared__ uint4 shared[(kPhases + 1) * S::kQuads];
+        const uint4* source = reinterpret_cast<const uint4*>(table);
+        for (int i = int(threadIdx.x); i < (kPhases + 1) * S::kQuads; i += int(blockDim.x)) {
+            const int row = i / S::kQuads, q = i % S::kQuads;
+            shared[row * S::kQuads + (q ^ (row & 3))] = source[i];
         }
         __syncthreads();
-        body(S{reinterpret_cast<const uint32_t*>(shared)});
+        body(S{shared});
     } else {
         body(IntTapsGlobal<T>{table});
     }
