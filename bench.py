@@ -5,9 +5,15 @@ A step renders one batch of jobs completely on the device: every voice span
 PCM16 finishing, with the plans, the int16 sample bank and the sinc table
 already resident in HBM (no host transfers inside the timed region).
 
-Default workload = configuration 2: 64 jobs x 60 s MIDI on the 176-region
-synthetic instrument (lpf_2p + CC1, pitch-bend walk, loops), 32-tap sinc,
-natural tail policy, 44.1 kHz. Prints one JSON line.
+Default workload = configuration 3: 512 jobs in the uneven batched form on the
+configuration-1 instrument (job lengths lognormal, clipped to 5-300 s,
+polyphony 4-64), 16-tap sinc, natural tail policy, 44.1 kHz. A step renders
+all 512 jobs resident on the device (one batch, plain form). The same line
+reports the render windows of the plain and persistent forms at
+--gpu-batch-jobs 128 (host transfers included) and whether their PCM16 is
+identical, the roofline of the resample-mix kernel (clayMixWaves) on the
+algorithmic bytes of its source windows, mix writes and span records, and
+the FP64 oracle on the host cores as the CPU baseline. Prints one JSON line.
 """
 
 import argparse
@@ -55,7 +61,7 @@ def kernel_bytes(plans, taps, natural):
     wet_samples = 0
     spans = 0
     frames = 0
-    span_bytes = 208  # device span record (hot part read per voice)
+    span_bytes = 216  # device span record (the hot part the mix reads per voice)
     for p in plans:
         frames += p.frames
         spans += p.span_count
@@ -99,14 +105,27 @@ def measured_traffic(config, kernel):
     return None
 
 
-def cpu_baseline(sfz, mids, taps, jobs, threads):
-    """FP64 oracle on a bounded sample of the same workload, on host cores."""
+def cpu_baseline(sfz, mids, taps, jobs, threads, voice_frame_budget=None):
+    """FP64 oracle on a bounded sample of the same workload, on host cores:
+    the first `jobs` jobs, or as many leading jobs as stay within
+    `voice_frame_budget` voice-frames (about 10-30 s of CPU work)."""
     import oracle
 
     s = ca.Session()
-    for m in mids[:jobs]:
-        s.add_job(sfz, m)
+    if voice_frame_budget is not None:
+        total = 0
+        for m in mids:
+            index = s.add_job(sfz, m)
+            v = np.ctypeslib.as_array(s.plan(index).spans, shape=(s.plan(index).span_count,))
+            total += int((v["end"] - v["begin"]).astype(np.int64).sum())
+            if total >= voice_frame_budget:
+                break
+        jobs = s.lib.clay_session_job_count(s.handle)
+    else:
+        for m in mids[:jobs]:
+            s.add_job(sfz, m)
     plans = s.plans()
+    threads = min(threads, len(plans))  # one job per thread at most
     bank_ptr, bank_count = s.bank()
     cfg = oracle.config(taps=taps, threads=threads)
     t0 = time.perf_counter()
@@ -392,12 +411,13 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4", "c5"])
+    ap.add_argument("--config", default="c3", choices=["c1", "c2", "c3", "c4", "c5"])
     ap.add_argument("--jobs", type=int, default=None)
     ap.add_argument("--seconds", type=float, default=None, help="job length (configuration 2: 60, 5: 240)")
     ap.add_argument("--synth-dir", default=os.environ.get("CLAY_SYNTH_DIR", "/tmp/clay_synth"))
     ap.add_argument("--cpu-jobs", type=int, default=8)
-    ap.add_argument("--cpu-threads", type=int, default=4)
+    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--no-forms", action="store_true", help="configuration 3: skip the render-window comparison")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--forms", action="store_true",
                     help="plain vs persistent form through render_window (configuration 3)")
@@ -458,7 +478,10 @@ def main():
         ms, n = batch.kernel_time(name)
         if n:
             kernels[name] = {"ms": ms, "launches": n}
-    dominant = max((k for k in kernels if k in bytes_per), key=lambda k: kernels[k]["ms"])
+    # The roofline goes on the resample-mix kernel where it renders the voices
+    # (configurations 1 and 3), else on the kernel that takes longest.
+    mix_kernel = "clayMixWaves" if "clayMixWaves" in kernels else None
+    dominant = mix_kernel or max((k for k in kernels if k in bytes_per), key=lambda k: kernels[k]["ms"])
     avg_s = kernels[dominant]["ms"] / kernels[dominant]["launches"] / 1e3
     achieved = bytes_per[dominant] / kernels[dominant]["launches"] / avg_s / 1e9
     results, _ = batch.results()
@@ -466,6 +489,10 @@ def main():
 
     audio_seconds = frames / 44100.0
     total_voice_frames = voice_frames["dry"] + voice_frames["wet"]
+    full_size = jobs == default_jobs
+    workload_name = {"c1": "config 1: 1 job, 32 notes, 16-tap sinc",
+                     "c2": f"config 2: {jobs} jobs x {args.seconds:g} s, 32-tap sinc, lpf_2p+CC1, bends",
+                     "c3": f"config 3: {jobs} jobs, lognormal 5-300 s, polyphony 4-64, 16-tap sinc"}[args.config]
     line = {
         "metric": "rendered audio-sec/sec and voice-samples/sec on 1 GPU; % of HBM roofline",
         "value": audio_seconds / (ms_per_step / 1e3),
@@ -479,24 +506,63 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": {"c1": "config 1: 1 job, 32 notes, 16-tap sinc",
-                                "c2": f"config 2: {jobs} jobs x {args.seconds:g} s, 32-tap sinc, lpf_2p+CC1, bends",
-                                "c3": f"config 3 (plain form): {jobs} jobs, 16-tap sinc"}[args.config],
-                   "jobs": jobs, "sinc_taps": taps, "tail_policy": "natural", "rate": 44100, "form": "plain"},
+        "config": {"workload": workload_name, "jobs": jobs, "sinc_taps": taps, "tail_policy": "natural",
+                   "rate": 44100, "step": "every job rendered and finished resident on the device, plain form"},
         "voice_samples_per_s": total_voice_frames / (ms_per_step / 1e3),
         "audio_seconds": audio_seconds,
         "kernels_ms": {k: round(v["ms"], 3) for k, v in kernels.items()},
         "statuses": statuses,
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": measured_traffic(args.config, dominant) if jobs == default_jobs else None},
+                     "frac": achieved / HBM_PEAK_GBS, "algorithmic_bytes": bytes_per[dominant],
+                     "traffic": measured_traffic(args.config, dominant) if full_size else None},
     }
+    valu = measured_valu(args.config, dominant) if full_size else None
+    if valu:
+        line["roofline"]["valu"] = valu
     batch.close()
     renderer.close()
+    if args.config == "c3" and not args.no_forms:
+        line["forms"] = forms_compare(args, plans, (bank_ptr, bank_count), taps, frames)
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(sfz, mids, taps, min(args.cpu_jobs, jobs),
-                                            min(args.cpu_threads, args.cpu_jobs, jobs))
+        threads = max(1, args.cpu_threads)
+        budget = None if args.config != "c3" else threads * 8e7  # about 15 s of FP64 oracle work
+        line["cpu_baseline"] = cpu_baseline(sfz, mids, taps, min(args.cpu_jobs, jobs), min(threads, jobs), budget)
     print(json.dumps(line), flush=True)
+
+
+def measured_valu(config, kernel):
+    """VALU figures of `kernel` from the committed SQ counter summary of this
+    workload (profiles/valu_<config>.txt: `key value` lines), or None."""
+    path = os.path.join(ROOT, "profiles", f"valu_{config}.txt")
+    if not os.path.exists(path):
+        return None
+    out = {}
+    for line in open(path):
+        parts = line.split()
+        if len(parts) == 3 and parts[0] == kernel and not line.startswith("#"):
+            out[parts[1]] = float(parts[2])
+    return out or None
+
+
+def forms_compare(args, plans, bank, taps, frames):
+    """The plain and persistent forms through render_window on the same jobs
+    (--gpu-batch-jobs as given), wall time each including host transfers,
+    and per-job PCM16 checksums compared."""
+    warm = (abi.PlanView * min(8, len(plans)))(*plans[:8])
+    for form in (abi.FORM_PLAIN, abi.FORM_PERSISTENT):
+        window_pass(abi.POLICY_NATURAL, taps, form, args.batch_jobs, warm, bank)
+    plain, _, plain_stats = window_pass(abi.POLICY_NATURAL, taps, abi.FORM_PLAIN, args.batch_jobs, plans, bank)
+    rolled, _, stats = window_pass(abi.POLICY_NATURAL, taps, abi.FORM_PERSISTENT, args.batch_jobs, plans, bank)
+    _, want, _ = window_pass(abi.POLICY_NATURAL, taps, abi.FORM_PLAIN, args.batch_jobs, plans, bank, True)
+    _, got, _ = window_pass(abi.POLICY_NATURAL, taps, abi.FORM_PERSISTENT, args.batch_jobs, plans, bank, True)
+    log(f"windows: plain {plain:.3f} s, persistent {rolled:.3f} s, identical PCM16 {want == got}")
+    audio_seconds = frames / 44100.0
+    return {"batch_jobs": args.batch_jobs, "plain_window_s": plain, "persistent_window_s": rolled,
+            "plain_audio_s_per_s": audio_seconds / plain, "persistent_audio_s_per_s": audio_seconds / rolled,
+            "persistent_vs_plain": plain / rolled, "identical_pcm16": want == got,
+            "persistent": {"rounds": stats.rounds, "segment_frames": stats.segment_frames,
+                           "graph_launches": stats.graph_launches},
+            "note": "render windows include host planning handoff, bank-resident uploads of the plans and PCM downloads"}
 
 
 if __name__ == "__main__":
