@@ -63,7 +63,7 @@ HIP_SYMBOLS = [
     "clay_hip_release_pcm", "clay_hip_get_stats", "clay_hip_last_error", "clay_hip_sinc_table",
     "clay_hip_batch_prepare", "clay_hip_batch_run", "clay_hip_batch_results", "clay_hip_batch_kernel_time",
     "clay_hip_batch_destroy", "clay_hip_batch_voice_tails", "clay_mlp_init", "clay_hip_mlp_gradient",
-    "clay_hip_mlp_train", "clay_hip_mlp_predict",
+    "clay_hip_mlp_train", "clay_hip_mlp_predict", "clay_hip_mlp_gradient_noalloc",
 ]
 
 

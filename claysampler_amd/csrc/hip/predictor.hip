@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <numeric>
 #include <stdexcept>
 #include <string>
@@ -118,6 +119,63 @@ T* upload(const std::vector<T>& v, hipStream_t s)
 
 int blocksFor(size_t n) { return int((n + kTile - 1) / kTile); }
 
+// ---------------------------------------------------------------- no-alloc gradient
+// clay_hip_mlp_gradient_noalloc keeps every device buffer of a gradient
+// evaluation in one file-scope array: the standardised samples, the
+// parameters, the per-block partial gradients, the summed gradient and the
+// loss. Each device holds its own copy for the life of the process; host code
+// uses the per-device address from hipGetSymbolAddress, never the array's name.
+constexpr size_t kNoAllocSamples = 65536;
+constexpr size_t kNoAllocBlocks = (kNoAllocSamples + kTile - 1) / kTile;
+constexpr size_t kNoAllocX = 0;
+constexpr size_t kNoAllocY = kNoAllocX + kNoAllocSamples * kIn;
+constexpr size_t kNoAllocParams = kNoAllocY + kNoAllocSamples;
+constexpr size_t kNoAllocGrad = kNoAllocParams + kParams;
+constexpr size_t kNoAllocLoss = kNoAllocGrad + kParams;
+constexpr size_t kNoAllocPartial = (kNoAllocLoss + 1 + 63) / 64 * 64;
+constexpr size_t kNoAllocFloats = kNoAllocPartial + kNoAllocBlocks * (kParams + 1);
+
+} // namespace
+
+// Input and target standardisation of a model, by value to the kernel.
+struct ClayMlpStandardisation {
+    float inputMean[CLAY_MLP_INPUTS], inputScale[CLAY_MLP_INPUTS];
+    float outputMean, outputScale;
+};
+
+static __device__ __align__(256) float scratch[kNoAllocFloats];
+
+// In place: x' = (x - mean) * scale per input, y' = (y - mean) * scale; the
+// same float operations as the host standardisation of clay_hip_mlp_gradient.
+__global__ void clayMlpStandardize(float* __restrict__ x, float* __restrict__ y, int n, ClayMlpStandardisation st)
+{
+    const int s = int(blockIdx.x * blockDim.x + threadIdx.x);
+    if (s >= n)
+        return;
+    for (int i = 0; i < kIn; ++i)
+        x[size_t(s) * kIn + size_t(i)] = (x[size_t(s) * kIn + size_t(i)] - st.inputMean[i]) * st.inputScale[i];
+    y[s] = (y[s] - st.outputMean) * st.outputScale;
+}
+
+namespace {
+
+// Address of `scratch` on `device`, looked up once per device with that
+// device current (the library's calls run on the renderer's device).
+float* scratchOn(int device)
+{
+    static std::mutex lock;
+    static float* address[64] = {};
+    if (device < 0 || device >= 64)
+        throw Error(CLAY_E_DEVICE, "device index beyond the no-alloc scratch table");
+    std::lock_guard<std::mutex> guard(lock);
+    if (!address[device]) {
+        void* p = nullptr;
+        check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+        address[device] = static_cast<float*>(p);
+    }
+    return address[device];
+}
+
 } // namespace
 
 extern "C" {
@@ -185,6 +243,51 @@ int clay_hip_mlp_gradient(clay_hip* h, const clay_tail_model* model, const float
         check(hipMemcpyAsync(gradient, b.grad, kParams * sizeof(float), hipMemcpyDeviceToHost, s), "gradient");
         float l = 0;
         check(hipMemcpyAsync(&l, b.loss, sizeof(float), hipMemcpyDeviceToHost, s), "loss");
+        check(hipStreamSynchronize(s), "gradient");
+        if (loss)
+            *loss = l;
+    });
+}
+
+int clay_hip_mlp_gradient_noalloc(clay_hip* h, const clay_tail_model* model, const float* x, const float* y,
+                                  size_t n, float* gradient, float* loss)
+{
+    if (!h || !model || !x || !y || !n || !gradient)
+        return CLAY_E_INVALID;
+    if (n > kNoAllocSamples) {
+        claySetError(h, "clay_hip_mlp_gradient_noalloc: more samples than its scratch holds (65536)");
+        return CLAY_E_INVALID;
+    }
+    return guarded(h, [&] {
+        const int device = clayDeviceOf(h);
+        check(hipSetDevice(device), "hipSetDevice");
+        hipStream_t s = clayStreamOf(h);
+        float* base = scratchOn(device);
+        float *dx = base + kNoAllocX, *dy = base + kNoAllocY, *params = base + kNoAllocParams;
+        float *grad = base + kNoAllocGrad, *dloss = base + kNoAllocLoss, *partial = base + kNoAllocPartial;
+        check(hipMemcpyAsync(dx, x, n * kIn * sizeof(float), hipMemcpyHostToDevice, s), "samples");
+        check(hipMemcpyAsync(dy, y, n * sizeof(float), hipMemcpyHostToDevice, s), "targets");
+        check(hipMemcpyAsync(params, model->params, kParams * sizeof(float), hipMemcpyHostToDevice, s), "parameters");
+        ClayMlpStandardisation st;
+        std::memcpy(st.inputMean, model->input_mean, sizeof st.inputMean);
+        std::memcpy(st.inputScale, model->input_scale, sizeof st.inputScale);
+        st.outputMean = model->output_mean;
+        st.outputScale = model->output_scale;
+        hipLaunchKernelGGL(clayMlpStandardize, dim3((n + kThreads - 1) / kThreads), dim3(kThreads), 0, s, dx, dy,
+                           int(n), st);
+        check(hipGetLastError(), "clayMlpStandardize");
+        // Every partial entry of every block is written by the pass, and the
+        // reduce writes the whole gradient and the loss: nothing to clear.
+        const int blocks = blocksFor(n);
+        hipLaunchKernelGGL(clayMlpPass<true>, dim3(blocks), dim3(kThreads), 0, s, params, dx, dy, nullptr, int(n), 0,
+                           float(1.0 / double(n)), partial, nullptr);
+        check(hipGetLastError(), "clayMlpPass");
+        hipLaunchKernelGGL(clayMlpReduce, dim3((kParams + 1 + kThreads - 1) / kThreads), dim3(kThreads), 0, s,
+                           partial, blocks, params, nullptr, nullptr, grad, dloss, 0, 0.f, 0.f, 0.f, 0.f, 0);
+        check(hipGetLastError(), "clayMlpReduce");
+        float l = 0;
+        check(hipMemcpyAsync(gradient, grad, kParams * sizeof(float), hipMemcpyDeviceToHost, s), "gradient");
+        check(hipMemcpyAsync(&l, dloss, sizeof(float), hipMemcpyDeviceToHost, s), "loss");
         check(hipStreamSynchronize(s), "gradient");
         if (loss)
             *loss = l;

@@ -317,6 +317,15 @@ int clay_mlp_init(clay_tail_model* model, const float* x, const float* y, size_t
  * with respect to the parameters (CLAY_MLP_PARAMS floats). */
 int clay_hip_mlp_gradient(clay_hip* h, const clay_tail_model* model, const float* x, const float* y, size_t n,
                           float* gradient, float* loss);
+/* clay_hip_mlp_gradient without allocating device memory. No reference
+ * counterpart: for callers that cannot allocate (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc). Same arguments and
+ * results as clay_hip_mlp_gradient (bit-identical); every device buffer lives
+ * in a fixed scratch array of about 45 MB that each device holds until the
+ * process ends. n <= 65536 samples, else CLAY_E_INVALID. Calls on one device
+ * share that array: calls that overlap on one GPU race. */
+int clay_hip_mlp_gradient_noalloc(clay_hip* h, const clay_tail_model* model, const float* x, const float* y,
+                                  size_t n, float* gradient, float* loss);
 /* Adam training in place; epoch_loss (epochs floats, may be NULL) = mean loss per epoch. */
 int clay_hip_mlp_train(clay_hip* h, clay_tail_model* model, const float* x, const float* y, size_t n,
                        const clay_mlp_train_config* config, float* epoch_loss);
