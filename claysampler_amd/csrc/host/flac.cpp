@@ -7,6 +7,7 @@
 #include "audio.h"
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <fstream>
 #include <iterator>
@@ -349,6 +350,83 @@ void readSubframe(BitReader& br, size_t n, int bps, std::vector<int64_t>& x)
 
 } // namespace
 
+// MD5 message digest (R. Rivest, RFC 1321) for the STREAMINFO signature of
+// the unencoded audio: 64-byte blocks, four rounds of sixteen steps with the
+// sine-derived constants T[i] = floor(2^32 |sin(i + 1)|).
+class Md5 {
+public:
+    Md5()
+    {
+        for (int i = 0; i < 64; ++i)
+            t_[i] = uint32_t(std::floor(std::fabs(std::sin(double(i + 1))) * 4294967296.0));
+    }
+    void update(const uint8_t* p, size_t n)
+    {
+        length_ += n;
+        while (n) {
+            const size_t take = std::min(n, size_t(64) - used_);
+            std::memcpy(buffer_ + used_, p, take);
+            used_ += take;
+            p += take;
+            n -= take;
+            if (used_ == 64) {
+                block(buffer_);
+                used_ = 0;
+            }
+        }
+    }
+    void finish(uint8_t out[16])
+    {
+        const uint64_t bits = length_ * 8;
+        const uint8_t one = 0x80, zero = 0;
+        update(&one, 1);
+        while (used_ != 56)
+            update(&zero, 1);
+        uint8_t tail[8];
+        for (int i = 0; i < 8; ++i)
+            tail[i] = uint8_t(bits >> (8 * i));
+        update(tail, 8);
+        const uint32_t words[4] = {a_, b_, c_, d_};
+        for (int i = 0; i < 16; ++i)
+            out[i] = uint8_t(words[i / 4] >> (8 * (i % 4)));
+    }
+
+private:
+    static uint32_t rotate(uint32_t x, int s) { return (x << s) | (x >> (32 - s)); }
+    void block(const uint8_t* p)
+    {
+        static const int shifts[4][4] = {{7, 12, 17, 22}, {5, 9, 14, 20}, {4, 11, 16, 23}, {6, 10, 15, 21}};
+        uint32_t m[16];
+        for (int i = 0; i < 16; ++i)
+            m[i] = uint32_t(p[4 * i]) | uint32_t(p[4 * i + 1]) << 8 | uint32_t(p[4 * i + 2]) << 16 |
+                   uint32_t(p[4 * i + 3]) << 24;
+        uint32_t a = a_, b = b_, c = c_, d = d_;
+        for (int i = 0; i < 64; ++i) {
+            const int round = i / 16;
+            uint32_t f;
+            int g;
+            if (round == 0) { f = (b & c) | (~b & d); g = i; }
+            else if (round == 1) { f = (b & d) | (c & ~d); g = (5 * i + 1) % 16; }
+            else if (round == 2) { f = b ^ c ^ d; g = (3 * i + 5) % 16; }
+            else { f = c ^ (b | ~d); g = (7 * i) % 16; }
+            const uint32_t next = b + rotate(a + f + t_[i] + m[g], shifts[round][i % 4]);
+            a = d;
+            d = c;
+            c = b;
+            b = next;
+        }
+        a_ += a;
+        b_ += b;
+        c_ += c;
+        d_ += d;
+    }
+    uint32_t t_[64];
+    uint32_t a_ = 0x67452301u, b_ = 0xefcdab89u, c_ = 0x98badcfeu, d_ = 0x10325476u;
+    uint64_t length_ = 0;
+    uint8_t buffer_[64];
+    size_t used_ = 0;
+};
+
 void writeFlac(const std::string& path, const int16_t* pcm, int dataFrames, int frames, int rate, int channels, int level)
 {
     if (frames <= 0 || dataFrames < 0 || dataFrames > frames || (channels != 1 && channels != 2) || level < 0 || level > 8)
@@ -367,8 +445,23 @@ void writeFlac(const std::string& path, const int16_t* pcm, int dataFrames, int 
     out.put(uint64_t(channels - 1), 3);
     out.put(15, 5);
     out.put(uint64_t(frames), 36);
-    for (int i = 0; i < 16; ++i)
-        out.put(0, 8);
+    // MD5 of the audio as encoded: `frames` frames of little-endian int16,
+    // channels interleaved, zeros past the data frames.
+    {
+        Md5 md5;
+        uint8_t sample[2] = {0, 0};
+        for (int f = 0; f < frames; ++f)
+            for (int c = 0; c < channels; ++c) {
+                const int16_t v = f < dataFrames ? pcm[size_t(f) * size_t(channels) + size_t(c)] : int16_t(0);
+                sample[0] = uint8_t(uint16_t(v) & 0xffu);
+                sample[1] = uint8_t(uint16_t(v) >> 8);
+                md5.update(sample, 2);
+            }
+        uint8_t digest[16];
+        md5.finish(digest);
+        for (int i = 0; i < 16; ++i)
+            out.put(digest[i], 8);
+    }
     std::vector<int64_t> ch[2], side, mid, best[4];
     uint64_t frameNumber = 0;
     for (int first = 0; first < frames; first += kBlock, ++frameNumber) {

@@ -92,6 +92,10 @@ def render(cfg, bank_ptr, bank_count, plans):
     return list(res), pcm
 
 
+def ref_midi_built():
+    return os.path.exists(REF_MIDI)
+
+
 def ref_midi():
     """The reference's own MIDI loader (oracle/_ref), or None where it is not built."""
     if not os.path.exists(REF_MIDI):

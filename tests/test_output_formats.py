@@ -2,9 +2,10 @@
 own `wave` reader; FLAC (this project's RFC 9639 encoder, replacing the
 reference's libsndfile/libFLAC) round-tripped through this project's decoder
 at every level, mono and stereo, padded and partial frames, and a corrupted
-frame is refused by its CRC. No independent FLAC decoder is installed here
-(no libFLAC, flac or sox), so FLAC bit-stream conformance beyond this decoder
-is not pinned."""
+frame is refused by its CRC. The STREAMINFO MD5 equals hashlib's digest of
+the audio. No independent FLAC decoder is installed here (no libFLAC, flac or
+sox), so the frame and subframe coding is pinned by this decoder, the CRCs
+and the MD5 only."""
 
 import ctypes as C
 import os
@@ -76,6 +77,23 @@ def test_flac_extremes_and_compression(tmp_path):
     tone = signal(44100, 1, 44100)
     assert write("flac", tmp_path / "t.flac", tone, 44100, 44100, 1) == abi.OK
     assert os.path.getsize(tmp_path / "t.flac") < 0.6 * 2 * 44100         # predictors do compress
+
+
+@pytest.mark.parametrize("channels,data,frames", [(1, 30000, 30000), (2, 12345, 20000), (2, 0, 4096)])
+def test_flac_streaminfo_md5_is_the_audio_digest(tmp_path, channels, data, frames):
+    """STREAMINFO carries the MD5 of the unencoded audio (RFC 9639 8.2):
+    frames x channels little-endian int16, zeros past the data frames, as
+    hashlib computes it independently of this encoder."""
+    import hashlib
+
+    rng = np.random.default_rng(channels + data)
+    x = rng.integers(-32768, 32767, max(1, data) * channels).astype(np.int16)
+    assert write("flac", tmp_path / "m.flac", x, data, frames, channels, 5) == abi.OK
+    raw = (tmp_path / "m.flac").read_bytes()
+    assert raw[:4] == b"fLaC" and raw[4] & 0x7F == 0 and int.from_bytes(raw[5:8], "big") == 34
+    audio = np.zeros(frames * channels, "<i2")
+    audio[:data * channels] = x[:data * channels]
+    assert raw[26:42] == hashlib.md5(audio.tobytes()).digest()
 
 
 def test_flac_crc_rejects_corruption(tmp_path):
