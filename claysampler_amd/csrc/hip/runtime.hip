@@ -21,6 +21,7 @@
 #include <hip/hip_runtime.h>
 
 #include "device_types.h"
+#include "staging.h"
 
 #include <algorithm>
 #include <atomic>
@@ -104,6 +105,36 @@ template<class T>
 size_t bytesOf(const std::vector<T>& v) { return v.size() * sizeof(T); }
 
 uint64_t ceilDiv(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
+
+// Staging policy of the render core (staging.h): a hipEvent behind each copy
+// on the renderer's stream, waited with hipEventSynchronize; pinned buffers
+// from hipHostMalloc. A failed wait leaves the error to the next HIP call.
+struct HipFences {
+    const hipStream_t* stream = nullptr;
+    using Fence = hipEvent_t;
+    Fence record()
+    {
+        hipEvent_t e = nullptr;
+        check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
+        check(hipEventRecord(e, *stream), "hipEventRecord");
+        return e;
+    }
+    void wait(Fence& e)
+    {
+        if (!e)
+            return;
+        (void)hipEventSynchronize(e);
+        (void)hipEventDestroy(e);
+        e = nullptr;
+    }
+    char* allocate(size_t bytes)
+    {
+        void* p = nullptr;
+        check(hipHostMalloc(&p, bytes, hipHostMallocDefault), "pinned staging");
+        return static_cast<char*>(p);
+    }
+    void free(char* p) { (void)hipHostFree(p); }
+};
 
 // std::vector of trivially copyable records -> device, in one arena.
 struct ArenaBuilder {
@@ -274,85 +305,13 @@ struct clay_hip {
     };
     std::vector<Graph> graphs;
     // Pinned image of a plain batch's records (grown on demand): the copy runs
-    // at DMA speed instead of through a pageable bounce.
-    char* staging = nullptr;
-    size_t stagingBytes = 0;
-    // Recorded behind the last copy out of `staging`: the image is rewritten or
-    // freed only after it (a caller may prepare a batch while another's
-    // upload is still in flight).
-    hipEvent_t stagingCopied = nullptr;
-    bool stagingPending = false;
+    // at DMA speed instead of through a pageable bounce; rewritten or freed only
+    // after the copy that last read it (a caller may prepare a batch while
+    // another's upload is still in flight).
+    clay_staging::Image<HipFences> staging{HipFences{&stream}};
     // Pinned staging of persistent-form uploads: a ring whose regions are
-    // reused once the copy that read them has completed.
-    struct Ring {
-        char* base = nullptr;
-        size_t capacity = 0, head = 0;
-        struct Use {
-            size_t begin, end;
-            hipEvent_t done;
-        };
-        std::deque<Use> uses;
-        size_t pendingBegin = 0, pendingEnd = 0;
-
-        void retire(const Use& u)
-        {
-            check(hipEventSynchronize(u.done), "staging ring");
-            (void)hipEventDestroy(u.done);
-        }
-
-        char* take(size_t bytes)
-        {
-            bytes = std::max<size_t>(bytes, 1);
-            if (bytes > capacity) {
-                for (auto& u : uses)
-                    retire(u);
-                uses.clear();
-                if (base)
-                    check(hipHostFree(base), "hipHostFree");
-                base = nullptr;
-                capacity = std::max<size_t>(bytes, size_t(64) << 20);
-                check(hipHostMalloc(reinterpret_cast<void**>(&base), capacity, hipHostMallocDefault), "staging ring");
-                head = 0;
-            }
-            if (head + bytes > capacity)
-                head = 0;
-            const size_t begin = head, end = head + bytes;
-            // Every earlier region that overlaps must have been read by its
-            // copy; after wrap-arounds such regions need not be the oldest.
-            for (auto it = uses.begin(); it != uses.end();) {
-                if (it->begin < end && begin < it->end) {
-                    retire(*it);
-                    it = uses.erase(it);
-                } else {
-                    ++it;
-                }
-            }
-            pendingBegin = begin;
-            pendingEnd = end;
-            head = end;
-            return base + begin;
-        }
-
-        void commit(hipStream_t stream)
-        {
-            hipEvent_t e = nullptr;
-            check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
-            check(hipEventRecord(e, stream), "hipEventRecord");
-            uses.push_back({pendingBegin, pendingEnd, e});
-        }
-
-        void release()
-        {
-            for (auto& u : uses) {
-                (void)hipEventSynchronize(u.done);
-                (void)hipEventDestroy(u.done);
-            }
-            uses.clear();
-            if (base)
-                (void)hipHostFree(base);
-            base = nullptr;
-        }
-    } ring;
+    // reused once the copies that read them have completed.
+    clay_staging::Ring<HipFences> ring{HipFences{&stream}};
     // Small pinned buffers (per-unit results) kept for reuse.
     std::vector<std::pair<void*, size_t>> pinnedSpare;
 
@@ -862,25 +821,7 @@ void upload(clay_hip* h, clay_hip_batch& b, bool streamOrdered = false)
     // Host image of the records: pinned ring (persistent form, the copy stays
     // asynchronous) or the renderer's one pinned image, grown as needed (plain
     // form, whose batches upload one at a time).
-    char* image = nullptr;
-    if (streamOrdered) {
-        image = h->ring.take(recordBytes);
-    } else {
-        if (h->stagingPending) {
-            check(hipEventSynchronize(h->stagingCopied), "record staging");
-            h->stagingPending = false;
-        }
-        if (h->stagingBytes < recordBytes) {
-            if (h->staging)
-                (void)hipHostFree(h->staging);
-            h->staging = nullptr;
-            h->stagingBytes = 0;
-            check(hipHostMalloc(reinterpret_cast<void**>(&h->staging), std::max<size_t>(recordBytes, 1)),
-                  "record staging");
-            h->stagingBytes = recordBytes;
-        }
-        image = h->staging;
-    }
+    char* image = streamOrdered ? h->ring.take(recordBytes) : h->staging.take(recordBytes);
     // Large arrays are copied into the image by several host threads.
     const auto put = [&](size_t offset, const auto& v) {
         const size_t bytes = bytesOf(v);
@@ -909,14 +850,10 @@ void upload(clay_hip* h, clay_hip_batch& b, bool streamOrdered = false)
     b.dOnsets = reinterpret_cast<int*>(put(oOnsets, b.onsets));
     b.dGroups = reinterpret_cast<int2*>(put(oGroups, b.groups));
     check(hipMemcpyAsync(base, image, recordBytes, hipMemcpyHostToDevice, h->stream), "batch upload");
-    if (streamOrdered) {
-        h->ring.commit(h->stream);
-    } else {
-        if (!h->stagingCopied)
-            check(hipEventCreateWithFlags(&h->stagingCopied, hipEventDisableTiming), "hipEventCreate");
-        check(hipEventRecord(h->stagingCopied, h->stream), "hipEventRecord");
-        h->stagingPending = true;
-    }
+    if (streamOrdered)
+        h->ring.commit();
+    else
+        h->staging.commit();
     b.dStates = reinterpret_cast<VoiceState*>(base + oStates);
     b.dMix = base + oMix;
     b.dFiltered = reinterpret_cast<float*>(base + oFiltered);
@@ -1471,10 +1408,7 @@ int clay_hip_destroy(clay_hip* h)
     for (auto& g : h->graphs)
         (void)hipGraphExecDestroy(g.exec);
     h->ring.release();
-    if (h->staging)
-        (void)hipHostFree(h->staging);
-    if (h->stagingCopied)
-        (void)hipEventDestroy(h->stagingCopied);
+    h->staging.release();
     for (auto& p : h->pinnedSizes)
         (void)hipHostFree(p.first);
     for (auto& stage : h->stages) {
@@ -1680,6 +1614,9 @@ int renderWindowPlain(clay_hip* h, const clay_plan_view* plans, size_t count, cl
     // halved batch) or a failed flatten falls back to the plain preparation.
     struct Ahead {
         size_t first = 0, end = 0;
+        // The views of the batch being flattened, copied: the window's queue
+        // may grow (refill) and move while the worker runs.
+        std::vector<clay_plan_view> views;
         clay_hip_batch* b = nullptr;
         std::thread worker;
         bool ok = false;
@@ -1762,11 +1699,11 @@ int renderWindowPlain(clay_hip* h, const clay_plan_view* plans, size_t count, cl
                 ahead.end = batchEnd(end);
                 ahead.b = new clay_hip_batch();
                 ahead.ok = false;
-                const clay_plan_view* next = queue.data() + ahead.first;
-                const size_t n = ahead.end - ahead.first;
-                ahead.worker = std::thread([h, next, n, &ahead] {
+                ahead.views.assign(queue.begin() + std::ptrdiff_t(ahead.first),
+                                   queue.begin() + std::ptrdiff_t(ahead.end));
+                ahead.worker = std::thread([h, &ahead] {
                     try {
-                        flatten(h, next, n, *ahead.b);
+                        flatten(h, ahead.views.data(), ahead.views.size(), *ahead.b);
                         ahead.ok = true;
                     } catch (...) {
                         ahead.ok = false;

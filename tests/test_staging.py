@@ -1,0 +1,41 @@
+"""The staging rules of the render core's pinned upload buffers
+(claysampler_amd/csrc/hip/staging.h), on the CPU: tests/staging_check.cpp
+replays random uploads of uneven sizes through a model of the copy queue in
+which queued copies run in stream order at random times, and checks that
+every copy reads the bytes it was queued with and no buffer is freed under a
+queued copy. The ring of the persistent form and the single image of the
+plain form pass; a ring that waits only for the oldest overlapping region
+(the mistake behind a GPU memory fault once) is caught."""
+
+import os
+import subprocess
+
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+@pytest.fixture(scope="module")
+def checker(tmp_path_factory):
+    exe = str(tmp_path_factory.mktemp("staging") / "staging_check")
+    subprocess.run(["g++", "-O2", "-std=c++17", "-Wall", "-o", exe, os.path.join(HERE, "staging_check.cpp")],
+                   check=True)
+    return exe
+
+
+def run(exe, kind, seed, count):
+    out = subprocess.run([exe, kind, str(seed), str(count)], check=True, capture_output=True, text=True).stdout
+    fields = out.split()
+    return int(fields[2]), int(fields[4])
+
+
+@pytest.mark.parametrize("kind", ["ring", "image"])
+def test_no_copy_reads_rewritten_bytes(checker, kind):
+    for seed in (1, 2):
+        executed, failures = run(checker, kind, seed, 100000)
+        assert executed == 100000 and failures == 0, (seed, failures)
+
+
+def test_the_check_catches_a_ring_that_waits_only_for_the_oldest_overlap(checker):
+    _, failures = run(checker, "planted", 1, 100000)
+    assert failures > 0
