@@ -12,7 +12,7 @@
  *   - Frames are output frames at the renderer's rate. Sample offsets index the
  *     int16 sample bank uploaded with clay_hip_upload_bank (interleaved when
  *     stereo). The bank is int16 PCM: value/32768 equals the float32 bank of the
- *     reference (src/io/audio.cpp:182-201, libsndfile PCM16 decode) exactly.
+ *     reference (src/io/audio.cpp:77,138-142, libsndfile's PCM16 decode) exactly.
  *   - One clay_hip handle belongs to one host thread at a time (the reference
  *     drives its renderer from one thread: src/main.cpp:468-553).
  */
