@@ -137,45 +137,62 @@ def cpu_baseline(sfz, mids, taps, jobs, threads, voice_frame_budget=None):
                       f"{threads} threads, {dt:.1f} s"}
 
 
-def window_pass(policy, taps, form, batch_jobs, plans, bank, checksums=False, flags=0, statuses=None, **renderer):
-    """One clay_hip_render_window over every plan. Returns (wall s, per-job
-    checksums or None, stats). The publish callback only releases the PCM
-    (and optionally checksums it), so the time is the render core's. Extra
+def window_pass(policy, taps, form, batch_jobs, plans, bank, checksums=False, flags=0, statuses=None, warm=None,
+                **renderer):
+    """clay_hip_render_window over every plan on one renderer: an untimed
+    window over `warm` first (graph capture, pinned staging and PCM slots),
+    then the timed window, then, with `checksums`, an untimed window that
+    checksums every job's PCM16. Returns (wall s of the timed window, per-job
+    checksums or None, stats of the timed window). The timed window's publish
+    callback only releases the PCM, so the time is the render core's. Extra
     keyword arguments go to ca.Renderer; `statuses`, a dict, receives each
     job's tail status."""
     import ctypes as C
 
     r = ca.Renderer(policy=policy, taps=taps, form=form, batch_jobs=batch_jobs, flags=flags, **renderer)
     r.upload_bank(*bank)
-    sums = {} if checksums else None
-    published = [0]
 
-    def publish(user, batch_ptr, first):
-        b = batch_ptr.contents
-        if sums is not None:
-            for k in range(b.result_count):
-                res = b.results[k]
-                n = res.data_frames * r.channels
-                view = np.ctypeslib.as_array(C.cast(C.addressof(b.pcm.contents) + 2 * res.offset * r.channels,
-                                                    C.POINTER(C.c_int16)), shape=(n,)) if n else np.zeros(0, np.int16)
-                sums[first + k] = (res.status, res.frames, res.data_frames, int(view.astype(np.int64).sum()),
-                                   int((view.astype(np.int64) * np.arange(n, dtype=np.int64) % 65521).sum()))
-        if statuses is not None:
-            for k in range(b.result_count):
-                statuses[first + k] = abi.TAIL_STATUS[b.results[k].status]
-        published[0] += max(1, b.result_count)
-        if b.pcm_handle:
-            r.lib.clay_hip_release_pcm(r.handle, b.pcm_handle)
-        return 0
+    def run(plan_array, sums, statuses):
+        published = [0]
 
-    cb = abi.PUBLISH_FN(publish)
-    t0 = time.perf_counter()
-    status = r.lib.clay_hip_render_window(r.handle, plans, len(plans), cb, abi.REFILL_FN(), None)
-    wall = time.perf_counter() - t0
-    r._check(status)
+        def publish(user, batch_ptr, first):
+            b = batch_ptr.contents
+            if sums is not None:
+                for k in range(b.result_count):
+                    res = b.results[k]
+                    n = res.data_frames * r.channels
+                    view = np.ctypeslib.as_array(C.cast(C.addressof(b.pcm.contents) + 2 * res.offset * r.channels,
+                                                        C.POINTER(C.c_int16)), shape=(n,)) if n else np.zeros(0, np.int16)
+                    sums[first + k] = (res.status, res.frames, res.data_frames, int(view.astype(np.int64).sum()),
+                                       int((view.astype(np.int64) * np.arange(n, dtype=np.int64) % 65521).sum()))
+            if statuses is not None:
+                for k in range(b.result_count):
+                    statuses[first + k] = abi.TAIL_STATUS[b.results[k].status]
+            published[0] += max(1, b.result_count)
+            if b.pcm_handle:
+                r.lib.clay_hip_release_pcm(r.handle, b.pcm_handle)
+            return 0
+
+        cb = abi.PUBLISH_FN(publish)
+        t0 = time.perf_counter()
+        status = r.lib.clay_hip_render_window(r.handle, plan_array, len(plan_array), cb, abi.REFILL_FN(), None)
+        wall = time.perf_counter() - t0
+        r._check(status)
+        assert published[0] == len(plan_array)
+        return wall
+
+    if warm is not None:
+        run(warm, None, None)
+    before = r.stats()
+    wall = run(plans, None, statuses)
     stats = r.stats()
+    for name in ("rounds", "admissions", "segments", "launches", "graph_launches", "downloaded_bytes"):
+        setattr(stats, name, getattr(stats, name) - getattr(before, name))
+    sums = None
+    if checksums:
+        sums = {}
+        run(plans, sums, None)
     r.close()
-    assert published[0] == len(plans)
     return wall, sums, stats
 
 
@@ -203,12 +220,10 @@ def c5_main(args):
     log(f"planned {jobs} jobs, {sum(p.span_count for p in plans)} spans, {bank[1] * 2 / 1e9:.2f} GB bank "
         f"in {time.perf_counter() - t0:.1f} s")
     kw = dict(rate=rate, group_minimum=8)
-    warm = (abi.PlanView * min(2, len(plans)))(*plans[:2])
-    window_pass(abi.POLICY_NATURAL, 64, abi.FORM_PERSISTENT, args.batch_jobs, warm, bank, **kw)
-    log("warmup done")
+    warm = (abi.PlanView * min(8, len(plans)))(*plans[:8])
     statuses = {}
     rolled, got, stats = window_pass(abi.POLICY_NATURAL, 64, abi.FORM_PERSISTENT, args.batch_jobs, plans, bank,
-                                     checksums=args.forms, statuses=statuses, **kw)
+                                     checksums=args.forms, statuses=statuses, warm=warm, **kw)
     log(f"persistent: {rolled:.3f} s, {stats.rounds} rounds, segment {stats.segment_frames}")
     line = {
         "metric": "rendered audio-sec/sec and voice-samples/sec on 1 GPU; % of HBM roofline",
@@ -236,7 +251,7 @@ def c5_main(args):
     }
     if args.forms:
         plain, want, _ = window_pass(abi.POLICY_NATURAL, 64, abi.FORM_PLAIN, args.batch_jobs, plans, bank,
-                                     checksums=True, **kw)
+                                     checksums=True, warm=warm, **kw)
         line["plain_value"] = frames / rate / plain
         line["identical_pcm16"] = want == got
         log(f"plain: {plain:.3f} s, identical PCM16: {want == got}")
@@ -248,17 +263,14 @@ def forms_main(args, jobs, sfz, mids, taps, plans, bank):
     form on the same jobs, whole-window wall time, identical PCM16 required."""
     frames = sum(p.frames for p in plans)
     audio_seconds = frames / 44100.0
-    warm = (abi.PlanView * min(8, len(plans)))(*plans[:8])
-    for form in (abi.FORM_PLAIN, abi.FORM_PERSISTENT):
-        window_pass(abi.POLICY_NATURAL, taps, form, args.batch_jobs, warm, bank)
-    log("warmup done")
-    plain, _, plain_stats = window_pass(abi.POLICY_NATURAL, taps, abi.FORM_PLAIN, args.batch_jobs, plans, bank)
+    warm = (abi.PlanView * len(plans))(*plans)
+    plain, want, plain_stats = window_pass(abi.POLICY_NATURAL, taps, abi.FORM_PLAIN, args.batch_jobs, plans, bank,
+                                           checksums=True, warm=warm)
     log(f"plain: {plain:.3f} s, {plain_stats.segments} segments")
-    rolled, _, stats = window_pass(abi.POLICY_NATURAL, taps, abi.FORM_PERSISTENT, args.batch_jobs, plans, bank)
+    rolled, got, stats = window_pass(abi.POLICY_NATURAL, taps, abi.FORM_PERSISTENT, args.batch_jobs, plans, bank,
+                                     checksums=True, warm=warm)
     log(f"persistent: {rolled:.3f} s, {stats.rounds} rounds, segment {stats.segment_frames}, "
         f"mix blocks {stats.mix_blocks}, graph launches {stats.graph_launches}")
-    _, want, _ = window_pass(abi.POLICY_NATURAL, taps, abi.FORM_PLAIN, args.batch_jobs, plans, bank, True)
-    _, got, _ = window_pass(abi.POLICY_NATURAL, taps, abi.FORM_PERSISTENT, args.batch_jobs, plans, bank, True)
     identical = want == got
     log(f"identical PCM16: {identical}")
     line = {
@@ -548,13 +560,11 @@ def forms_compare(args, plans, bank, taps, frames):
     """The plain and persistent forms through render_window on the same jobs
     (--gpu-batch-jobs as given), wall time each including host transfers,
     and per-job PCM16 checksums compared."""
-    warm = (abi.PlanView * min(8, len(plans)))(*plans[:8])
-    for form in (abi.FORM_PLAIN, abi.FORM_PERSISTENT):
-        window_pass(abi.POLICY_NATURAL, taps, form, args.batch_jobs, warm, bank)
-    plain, _, plain_stats = window_pass(abi.POLICY_NATURAL, taps, abi.FORM_PLAIN, args.batch_jobs, plans, bank)
-    rolled, _, stats = window_pass(abi.POLICY_NATURAL, taps, abi.FORM_PERSISTENT, args.batch_jobs, plans, bank)
-    _, want, _ = window_pass(abi.POLICY_NATURAL, taps, abi.FORM_PLAIN, args.batch_jobs, plans, bank, True)
-    _, got, _ = window_pass(abi.POLICY_NATURAL, taps, abi.FORM_PERSISTENT, args.batch_jobs, plans, bank, True)
+    warm = (abi.PlanView * len(plans))(*plans)
+    plain, want, plain_stats = window_pass(abi.POLICY_NATURAL, taps, abi.FORM_PLAIN, args.batch_jobs, plans, bank,
+                                           checksums=True, warm=warm)
+    rolled, got, stats = window_pass(abi.POLICY_NATURAL, taps, abi.FORM_PERSISTENT, args.batch_jobs, plans, bank,
+                                     checksums=True, warm=warm)
     log(f"windows: plain {plain:.3f} s, persistent {rolled:.3f} s, identical PCM16 {want == got}")
     audio_seconds = frames / 44100.0
     return {"batch_jobs": args.batch_jobs, "plain_window_s": plain, "persistent_window_s": rolled,

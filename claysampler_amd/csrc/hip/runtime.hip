@@ -127,6 +127,15 @@ struct HipFences {
         (void)hipEventDestroy(e);
         e = nullptr;
     }
+    bool done(Fence& e)
+    {
+        if (e && hipEventQuery(e) == hipErrorNotReady)
+            return false;
+        if (e)
+            (void)hipEventDestroy(e);
+        e = nullptr;
+        return true;
+    }
     char* allocate(size_t bytes)
     {
         void* p = nullptr;
@@ -284,9 +293,10 @@ struct clay_hip {
     };
     std::vector<Slot> slots;
     size_t residentBytes = 0;
+    int tunedSegment = 0;  // persistent form: segment frames chosen by the round tuner, 0 = not yet
     // Round descriptors: a header and up to unitCapacity unit views on the
     // device, written through a ring of pinned staging buffers.
-    static constexpr int kStages = 8;
+    static constexpr int kStages = 16;
     struct Stage {
         void* host = nullptr;
         hipEvent_t copied = nullptr;
@@ -309,9 +319,9 @@ struct clay_hip {
     // after the copy that last read it (a caller may prepare a batch while
     // another's upload is still in flight).
     clay_staging::Image<HipFences> staging{HipFences{&stream}};
-    // Pinned staging of persistent-form uploads: a ring whose regions are
-    // reused once the copies that read them have completed.
-    clay_staging::Ring<HipFences> ring{HipFences{&stream}};
+    // Pinned images of persistent-form records, packed by the threads that
+    // flatten cohorts and reused once the copies out of them have completed.
+    clay_staging::Pool<HipFences> images{HipFences{&stream}};
     // Small pinned buffers (per-unit results) kept for reuse.
     std::vector<std::pair<void*, size_t>> pinnedSpare;
 
@@ -338,6 +348,16 @@ struct clay_hip {
     std::map<void*, size_t> pinnedSizes;
 
     int channels() const { return natural ? 1 : 2; }
+
+    // Second stream for the persistent form's downloads (default priority),
+    // created on first use.
+    hipStream_t copies = nullptr;
+    hipStream_t copyStream()
+    {
+        if (!copies)
+            check(hipStreamCreateWithFlags(&copies, hipStreamNonBlocking), "hipStreamCreate");
+        return copies;
+    }
 
     hipEvent_t event()
     {
@@ -785,96 +805,144 @@ void release(clay_hip* h, clay_hip_batch& b)
     b.arena = nullptr;
 }
 
-// One device arena for all records, scratch and finishing buffers. The
-// persistent form allocates it stream-ordered and does not wait for the
-// copies (the host records live as long as the batch).
-void upload(clay_hip* h, clay_hip_batch& b, bool streamOrdered = false)
+// Where each record array and buffer of a batch sits in its one device arena:
+// the records first ([0, records), copied from one host image), then the
+// scratch and finishing buffers.
+struct ArenaLayout {
+    size_t spans, cold, mods, points, jobs, tiles, voices, pitch, chains, byStart, audio, onsets, groups;
+    size_t records;
+    size_t states, mix, filtered, phase, mono, stereo, windows, body, onsetPower, pcmJobs, output, results;
+    size_t total;
+};
+
+ArenaLayout layoutOf(const clay_hip* h, const clay_hip_batch& b)
 {
     ArenaBuilder a;
+    ArenaLayout L{};
     const int ch = h->channels();
-    const size_t oSpans = a.add(bytesOf(b.spans)), oCold = a.add(bytesOf(b.cold)), oMods = a.add(bytesOf(b.mods));
-    const size_t oPoints = a.add(bytesOf(b.points)), oJobs = a.add(bytesOf(b.jobs)), oTiles = a.add(bytesOf(b.tiles));
-    const size_t oVoices = a.add(bytesOf(b.voices));
-    const size_t oPitch = a.add(bytesOf(b.pitchHeads)), oChains = a.add(bytesOf(b.chainHeads));
-    const size_t oByStart = a.add(bytesOf(b.chainByStart));
-    const size_t oAudio = a.add(bytesOf(b.audioJobs)), oOnsets = a.add(bytesOf(b.onsets)), oGroups = a.add(bytesOf(b.groups));
-    const size_t recordBytes = a.bytes;  // records first: one host image, one copy
-    const size_t oStates = a.add(size_t(b.chains) * sizeof(VoiceState));
-    const size_t oMix = a.add(size_t(b.frames) * (h->natural ? sizeof(float2) : sizeof(short2)));
-    const size_t oFiltered = a.add(size_t(b.filteredSamples) * sizeof(float));
-    const size_t oPhase = a.add(size_t(b.pitchFrames) * sizeof(double));
-    const size_t oMono = a.add(h->natural ? size_t(b.frames) * sizeof(int16_t) : 0);
-    const size_t oStereo = a.add(h->natural ? size_t(b.blocks) * sizeof(StereoStats) : 0);
-    const size_t oWindows = a.add(size_t(b.windows) * sizeof(MonoStats));
-    const size_t oBody = a.add(size_t(b.bodyBlocks) * sizeof(unsigned long long));
-    const size_t oOnsetPower = a.add(b.onsets.size() * sizeof(unsigned long long));
-    const size_t oPcmJobs = a.add(b.assess ? b.jobs.size() * sizeof(PcmJob) : 0);
-    const size_t oOutput = a.add(b.assess ? size_t(b.frames) * size_t(ch) * sizeof(int16_t) : 0);
-    const size_t oResults = a.add(b.jobs.size() * sizeof(clay_tail_result));
-    if (streamOrdered)
-        check(hipMallocAsync(&b.arena, a.bytes, h->stream), "batch arena");
-    else
-        check(hipMalloc(&b.arena, a.bytes), "batch arena");
-    b.streamOrdered = streamOrdered;
-    b.arenaBytes = a.bytes;
-    auto* base = static_cast<char*>(b.arena);
-    // Host image of the records: pinned ring (persistent form, the copy stays
-    // asynchronous) or the renderer's one pinned image, grown as needed (plain
-    // form, whose batches upload one at a time).
-    char* image = streamOrdered ? h->ring.take(recordBytes) : h->staging.take(recordBytes);
-    // Large arrays are copied into the image by several host threads.
+    L.spans = a.add(bytesOf(b.spans));
+    L.cold = a.add(bytesOf(b.cold));
+    L.mods = a.add(bytesOf(b.mods));
+    L.points = a.add(bytesOf(b.points));
+    L.jobs = a.add(bytesOf(b.jobs));
+    L.tiles = a.add(bytesOf(b.tiles));
+    L.voices = a.add(bytesOf(b.voices));
+    L.pitch = a.add(bytesOf(b.pitchHeads));
+    L.chains = a.add(bytesOf(b.chainHeads));
+    L.byStart = a.add(bytesOf(b.chainByStart));
+    L.audio = a.add(bytesOf(b.audioJobs));
+    L.onsets = a.add(bytesOf(b.onsets));
+    L.groups = a.add(bytesOf(b.groups));
+    L.records = a.bytes;
+    L.states = a.add(size_t(b.chains) * sizeof(VoiceState));
+    L.mix = a.add(size_t(b.frames) * (h->natural ? sizeof(float2) : sizeof(short2)));
+    L.filtered = a.add(size_t(b.filteredSamples) * sizeof(float));
+    L.phase = a.add(size_t(b.pitchFrames) * sizeof(double));
+    L.mono = a.add(h->natural ? size_t(b.frames) * sizeof(int16_t) : 0);
+    L.stereo = a.add(h->natural ? size_t(b.blocks) * sizeof(StereoStats) : 0);
+    L.windows = a.add(size_t(b.windows) * sizeof(MonoStats));
+    L.body = a.add(size_t(b.bodyBlocks) * sizeof(unsigned long long));
+    L.onsetPower = a.add(b.onsets.size() * sizeof(unsigned long long));
+    L.pcmJobs = a.add(b.assess ? b.jobs.size() * sizeof(PcmJob) : 0);
+    L.output = a.add(b.assess ? size_t(b.frames) * size_t(ch) * sizeof(int16_t) : 0);
+    L.results = a.add(b.jobs.size() * sizeof(clay_tail_result));
+    L.total = a.bytes;
+    return L;
+}
+
+// The batch's records at their arena offsets in a host image of L.records
+// bytes; large arrays are copied by several threads when `threaded`.
+void packRecords(const clay_hip_batch& b, const ArenaLayout& L, char* image, bool threaded)
+{
     const auto put = [&](size_t offset, const auto& v) {
         const size_t bytes = bytesOf(v);
         const auto* src = reinterpret_cast<const char*>(v.data());
         constexpr size_t kPiece = size_t(8) << 20;
-        if (bytes > 2 * kPiece)
+        if (threaded && bytes > 2 * kPiece)
             parallelFor(ceilDiv(bytes, kPiece), [&](size_t k) {
                 const size_t at = k * kPiece;
                 std::memcpy(image + offset + at, src + at, std::min(kPiece, bytes - at));
             });
         else if (bytes)
             std::memcpy(image + offset, src, bytes);
-        return base + offset;
     };
-    b.dSpans = reinterpret_cast<Span*>(put(oSpans, b.spans));
-    b.dCold = reinterpret_cast<SpanCold*>(put(oCold, b.cold));
-    b.dMods = reinterpret_cast<Mod*>(put(oMods, b.mods));
-    b.dPoints = reinterpret_cast<clay_control_point*>(put(oPoints, b.points));
-    b.dJobs = reinterpret_cast<Job*>(put(oJobs, b.jobs));
-    b.dTiles = reinterpret_cast<Tile*>(put(oTiles, b.tiles));
-    b.dVoices = reinterpret_cast<int*>(put(oVoices, b.voices));
-    b.dPitchHeads = reinterpret_cast<int*>(put(oPitch, b.pitchHeads));
-    b.dChainHeads = reinterpret_cast<int*>(put(oChains, b.chainHeads));
-    b.dChainByStart = reinterpret_cast<int*>(put(oByStart, b.chainByStart));
-    b.dAudioJobs = reinterpret_cast<AudioJobDesc*>(put(oAudio, b.audioJobs));
-    b.dOnsets = reinterpret_cast<int*>(put(oOnsets, b.onsets));
-    b.dGroups = reinterpret_cast<int2*>(put(oGroups, b.groups));
-    check(hipMemcpyAsync(base, image, recordBytes, hipMemcpyHostToDevice, h->stream), "batch upload");
+    put(L.spans, b.spans);
+    put(L.cold, b.cold);
+    put(L.mods, b.mods);
+    put(L.points, b.points);
+    put(L.jobs, b.jobs);
+    put(L.tiles, b.tiles);
+    put(L.voices, b.voices);
+    put(L.pitch, b.pitchHeads);
+    put(L.chains, b.chainHeads);
+    put(L.byStart, b.chainByStart);
+    put(L.audio, b.audioJobs);
+    put(L.onsets, b.onsets);
+    put(L.groups, b.groups);
+}
+
+// The arena: stream-ordered (persistent form) or plain.
+void allocateArena(clay_hip* h, clay_hip_batch& b, const ArenaLayout& L, bool streamOrdered)
+{
     if (streamOrdered)
-        h->ring.commit();
+        check(hipMallocAsync(&b.arena, L.total, h->stream), "batch arena");
     else
-        h->staging.commit();
-    b.dStates = reinterpret_cast<VoiceState*>(base + oStates);
-    b.dMix = base + oMix;
-    b.dFiltered = reinterpret_cast<float*>(base + oFiltered);
-    b.dPhase = reinterpret_cast<double*>(base + oPhase);
-    b.dMono = reinterpret_cast<int16_t*>(base + oMono);
-    b.dStereo = reinterpret_cast<StereoStats*>(base + oStereo);
-    b.dWindows = reinterpret_cast<MonoStats*>(base + oWindows);
-    b.dBody = reinterpret_cast<unsigned long long*>(base + oBody);
-    b.dOnsetPower = reinterpret_cast<unsigned long long*>(base + oOnsetPower);
-    b.dPcmJobs = reinterpret_cast<PcmJob*>(base + oPcmJobs);
+        check(hipMalloc(&b.arena, L.total), "batch arena");
+    b.streamOrdered = streamOrdered;
+    b.arenaBytes = L.total;
+}
+
+// Queues the copy of the packed records into the arena, points the batch's
+// device views into it and clears the filter states.
+void placeRecords(clay_hip* h, clay_hip_batch& b, const ArenaLayout& L, const char* image)
+{
+    auto* base = static_cast<char*>(b.arena);
+    check(hipMemcpyAsync(base, image, L.records, hipMemcpyHostToDevice, h->stream), "batch upload");
+    b.dSpans = reinterpret_cast<Span*>(base + L.spans);
+    b.dCold = reinterpret_cast<SpanCold*>(base + L.cold);
+    b.dMods = reinterpret_cast<Mod*>(base + L.mods);
+    b.dPoints = reinterpret_cast<clay_control_point*>(base + L.points);
+    b.dJobs = reinterpret_cast<Job*>(base + L.jobs);
+    b.dTiles = reinterpret_cast<Tile*>(base + L.tiles);
+    b.dVoices = reinterpret_cast<int*>(base + L.voices);
+    b.dPitchHeads = reinterpret_cast<int*>(base + L.pitch);
+    b.dChainHeads = reinterpret_cast<int*>(base + L.chains);
+    b.dChainByStart = reinterpret_cast<int*>(base + L.byStart);
+    b.dAudioJobs = reinterpret_cast<AudioJobDesc*>(base + L.audio);
+    b.dOnsets = reinterpret_cast<int*>(base + L.onsets);
+    b.dGroups = reinterpret_cast<int2*>(base + L.groups);
+    b.dStates = reinterpret_cast<VoiceState*>(base + L.states);
+    b.dMix = base + L.mix;
+    b.dFiltered = reinterpret_cast<float*>(base + L.filtered);
+    b.dPhase = reinterpret_cast<double*>(base + L.phase);
+    b.dMono = reinterpret_cast<int16_t*>(base + L.mono);
+    b.dStereo = reinterpret_cast<StereoStats*>(base + L.stereo);
+    b.dWindows = reinterpret_cast<MonoStats*>(base + L.windows);
+    b.dBody = reinterpret_cast<unsigned long long*>(base + L.body);
+    b.dOnsetPower = reinterpret_cast<unsigned long long*>(base + L.onsetPower);
+    b.dPcmJobs = reinterpret_cast<PcmJob*>(base + L.pcmJobs);
     // Natural: the packed mono PCM reuses the float mix once it is quantised.
     // orig/custom: finalize works in place on the short2 mix.
-    b.dOutput = b.assess ? reinterpret_cast<int16_t*>(base + oOutput) : reinterpret_cast<int16_t*>(b.dMix);
-    b.dResults = reinterpret_cast<clay_tail_result*>(base + oResults);
+    b.dOutput = b.assess ? reinterpret_cast<int16_t*>(base + L.output) : reinterpret_cast<int16_t*>(b.dMix);
+    b.dResults = reinterpret_cast<clay_tail_result*>(base + L.results);
     if (b.chains)
         check(hipMemsetAsync(b.dStates, 0, size_t(b.chains) * sizeof(VoiceState), h->stream), "state reset");
-    if (!streamOrdered)
-        check(hipStreamSynchronize(h->stream), "batch upload");
-    h->residentBytes = a.bytes;
-    h->stats.peak_bytes = std::max<uint64_t>(h->stats.peak_bytes, a.bytes + h->bankCapacity * 2 +
+    h->residentBytes = L.total;
+    h->stats.peak_bytes = std::max<uint64_t>(h->stats.peak_bytes, L.total + h->bankCapacity * 2 +
                                                                       size_t(h->cfg.sinc_taps) * (kPhases + 1) * 4);
+}
+
+// Plain form: the records through the renderer's one pinned image, grown as
+// needed (batches upload one at a time), and a wait for the copy.
+void upload(clay_hip* h, clay_hip_batch& b)
+{
+    const ArenaLayout L = layoutOf(h, b);
+    allocateArena(h, b, L, false);
+    char* image = h->staging.take(L.records);
+    packRecords(b, L, image, true);
+    placeRecords(h, b, L, image);
+    h->staging.commit();
+    check(hipStreamSynchronize(h->stream), "batch upload");
 }
 
 constexpr int kMaxBlocksPerLaunch = 1 << 20;
@@ -1114,17 +1182,11 @@ void renderRound(clay_hip* h, const Window& w)
     ++h->stats.segments;
 }
 
-// One round with fixed grids (persistent form): a replay of the captured
-// round for this launch shape, captured on first use.
-void renderRoundFixed(clay_hip* h, const std::vector<Window>& windows, const Shape& shape, bool useGraph)
+// The captured round for a launch shape, captured on first use. The round's
+// windows are read from the device descriptors at replay, so one graph
+// serves every round of that shape.
+const clay_hip::Graph& roundGraph(clay_hip* h, const Shape& shape)
 {
-    stageRound(h, windows);
-    ++h->stats.segments;
-    ++h->stats.rounds;
-    if (!useGraph) {
-        launchRoundTaps(h, shape, true);
-        return;
-    }
     const clay_hip::Graph* found = nullptr;
     for (const auto& g : h->graphs)
         if (g.mixBlocks == shape.mixBlocks && g.chainBlocks == shape.chainBlocks && g.simpleMix == shape.simpleMix &&
@@ -1149,6 +1211,21 @@ void renderRoundFixed(clay_hip* h, const std::vector<Window>& windows, const Sha
         h->graphs.push_back({exec, shape.mixBlocks, shape.chainBlocks, shape.simpleMix, h->bank});
         found = &h->graphs.back();
     }
+    return *found;
+}
+
+// One round with fixed grids (persistent form): a replay of the captured
+// round for this launch shape.
+void renderRoundFixed(clay_hip* h, const std::vector<Window>& windows, const Shape& shape, bool useGraph)
+{
+    stageRound(h, windows);
+    ++h->stats.segments;
+    ++h->stats.rounds;
+    if (!useGraph) {
+        launchRoundTaps(h, shape, true);
+        return;
+    }
+    const clay_hip::Graph* found = &roundGraph(h, shape);
     if (h->timing) {
         h->launch("round", [&] { check(hipGraphLaunch(found->exec, h->stream), "hipGraphLaunch"); });
     } else {
@@ -1252,25 +1329,38 @@ void finishBatch(clay_hip* h, clay_hip_batch& b)
 }
 
 // PCM16 of a finished batch into a pinned slot (renderer.cpp:672-690).
+// The smallest free slot that fits is reused. A new slot is a power of two
+// (at least 1 MiB) so that batches of varying size reuse slots; free slots
+// too small are given back (hipHostFree is slow) only when the slots hold
+// more than kSlotBudget.
 void* pinnedSlot(clay_hip* h, size_t bytes)
 {
-    for (auto& slot : h->slots)
-        if (!slot.busy && slot.capacity >= bytes) {
-            slot.busy = true;
-            return slot.host;
-        }
-    for (auto& slot : h->slots)
-        if (!slot.busy) {
-            check(hipHostFree(slot.host), "hipHostFree");
-            slot.host = nullptr;
-            check(hipHostMalloc(&slot.host, std::max<size_t>(bytes, 1), hipHostMallocDefault), "pinned PCM");
-            slot.capacity = std::max<size_t>(bytes, 1);
-            slot.busy = true;
-            return slot.host;
+    constexpr size_t kSlotBudget = size_t(16) << 30;
+    clay_hip::Slot* best = nullptr;
+    size_t held = 0;
+    for (auto& slot : h->slots) {
+        held += slot.capacity;
+        if (!slot.busy && slot.capacity >= bytes && (!best || slot.capacity < best->capacity))
+            best = &slot;
+    }
+    if (best) {
+        best->busy = true;
+        return best->host;
+    }
+    size_t capacity = size_t(1) << 20;
+    while (capacity < bytes)
+        capacity *= 2;
+    for (size_t i = 0; i < h->slots.size() && held + capacity > kSlotBudget;)
+        if (!h->slots[i].busy) {
+            check(hipHostFree(h->slots[i].host), "hipHostFree");
+            held -= h->slots[i].capacity;
+            h->slots.erase(h->slots.begin() + std::ptrdiff_t(i));
+        } else {
+            ++i;
         }
     clay_hip::Slot slot;
-    check(hipHostMalloc(&slot.host, std::max<size_t>(bytes, 1), hipHostMallocDefault), "pinned PCM");
-    slot.capacity = std::max<size_t>(bytes, 1);
+    check(hipHostMalloc(&slot.host, capacity, hipHostMallocDefault), "pinned PCM");
+    slot.capacity = capacity;
     slot.busy = true;
     h->slots.push_back(slot);
     return slot.host;
@@ -1400,6 +1490,8 @@ int clay_hip_destroy(clay_hip* h)
     (void)hipSetDevice(h->device);
     if (h->stream)
         (void)hipStreamSynchronize(h->stream);
+    if (h->copies)
+        (void)hipStreamSynchronize(h->copies);
     for (auto& slot : h->slots)
         if (slot.host)
             (void)hipHostFree(slot.host);
@@ -1407,7 +1499,7 @@ int clay_hip_destroy(clay_hip* h)
         (void)hipEventDestroy(e);
     for (auto& g : h->graphs)
         (void)hipGraphExecDestroy(g.exec);
-    h->ring.release();
+    h->images.release();
     h->staging.release();
     for (auto& p : h->pinnedSizes)
         (void)hipHostFree(p.first);
@@ -1425,6 +1517,8 @@ int clay_hip_destroy(clay_hip* h)
         (void)hipFree(h->sinc);
     if (h->sincInt)
         (void)hipFree(h->sincInt);
+    if (h->copies)
+        (void)hipStreamDestroy(h->copies);
     if (h->stream)
         (void)hipStreamDestroy(h->stream);
     delete h;
@@ -1744,18 +1838,22 @@ int renderWindowPlain(clay_hip* h, const clay_plan_view* plans, size_t count, cl
 
 
 // Persistent form (the role of renderRolling, renderer.cpp:1351-1746).
-// Up to batch_jobs units are resident; each round advances every unit by
-// `segment` frames with one replay of the captured round. A unit whose last
+// Jobs are admitted in cohorts of consecutive jobs (1, 2, 4, ... up to
+// batch_jobs / 8, never splitting a volume group), flattened on worker threads ahead of need. Up to
+// batch_jobs jobs are rendering at once (a job whose frames are all rendered
+// no longer counts, though its cohort stays resident until its last job is
+// done); each round advances every resident cohort by
+// `segment` frames with one replay of the captured round. A cohort whose last
 // window was rendered is finished on the device right behind that round; its
-// results, then its PCM, come back through pinned memory and it is published
-// while later rounds run. Its place is filled from the queue (and refill) at
-// the next round. The host only waits when a staging buffer is still in use,
-// or when nothing is resident and completions are outstanding.
+// results, then its PCM, come back on a second stream while later rounds run,
+// and it is published from the host loop. Its place is filled at the next
+// round boundary. The host waits only for a staging buffer still in use, for
+// a flatten when nothing is resident, or for the oldest completion when
+// nothing is resident.
 struct RoundTuner {
     std::vector<std::pair<int, int>> candidates;  // (segment frames, mix blocks)
     std::vector<double> cost;                     // ms per rendered frame, summed over trials
-    std::vector<int> trials;
-    size_t next = 0;
+    std::vector<int> trials, issued;              // measured rounds completed / launched per candidate
     int segment = 0, mixBlocks = 0;
     bool tuning = false;
     static constexpr int kTrials = 2;
@@ -1767,60 +1865,109 @@ struct RoundTuner {
         segment = fixedSegment > 0 ? fixedSegment : 1 << 17;
         mixBlocks = cus * 16;
         tuning = !(h->cfg.flags & CLAY_FLAG_NO_TUNE);
+        if (h->tunedSegment > 0) {  // tuned by an earlier window of this renderer
+            segment = h->tunedSegment;
+            tuning = false;
+        }
         if (!tuning)
             return;
-        for (int seg : {1 << 16, 1 << 17, 1 << 18})
+        for (int seg : {1 << 16, 1 << 17, 1 << 18, 1 << 19})
             if (fixedSegment <= 0 || seg == fixedSegment)
                 candidates.emplace_back(seg, mixBlocks);
         if (fixedSegment > 0 && candidates.empty())
             candidates.emplace_back(fixedSegment, mixBlocks);
         cost.assign(candidates.size(), 0);
         trials.assign(candidates.size(), 0);
+        issued.assign(candidates.size(), 0);
     }
 
-    // Launch shape of the next round.
-    std::pair<int, int> pick() const
+    // Launch shape of the next round and the candidate it measures (-1: none).
+    // Measurements complete asynchronously: until every candidate's trials
+    // have come back, rounds past the last trial use the current default.
+    std::pair<int, std::pair<int, int>> pick() const
     {
-        if (tuning && next < candidates.size())
-            return candidates[next];
-        return {segment, mixBlocks};
+        if (tuning)
+            for (size_t i = 0; i < candidates.size(); ++i)
+                if (issued[i] < kTrials)
+                    return {int(i), candidates[i]};
+        return {-1, {segment, mixBlocks}};
     }
 
-    void record(double ms, uint64_t frames)
+    void launched(int index) { ++issued[size_t(index)]; }
+
+    void record(int index, double ms, uint64_t frames)
     {
-        if (!tuning || next >= candidates.size() || !frames)
+        if (!tuning || index < 0 || !frames)
             return;
-        cost[next] += ms / double(frames);
-        if (++trials[next] == kTrials)
-            ++next;
-        if (next == candidates.size()) {
-            size_t best = 0;
-            for (size_t i = 1; i < cost.size(); ++i)
-                if (cost[i] < cost[best])
-                    best = i;
-            segment = candidates[best].first;
-            mixBlocks = candidates[best].second;
-            tuning = false;
-        }
+        cost[size_t(index)] += ms / double(frames);
+        ++trials[size_t(index)];
+        for (int t : trials)
+            if (t < kTrials)
+                return;
+        size_t best = 0;
+        for (size_t i = 1; i < cost.size(); ++i)
+            if (cost[i] < cost[best])
+                best = i;
+        segment = candidates[best].first;
+        mixBlocks = candidates[best].second;
+        tuning = false;
+    }
+
+    // Keeps the choice for the renderer's later windows.
+    void keep(clay_hip* h) const
+    {
+        if (!tuning && !(h->cfg.flags & CLAY_FLAG_NO_TUNE))
+            h->tunedSegment = segment;
     }
 };
 
 int renderWindowPersistent(clay_hip* h, const clay_plan_view* plans, size_t count, clay_publish_fn publish,
                            clay_refill_fn refill, void* user)
 {
-    struct Unit {
+    // Consecutive queued jobs (never splitting a volume group) flattened into
+    // one batch: one arena, one record copy and one finishing sequence per
+    // cohort instead of per job.
+    struct Cohort {
         clay_hip_batch* b = nullptr;
         size_t first = 0, jobs = 0;
         int cursor = 0, chainLo = 0;
         int stage = 0;  // 0 rendering, 1 results in flight, 2 PCM in flight
-        hipEvent_t ready = nullptr, begin = nullptr;
+        hipEvent_t finished = nullptr, ready = nullptr, begin = nullptr;
         clay_tail_result* results = nullptr;  // pinned
         void* slot = nullptr;
         uint64_t samples = 0;
+        std::vector<int> ends;  // frames of each job, ascending
+        // Jobs still rendering: a finished job frees its place for admission
+        // while the rest of its cohort renders on.
+        size_t active() const { return size_t(ends.end() - std::upper_bound(ends.begin(), ends.end(), cursor)); }
     };
+    // A cohort whose records a worker thread is flattening. The views are
+    // copied: the window's queue may grow (refill) and move meanwhile.
+    struct Flattening {
+        size_t first = 0, end = 0;
+        std::vector<clay_plan_view> views;
+        clay_hip_batch* b = nullptr;
+        ArenaLayout layout{};
+        char* image = nullptr;  // packed records, pinned (h->images)
+        std::thread worker;
+        std::atomic<bool> done{false};
+        int status = CLAY_OK;
+        std::string message;
+        void join()
+        {
+            if (worker.joinable())
+                worker.join();
+        }
+    };
+    constexpr size_t kAhead = 4;            // cohorts flattened ahead of admission
+    constexpr size_t kCohortsPerWindow = 8; // resident cohorts when every one is full
+    const size_t cohortJobs = std::max<size_t>(1, size_t(h->cfg.batch_jobs) / kCohortsPerWindow);
+
     std::vector<clay_plan_view> queue(plans, plans + count);
-    std::vector<std::unique_ptr<Unit>> resident, completing;
-    const auto drop = [&](Unit& u) {
+    std::vector<std::unique_ptr<Cohort>> resident, completing;
+    std::deque<std::unique_ptr<Flattening>> ahead;
+    hipStream_t copies = h->copyStream();
+    const auto drop = [&](Cohort& u) {
         if (u.b) {
             release(h, *u.b);
             delete u.b;
@@ -1828,11 +1975,47 @@ int renderWindowPersistent(clay_hip* h, const clay_plan_view* plans, size_t coun
         }
         h->unpin(u.results);
         u.results = nullptr;
-        if (u.ready)
-            h->eventPool.push_back(u.ready);
-        if (u.begin)
-            h->eventPool.push_back(u.begin);
-        u.ready = u.begin = nullptr;
+        for (hipEvent_t* e : {&u.finished, &u.ready, &u.begin})
+            if (*e) {
+                h->eventPool.push_back(*e);
+                *e = nullptr;
+            }
+    };
+    const auto dropFlattening = [&](Flattening& f) {
+        f.join();
+        if (f.image)
+            h->images.discard(f.image);
+        f.image = nullptr;
+        if (f.b) {
+            release(h, *f.b);
+            delete f.b;
+            f.b = nullptr;
+        }
+    };
+    const auto startFlatten = [&](size_t first, size_t end) {
+        auto f = std::make_unique<Flattening>();
+        f->first = first;
+        f->end = end;
+        f->views.assign(queue.begin() + std::ptrdiff_t(first), queue.begin() + std::ptrdiff_t(end));
+        f->b = new clay_hip_batch();
+        Flattening* raw = f.get();
+        raw->worker = std::thread([h, raw] {
+            try {
+                flatten(h, raw->views.data(), raw->views.size(), *raw->b);
+                raw->layout = layoutOf(h, *raw->b);
+                check(hipSetDevice(h->device), "hipSetDevice");
+                raw->image = h->images.take(raw->layout.records);
+                packRecords(*raw->b, raw->layout, raw->image, true);
+            } catch (const Failure& e) {
+                raw->status = e.code;
+                raw->message = e.what();
+            } catch (const std::exception& e) {
+                raw->status = CLAY_E_INVALID;
+                raw->message = e.what();
+            }
+            raw->done = true;
+        });
+        return f;
     };
     const int status = guarded(h, [&] {
         check(hipSetDevice(h->device), "hipSetDevice");
@@ -1840,102 +2023,154 @@ int renderWindowPersistent(clay_hip* h, const clay_plan_view* plans, size_t coun
         tuner.init(h);
         const int cus = std::max(1, h->props.multiProcessorCount);
         const bool useGraph = !(h->cfg.flags & CLAY_FLAG_NO_GRAPH);
-        size_t next = 0;
+        const auto shapeOf = [&](int mixBlocks, bool simpleMix) {
+            Shape fixed;
+            fixed.mixBlocks = mixBlocks;
+            fixed.chainBlocks = cus * 6;
+            fixed.pitchBlocks = cus;
+            fixed.audioBlocks = cus;
+            fixed.simpleMix = simpleMix;
+            return fixed;
+        };
+        // Both round graphs are captured before any worker thread starts: no
+        // HIP call runs on another thread while a capture is open.
+        if (useGraph)
+            for (bool simple : {false, true})
+                roundGraph(h, shapeOf(tuner.pick().second.second, simple));
+        size_t next = 0;  // first queued job not yet handed to a cohort
+        size_t cohorts = 0;
         bool exhausted = !refill;
         std::vector<Window> windows;
-        hipEvent_t roundBegin = h->event(), roundEnd = h->event();
+        // Rounds timed for the tuner, read back once their events complete.
+        struct Measurement {
+            int index;
+            hipEvent_t begin, end;
+            uint64_t frames;
+        };
+        std::deque<Measurement> measurements;
+        const auto collect = [&](bool wait) {
+            while (!measurements.empty()) {
+                Measurement& m = measurements.front();
+                if (wait)
+                    check(hipEventSynchronize(m.end), "round timing");
+                else if (hipEventQuery(m.end) == hipErrorNotReady)
+                    return;
+                float ms = 0;
+                check(hipEventElapsedTime(&ms, m.begin, m.end), "hipEventElapsedTime");
+                tuner.record(m.index, ms, m.frames);
+                h->eventPool.push_back(m.begin);
+                h->eventPool.push_back(m.end);
+                measurements.pop_front();
+            }
+        };
+        // Keeps kAhead cohorts flattening, refilling the queue when it runs
+        // dry (waiting for the caller only when nothing else is in flight).
+        const auto topUp = [&] {
+            while (ahead.size() < kAhead) {
+                if (next == queue.size()) {
+                    if (exhausted)
+                        return;
+                    const clay_plan_view* more = nullptr;
+                    size_t n = 0;
+                    const int wait = resident.empty() && completing.empty() && ahead.empty();
+                    if (!refill(user, wait, &more, &n) || !n) {
+                        if (wait)
+                            exhausted = true;
+                        return;
+                    }
+                    queue.insert(queue.end(), more, more + n);
+                }
+                // The first cohorts grow from one job so that rendering starts
+                // after one job's flatten.
+                const size_t size = std::min(cohortJobs, size_t(1) << std::min<size_t>(cohorts, 20));
+                ++cohorts;
+                size_t end = std::min(queue.size(), next + size);
+                if (h->cfg.group_minimum)
+                    while (end < queue.size() && queue[end].volume_group &&
+                           queue[end].volume_group == queue[end - 1].volume_group)
+                        ++end;
+                ahead.push_back(startFlatten(next, end));
+                next = end;
+            }
+        };
+        const auto residentJobs = [&] {
+            size_t n = 0;
+            for (auto& u : resident)
+                n += u->active();
+            return n;
+        };
         try {
             for (;;) {
-                // 1. Admission: fill free places from the queue, refilling it when empty.
-                // The records of the units admitted together are flattened by
-                // several host threads; their uploads follow in queue order.
-                for (bool admitting = true; admitting && resident.size() < h->cfg.batch_jobs;) {
-                    std::vector<std::pair<size_t, size_t>> pending;  // [first, end) of each unit
-                    size_t scan = next;
-                    while (resident.size() + pending.size() < h->cfg.batch_jobs) {
-                        if (scan == queue.size()) {
-                            if (exhausted || !pending.empty())
-                                break;
-                            const clay_plan_view* more = nullptr;
-                            size_t n = 0;
-                            const int wait = resident.empty() && completing.empty();
-                            if (!refill(user, wait, &more, &n) || !n) {
-                                if (wait)
-                                    exhausted = true;
-                                break;
-                            }
-                            queue.insert(queue.end(), more, more + n);
-                        }
-                        size_t end = scan + 1;
-                        if (h->cfg.group_minimum)
-                            while (end < queue.size() && queue[scan].volume_group &&
-                                   queue[end].volume_group == queue[scan].volume_group)
-                                ++end;
-                        pending.emplace_back(scan, end);
-                        scan = end;
-                    }
-                    if (pending.empty())
+                topUp();
+                // 1. Admission at the round boundary: cohorts whose records are
+                // ready, while their jobs fit; wait for a flatten only when
+                // nothing is resident.
+                while (!ahead.empty()) {
+                    Flattening& f = *ahead.front();
+                    const size_t jobs = f.end - f.first;
+                    if (!resident.empty() && (residentJobs() + jobs > h->cfg.batch_jobs || !f.done))
                         break;
-                    std::vector<std::unique_ptr<Unit>> made(pending.size());
-                    std::vector<int> flattened(pending.size(), CLAY_OK);
-                    std::vector<std::string> message(pending.size());
-                    for (size_t k = 0; k < pending.size(); ++k) {
-                        made[k] = std::make_unique<Unit>();
-                        made[k]->first = pending[k].first;
-                        made[k]->jobs = pending[k].second - pending[k].first;
-                        made[k]->b = new clay_hip_batch();
+                    f.join();
+                    if (f.status != CLAY_OK) {
+                        h->error = f.message;
+                        throw Failure(f.status, f.message);
                     }
-                    parallelFor(pending.size(), [&](size_t k) {
-                        try {
-                            flatten(h, queue.data() + pending[k].first, made[k]->jobs, *made[k]->b, false);
-                        } catch (const Failure& f) {
-                            flattened[k] = f.code;
-                            message[k] = f.what();
-                        } catch (const std::exception& e) {
-                            flattened[k] = CLAY_E_INVALID;
-                            message[k] = e.what();
-                        }
+                    const int st = guarded(h, [&] {
+                        allocateArena(h, *f.b, f.layout, true);
+                        placeRecords(h, *f.b, f.layout, f.image);
+                        h->images.give(f.image);
+                        f.image = nullptr;
                     });
-                    for (size_t k = 0; k < pending.size(); ++k) {
-                        auto& u = made[k];
-                        const size_t end = pending[k].second;
-                        int st = flattened[k];
-                        if (st == CLAY_OK)
-                            st = guarded(h, [&] { upload(h, *u->b, true); });
-                        else
-                            h->error = message[k];
-                        if (st == CLAY_E_MEMORY && (!resident.empty() || !completing.empty())) {
-                            for (size_t r = k; r < pending.size(); ++r)
-                                drop(*made[r]);  // retry once memory is returned
-                            admitting = false;
-                            break;
-                        }
-                        if (st == CLAY_E_MEMORY) {
-                            drop(*u);
-                            clay_render_batch rejected{};
-                            rejected.failed_scratch_bytes = 1;
-                            if (publish(user, &rejected, next))
-                                throw Failure(CLAY_E_CALLBACK, "publish aborted the window");
-                            next = end;
+                    if (st == CLAY_E_MEMORY) {
+                        release(h, *f.b);
+                        if (!resident.empty() || !completing.empty())
+                            break;  // retried once memory is returned
+                        auto failed = std::move(ahead.front());
+                        ahead.pop_front();
+                        dropFlattening(*failed);
+                        if (jobs > 1) {
+                            // Halve the cohort (at a volume-group boundary when one exists).
+                            size_t half = failed->first + jobs / 2;
+                            if (h->cfg.group_minimum)
+                                while (half < failed->end && queue[half].volume_group &&
+                                       queue[half].volume_group == queue[half - 1].volume_group)
+                                    ++half;
+                            if (half == failed->end)
+                                half = failed->first + jobs / 2;
+                            ahead.push_front(startFlatten(half, failed->end));
+                            ahead.push_front(startFlatten(failed->first, half));
                             continue;
                         }
-                        if (st != CLAY_OK) {
-                            for (size_t r = k; r < pending.size(); ++r)
-                                drop(*made[r]);
-                            throw Failure(st, h->error);
-                        }
-                        u->begin = h->event();
-                        check(hipEventRecord(u->begin, h->stream), "hipEventRecord");
-                        ++h->stats.admissions;
-                        next = end;
-                        resident.push_back(std::move(u));
+                        clay_render_batch rejected{};
+                        rejected.failed_scratch_bytes = 1;
+                        if (publish(user, &rejected, failed->first))
+                            throw Failure(CLAY_E_CALLBACK, "publish aborted the window");
+                        continue;
                     }
+                    if (st != CLAY_OK)
+                        throw Failure(st, h->error);
+                    auto u = std::make_unique<Cohort>();
+                    u->b = f.b;
+                    f.b = nullptr;
+                    u->first = f.first;
+                    u->jobs = jobs;
+                    for (const Job& j : u->b->jobs)
+                        u->ends.push_back(j.frames);
+                    std::sort(u->ends.begin(), u->ends.end());
+                    u->begin = h->event();
+                    check(hipEventRecord(u->begin, h->stream), "hipEventRecord");
+                    ++h->stats.admissions;
+                    resident.push_back(std::move(u));
+                    ahead.pop_front();
+                    topUp();
                 }
-                if (resident.empty() && completing.empty() && (exhausted || next == queue.size()))
+                if (resident.empty() && completing.empty() && ahead.empty() && (exhausted || next == queue.size()))
                     break;
-                // 2. One round over every resident unit.
+                // 2. One round over every resident cohort.
                 if (!resident.empty()) {
-                    const auto shape = tuner.pick();
+                    collect(false);
+                    const auto picked = tuner.pick();
+                    const auto shape = picked.second;
                     windows.clear();
                     uint64_t frames = 0;
                     bool simpleMix = true;
@@ -1948,26 +2183,24 @@ int renderWindowPersistent(clay_hip* h, const clay_plan_view* plans, size_t coun
                         frames += uint64_t(end - first) * u->b->jobs.size();
                         u->cursor = end;
                     }
-                    Shape fixed;
-                    fixed.mixBlocks = shape.second;
-                    fixed.chainBlocks = cus * 6;
-                    fixed.pitchBlocks = cus;
-                    fixed.audioBlocks = cus;
-                    fixed.simpleMix = simpleMix;
-                    const bool measure = tuner.tuning && h->stats.rounds > 0;
-                    if (measure)
-                        check(hipEventRecord(roundBegin, h->stream), "hipEventRecord");
+                    const Shape fixed = shapeOf(shape.second, simpleMix);
+                    const bool measure = picked.first >= 0;
+                    Measurement m{picked.first, nullptr, nullptr, frames};
+                    if (measure) {
+                        m.begin = h->event();
+                        check(hipEventRecord(m.begin, h->stream), "hipEventRecord");
+                    }
                     renderRoundFixed(h, windows, fixed, useGraph);
                     if (measure) {
-                        check(hipEventRecord(roundEnd, h->stream), "hipEventRecord");
-                        check(hipEventSynchronize(roundEnd), "round timing");
-                        float ms = 0;
-                        check(hipEventElapsedTime(&ms, roundBegin, roundEnd), "hipEventElapsedTime");
-                        tuner.record(ms, frames);
+                        m.end = h->event();
+                        check(hipEventRecord(m.end, h->stream), "hipEventRecord");
+                        measurements.push_back(m);
+                        tuner.launched(m.index);
                     }
                     h->stats.segment_frames = shape.first;
                     h->stats.mix_blocks = shape.second;
-                    // Units whose last window was rendered: finish and fetch results.
+                    // Cohorts whose last window was rendered: finish them behind
+                    // this round; their results come back on the copy stream.
                     for (size_t i = 0; i < resident.size();) {
                         auto& u = resident[i];
                         if (u->cursor < u->b->lastFrame) {
@@ -1975,11 +2208,14 @@ int renderWindowPersistent(clay_hip* h, const clay_plan_view* plans, size_t coun
                             continue;
                         }
                         finishBatch(h, *u->b);
+                        u->finished = h->event();
+                        check(hipEventRecord(u->finished, h->stream), "hipEventRecord");
+                        check(hipStreamWaitEvent(copies, u->finished, 0), "hipStreamWaitEvent");
                         u->results = static_cast<clay_tail_result*>(h->pinned(u->jobs * sizeof(clay_tail_result)));
                         check(hipMemcpyAsync(u->results, u->b->dResults, u->jobs * sizeof(clay_tail_result),
-                                             hipMemcpyDeviceToHost, h->stream), "results download");
+                                             hipMemcpyDeviceToHost, copies), "results download");
                         u->ready = h->event();
-                        check(hipEventRecord(u->ready, h->stream), "hipEventRecord");
+                        check(hipEventRecord(u->ready, copies), "hipEventRecord");
                         u->stage = 1;
                         completing.push_back(std::move(u));
                         resident.erase(resident.begin() + std::ptrdiff_t(i));
@@ -1989,8 +2225,7 @@ int renderWindowPersistent(clay_hip* h, const clay_plan_view* plans, size_t coun
                 for (size_t i = 0; i < completing.size();) {
                     auto& u = completing[i];
                     hipError_t q = hipEventQuery(u->ready);
-                    if (q == hipErrorNotReady && resident.empty() && i == 0 &&
-                        (exhausted || next == queue.size() || resident.size() >= h->cfg.batch_jobs)) {
+                    if (q == hipErrorNotReady && resident.empty() && i == 0) {
                         check(hipEventSynchronize(u->ready), "completion");
                         q = hipSuccess;
                     }
@@ -2006,12 +2241,11 @@ int renderWindowPersistent(clay_hip* h, const clay_plan_view* plans, size_t coun
                         u->slot = pinnedSlot(h, u->samples * sizeof(int16_t));
                         if (u->samples)
                             check(hipMemcpyAsync(u->slot, u->b->dOutput, u->samples * sizeof(int16_t),
-                                                 hipMemcpyDeviceToHost, h->stream), "pcm download");
+                                                 hipMemcpyDeviceToHost, copies), "pcm download");
                         h->stats.downloaded_bytes += u->samples * sizeof(int16_t);
-                        check(hipEventRecord(u->ready, h->stream), "hipEventRecord");
+                        check(hipEventRecord(u->ready, copies), "hipEventRecord");
                         u->stage = 2;
-                        ++i;
-                        continue;
+                        continue;  // its PCM may already have arrived
                     }
                     float ms = 0;
                     (void)hipEventElapsedTime(&ms, u->begin, u->ready);
@@ -2030,15 +2264,23 @@ int renderWindowPersistent(clay_hip* h, const clay_plan_view* plans, size_t coun
                 }
             }
         } catch (...) {
-            h->eventPool.push_back(roundBegin);
-            h->eventPool.push_back(roundEnd);
+            (void)hipStreamSynchronize(h->stream);
+            for (auto& m : measurements) {
+                h->eventPool.push_back(m.begin);
+                h->eventPool.push_back(m.end);
+            }
             throw;
         }
-        h->eventPool.push_back(roundBegin);
-        h->eventPool.push_back(roundEnd);
+        collect(true);
+        tuner.keep(h);
     });
     if (status != CLAY_OK) {
+        (void)hipStreamSynchronize(copies);
         (void)hipStreamSynchronize(h->stream);
+    }
+    for (auto& f : ahead)
+        dropFlattening(*f);
+    if (status != CLAY_OK) {
         for (auto& u : resident)
             drop(*u);
         for (auto& u : completing) {

@@ -1,13 +1,15 @@
-// CPU model of the copy queue behind clay_staging::Ring and ::Image
+// CPU model of the copy queue behind clay_staging::Image and ::Pool
 // (claysampler_amd/csrc/hip/staging.h). Every copy is queued with a stamp
-// written into its whole region; the model's "device" executes queued copies
+// written into its whole buffer; the model's "device" executes queued copies
 // in order, at random times or when the host waits, and checks that each
-// copy still reads its own stamp. A region rewritten before its copy ran, or
-// a buffer freed under a queued copy, fails the check.
+// copy still reads its own stamp. A buffer rewritten before its copy ran, or
+// freed under a queued copy, fails the check.
 //
-// usage: staging_check ring|image|planted SEED COUNT
-//   planted: a ring that waits only for the oldest overlapping region (a
-//   plausible mistake); the check must catch it.
+// usage: staging_check image|pool|planted SEED COUNT
+//   pool: up to four producers hold buffers at once (threads packing records)
+//   and queue their copies in random order.
+//   planted: a pool that hands a buffer out again as soon as it comes back,
+//   before its copy has run (a plausible mistake); the check must catch it.
 #include "../claysampler_amd/csrc/hip/staging.h"
 
 #include <cstdint>
@@ -15,9 +17,10 @@
 #include <cstdlib>
 #include <cstring>
 #include <deque>
-#include <random>
 #include <map>
+#include <random>
 #include <string>
+#include <vector>
 
 namespace {
 
@@ -59,6 +62,7 @@ struct Model {
             queue.pop_front();
         }
     }
+    bool ran(int fence) const { return queue.empty() || queue.front().fence > fence; }
 };
 
 Model* g_model = nullptr;
@@ -67,6 +71,7 @@ struct ModelFences {
     using Fence = int;
     Fence record() { return g_model->queue.empty() ? g_model->nextFence - 1 : g_model->queue.back().fence; }
     void wait(Fence& f) { g_model->waitFor(f); }
+    bool done(Fence& f) { return g_model->ran(f); }
     char* allocate(size_t bytes)
     {
         char* p = static_cast<char*>(std::malloc(bytes));
@@ -92,77 +97,99 @@ struct ModelFences {
     }
 };
 
-// The mistake the ring once had: after a wrap-around, wait only for the
-// oldest region that overlaps the new one.
-class PlantedRing {
+// The mistake: a returned buffer is free again at once, its copy still queued.
+class PlantedPool {
 public:
     char* take(size_t bytes)
     {
-        if (bytes > capacity_) {
-            for (auto& u : uses_)
-                fences_.wait(u.fence);
-            uses_.clear();
-            if (base_)
-                fences_.free(base_);
-            capacity_ = std::max<size_t>(bytes, 1 << 16);
-            base_ = fences_.allocate(capacity_);
-            head_ = 0;
-        }
-        if (head_ + bytes > capacity_)
-            head_ = 0;
-        const size_t begin = head_, end = head_ + bytes;
-        for (auto it = uses_.begin(); it != uses_.end(); ++it)
-            if (it->begin < end && begin < it->end) {
-                fences_.wait(it->fence);
-                uses_.erase(it);
-                break;
+        for (auto& b : buffers_)
+            if (!b.busy && b.capacity >= bytes) {
+                b.busy = true;
+                return b.host;
             }
-        pending_ = {begin, end, 0};
-        head_ = end;
-        return base_ + begin;
+        buffers_.push_back({fences_.allocate(bytes), bytes, true});
+        return buffers_.back().host;
     }
-    void commit()
+    void give(char* host)
     {
-        pending_.fence = fences_.record();
-        uses_.push_back(pending_);
+        for (auto& b : buffers_)
+            if (b.host == host)
+                b.busy = false;
     }
     void release()
     {
-        for (auto& u : uses_)
-            fences_.wait(u.fence);
-        uses_.clear();
-        if (base_)
-            fences_.free(base_);
-        base_ = nullptr;
+        g_model->waitFor(g_model->nextFence);
+        for (auto& b : buffers_)
+            fences_.free(b.host);
+        buffers_.clear();
     }
 
 private:
-    struct Use {
-        size_t begin, end;
-        int fence;
+    struct Buffer {
+        char* host;
+        size_t capacity;
+        bool busy;
     };
     ModelFences fences_;
-    char* base_ = nullptr;
-    size_t capacity_ = 0, head_ = 0;
-    Use pending_{};
-    std::deque<Use> uses_;
+    std::vector<Buffer> buffers_;
 };
 
-template<class Buffer>
-void run(Buffer& buffer, long count, size_t maxBytes)
+void queueCopy(char* p, size_t bytes, uint8_t stamp)
+{
+    // The copy is queued behind any earlier work; the fence marks it.
+    g_model->queue.push_back({g_model->nextFence++, p, bytes, stamp});
+}
+
+void runImage(long count)
 {
     Model& m = *g_model;
+    clay_staging::Image<ModelFences> image;
     for (long k = 0; k < count; ++k) {
-        const size_t bytes = 1 + m.rng() % maxBytes;
-        char* p = buffer.take(bytes);
+        const size_t bytes = 1 + m.rng() % 100000;
+        char* p = image.take(bytes);
         const uint8_t stamp = uint8_t(1 + k % 251);
         std::memset(p, stamp, bytes);
-        // The copy is queued behind any earlier work; the fence marks it.
-        m.queue.push_back({m.nextFence++, p, bytes, stamp});
-        buffer.commit();
+        queueCopy(p, bytes, stamp);
+        image.commit();
         m.progress();
     }
-    buffer.release();
+    image.release();
+    m.waitFor(m.nextFence);
+}
+
+template<class P>
+void runPool(P& pool, long count)
+{
+    Model& m = *g_model;
+    struct Held {
+        char* p;
+        size_t bytes;
+        uint8_t stamp;
+    };
+    std::vector<Held> held;
+    for (long k = 0; k < count;) {
+        if (held.size() < 4 && (held.empty() || m.rng() % 2)) {
+            // A producer packs a new image (sizes spanning several powers of two).
+            const size_t bytes = 1 + m.rng() % (size_t(3) << 20);
+            const uint8_t stamp = uint8_t(1 + k % 251);
+            char* p = pool.take(bytes);
+            std::memset(p, stamp, bytes);
+            held.push_back({p, bytes, stamp});
+            ++k;
+        } else {
+            // One of the packed images is copied, in random order.
+            const size_t i = m.rng() % held.size();
+            queueCopy(held[i].p, held[i].bytes, held[i].stamp);
+            pool.give(held[i].p);
+            held.erase(held.begin() + std::ptrdiff_t(i));
+        }
+        m.progress();
+    }
+    for (auto& h : held) {
+        queueCopy(h.p, h.bytes, h.stamp);
+        pool.give(h.p);
+    }
+    pool.release();
     m.waitFor(m.nextFence);
 }
 
@@ -177,16 +204,14 @@ int main(int argc, char** argv)
     model.rng.seed(std::strtoull(argv[2], nullptr, 10));
     g_model = &model;
     const long count = std::strtol(argv[3], nullptr, 10);
-    if (kind == "ring") {
-        clay_staging::Ring<ModelFences> ring;
-        ring.setMinimum(1 << 16);
-        run(ring, count, 24000);
-    } else if (kind == "image") {
-        clay_staging::Image<ModelFences> image;
-        run(image, count, 100000);
+    if (kind == "image") {
+        runImage(count);
+    } else if (kind == "pool") {
+        clay_staging::Pool<ModelFences> pool{ModelFences{}, size_t(24) << 20};
+        runPool(pool, count);
     } else if (kind == "planted") {
-        PlantedRing ring;
-        run(ring, count, 24000);
+        PlantedPool pool;
+        runPool(pool, count);
     } else {
         return 2;
     }
