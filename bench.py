@@ -341,8 +341,9 @@ def predictor_main(args, taps=32, db=-90.0):
         x, rel, end, ok = tails.chain_records(plans, s.bank_array())
         xs.append(x[ok])
         ys.append(tails.to_target(exact[ok], rel[ok]))
+        validation = (x[ok], rel[ok], end[ok], exact[ok])  # the last batch sets the scratch margin
         log(f"tail batch {k + 1}/{args.tail_batches}: {ok.sum()} voices, {time.perf_counter() - t0:.1f} s")
-    x, y = np.concatenate(xs), np.concatenate(ys)
+    x, y = np.concatenate(xs[:-1]), np.concatenate(ys[:-1])
     renderer = ca.Renderer(policy=abi.POLICY_NATURAL, taps=taps, voice_silence_db=db)
     model = tails.init(x, y, seed=1)
     t0 = time.perf_counter()
@@ -366,6 +367,44 @@ def predictor_main(args, taps=32, db=-90.0):
     for name, p in (("network", pred), ("envelope", base)):
         scratch[name], margin = tails.scratch_for_coverage(p, exact_ok, rel, end)
         scratch[name + "_margin"] = margin
+    # Scratch sized from the predictions on the held-out batch. A job falls
+    # back when any of its ~500 chains overflows, so the margin holds 99.99 %
+    # of the validation batch's chain tails; jobs that overflow are rendered
+    # again unbounded, and every job must equal the unbounded render.
+    xv, relv, endv, exactv = validation
+    _, margin = tails.scratch_for_coverage(tails.from_target(tails.predict(renderer, model, xv), relv, endv),
+                                           exactv, relv, endv, coverage=0.9999)
+    _, margin_env = tails.scratch_for_coverage(tails.envelope_baseline(xv, relv, endv, db), exactv, relv, endv,
+                                               coverage=0.9999)
+    xa, rela, enda, oka = tails.chain_records(plans, held.bank_array())
+    pred_all = tails.from_target(tails.predict(renderer, model, xa), rela, enda)
+    env_all = tails.envelope_baseline(xa, rela, enda, db)
+    realised = {}
+    r = ca.Renderer(policy=abi.POLICY_NATURAL, taps=taps, voice_silence_db=db)
+    r.upload_bank(*bank)
+    b = r.prepare(plans)
+    b.run()
+    want_res, want_pcm = b.results()
+    want = [(q.status, q.frames, q.data_frames, want_pcm[q.offset:q.offset + q.data_frames].tobytes()) for q in want_res]
+    full_filtered, full_arena = b.scratch_bytes()
+    b.close()
+    # The same path (prepare, run, results) with every chain's bound at its end, for the time.
+    tails.render_bounded(r, plans, enda.astype(np.int32))
+    _, info = tails.render_bounded(r, plans, enda.astype(np.int32))
+    realised["unbounded"] = {"filtered_bytes": full_filtered, "arena_bytes": full_arena,
+                             "seconds": info["bounded_s"]}
+    for name, p, m in (("network", pred_all, margin), ("envelope", env_all, margin_env)):
+        bounds = tails.scratch_bounds(p, rela, enda, oka, m)
+        tails.render_bounded(r, plans, bounds)  # warm
+        jobs, info = tails.render_bounded(r, plans, bounds)
+        got = [(q.status, q.frames, q.data_frames, pcm.tobytes()) for q, pcm in jobs]
+        realised[name] = {"margin": m, "filtered_bytes": info["filtered_bytes"], "arena_bytes": info["arena_bytes"],
+                          "overflowed_jobs": len(info["overflowed_jobs"]),
+                          "seconds": info["bounded_s"] + info["fallback_s"], "fallback_s": info["fallback_s"],
+                          "identical_pcm16": got == want}
+        log(f"bounded scratch ({name}): {info['filtered_bytes'] / 1e9:.3f} GB vs {full_filtered / 1e9:.3f} GB, "
+            f"{len(info['overflowed_jobs'])} jobs rendered again, identical {got == want}")
+    r.close()
     renderer.close()
     # Throughput on the held-out batch, policy off and on (plain form, resident batch).
     rates = {}
@@ -411,7 +450,8 @@ def predictor_main(args, taps=32, db=-90.0):
                    "training_tails": int(len(y)), "epochs": args.epochs, "held_out_voices": int(len(truth))},
         "predictor": {"train_seconds": train_s, "loss_first_epoch": float(losses[0]),
                       "loss_last_epoch": float(losses[-1]), "median_abs_frame_error": frames_err,
-                      "mean_abs_log_error": log_err, "post_release_scratch_bytes_99pct": scratch},
+                      "mean_abs_log_error": log_err, "post_release_scratch_bytes_99pct": scratch,
+                      "realised_scratch": realised},
         "throughput_audio_s_per_s": rates,
         "identical_pcm16_plain_vs_persistent": sums[0] == sums[1],
     }

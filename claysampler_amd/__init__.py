@@ -64,6 +64,7 @@ HIP_SYMBOLS = [
     "clay_hip_batch_prepare", "clay_hip_batch_run", "clay_hip_batch_results", "clay_hip_batch_kernel_time",
     "clay_hip_batch_destroy", "clay_hip_batch_voice_tails", "clay_mlp_init", "clay_hip_mlp_gradient",
     "clay_hip_mlp_train", "clay_hip_mlp_predict", "clay_hip_mlp_gradient_noalloc",
+    "clay_hip_batch_prepare_bounded", "clay_hip_batch_overflows", "clay_hip_batch_scratch_bytes",
 ]
 
 
@@ -99,6 +100,11 @@ def hip():
         _bind(lib, "clay_hip_batch_kernel_time", C.c_int, P, P, C.c_char_p, C.POINTER(C.c_double),
               C.POINTER(C.c_int))
         _bind(lib, "clay_hip_batch_destroy", C.c_int, P, P)
+        _bind(lib, "clay_hip_batch_prepare_bounded", C.c_int, P, C.POINTER(abi.PlanView), C.c_size_t,
+              C.POINTER(C.c_int32), C.c_size_t, C.POINTER(P))
+        _bind(lib, "clay_hip_batch_overflows", C.c_int, P, P, C.POINTER(C.c_int32), C.c_size_t,
+              C.POINTER(C.c_size_t))
+        _bind(lib, "clay_hip_batch_scratch_bytes", C.c_int, P, P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64))
         _hip = lib
     return _hip
 
@@ -182,9 +188,16 @@ class Renderer:
     def upload_bank(self, ptr, count, fresh=False):
         self._check(self.lib.clay_hip_upload_bank(self.handle, ptr, count, int(fresh)))
 
-    def prepare(self, plans):
+    def prepare(self, plans, scratch_end=None):
+        """A resident batch; `scratch_end` (one absolute frame per voice chain)
+        bounds each chain's filtered-voice scratch (voice-silence policy)."""
         b = C.c_void_p()
-        self._check(self.lib.clay_hip_batch_prepare(self.handle, plans, len(plans), C.byref(b)))
+        if scratch_end is None:
+            self._check(self.lib.clay_hip_batch_prepare(self.handle, plans, len(plans), C.byref(b)))
+        else:
+            ends = np.ascontiguousarray(scratch_end, np.int32)
+            self._check(self.lib.clay_hip_batch_prepare_bounded(
+                self.handle, plans, len(plans), ends.ctypes.data_as(C.POINTER(C.c_int32)), ends.size, C.byref(b)))
         return Batch(self, b, len(plans))
 
     def stats(self):
@@ -275,6 +288,22 @@ class Batch:
                                                             out.ctypes.data_as(C.POINTER(C.c_int32)), out.size,
                                                             C.byref(n)))
         return out[:n.value]
+
+    def overflows(self):
+        """Jobs whose voice chains reached their scratch bound in the last run."""
+        n = C.c_size_t()
+        self.r._check(self.r.lib.clay_hip_batch_overflows(self.r.handle, self.handle, None, 0, C.byref(n)))
+        out = np.zeros(max(1, n.value), np.int32)
+        self.r._check(self.r.lib.clay_hip_batch_overflows(self.r.handle, self.handle,
+                                                          out.ctypes.data_as(C.POINTER(C.c_int32)), out.size,
+                                                          C.byref(n)))
+        return out[:n.value]
+
+    def scratch_bytes(self):
+        """(filtered-voice scratch bytes, whole device arena bytes) of the batch."""
+        f, a = C.c_uint64(), C.c_uint64()
+        self.r._check(self.r.lib.clay_hip_batch_scratch_bytes(self.r.handle, self.handle, C.byref(f), C.byref(a)))
+        return f.value, a.value
 
     def kernel_time(self, name):
         ms, n = C.c_double(), C.c_int()

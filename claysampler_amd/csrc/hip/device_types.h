@@ -43,7 +43,10 @@ struct Span {
     // generator, variable pitch or stereo matrix): the mix renders it with the
     // integer FIR of the packed int16 weight table.
     int32_t plain;
-    int32_t pad;
+    // Filtered voices: frames from here on have no scratch (a bound from the
+    // predicted tail of the chain; INT32_MAX = the whole span). A chain that
+    // reaches it without retiring is stopped there and flagged (overflow).
+    int32_t scratchEnd;
 };
 
 // Cold span record: filter slots and the filter/pitch envelopes.
@@ -90,7 +93,7 @@ struct VoiceState {
     double position, step;
     int32_t pitchSpan, previousLoopEnd, previousLoopStart;
     int32_t pitchInitialized, filterInitialized;
-    int32_t pad;
+    int32_t overflow;  // 1: the chain reached its scratch bound before it retired
     double z1[CLAY_FILTER_SLOTS][3][2], z2[CLAY_FILTER_SLOTS][3][2];
     double coefficients[CLAY_FILTER_SLOTS][5];
     double target[CLAY_FILTER_SLOTS][5];

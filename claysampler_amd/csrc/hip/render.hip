@@ -219,7 +219,9 @@ __device__ __forceinline__ void chainAudio(const int16_t* __restrict__ bank, con
         if (v.end <= segFirst || v.begin >= segEnd)
             continue;
         const Job& j = jobs[v.job];
-        const int first = max(v.begin, segFirst), stop = min(v.end, segEnd);
+        const int first = max(v.begin, segFirst), stop = min(min(v.end, segEnd), v.scratchEnd);
+        if (stop <= first)
+            continue;  // past the chain's scratch bound
         // Whether any modulation reaches the voice audio (filter targets do
         // not), so that frames skip the scan of the span's modulations.
         bool audioMods = false;
@@ -350,6 +352,7 @@ __device__ __forceinline__ void voiceChain(const Span* __restrict__ spans, const
     SilenceTrack track{st.retired, st.silence, st.silentChunks, st.openChunk, st.openPeak, st.openEligible};
     if (track.retired)
         return;  // retired in an earlier segment: nothing more to render
+    int overflow = st.overflow;
     int sectionCount[CLAY_FILTER_SLOTS], total = 0;
     for (int f = 0; f < CLAY_FILTER_SLOTS; ++f) {
         sectionCount[f] = sections(cold[head].filters[f].type);
@@ -385,7 +388,15 @@ __device__ __forceinline__ void voiceChain(const Span* __restrict__ spans, const
         if (v.end <= segFirst || v.begin >= segEnd)
             continue;
         const SpanCold& cv = cold[s];
-        const int first = max(v.begin, segFirst), stop = min(v.end, segEnd);
+        const int first = max(v.begin, segFirst), want = min(v.end, segEnd), stop = min(want, v.scratchEnd);
+        if (stop <= first) {
+            // The chain needs frames past its scratch bound: it is stopped at
+            // the bound and flagged (its job is rendered again without one).
+            track.retired = 1;
+            track.silence = first;
+            overflow = 1;
+            break;
+        }
         constexpr bool stereo = Stereo;
         const int rate = cv.sampleRate;
         const double amount = 1 - exp(-1000.0 / rate), keep = 1 - amount;
@@ -871,12 +882,18 @@ __device__ __forceinline__ void voiceChain(const Span* __restrict__ spans, const
                 silencePass(track, silenceLevel, f0, lo, hi, act, outL, outR, base, v.releaseStart))
                 break;
         }
+        if (!track.retired && stop < want) {
+            track.retired = 1;  // reached the scratch bound without retiring
+            track.silence = stop;
+            overflow = 1;
+        }
         if (track.retired)
             break;
     }
     // Save the chain state for the next segment.
     __builtin_amdgcn_wave_barrier();
     if (lane == 0) {
+        st.overflow = overflow;
         st.retired = track.retired;
         st.silence = track.silence;
         st.silentChunks = track.silentChunks;

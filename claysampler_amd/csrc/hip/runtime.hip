@@ -234,6 +234,9 @@ struct clay_hip_batch {
     std::vector<int2> groups;
     uint64_t frames = 0, filteredSamples = 0, pitchFrames = 0;
     int blocks = 0, windows = 0, bodyBlocks = 0, chains = 0;
+    // Optional per-chain scratch bounds (absolute frames, batch chain order),
+    // read by flatten; empty = every filtered span backed in full.
+    std::vector<int32_t> scratchEnd;
     bool assess = false;
     bool allWet = false;  // every span renders through a voice chain (no dry voice in the mix)
     bool simpleMix = false;  // every dry voice is plain with no loop and no bend (the mix's Simple variant)
@@ -509,6 +512,21 @@ void flatten(clay_hip* h, const clay_plan_view* plans, size_t count, clay_hip_ba
         std::string error;
     };
     std::vector<Sizes> sizes(count);
+    // Chain ids of each plan start after the earlier plans' chains.
+    std::vector<int> chainBase(count, 0);
+    for (size_t i = 1; i < count; ++i)
+        chainBase[i] = chainBase[i - 1] + std::max(0, plans[i - 1].filter_tracks);
+    const bool bounded = !b.scratchEnd.empty();
+    if (bounded && (count == 0 ? 0 : size_t(chainBase[count - 1] + std::max(0, plans[count - 1].filter_tracks))) !=
+                       b.scratchEnd.size())
+        throw Failure(CLAY_E_INVALID, "scratch bounds do not match the voice chains of the batch");
+    // Frames of a filtered span backed by scratch: up to its chain's bound.
+    const auto scratchFrames = [&](size_t i, const clay_voice_span& v) {
+        if (!bounded)
+            return int64_t(v.end - v.begin);
+        const int32_t bound = b.scratchEnd[size_t(chainBase[i] + v.filter_track)];
+        return std::max<int64_t>(0, int64_t(std::min(v.end, bound)) - v.begin);
+    };
     forEachPlan([&](size_t i) {
         const auto& p = plans[i];
         Sizes& z = sizes[i];
@@ -522,7 +540,7 @@ void flatten(clay_hip* h, const clay_plan_view* plans, size_t count, clay_hip_ba
             const auto& v = p.spans[s];
             z.voices += (v.end - 1) / kTileFrames - v.begin / kTileFrames + 1;
             if (v.filter_track >= 0) {
-                z.filtered += uint64_t(v.end - v.begin) * uint64_t(v.channels);
+                z.filtered += uint64_t(scratchFrames(i, v)) * uint64_t(v.channels);
                 if (v.variable_pitch)
                     z.pitch += uint64_t(v.end - v.begin);
             }
@@ -674,6 +692,7 @@ void flatten(clay_hip* h, const clay_plan_view* plans, size_t count, clay_hip_ba
                 d.filterTrack = v.filter_track >= 0 ? o.chain + v.filter_track : -1;
                 d.job = jobIndex;
                 d.next = -1;
+                d.scratchEnd = INT32_MAX;
                 d.attackStart = v.attack_start;
                 d.holdStart = v.hold_start;
                 d.decayStart = v.decay_start;
@@ -731,7 +750,9 @@ void flatten(clay_hip* h, const clay_plan_view* plans, size_t count, clay_hip_ba
                     info.end = std::max(info.end, v.end);
                     tail = index;
                     d.filtered = filtered;
-                    filtered += uint64_t(v.end - v.begin) * uint64_t(v.channels);
+                    filtered += uint64_t(scratchFrames(i, v)) * uint64_t(v.channels);
+                    if (bounded)
+                        d.scratchEnd = std::min(v.end, b.scratchEnd[size_t(o.chain + v.filter_track)]);
                     if (v.variable_pitch) {
                         d.phase = pitch;
                         pitch += uint64_t(v.end - v.begin);
@@ -1678,6 +1699,69 @@ int clay_hip_batch_voice_tails(clay_hip* h, clay_hip_batch* b, int32_t* silence,
         for (size_t i = 0; i < n; ++i)
             silence[i] = states[i].retired ? states[i].silence : end[i];
     });
+}
+
+int clay_hip_batch_prepare_bounded(clay_hip* h, const clay_plan_view* plans, size_t count, const int32_t* scratch_end,
+                                   size_t chains, clay_hip_batch** out)
+{
+    if (!h || !out || (count && !plans) || (chains && !scratch_end))
+        return CLAY_E_INVALID;
+    auto b = std::make_unique<clay_hip_batch>();
+    const int status = guarded(h, [&] {
+        if (!count)
+            throw Failure(CLAY_E_INVALID, "empty batch");
+        if (chains && h->silenceLevel <= 0.f)
+            throw Failure(CLAY_E_INVALID, "scratch bounds need the voice-silence policy");
+        check(hipSetDevice(h->device), "hipSetDevice");
+        b->scratchEnd.assign(scratch_end, scratch_end + chains);
+        flatten(h, plans, count, *b);
+        upload(h, *b);
+    });
+    if (status != CLAY_OK) {
+        release(h, *b);
+        return status;
+    }
+    *out = b.release();
+    return CLAY_OK;
+}
+
+int clay_hip_batch_overflows(clay_hip* h, clay_hip_batch* b, int32_t* jobs, size_t capacity, size_t* count)
+{
+    if (!h || !b)
+        return CLAY_E_INVALID;
+    return guarded(h, [&] {
+        check(hipSetDevice(h->device), "hipSetDevice");
+        const size_t n = size_t(b->chains);
+        std::vector<VoiceState> states(n);
+        if (n)
+            check(hipMemcpy(states.data(), b->dStates, n * sizeof(VoiceState), hipMemcpyDeviceToHost), "overflows");
+        std::vector<char> flagged(b->jobs.size(), 0);
+        for (const auto& sp : b->spans)
+            if (sp.filterTrack >= 0 && states[size_t(sp.filterTrack)].overflow)
+                flagged[size_t(sp.job)] = 1;
+        size_t found = 0;
+        for (size_t j = 0; j < flagged.size(); ++j)
+            if (flagged[j]) {
+                if (jobs && found < capacity)
+                    jobs[found] = int32_t(j);
+                ++found;
+            }
+        if (count)
+            *count = found;
+        if (jobs && found > capacity)
+            throw Failure(CLAY_E_INVALID, "overflow capacity too small");
+    });
+}
+
+int clay_hip_batch_scratch_bytes(clay_hip* h, clay_hip_batch* b, uint64_t* filtered, uint64_t* arena)
+{
+    if (!h || !b)
+        return CLAY_E_INVALID;
+    if (filtered)
+        *filtered = b->filteredSamples * sizeof(float);
+    if (arena)
+        *arena = b->arenaBytes;
+    return CLAY_OK;
 }
 
 int clay_hip_batch_destroy(clay_hip* h, clay_hip_batch* b)

@@ -203,3 +203,51 @@ def scratch_for_coverage(pred, exact, release, end, channels=1, coverage=0.99):
     margin = max(0.0, float(ratio[min(len(ratio) - 1, int(np.ceil(coverage * len(ratio))) - 1)]) - 1.0)
     alloc = np.minimum(end - release, np.ceil(have * (1 + margin)).astype(np.int64))
     return float(4 * channels * alloc.sum()), margin
+
+
+def scratch_bounds(pred, release, end, usable, margin, chunk=64):
+    """Per-chain scratch bounds (absolute frames, batch chain order) for
+    clay_hip_batch_prepare_bounded: a usable chain is backed up to its
+    predicted tail plus `margin` (a fraction of the predicted post-release
+    length), rounded up to whole 64-frame silence chunks plus one; any other
+    chain, or a bound past the chain's end, keeps its whole span. `pred`,
+    `release` and `end` are per chain (all chains, as chain_records gives)."""
+    pred = np.asarray(pred, np.int64)
+    have = np.maximum(pred - release, 1).astype(np.float64)
+    bound = release + np.ceil(have * (1.0 + margin)).astype(np.int64)
+    bound = (bound + chunk - 1) // chunk * chunk + chunk
+    bound = np.where(usable, np.minimum(bound, end), end)
+    return np.minimum(bound, np.iinfo(np.int32).max).astype(np.int32)
+
+
+def render_bounded(renderer, plans, bounds):
+    """Renders `plans` as one resident batch with per-chain scratch bounds,
+    then renders the jobs that overflowed their bounds again without bounds.
+    Returns (per-job (result, pcm) list, info) where info holds the bounded
+    batch's filtered scratch bytes, the overflowed jobs and the seconds of
+    both passes."""
+    import time
+
+    def per_job(results, pcm, channels):
+        return [(r, pcm[r.offset * channels:(r.offset + r.data_frames) * channels].copy()) for r in results]
+
+    t0 = time.perf_counter()
+    b = renderer.prepare(plans, bounds)
+    b.run()
+    results, pcm = b.results()
+    overflowed = [int(j) for j in b.overflows()]
+    filtered, arena = b.scratch_bytes()
+    b.close()
+    jobs = per_job(results, pcm, renderer.channels)
+    t1 = time.perf_counter()
+    if overflowed:
+        again = (abi.PlanView * len(overflowed))(*[plans[j] for j in overflowed])
+        b = renderer.prepare(again)
+        b.run()
+        redo = per_job(*b.results(), renderer.channels)
+        b.close()
+        for k, j in enumerate(overflowed):
+            jobs[j] = redo[k]
+    t2 = time.perf_counter()
+    return jobs, {"filtered_bytes": filtered, "arena_bytes": arena, "overflowed_jobs": overflowed,
+                  "bounded_s": t1 - t0, "fallback_s": t2 - t1}

@@ -355,6 +355,22 @@ int clay_hip_batch_destroy(clay_hip* h, clay_hip_batch* b);
  * training labels and the judge of the learned tail predictor. */
 int clay_hip_batch_voice_tails(clay_hip* h, clay_hip_batch* b, int32_t* silence, size_t capacity, size_t* count);
 
+/* clay_hip_batch_prepare with each voice chain's filtered-voice scratch
+ * bounded: chain c (batch order, as in clay_hip_batch_voice_tails) is backed
+ * for frames before scratch_end[c] only (chains = the batch's chain count).
+ * Needs the voice-silence policy. A chain that has not retired by its bound
+ * is stopped there, and its job is reported by clay_hip_batch_overflows: that
+ * job's output is not valid and is rendered again with no bound. Jobs with no
+ * overflow are bit-identical to an unbounded render. No reference
+ * counterpart: scratch sized from the learned tail predictor. */
+int clay_hip_batch_prepare_bounded(clay_hip* h, const clay_plan_view* plans, size_t count, const int32_t* scratch_end,
+                                   size_t chains, clay_hip_batch** out);
+/* Jobs of the batch (indices, ascending) whose chains reached their scratch
+ * bound in the last run; `count` receives how many. */
+int clay_hip_batch_overflows(clay_hip* h, clay_hip_batch* b, int32_t* jobs, size_t capacity, size_t* count);
+/* Bytes of the batch's filtered-voice scratch and of its whole device arena. */
+int clay_hip_batch_scratch_bytes(clay_hip* h, clay_hip_batch* b, uint64_t* filtered, uint64_t* arena);
+
 #ifdef __cplusplus
 }
 #endif

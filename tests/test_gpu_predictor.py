@@ -88,3 +88,63 @@ def test_network_beats_the_envelope_estimate(renderer, synth_root):
     net_bytes, _ = tails.scratch_for_coverage(pred, exactt, relt, endt)
     base_bytes, _ = tails.scratch_for_coverage(base, exactt, relt, endt)
     assert net_bytes < base_bytes and net_bytes < planned, (net_bytes, base_bytes, planned)
+
+
+def per_job(results, pcm):
+    return [(r.status, r.frames, r.data_frames, pcm[r.offset:r.offset + r.data_frames].tobytes()) for r in results]
+
+
+def test_scratch_bounds_from_tails_are_exact_or_fall_back(synth_root):
+    """Filtered-voice scratch bounded per chain (clay_hip_batch_prepare_bounded):
+    at the exact tails no job overflows, less scratch is allocated and every
+    job is bit-identical to the unbounded render; with bounds far too tight the
+    overflowing jobs are reported, rendered again unbounded, and the result is
+    again bit-identical."""
+    sfz = synth.c2_instrument(f"{synth_root}/c2")
+    s = ca.Session()
+    for seed in range(4000, 4006):
+        s.add_job(sfz, synth.c2_midi(f"{synth_root}/c2/bounds_{seed}.mid", seed, seconds=8.0))
+    plans = s.plans()
+    r = ca.Renderer(policy=abi.POLICY_NATURAL, taps=32, voice_silence_db=DB)
+    r.upload_bank(*s.bank())
+    b = r.prepare(plans)
+    b.run()
+    want = per_job(*b.results())
+    exact = b.voice_tails().astype(np.int64)
+    full, _ = b.scratch_bytes()
+    assert len(b.overflows()) == 0
+    b.close()
+    x, rel, end, ok = tails.chain_records(plans, s.bank_array())
+    assert ok.sum() > 50 and len(exact) == len(rel)
+
+    bounds = tails.scratch_bounds(exact, rel, end, ok, 0.0)
+    assert (bounds >= np.minimum(exact, end)).all()
+    jobs, info = tails.render_bounded(r, plans, bounds)
+    assert info["overflowed_jobs"] == []
+    assert info["filtered_bytes"] < full
+    assert [(x.status, x.frames, x.data_frames, p.tobytes()) for x, p in jobs] == want
+
+    tight = np.where(ok, np.minimum(rel + 64, end), end).astype(np.int32)
+    jobs, info = tails.render_bounded(r, plans, tight)
+    assert len(info["overflowed_jobs"]) > 0
+    assert info["filtered_bytes"] < full
+    assert [(x.status, x.frames, x.data_frames, p.tobytes()) for x, p in jobs] == want
+    r.close()
+
+
+def test_scratch_bounds_need_the_voice_silence_policy(synth_root):
+    sfz = synth.c2_instrument(f"{synth_root}/c2")
+    s = ca.Session()
+    s.add_job(sfz, synth.c2_midi(f"{synth_root}/c2/bounds_4000.mid", 4000, seconds=8.0))
+    plans = s.plans()
+    r = ca.Renderer(policy=abi.POLICY_NATURAL, taps=32)
+    r.upload_bank(*s.bank())
+    chains = sum(p.filter_tracks for p in plans)
+    with pytest.raises(ca.ClayError, match="voice-silence"):
+        r.prepare(plans, np.full(chains, 1 << 20, np.int32))
+    r2 = ca.Renderer(policy=abi.POLICY_NATURAL, taps=32, voice_silence_db=DB)
+    r2.upload_bank(*s.bank())
+    with pytest.raises(ca.ClayError, match="do not match"):
+        r2.prepare(plans, np.full(chains + 1, 1 << 20, np.int32))
+    r.close()
+    r2.close()
