@@ -1,15 +1,23 @@
-// CPU planner. It walks the frame-ordered MIDI events, keeps controller,
-// pedal, keyswitch and voice state, and describes every sounding voice as one
-// or more spans with fixed playback parameters plus piecewise-linear control
-// tracks. Semantics (and therefore every span the renderer sees) follow
-// src/render/plan.cpp of the reference; random draws use the same
-// std::mt19937 streams and seeds so region selection is identical.
+// Job planner: one MIDI performance through one instrument, turned into the
+// voice spans and control tracks the render core consumes.
+//
+// The planner is an event loop over the job's MIDI (frames are already
+// integers). Between events it keeps three kinds of state:
+//   - the channel: controller values, held keys, pedals, the bend in force;
+//   - the voices sounding, each with the span it is extending;
+//   - the plan being written: finished spans, control tracks, bindings.
+// A span is cut wherever its parameters change mid-voice: at a bend change
+// (the pitch ramp restarts) and where a sustain loop is left at release.
+// Control tracks are piecewise-linear point lists with an optional smoothing
+// filter, emitted in the form and with the timing rules of the reference
+// (src/render/plan.cpp:114-216); voice parameters follow its trigger rules
+// (:693-830) and release rules (:832-870). The features the instrument loader
+// accepts are the ones planned here (see instrument.h).
 #include "plan.h"
 
 #include <algorithm>
-#include <array>
+#include <climits>
 #include <cmath>
-#include <limits>
 #include <map>
 #include <random>
 #include <stdexcept>
@@ -18,1059 +26,657 @@
 namespace clay {
 namespace {
 
-constexpr int kForever = std::numeric_limits<int>::max();
-constexpr float kSilence = 1e-4f;      // release level that starts the final 50 ms fade
-constexpr int kBendBlock = 1024;       // sfizz_render block used for bend/controller ramps
-constexpr float kMasterGainDb = -7.35f;
+constexpr int kNoEnd = INT_MAX;
+constexpr float kSilentLevel = 1e-4f;  // a release below this level has reached silence
+constexpr int kBlock = 1024;           // sfizz_render's processing block: events ramp within it
+constexpr float kMasterDb = -7.35f;    // output level of the orig/custom policies (sfizz's default volume)
 constexpr float kQuarterPi = .7853981633974483f;
+constexpr float kPanCompensation = 1.4125375446227544f;  // +3 dB of the stereo matrix
+constexpr float kStereoGain = .99881488f;                // sfizz's stereo position gain at centre
 
-enum class VoiceKind { attack, release, controller };
+// ------------------------------------------------------------ control tracks
 
-struct Voice {
-    clay_voice_span span;
-    int firstLfo = 0, lfoCount = 0;
-    int key = 0, velocity = 0, triggerFrame = 0;
-    const Region* region = nullptr;
-    double baseStep = 0;
-    float releaseSeconds = 0;
-    bool noteOff = false;
-    VoiceKind kind = VoiceKind::attack;
-    bool choked = false, sostenuto = false;
-    std::vector<int> flexShapes;
+// A controller change as sfizz_render applies it: a linear ramp from the
+// start of the event's 1024-frame block (not before the previous event) to
+// the event itself.
+struct Ramp {
+    int first, last;
+    float from, to;
 };
 
-struct DelayedRelease {
-    int key, velocity, duration;
-    const Region* region;
-    bool sostenuto;
-};
-
-struct Bend {
-    int begin = 0, end = 0, from = 0, to = 0;
-};
-
-struct LfoInput {
-    int track;  // -1: constant `depth`
-    ModTarget target;
-    float depth;
-};
-
-struct LfoTrack {
-    const Lfo* source;
-    int begin, end, delay, fade;
-    std::vector<LfoInput> inputs;
-    std::vector<std::pair<ModTarget, int>> outputs;
-};
-
-using Track = std::vector<clay_control_point>;
-
-float trackAt(const Track& track, int frame)
+// Source 128 is the pitch bend; CC 121 resets a controller to its initial value.
+template<class Map>
+std::pair<float, std::vector<Ramp>> controllerRamps(const Midi& midi, int source, float initial, Map map, bool settle)
 {
-    auto it = std::upper_bound(track.begin(), track.end(), frame,
-                               [](int f, const clay_control_point& p) { return f < p.frame; });
-    const auto& p = it == track.begin() ? track.front() : *std::prev(it);
-    return p.value + float(frame - p.frame) * p.step;
+    float value = map(initial), start = value;
+    std::vector<Ramp> ramps;
+    int previousFrame = 0;
+    for (const auto& e : midi.events) {
+        float raw;
+        if (e.kind == EventKind::control && e.a == 121)
+            raw = initial;
+        else if (source == 128 && e.kind == EventKind::bend)
+            raw = float(e.a);
+        else if (e.kind == EventKind::control && e.a == source)
+            raw = float(e.b);
+        else
+            continue;
+        const float next = map(raw);
+        if (settle && e.frame == 0)
+            start = next;  // the natural policy's leading defaults apply from frame 0
+        else if (next != value)
+            ramps.push_back({std::max(previousFrame, e.frame / kBlock * kBlock), e.frame, value, next});
+        value = next;
+        previousFrame = e.frame;
+    }
+    return {start, ramps};
 }
 
-float lfoWave(int wave, double phase)
+// Appends a point unless the track already passes through it with that slope.
+void appendPoint(std::vector<clay_control_point>& track, int frame, float value, float step)
 {
-    const float p = float(phase - std::floor(phase));
-    switch (wave) {
-    case 0: return p < .25f ? 4 * p : p > .75f ? 4 * p - 4 : 2 - 4 * p;
-    case 1: {
-        const float x = 2 * p - 1;
-        return -4 * x * (1 - std::abs(x));
-    }
-    case 2: return p < .75f;
-    case 3: return p < .5f;
-    case 4: return p < .25f;
-    case 5: return p < .125f;
-    case 6: return 2 * p - 1;
-    case 7: return 1 - 2 * p;
-    default: return 0;
-    }
+    clay_control_point& last = track.back();
+    if (last.frame == frame)
+        last = {frame, value, step};
+    else if (last.step != step || std::abs(last.value + (frame - last.frame) * step - value) > 1e-6f)
+        track.push_back({frame, value, step});
 }
 
-// amp_veltrack with the default (v/127)^2 or amp_velcurve_N points (plan.cpp:80-97).
-float velocityGain(const Region& r, int velocity)
+// The track of a controller's ramps; with smoothing (sfizz's one-pole
+// controller smoother, `smooth` frames) the target moves in 16-frame groups
+// and the value follows it linearly.
+std::vector<clay_control_point> trackOf(float start, const std::vector<Ramp>& ramps, int smooth)
 {
-    const float normalized = float(velocity) / 127;
-    float gain = normalized * normalized;
-    if (!r.velocityCurve.empty()) {
-        const auto hi = std::lower_bound(r.velocityCurve.begin(), r.velocityCurve.end(), velocity,
-                                         [](const auto& p, int v) { return p.first < v; });
-        if (hi->first == velocity) {
-            gain = hi->second;
+    std::vector<clay_control_point> track{{0, start, 0}};
+    if (smooth < 2) {
+        for (const auto& r : ramps) {
+            if (r.first < r.last)
+                appendPoint(track, r.first, r.from, (r.to - r.from) / (r.last - r.first));
+            appendPoint(track, r.last, r.to, 0);
+        }
+        return track;
+    }
+    constexpr int kGroup = 16;
+    const int span = std::max(kGroup, smooth);
+    float current = start, target = start, slope = 0;
+    size_t next = 0;
+    for (int frame = 0; next < ramps.size() || current != target; frame += kGroup) {
+        if (current == target && next < ramps.size() && frame < ramps[next].first)
+            frame = std::max(frame, ramps[next].first / kGroup * kGroup);  // idle until the next ramp
+        // Where the controller stands at the end of this group.
+        const int groupEnd = frame + kGroup - 1;
+        float aim = target;
+        while (next < ramps.size() && groupEnd >= ramps[next].last)
+            aim = ramps[next++].to;
+        if (next < ramps.size() && groupEnd >= ramps[next].first) {
+            const Ramp& r = ramps[next];
+            aim = r.from + (r.to - r.from) * (float(groupEnd - r.first) / (r.last - r.first));
+        }
+        if (aim != target) {
+            target = aim;
+            slope = (target - current) / span;
+        }
+        if (current == target || slope == 0) {
+            appendPoint(track, frame, current, 0);
+            continue;
+        }
+        appendPoint(track, frame, current + slope, slope);
+        const float groups = std::abs((target - current) / slope);
+        if (groups <= kGroup) {
+            appendPoint(track, frame + std::max(0, int(std::ceil(groups)) - 1), target, 0);
+            current = target;
         } else {
-            const auto lo = std::prev(hi);
-            const float t = float(velocity - lo->first) / float(hi->first - lo->first);
-            gain = lo->second + t * (hi->second - lo->second);
+            current += kGroup * slope;
         }
     }
-    const float track = std::abs(r.velocityTrack) * (1 - gain);
-    return r.velocityTrack < 0 ? track : 1 - track;
+    return track;
 }
 
-float crossfadeGain(const Crossfade& fade, float value, bool controllerScale)
-{
-    const auto top = [controllerScale](int high) { return controllerScale && high == 127 ? 126 : high; };
-    const int inHigh = top(fade.inHigh), outHigh = top(fade.outHigh);
-    float up = 1, down = 1;
-    if (value < float(fade.inLow))
-        up = 0;
-    else if (inHigh > fade.inLow)
-        up = std::min(1.f, (value - float(fade.inLow)) / float(inHigh - fade.inLow));
-    if (outHigh > fade.outLow)
-        down = std::clamp((float(outHigh) - value) / float(outHigh - fade.outLow), 0.f, 1.f);
-    return fade.power ? std::sqrt(up * down) : up * down;
-}
+// ------------------------------------------------------------ bends
 
-// Pitch bend ramps within sfizz_render's 1024-frame blocks (plan.cpp:114-134).
-std::vector<Bend> bendTimeline(const Midi& midi)
+struct BendChange {
+    int begin = 0, end = 0, from = 0, to = 0;  // a ramp from `from` at begin to `to` at end
+};
+
+// Each bend value change ramps within its block, then holds (src/render/plan.cpp:114-134).
+std::vector<BendChange> bendChanges(const Midi& midi)
 {
-    std::vector<Bend> bends;
-    int lastFrame = 0, lastValue = 0;
+    std::vector<BendChange> out;
+    int previousFrame = 0, value = 0;
     for (const auto& e : midi.events) {
         const bool reset = e.kind == EventKind::control && e.a == 121;
         if (e.kind != EventKind::bend && !reset)
             continue;
-        const int value = reset ? 0 : e.a;
-        const int begin = std::max(lastFrame, e.frame / kBendBlock * kBendBlock);
-        if (lastValue != value) {
-            bends.push_back({begin, e.frame, lastValue, value});
-            bends.push_back({e.frame, e.frame, value, value});
+        const int next = reset ? 0 : e.a;
+        if (next != value) {
+            out.push_back({std::max(previousFrame, e.frame / kBlock * kBlock), e.frame, value, next});
+            out.push_back({e.frame, e.frame, next, next});
         }
-        lastFrame = e.frame;
-        lastValue = value;
+        previousFrame = e.frame;
+        value = next;
     }
-    return bends;
+    return out;
 }
 
-// Controller automation track: block-aligned linear ramps to each event, then
-// optional one-pole-like smoothing evaluated in 16-frame groups (plan.cpp:136-216).
-template<class Transform>
-Track automation(const Midi& midi, int cc, float initial, int smoothFrames, Transform transform, bool settle)
-{
-    float previous = transform(initial);
-    Track points{{0, previous, 0}};
-    struct Ramp {
-        int first, last;
-        float from, to;
-    };
-    std::vector<Ramp> ramps;
-    int lastFrame = 0;
-    for (const auto& e : midi.events) {
-        float value;
-        if (e.kind == EventKind::control && e.a == 121) value = initial;
-        else if (cc == 128 && e.kind == EventKind::bend) value = float(e.a);
-        else if (e.kind == EventKind::control && e.a == cc) value = float(e.b);
-        else continue;
-        const float target = transform(value);
-        if (settle && e.frame == 0) {
-            points.front().value = target;
-        } else if (target != previous) {
-            ramps.push_back({std::max(lastFrame, e.frame / kBendBlock * kBendBlock), e.frame, previous, target});
-        }
-        previous = target;
-        lastFrame = e.frame;
-    }
-    const auto add = [&](int frame, float value, float step) {
-        auto& back = points.back();
-        if (back.frame == frame)
-            back = {frame, value, step};
-        else if (back.step != step || std::abs(back.value + float(frame - back.frame) * step - value) > 1e-6f)
-            points.push_back({frame, value, step});
-    };
-    if (smoothFrames < 2) {
-        for (const auto& ramp : ramps) {
-            if (ramp.first < ramp.last)
-                add(ramp.first, ramp.from, (ramp.to - ramp.from) / float(ramp.last - ramp.first));
-            add(ramp.last, ramp.to, 0);
-        }
-        return points;
-    }
-    constexpr int group = 16;
-    const int smooth = std::max(group, smoothFrames);
-    float current = points.front().value, target = current, step = 0;
-    size_t next = 0;
-    for (int frame = 0; next < ramps.size() || current != target; frame += group) {
-        if (current == target && next < ramps.size() && frame < ramps[next].first)
-            frame = std::max(frame, ramps[next].first / group * group);
-        float goal = target;
-        const int end = frame + group - 1;
-        while (next < ramps.size() && end >= ramps[next].last)
-            goal = ramps[next++].to;
-        if (next < ramps.size() && end >= ramps[next].first) {
-            const auto& ramp = ramps[next];
-            goal = ramp.from + (ramp.to - ramp.from) * (float(end - ramp.first) / float(ramp.last - ramp.first));
-        }
-        if (goal != target) {
-            target = goal;
-            step = (target - current) / float(smooth);
-        }
-        if (current == target || step == 0) {
-            add(frame, current, 0);
-            continue;
-        }
-        add(frame, current + step, step);
-        const float remaining = std::abs((target - current) / step);
-        if (remaining <= group) {
-            add(frame + std::max(0, int(std::ceil(remaining)) - 1), target, 0);
-            current = target;
-        } else {
-            current += float(group) * step;
-        }
-    }
-    return points;
-}
+// ------------------------------------------------------------ envelopes and source travel
+// Span and envelope-shape records share these field names (clay_voice_span,
+// clay_envelope_shape), so one template serves both.
 
-// Host twin of the device envelope (plan.cpp:218-235), float arithmetic.
 template<class E>
-float envelopeLevel(const E& v, int frame)
+float levelAt(const E& e, int frame)
 {
-    if (v.release_start >= 0 && frame >= v.release_start) {
-        if (frame >= v.fade_start)
-            return std::max(0.f, v.fade_value - float(frame - v.fade_start + 1) * v.fade_step);
-        return v.release_value * std::exp(float(frame - v.release_start + 1) * v.release_rate);
+    if (e.release_start >= 0 && frame >= e.release_start) {
+        if (frame >= e.fade_start)
+            return std::max(0.f, e.fade_value - (frame - e.fade_start + 1) * e.fade_step);
+        return e.release_value * std::exp((frame - e.release_start + 1) * e.release_rate);
     }
-    if (frame < v.attack_start)
-        return v.start;
-    if (frame < v.hold_start)
-        return std::min(1.f, v.start + float(frame - v.attack_start + 1) * v.attack_step);
-    if (frame < v.decay_start)
+    if (frame < e.attack_start)
+        return e.start;
+    if (frame < e.hold_start)
+        return std::min(1.f, e.start + (frame - e.attack_start + 1) * e.attack_step);
+    if (frame < e.decay_start)
         return 1;
-    if (v.decay_rate == 0)
-        return v.sustain;
-    return std::max(v.sustain, std::exp(float(frame - v.decay_start + 1) * v.decay_rate));
+    if (e.decay_rate == 0)
+        return e.sustain;
+    return std::max(e.sustain, std::exp((frame - e.decay_start + 1) * e.decay_rate));
 }
 
-// Exponential release to 1e-4, then a linear 50 ms fade (plan.cpp:237-248).
+// Exponential release from `level` (-9 nepers over `seconds`), then a linear
+// 50 ms fade from where it falls below the silent level.
 template<class E>
-void beginRelease(E& e, int frame, int rate, float level, float seconds)
+void startReleaseAt(E& e, int frame, int rate, float level, float seconds)
 {
     e.release_start = frame;
     e.release_value = level;
-    e.release_rate = -9 / (std::clamp(seconds, .006f, 100.f) * float(rate));
-    const int duration = level > kSilence ? int(std::log(kSilence / level) / e.release_rate) : 0;
-    e.fade_start = frame + duration;
-    e.fade_value = level * std::exp(e.release_rate * float(duration));
-    e.fade_step = std::max(kSilence, e.fade_value) / (float(rate) * .05f);
+    e.release_rate = -9 / (std::clamp(seconds, .006f, 100.f) * rate);
+    const int toSilence = level > kSilentLevel ? int(std::log(kSilentLevel / level) / e.release_rate) : 0;
+    e.fade_start = frame + toSilence;
+    e.fade_value = level * std::exp(e.release_rate * toSilence);
+    e.fade_step = std::max(kSilentLevel, e.fade_value) / (rate * .05f);
 }
 
-double travel(const clay_voice_span& v, int frames)
+// Source frames covered in `frames` output frames (the bend ramp is a
+// geometric progression of the step).
+double sourceTravel(const clay_voice_span& s, int frames)
 {
-    if (v.pitch_slope == 0)
-        return v.step * v.minimum_pitch_ratio * frames;
-    return v.step * v.minimum_pitch_ratio * std::expm1(v.pitch_slope * frames) / std::expm1(v.pitch_slope);
+    const double step = s.step * s.minimum_pitch_ratio;
+    if (s.pitch_slope == 0)
+        return step * frames;
+    return step * std::expm1(s.pitch_slope * frames) / std::expm1(s.pitch_slope);
 }
 
-double wrapLoop(const clay_voice_span& v, double position)
+double insideLoop(const clay_voice_span& s, double position)
 {
-    if (v.loop_end > 0 && position >= v.loop_end)
-        return v.loop_first + std::fmod(position - v.loop_first, double(v.loop_end - v.loop_first));
+    if (s.loop_end > 0 && position >= s.loop_end)
+        return s.loop_first + std::fmod(position - s.loop_first, s.loop_end - s.loop_first);
     return position;
 }
 
-// Frame where an unlooped sample runs out, bounded by the envelope end (plan.cpp:264-280).
-int voiceEnd(const clay_voice_span& v, int envelopeEnd)
+// The frame where an unlooped sample runs out, or `envelopeEnd` if sooner
+// (src/render/plan.cpp:264-280).
+int spanEnd(const clay_voice_span& s, int envelopeEnd)
 {
-    if (v.loop_end > 0 || v.generator != CLAY_GEN_NONE)
+    if (s.loop_end > 0 || s.generator != CLAY_GEN_NONE)
         return envelopeEnd;
-    const int rampFrames = std::max(0, v.bend_end - v.begin);
-    const double rampTravel = travel(v, rampFrames);
-    const double remaining = v.source_end - v.position;
-    const double step = v.step * v.minimum_pitch_ratio;
-    const double length = rampFrames && v.pitch_slope != 0 && remaining <= rampTravel
-        ? std::ceil(std::log1p(std::max(0., remaining) * std::expm1(v.pitch_slope) / step) / v.pitch_slope)
-        : rampFrames + std::ceil((remaining - rampTravel) / (step * std::exp(v.pitch_slope * rampFrames)));
-    if (length > double(kForever - v.begin))
+    const int rampFrames = std::max(0, s.bend_end - s.begin);
+    const double rampTravel = sourceTravel(s, rampFrames);
+    const double left = s.source_end - s.position;
+    const double step = s.step * s.minimum_pitch_ratio;
+    const double frames =
+        rampFrames && s.pitch_slope != 0 && left <= rampTravel
+            ? std::ceil(std::log1p(std::max(0., left) * std::expm1(s.pitch_slope) / step) / s.pitch_slope)
+            : rampFrames + std::ceil((left - rampTravel) / (step * std::exp(s.pitch_slope * rampFrames)));
+    if (frames > kNoEnd - s.begin)
         return envelopeEnd;
-    return std::min(envelopeEnd, v.begin + std::max(0, int(length)));
+    return std::min(envelopeEnd, s.begin + std::max(0, int(frames)));
 }
 
-class Planner {
+// Velocity to gain: squared velocity, or the region's curve, scaled by amp_veltrack.
+float velocityGain(const Region& r, int velocity)
+{
+    const float v = float(velocity) / 127;
+    float gain = v * v;
+    if (!r.velocityCurve.empty()) {
+        const auto hi = std::lower_bound(r.velocityCurve.begin(), r.velocityCurve.end(), velocity,
+                                         [](const auto& p, int x) { return p.first < x; });
+        if (hi->first == velocity)
+            gain = hi->second;
+        else {
+            const auto lo = hi - 1;
+            gain = lo->second + float(velocity - lo->first) / (hi->first - lo->first) * (hi->second - lo->second);
+        }
+    }
+    const float reduction = std::abs(r.velocityTrack) * (1 - gain);
+    return r.velocityTrack < 0 ? reduction : 1 - reduction;
+}
+
+// ------------------------------------------------------------ the planner
+
+class JobPlanner {
 public:
-    Planner(const Instrument& instrument, AudioBank& bank, int rate, unsigned seed, int limit, TailPolicy tail)
-        : inst_(instrument), bank_(bank), rate_(rate), limit_(limit), tail_(tail), cc_(instrument.initialCC),
-          sequence_(instrument.regions.size(), 0u), random_(seed), variation_(seed ^ 0x85ebca6bu),
-          currentSwitch_(instrument.initialSwitch)
+    JobPlanner(const Instrument& instrument, AudioBank& bank, int rate, unsigned seed, int limit, TailPolicy tail)
+        : inst_(instrument), bank_(bank), rate_(rate), limit_(limit), natural_(tail == TailPolicy::natural),
+          controllers_(instrument.ccDefault), noise_(seed ^ 0x85ebca6b)
     {
     }
 
-    Plan render(const Midi& midi)
+    Plan run(const Midi& midi)
     {
         midi_ = &midi;
-        hasBend_ = std::any_of(midi.events.begin(), midi.events.end(),
+        anyBend_ = std::any_of(midi.events.begin(), midi.events.end(),
                                [](const Event& e) { return e.kind == EventKind::bend && e.a != 0; });
+        // Tracks 0..2: volume (CC7), expression (CC11), pan (CC10), constant
+        // when the instrument binds that controller itself.
         for (int cc : {7, 11, 10}) {
-            if (inst_.usedCC[size_t(cc)]) {
-                plan_.controls.push_back({{0, cc == 10 ? 0.f : 1.f, 0.f}});
-            } else {
-                const int curve = cc == 10 ? 1 : 4;
-                plan_.controls.push_back(automation(midi, cc, inst_.initialCC[size_t(cc)], rate_ / 100,
-                    [&](float v) { return curveValue(inst_, curve, v); }, natural()));
-            }
+            if (inst_.ccUsed[size_t(cc)])
+                plan_.controls.push_back({{0, cc == 10 ? 0.f : 1.f, 0}});
+            else
+                plan_.controls.push_back(controllerTrack(cc, cc == 10 ? 1 : 4, rate_ / 100));
         }
-        const auto bends = bendTimeline(midi);
-        size_t nextBend = 0;
+        const auto bends = bendChanges(midi);
+        size_t bend = 0;
         for (const auto& e : midi.events) {
-            while (nextBend < bends.size() && bends[nextBend].begin <= e.frame)
-                changeBend(bends[nextBend++]);
-            prune(e.frame);
-            switch (e.kind) {
-            case EventKind::on: noteOn(e); break;
-            case EventKind::off: noteOff(e); break;
-            case EventKind::control: control(e); break;
-            case EventKind::bend: break;
-            }
+            for (; bend < bends.size() && bends[bend].begin <= e.frame; ++bend)
+                applyBendChange(bends[bend]);
+            retireEnded(e.frame);
+            if (e.kind == EventKind::on)
+                noteOn(e);
+            else if (e.kind == EventKind::off)
+                noteOff(e);
+            else if (e.kind == EventKind::control)
+                controller(e);
         }
-        prune(midi.end);
-        for (auto& voice : active_) {
-            if (!natural() && voice.kind == VoiceKind::attack && voice.region->loop != Loop::oneShot)
-                release(voice, midi.end);
-            if (voice.span.end == kForever && limit_ == kForever)
+        retireEnded(midi.end);
+        for (auto& v : voices_) {
+            // The natural policy's MIDI ends its notes itself (pedals lifted at the end).
+            if (!natural_)
+                release(v, midi.end);
+            if (v.span.end == kNoEnd && limit_ == kNoEnd)
                 throw std::runtime_error("a looping one-shot or release sample has no finite end");
-            emit(voice, voice.span.end);
+            finish(v.span, v.span.end);
         }
-        if (!natural())
+        if (!natural_)
             plan_.frames = std::max(plan_.frames, midi.end);
         if (plan_.spans.empty())
             throw std::runtime_error("no playable notes for this SFZ and MIDI");
-        buildLfoTracks();
         return std::move(plan_);
     }
 
 private:
-    bool natural() const { return tail_ == TailPolicy::natural; }
+    struct Voice {
+        clay_voice_span span;
+        const Region* region;
+        int key, velocity;
+        double baseStep;        // source frames per output frame without bend
+        float releaseSeconds;
+        bool keyReleased = false;  // note-off seen (the pedals may still hold it)
+        bool sostenuto = false;    // held by the sostenuto pedal
+    };
 
-    void emit(const Voice& voice, int end)
+    // ---------------------------------------------------- plan output
+
+    // Writes a span (cut at `end` and at the render limit) to the plan.
+    void finish(clay_voice_span s, int end)
     {
-        auto span = voice.span;
-        span.end = std::min(span.end, end);
-        plan_.tailClipped |= span.end > limit_ && span.begin < span.end;
-        span.end = std::min(span.end, limit_);
-        if (span.begin < span.end) {
-            for (int i = voice.firstLfo; i < voice.firstLfo + voice.lfoCount; ++i)
-                lfos_[size_t(i)].end = std::max(lfos_[size_t(i)].end, span.end);
-            plan_.frames = std::max(plan_.frames, span.end);
-            plan_.spans.push_back(span);
-        }
-    }
-
-    void prune(int frame)
-    {
-        auto keep = std::remove_if(active_.begin(), active_.end(), [&](const Voice& v) {
-            if (v.span.end > frame)
-                return false;
-            emit(v, v.span.end);
-            return true;
-        });
-        active_.erase(keep, active_.end());
-    }
-
-    void split(Voice& voice, int frame, double position)
-    {
-        emit(voice, frame);
-        voice.span.begin = frame;
-        voice.span.position = position;
-    }
-
-    bool pedal(const Region& r) const { return r.sustainEnabled && cc_[size_t(r.sustainCC)] >= float(r.sustainLow); }
-
-    bool switchesMatch(const Region& r) const
-    {
-        return (r.switchLow < 0 || (currentSwitch_ >= r.switchLow && currentSwitch_ <= r.switchHigh)) &&
-               (r.switchDown < 0 || down_[size_t(r.switchDown)]) && (r.switchUp < 0 || !down_[size_t(r.switchUp)]) &&
-               (r.switchPrevious < 0 || previousNote_ == r.switchPrevious);
-    }
-
-    bool controllersMatch(const Region& r) const
-    {
-        return std::all_of(r.conditions.begin(), r.conditions.end(), [&](const Condition& c) {
-            return cc_[c.cc] >= float(c.low) && cc_[c.cc] <= float(c.high);
-        });
-    }
-
-    bool matches(const Region& r, int velocity, float draw) const
-    {
-        return velocity >= r.lowVelocity && velocity <= r.highVelocity && draw >= r.randomLow && draw < r.randomHigh &&
-               switchesMatch(r) && controllersMatch(r);
-    }
-
-    // Plain pitch bend: a ramp span with exponential step change, or a constant step (plan.cpp:396-420).
-    void applyBend(Voice& voice) const
-    {
-        auto& s = voice.span;
-        if (voice.region->bendSmooth > 0) {
-            s.bend_end = s.begin;
-            s.pitch_slope = 0;
-            s.step = voice.baseStep;
+        s.end = std::min(s.end, end);
+        plan_.tailClipped |= s.end > limit_ && s.begin < s.end;
+        s.end = std::min(s.end, limit_);
+        if (s.begin >= s.end)
             return;
-        }
-        const auto cents = [&](int value) {
-            const double normalized = std::max(-8191, value) / 8191.0;
-            return normalized >= 0 ? normalized * voice.region->bendUp : -normalized * voice.region->bendDown;
-        };
-        const double to = cents(bend_.to);
-        s.bend_end = bend_.end;
-        s.pitch_slope = 0;
-        if (s.begin < bend_.end) {
-            const double from = cents(bend_.from);
-            const double slope = (to - from) / (bend_.end - bend_.begin);
-            s.pitch_slope = slope * (std::log(2.) / 1200);
-            s.step = voice.baseStep * std::exp2((from + (s.begin - bend_.begin) * slope) / 1200);
-        } else {
-            s.step = voice.baseStep * std::exp2(to / 1200);
-        }
+        plan_.frames = std::max(plan_.frames, s.end);
+        plan_.spans.push_back(s);
     }
 
-    void changeBend(const Bend& bend)
+    // Voices whose spans have ended by `frame` leave the pool, in order.
+    void retireEnded(int frame)
     {
-        prune(bend.begin);
-        bend_ = bend;
-        for (auto& voice : active_) {
-            if (voice.region->bendSmooth > 0)
-                continue;
-            const int start = std::max(bend.begin, voice.span.begin);
-            const double phase = voice.span.position + travel(voice.span, start - voice.span.begin);
-            split(voice, start, phase);
-            applyBend(voice);
-            voice.span.end = voiceEnd(voice.span, fadeEnd(voice.span));
+        auto keep = voices_.begin();
+        for (auto& v : voices_) {
+            if (v.span.end > frame)
+                *keep++ = v;
+            else
+                finish(v.span, v.span.end);
         }
+        voices_.erase(keep, voices_.end());
     }
 
-    int fadeEnd(const clay_voice_span& s) const { return s.release_start < 0 ? kForever : s.fade_start + rate_ / 20 + 1; }
-
-    float controllerValue(const Modulation& m, int key, int velocity, VoiceKind kind) const
+    // Ends the voice's current span at `frame` and continues it from there.
+    void cut(Voice& v, int frame, double position)
     {
-        if (m.cc == 140)
-            return keyDelta_;
-        if (m.cc == 141)
-            return std::abs(keyDelta_);
-        float value = 0;
-        if (m.cc < 128) value = cc_[size_t(m.cc)];
-        else if (m.cc == 133) value = float(key);
-        else if ((m.cc == 131 && kind == VoiceKind::attack) || (m.cc == 132 && kind == VoiceKind::release)) value = float(velocity);
-        else if (m.cc == 134) value = keysDown_ > 0 ? 127.f : 0.f;
-        else if (m.cc == 135) value = 127 * randomValue_;
-        else if (m.cc == 136) value = 127 * (2 * randomValue_ - 1);
-        else if (m.cc == 137) value = alternate_ ? 127.f : 0.f;
-        else if (const auto it = inst_.extendedCCDefaults.find(m.cc); it != inst_.extendedCCDefaults.end()) value = it->second;
-        return curveValue(inst_, m.curve, value);
+        finish(v.span, frame);
+        v.span.begin = frame;
+        v.span.position = position;
     }
 
-    int controllerTrack(const Modulation& m)
+    std::vector<clay_control_point> controllerTrack(int cc, int curve, int smooth)
     {
-        const auto key = std::make_tuple(m.cc, m.curve, m.smooth);
-        const auto [it, inserted] = tracks_.emplace(key, int(plan_.controls.size()));
-        if (inserted)
-            plan_.controls.push_back(automation(*midi_, m.cc, inst_.initialCC[size_t(m.cc)], m.smooth * rate_ / 1000,
-                [&](float v) { return curveValue(inst_, m.curve, v); }, natural()));
-        return it->second;
+        const auto [start, ramps] = controllerRamps(*midi_, cc, inst_.ccDefault[size_t(cc)],
+                                                    [curve](float x) { return curveValue(curve, x); }, natural_);
+        return trackOf(start, ramps, smooth);
     }
 
+    // Index of the plan track for a controller binding (shared by equal bindings).
+    int bindingTrack(const Binding& b)
+    {
+        const auto key = std::make_tuple(b.source, b.curve, b.smoothMs);
+        const auto found = bindingTracks_.find(key);
+        if (found != bindingTracks_.end())
+            return found->second;
+        const int index = int(plan_.controls.size());
+        bindingTracks_.emplace(key, index);
+        plan_.controls.push_back(controllerTrack(b.source, b.curve, b.smoothMs * rate_ / 1000));
+        return index;
+    }
+
+    // Pitch track (cents) of a smoothed bend (bend_smooth).
     int smoothBendTrack(const Region& r)
     {
-        const auto key = std::make_tuple(r.bendUp, r.bendDown, r.bendSmooth);
-        const auto [it, inserted] = bendTracks_.emplace(key, int(plan_.controls.size()));
-        if (inserted)
-            plan_.controls.push_back(automation(*midi_, 128, 0, r.bendSmooth * rate_ / 1000, [&](float v) {
-                const float bend = std::max(-8191.f, v) / 8191;
-                return bend >= 0 ? bend * r.bendUp : -bend * r.bendDown;
-            }, natural()));
-        return it->second;
+        const auto key = std::make_tuple(r.bendUp, r.bendDown, r.bendSmoothMs);
+        const auto found = bendTracks_.find(key);
+        if (found != bendTracks_.end())
+            return found->second;
+        const int index = int(plan_.controls.size());
+        bendTracks_.emplace(key, index);
+        const auto cents = [&r](float value) {
+            const float bend = std::max(-8191.f, value) / 8191;
+            return bend >= 0 ? bend * r.bendUp : -bend * r.bendDown;
+        };
+        const auto [start, ramps] = controllerRamps(*midi_, 128, 0, cents, natural_);
+        plan_.controls.push_back(trackOf(start, ramps, r.bendSmoothMs * rate_ / 1000));
+        return index;
     }
 
-    int controllerFadeTrack(int cc, const Crossfade& fade)
+    // ---------------------------------------------------- controllers and parameters
+
+    // A binding's source value through its curve (the key steps 140/141 unshaped).
+    float sourceValue(const Binding& b, int key, int velocity) const
     {
-        const auto key = std::make_tuple(cc, fade.inLow, fade.inHigh, fade.outLow, fade.outHigh, fade.power);
-        const auto [it, inserted] = fadeTracks_.emplace(key, int(plan_.controls.size()));
-        if (inserted)
-            plan_.controls.push_back(automation(*midi_, cc, inst_.initialCC[size_t(cc)], rate_ / 200,
-                [&](float v) { return crossfadeGain(fade, v, true); }, natural()));
-        return it->second;
+        if (b.source == 140)
+            return keyStep_;
+        if (b.source == 141)
+            return std::abs(keyStep_);
+        float raw = 0;
+        if (b.source < 128)
+            raw = controllers_[size_t(b.source)];
+        else if (b.source == 133)
+            raw = float(key);
+        else if (b.source == 131)
+            raw = float(velocity);  // note-on velocity (every voice here starts on a note)
+        else if (b.source == 134)
+            raw = keysHeld_ > 0 ? 127.f : 0.f;
+        else if (const auto d = inst_.extendedDefaults.find(b.source); d != inst_.extendedDefaults.end())
+            raw = d->second;
+        return curveValue(b.curve, raw);
     }
 
-    float param(const Region& r, ModTarget target, float base, int key, int velocity, VoiceKind kind) const
+    // A voice parameter at note-on: base value, amplitude-envelope velocity
+    // offset, plus every binding to it.
+    float parameter(const Region& r, ModTarget target, float base, int key, int velocity) const
     {
         if (target >= ModTarget::envDelay && target <= ModTarget::envRelease)
-            base += r.velocityEnvelope[size_t(int(target) - int(ModTarget::envDelay))] * (float(velocity) / 127.f);
-        for (const auto& m : r.modulation)
-            if (m.target == target)
-                base += m.depth * controllerValue(m, key, velocity, kind);
+            base += r.ampVelocity[size_t(int(target) - int(ModTarget::envDelay))] * (velocity / 127.f);
+        for (const auto& b : r.bindings)
+            if (b.target == target)
+                base += b.depth * sourceValue(b, key, velocity);
         return base;
     }
 
-    // Filter EG (slot 0) and pitch EG (slot 1) shapes (plan.cpp:507-536).
-    void initSecondary(Voice& voice)
+    bool pedalDown(const Region& r) const
     {
-        auto& s = voice.span;
-        for (int slot = 0; slot < 2; ++slot) {
-            const auto& src = voice.region->secondary[size_t(slot)];
-            if (!src.enabled)
+        return r.sustainSwitch && controllers_[size_t(r.sustainCC)] >= r.sustainLow;
+    }
+
+    bool selected(const Region& r, int velocity) const
+    {
+        if (velocity < r.velocityLow || velocity > r.velocityHigh)
+            return false;
+        for (const auto& w : r.conditions)
+            if (controllers_[size_t(w.cc)] < w.low || controllers_[size_t(w.cc)] > w.high)
+                return false;
+        return true;
+    }
+
+    // ---------------------------------------------------- pitch
+
+    // Step and pitch ramp of the voice's current span under the bend in force.
+    void bendSpan(Voice& v) const
+    {
+        clay_voice_span& s = v.span;
+        const Region& r = *v.region;
+        s.pitch_slope = 0;
+        if (r.bendSmoothMs > 0) {  // the bend reaches the voice through its smoothed pitch track
+            s.bend_end = s.begin;
+            s.step = v.baseStep;
+            return;
+        }
+        const auto cents = [&r](int value) {
+            const double bend = std::max(-8191, value) / 8191.0;
+            return bend >= 0 ? bend * r.bendUp : -bend * r.bendDown;
+        };
+        const double to = cents(bend_.to);
+        s.bend_end = bend_.end;
+        if (s.begin >= bend_.end) {
+            s.step = v.baseStep * std::exp2(to / 1200);
+            return;
+        }
+        const double from = cents(bend_.from);
+        const double slope = (to - from) / (bend_.end - bend_.begin);  // cents per frame
+        s.pitch_slope = slope * (std::log(2.) / 1200);
+        s.step = v.baseStep * std::exp2((from + (s.begin - bend_.begin) * slope) / 1200);
+    }
+
+    int fadeEnd(const clay_voice_span& s) const { return s.release_start < 0 ? kNoEnd : s.fade_start + rate_ / 20 + 1; }
+
+    void applyBendChange(const BendChange& change)
+    {
+        retireEnded(change.begin);
+        bend_ = change;
+        for (auto& v : voices_) {
+            if (v.region->bendSmoothMs > 0)
                 continue;
-            const auto first = slot == 0 ? ModTarget::filterEnvDelay : ModTarget::pitchEnvDelay;
-            std::array<float, 7> values;
-            for (size_t p = 0; p < values.size(); ++p) {
-                const float base = src.values[p] + src.velocity[p] * (float(voice.velocity) / 127.f);
-                const float value = param(*voice.region, static_cast<ModTarget>(int(first) + int(p)), base, voice.key,
-                                          voice.velocity, voice.kind);
-                values[p] = std::clamp(value, 0.f, p == 4 || p == 5 ? 1.f : 100.f);
+            const int from = std::max(change.begin, v.span.begin);
+            cut(v, from, v.span.position + sourceTravel(v.span, from - v.span.begin));
+            bendSpan(v);
+            v.span.end = spanEnd(v.span, fadeEnd(v.span));
+        }
+    }
+
+    // ---------------------------------------------------- starting voices
+
+    // Filter and pitch envelope shapes (fileg_, pitcheg_) of a new voice.
+    void modEnvelopes(Voice& v) const
+    {
+        const Region& r = *v.region;
+        for (int slot = 0; slot < 2; ++slot) {
+            const ModEnvelope& src = r.modEnvelopes[size_t(slot)];
+            if (!src.used)
+                continue;
+            const int first = int(slot == 0 ? ModTarget::filterEnvDelay : ModTarget::pitchEnvDelay);
+            Stages st;
+            for (int k = 0; k < stageCount; ++k) {
+                const float base = src.stages[size_t(k)] + src.velocity[size_t(k)] * (v.velocity / 127.f);
+                const float value = parameter(r, ModTarget(first + k), base, v.key, v.velocity);
+                st[size_t(k)] = std::clamp(value, 0.f, k == stageSustain || k == stageStart ? 1.f : 100.f);
             }
-            auto& e = s.extra_envelope[slot];
-            s.active_envelope[slot] = 1;
-            e.start = values[5];
-            e.attack_start = s.begin + int(values[0] * float(rate_));
-            const float attack = values[1] * float(rate_);
+            clay_envelope_shape& e = v.span.extra_envelope[slot];
+            v.span.active_envelope[slot] = 1;
+            e.start = st[stageStart];
+            e.attack_start = v.span.begin + int(st[stageDelay] * rate_);
+            const float attack = st[stageAttack] * rate_;
             e.attack_step = attack > 0 ? 1 / attack : 1;
             e.hold_start = e.attack_start + std::max(0, int(std::ceil((1 - e.start) * attack)) - 1);
-            e.decay_start = e.hold_start + int(values[2] * float(rate_));
-            e.decay_rate = values[3] > 0 ? -9 / (std::max(.006f, values[3]) * float(rate_)) : 0;
-            e.sustain = values[4];
+            e.decay_start = e.hold_start + int(st[stageHold] * rate_);
+            e.decay_rate = st[stageDecay] > 0 ? -9 / (std::max(.006f, st[stageDecay]) * rate_) : 0;
+            e.sustain = st[stageSustain];
             e.release_start = -1;
-            e.release_seconds = values[6];
-            e.depth = src.depth + src.velocityDepth * (float(voice.velocity) / 127.f);
+            e.release_seconds = st[stageRelease];
+            e.depth = src.depth + src.velocityDepth * (v.velocity / 127.f);
         }
     }
 
-    float appendLfos(Voice& voice, int frame, bool& modulatedStereo)
+    int newFilterTrack(clay_voice_span& s)
     {
-        voice.firstLfo = int(lfos_.size());
-        float pitchBound = 0;
-        for (const auto& entry : voice.region->lfos) {
-            const auto& lfo = entry.second;
-            LfoTrack track{&lfo, frame, frame, 0, 0, {}, {}};
-            float delay = lfo.delay, fade = lfo.fade;
-            auto bounds = lfo.depths;
-            for (auto& d : bounds)
-                d.second = std::abs(d.second);
-            for (const auto& m : lfo.modulation) {
-                const float value = controllerValue(m, voice.key, voice.velocity, voice.kind);
-                if (m.target == ModTarget::lfoDelay) {
-                    delay += m.depth * value;
-                } else if (m.target == ModTarget::lfoFade) {
-                    fade += m.depth * value;
-                } else {
-                    const int source = m.cc < 128 ? controllerTrack(m) : -1;
-                    track.inputs.push_back({source, m.target, source < 0 ? m.depth * value : m.depth});
-                    if (m.target < ModTarget::lfoFrequency || m.target >= ModTarget::eq1Frequency) {
-                        float maximum = std::abs(value);
-                        if (source >= 0)
-                            for (const auto& p : plan_.controls[size_t(source)])
-                                maximum = std::max(maximum, std::abs(p.value));
-                        bounds[m.target] += std::abs(m.depth) * maximum;
-                    }
-                }
-            }
-            track.delay = int(std::clamp(delay, 0.f, 100.f) * float(rate_));
-            track.fade = int(std::clamp(fade, 0.f, 100.f) * float(rate_));
-            for (const auto& [target, bound] : bounds) {
-                if (bound == 0)
-                    continue;
-                modulatedStereo |= target == ModTarget::width || target == ModTarget::position;
-                if (target == ModTarget::pitch)
-                    pitchBound += bound;
-                const int index = int(plan_.controls.size());
-                plan_.controls.emplace_back();
-                track.outputs.emplace_back(target, index);
-                plan_.modulation.push_back({index, int(target), 1.f, -1});
-            }
-            if (!track.outputs.empty())
-                lfos_.push_back(std::move(track));
-        }
-        voice.lfoCount = int(lfos_.size()) - voice.firstLfo;
-        return pitchBound;
+        if (s.filter_track < 0)
+            s.filter_track = plan_.filterTracks++;
+        return s.filter_track;
     }
 
-    // LFO output tracks, sampled every 16 frames for triangle/sine (plan.cpp:589-639).
-    void buildLfoTracks()
+    void startVoice(const Region& r, int frame, int key, int velocity)
     {
-        std::mt19937 sampleHold(1);
-        for (const auto& lfo : lfos_) {
-            const auto& src = *lfo.source;
-            const int start = lfo.begin + lfo.delay;
-            const int stride = src.wave <= 1 ? 16 : 1;
-            double phase = 0;
-            int cycle = -1;
-            float held = 0;
-            std::array<float, int(ModTarget::eq3Gain) + 1> baseDepth{};
-            for (const auto& d : src.depths)
-                baseDepth[size_t(d.first)] = d.second;
-            for (const auto& out : lfo.outputs)
-                plan_.controls[size_t(out.second)].push_back({0, 0, 0});
-            for (int frame = start; frame < lfo.end; frame += stride) {
-                float frequency = src.frequency, shift = src.phase;
-                auto depth = baseDepth;
-                for (const auto& in : lfo.inputs) {
-                    const float v = in.track < 0 ? in.depth : in.depth * trackAt(plan_.controls[size_t(in.track)], frame);
-                    if (in.target == ModTarget::lfoFrequency) frequency += v;
-                    else if (in.target == ModTarget::lfoPhase) shift += v;
-                    else depth[size_t(in.target)] += v;
-                }
-                float wave = lfoWave(src.wave, phase + shift);
-                if (src.wave == 12) {
-                    const int nextCycle = int(phase);
-                    if (cycle != nextCycle) {
-                        held = 2 * std::generate_canonical<float, 24>(sampleHold) - 1;
-                        cycle = nextCycle;
-                    }
-                    wave = held;
-                }
-                const float fade = lfo.fade == 0 ? 1.f : std::min(1.f, float(frame - start) / float(lfo.fade));
-                for (const auto& out : lfo.outputs) {
-                    auto& points = plan_.controls[size_t(out.second)];
-                    const float value = depth[size_t(out.first)] * wave * fade;
-                    if (points.back().frame == frame) {
-                        points.back().value = value;
-                    } else {
-                        if (stride > 1 && frame - points.back().frame == stride)
-                            points.back().step = (value - points.back().value) / float(stride);
-                        if (stride > 1 || value != points.back().value)
-                            points.push_back({frame, value, 0});
-                    }
-                }
-                phase += double(std::clamp(frequency, 0.f, 1000.f)) * stride / rate_;
-            }
-        }
-    }
-
-    // egN pitch envelopes as control tracks plus pitch bindings (plan.cpp:641-691).
-    float appendPitchEnvelopes(Voice& voice)
-    {
-        float bound = 0;
-        for (const auto& entry : voice.region->pitchEnvelopes) {
-            const auto& src = entry.second;
-            std::vector<clay_modulation> bindings;
-            if (src.depth != 0) {
-                bindings.push_back({-1, int(ModTarget::pitch), src.depth, -1});
-                bound += std::abs(src.depth);
-            }
-            for (const auto& [cc, depth] : src.depthCC) {
-                const Modulation m{ModTarget::pitch, cc, depth};
-                const float value = controllerValue(m, voice.key, voice.velocity, voice.kind);
-                if (cc < 128) {
-                    bindings.push_back({controllerTrack(m), int(ModTarget::pitch), depth, -1});
-                    bound += std::abs(depth);
-                } else if (value != 0) {
-                    bindings.push_back({-1, int(ModTarget::pitch), depth * value, -1});
-                    bound += std::abs(depth * value);
-                }
-            }
-            if (bindings.empty())
-                continue;
-            const int shape = int(plan_.controls.size());
-            voice.flexShapes.push_back(shape);
-            plan_.controls.emplace_back();
-            auto& track = plan_.controls.back();
-            track.push_back({0, 0, 0});
-            const auto write = [&](int frame, float level, float step) {
-                if (track.back().frame == frame) track.back() = {frame, level, step};
-                else track.push_back({frame, level, step});
-            };
-            int frame = voice.triggerFrame;
-            float level = 0;
-            for (const auto& [index, point] : src.points) {
-                float time = point.seconds;
-                for (const auto& [cc, amount] : point.timeCC)
-                    time += amount * controllerValue({ModTarget::pitch, cc}, voice.key, voice.velocity, voice.kind);
-                const int frames = int(std::clamp(time, 0.f, 100.f) * float(rate_));
-                if (frames > 0)
-                    write(frame, level, (point.level - level) / float(frames));
-                level = point.level;
-                frame += frames;
-                write(frame, level, 0);
-            }
-            for (auto b : bindings) {
-                b.shape_track = shape;
-                plan_.modulation.push_back(b);
-            }
-        }
-        return bound;
-    }
-
-    // Start a voice (plan.cpp:693-830).
-    void trigger(const Region& r, int frame, int key, int velocity, VoiceKind kind, int duration = 0, int controller = -1)
-    {
-        if (inst_.hasOffBy)
-            choke(r, frame, controller >= 0 ? controller : key, kind);
-        if (r.notePolyphony)
-            limitNoteVoices(r, frame, key, velocity);
         const Sample* sample = r.generator == Generator::none ? &bank_.decode(r.sample) : nullptr;
-        const auto vary = [&](float extent) {
-            return extent == 0 ? 0.f : extent * std::generate_canonical<float, 24>(variation_);
-        };
-        Voice voice{};
-        voice.key = key;
-        voice.velocity = velocity;
-        voice.triggerFrame = frame;
-        voice.region = &r;
-        voice.kind = kind;
-        const auto modulated = [&](ModTarget t, float base) { return param(r, t, base, key, velocity, kind); };
-        const auto seconds = [&](ModTarget t, float base) { return std::clamp(modulated(t, base), 0.f, 100.f); };
-        voice.releaseSeconds = seconds(ModTarget::envRelease, r.release);
-        const double baseRate = sample ? sample->rate : 440 * std::exp2((r.rootKey - 69) / 12.);
-        voice.baseStep = (baseRate / rate_) *
-            std::exp2((r.keyTrack * float(key - r.rootKey) + r.tune + float(100 * r.transpose) + vary(r.pitchRandom)) / 1200.0);
-        auto& s = voice.span;
+        const auto param = [&](ModTarget t, float base) { return parameter(r, t, base, key, velocity); };
+        const auto seconds = [&](ModTarget t, float base) { return std::clamp(param(t, base), 0.f, 100.f); };
+        Voice v{};
+        v.region = &r;
+        v.key = key;
+        v.velocity = velocity;
+        v.releaseSeconds = seconds(ModTarget::envRelease, r.amp[stageRelease]);
+        const double rootRate = sample ? sample->rate : 440 * std::exp2((r.keyRoot - 69) / 12.);
+        const float cents = r.keyTrack * (key - r.keyRoot) + r.tune + 100 * r.transpose;
+        v.baseStep = (rootRate / rate_) * std::exp2(cents / 1200.0);
+
+        clay_voice_span& s = v.span;
         s.sample = sample ? sample->offset : 0;
         s.channels = sample ? sample->channels : 1;
-        s.generator = int(r.generator);
+        s.generator = int32_t(r.generator);
         if (r.generator == Generator::noise) {
-            s.noise_seed = variation_();
+            s.noise_seed = noise_();
             s.noise_start = frame;
         }
-        s.begin = frame + int(seconds(ModTarget::delay, r.delay) * float(rate_));
+        s.begin = frame + int(seconds(ModTarget::delay, r.delay) * rate_);
         s.source_end = r.end;
-        const int randomOffset = r.offsetRandom ? std::uniform_int_distribution<int>(0, r.offsetRandom)(variation_) : 0;
         if (sample) {
-            s.position = std::trunc(std::clamp(double(modulated(ModTarget::offset, float(r.offset))) + randomOffset, 0.,
-                                               double(r.end)));
+            const double offset = param(ModTarget::offset, float(r.offset));
+            s.position = std::trunc(std::clamp(offset, 0., double(r.end)));
         } else {
-            const double phase = r.oscillatorPhase >= 0 ? r.oscillatorPhase : randomValue_;
-            s.position = phase - std::floor(phase);
+            s.position = r.oscillatorPhase - std::floor(r.oscillatorPhase);
         }
-        applyBend(voice);
-        if (sample && (r.loop == Loop::continuous || r.loop == Loop::sustain)) {
+        bendSpan(v);
+        if (sample && (r.loop == LoopMode::continuous || r.loop == LoopMode::sustain)) {
             s.loop_first = r.loopStart;
             s.loop_end = r.loopEnd;
             s.loop_crossfade = int(std::lround(std::min<double>(r.loopStart, double(r.loopCrossfade) * sample->rate)));
         }
-        s.start = std::clamp(modulated(ModTarget::envStart, r.start), 0.f, 1.f);
-        s.attack_start = s.begin + int(seconds(ModTarget::envDelay, r.envelopeDelay) * float(rate_));
-        const float attackFrames = seconds(ModTarget::envAttack, r.attack) * float(rate_);
-        s.attack_step = attackFrames > 0 ? 1 / attackFrames : 1;
-        s.hold_start = s.attack_start + std::max(0, int(std::ceil((1 - s.start) * attackFrames)) - 1);
-        s.decay_start = s.hold_start + int(seconds(ModTarget::envHold, r.hold) * float(rate_));
-        const float decay = seconds(ModTarget::envDecay, r.decay);
-        s.decay_rate = decay > 0 ? -9 / (std::max(.006f, decay) * float(rate_)) : 0;
-        s.sustain = std::clamp(modulated(ModTarget::envSustain, r.sustain), 0.f, 1.f);
+        // Amplitude envelope.
+        s.start = std::clamp(param(ModTarget::envStart, r.amp[stageStart]), 0.f, 1.f);
+        s.attack_start = s.begin + int(seconds(ModTarget::envDelay, r.amp[stageDelay]) * rate_);
+        const float attack = seconds(ModTarget::envAttack, r.amp[stageAttack]) * rate_;
+        s.attack_step = attack > 0 ? 1 / attack : 1;
+        s.hold_start = s.attack_start + std::max(0, int(std::ceil((1 - s.start) * attack)) - 1);
+        s.decay_start = s.hold_start + int(seconds(ModTarget::envHold, r.amp[stageHold]) * rate_);
+        const float decay = seconds(ModTarget::envDecay, r.amp[stageDecay]);
+        s.decay_rate = decay > 0 ? -9 / (std::max(.006f, decay) * rate_) : 0;
+        s.sustain = std::clamp(param(ModTarget::envSustain, r.amp[stageSustain]), 0.f, 1.f);
         s.release_start = -1;
-        initSecondary(voice);
+        modEnvelopes(v);
+        // Filters and EQ bands.
         s.filter_track = -1;
         s.sample_rate = rate_;
-        for (size_t i = 0; i < r.filters.size(); ++i) {
-            const auto& f = r.filters[i];
-            if (f.type == FilterType::none)
+        for (int f = 0; f < 2; ++f) {
+            const FilterSettings& fs = r.filters[size_t(f)];
+            if (fs.type == FilterType::none)
                 continue;
-            if (s.filter_track < 0)
-                s.filter_track = plan_.filterTracks++;
-            const auto velocityTarget = i == 0 ? ModTarget::filterVelocity : ModTarget::filterVelocity2;
-            const float cents = f.keyTrack * float(key - f.keyCenter) + vary(f.random) +
-                                modulated(velocityTarget, f.velocityTrack) * float(velocity) / 127.f;
-            s.filters[i] = {int(f.type), std::clamp(f.cutoff * std::exp2(std::clamp(cents, -24000.f, 24000.f) / 1200), 0.f, 20000.f),
-                            f.resonance, 0, 0};
+            newFilterTrack(s);
+            const float tracking = fs.keyTrack * (key - fs.keyCenter) +
+                                   param(f == 0 ? ModTarget::filterVelocity : ModTarget::filterVelocity2,
+                                         fs.velocityTrack) * velocity / 127.f;
+            const float cutoff = fs.cutoff * std::exp2(std::clamp(tracking, -24000.f, 24000.f) / 1200);
+            s.filters[f] = {int32_t(fs.type), std::clamp(cutoff, 0.f, 20000.f), fs.resonance, 0, 0};
         }
-        static const float eqDefaults[3] = {50, 500, 5000};
-        for (size_t i = 0; i < r.eq.size(); ++i) {
-            const auto& eq = r.eq[i];
-            if (!eq.enabled)
+        static constexpr float kEqCentres[3] = {50, 500, 5000};
+        for (int band = 0; band < 3; ++band) {
+            const EqSettings& eq = r.eq[size_t(band)];
+            if (!eq.used)
                 continue;
-            if (s.filter_track < 0)
-                s.filter_track = plan_.filterTracks++;
-            s.filters[i + 2] = {int(FilterType::peak), eq.frequency == 0 ? eqDefaults[i] : eq.frequency, 0, eq.bandwidth, eq.gain};
+            newFilterTrack(s);
+            s.filters[band + 2] = {int32_t(FilterType::peak), eq.frequency == 0 ? kEqCentres[band] : eq.frequency, 0,
+                                   eq.bandwidth, eq.gain};
         }
+        // Bindings that modulate the voice while it sounds; a pitch modulation
+        // range makes the voice a variable-pitch chain.
         s.first_modulation = int(plan_.modulation.size());
-        bool modulatedStereo = false;
-        float pitchBound = std::abs(s.extra_envelope[1].depth);
-        for (const auto& m : r.modulation) {
-            if ((m.target >= ModTarget::envDelay && m.target < ModTarget::cutoff) ||
-                (m.target >= ModTarget::filterVelocity && m.target < ModTarget::filterEnvDepth))
+        bool stereoModulated = false;
+        float pitchRange = std::abs(s.extra_envelope[1].depth);
+        for (const auto& b : r.bindings) {
+            const bool atNoteOn = (b.target >= ModTarget::envDelay && b.target < ModTarget::cutoff) ||
+                                  (b.target >= ModTarget::filterVelocity && b.target < ModTarget::filterEnvDepth);
+            if (atNoteOn)
                 continue;
-            modulatedStereo |= m.target == ModTarget::width || m.target == ModTarget::position;
-            if (m.target == ModTarget::pitch || m.target == ModTarget::pitchEnvDepth)
-                pitchBound += std::abs(m.cc < 128 ? m.depth : m.depth * controllerValue(m, key, velocity, kind));
-            if (m.cc < 128)
-                plan_.modulation.push_back({controllerTrack(m), int(m.target), m.depth, -1});
+            stereoModulated |= b.target == ModTarget::width || b.target == ModTarget::position;
+            if (b.target == ModTarget::pitch || b.target == ModTarget::pitchEnvDepth)
+                pitchRange += std::abs(b.source < 128 ? b.depth : b.depth * sourceValue(b, key, velocity));
+            if (b.source < 128)
+                plan_.modulation.push_back({bindingTrack(b), int32_t(b.target), b.depth, -1});
             else
-                plan_.modulation.push_back({-1, int(m.target), m.depth * controllerValue(m, key, velocity, kind), -1});
+                plan_.modulation.push_back({-1, int32_t(b.target), b.depth * sourceValue(b, key, velocity), -1});
         }
-        for (const auto& [cc, fade] : r.controllerFades)
-            plan_.modulation.push_back({controllerFadeTrack(cc, fade), int(ModTarget::amplitude), 1.f, -1});
-        if (r.bendSmooth > 0 && hasBend_) {
-            pitchBound += std::max(std::abs(r.bendUp), std::abs(r.bendDown));
-            plan_.modulation.push_back({smoothBendTrack(r), int(ModTarget::pitch), 1.f, -1});
+        if (r.bendSmoothMs > 0 && anyBend_) {
+            pitchRange += std::max(std::abs(r.bendUp), std::abs(r.bendDown));
+            plan_.modulation.push_back({smoothBendTrack(r), CLAY_MOD_PITCH, 1, -1});
         }
-        pitchBound += appendLfos(voice, frame, modulatedStereo);
-        pitchBound += appendPitchEnvelopes(voice);
-        s.minimum_pitch_ratio = std::exp2(-std::min(24000.f, pitchBound) / 1200);
-        if (pitchBound > 0) {
+        s.minimum_pitch_ratio = std::exp2(-std::min(24000.f, pitchRange) / 1200);
+        if (pitchRange > 0) {
             s.variable_pitch = 1;
-            if (s.filter_track < 0)
-                s.filter_track = plan_.filterTracks++;
+            newFilterTrack(s);
         }
         s.modulation_count = int(plan_.modulation.size()) - s.first_modulation;
-        const float noteGain = kind == VoiceKind::controller ? 1.f :
-            velocityGain(r, velocity) * crossfadeGain(r.keyFade, float(key), false) *
-            crossfadeGain(r.velocityFade, float(velocity), true);
-        const float gain = r.amplitude * noteGain *
-            std::pow(10.f, ((natural() ? 0 : kMasterGainDb) + r.volume + vary(r.ampRandom) -
-                            r.releaseDecay * float(duration) / float(rate_) + 3) / 20);
-        constexpr float panCompensation = 1.4125375446227544f;
-        const bool stereoMatrix = s.channels == 2 && (modulatedStereo || r.width != 1 || r.position != 0);
-        s.stereo_matrix = stereoMatrix;
-        const float positionGain = s.channels == 1 ? 1.f : .99881488f;
-        s.gain = gain * (stereoMatrix ? panCompensation : positionGain);
+        // Level and stereo placement.
+        const float db = (natural_ ? 0.f : kMasterDb) + r.volumeDb + 3;
+        const float gain = r.amplitude * velocityGain(r, velocity) * std::pow(10.f, db / 20);
+        const bool matrix = s.channels == 2 && (stereoModulated || r.width != 1 || r.position != 0);
+        s.stereo_matrix = matrix;
+        s.gain = gain * (matrix ? kPanCompensation : s.channels == 1 ? 1.f : kStereoGain);
         s.pan = r.pan;
         s.width = r.width;
         s.stereo_position = r.position;
         s.left = s.gain * std::cos((1 + r.pan) * kQuarterPi);
         s.right = s.gain * std::cos((1 - r.pan) * kQuarterPi);
-        s.width_direct = stereoMatrix ? std::cos((1 - r.width) * kQuarterPi) : 1.f;
-        s.width_cross = stereoMatrix ? std::cos((1 + r.width) * kQuarterPi) : 0.f;
-        s.position_left = stereoMatrix ? std::cos((1 + r.position) * kQuarterPi) : 1.f;
-        s.position_right = stereoMatrix ? std::cos((1 - r.position) * kQuarterPi) : 1.f;
-        s.end = voiceEnd(s, kForever);
-        active_.push_back(std::move(voice));
+        s.width_direct = matrix ? std::cos((1 - r.width) * kQuarterPi) : 1.f;
+        s.width_cross = matrix ? std::cos((1 + r.width) * kQuarterPi) : 0.f;
+        s.position_left = matrix ? std::cos((1 + r.position) * kQuarterPi) : 1.f;
+        s.position_right = matrix ? std::cos((1 - r.position) * kQuarterPi) : 1.f;
+        s.end = spanEnd(s, kNoEnd);
+        voices_.push_back(v);
     }
 
-    void release(Voice& voice, int frame)
+    // ---------------------------------------------------- releasing voices
+
+    void release(Voice& v, int frame)
     {
-        if (voice.span.release_start >= 0 || voice.kind != VoiceKind::attack || voice.region->loop == Loop::oneShot)
+        if (v.span.release_start >= 0 || v.region->loop == LoopMode::oneShot)
             return;
-        if (voice.region->envelopeDynamic)
-            voice.releaseSeconds = param(*voice.region, ModTarget::envRelease, voice.region->release, voice.key,
-                                         voice.velocity, voice.kind);
-        startRelease(voice, frame, voice.releaseSeconds);
-    }
-
-    // plan.cpp:842-873
-    void startRelease(Voice& voice, int frame, float seconds)
-    {
-        auto& s = voice.span;
+        clay_voice_span& s = v.span;
         if (frame >= s.end)
             return;
-        if (frame < s.begin) {
+        if (frame < s.begin) {  // released during its delay: never sounds
             s.end = s.begin;
             return;
         }
-        const float level = envelopeLevel(s, frame - 1);
-        const bool exitLoop = voice.region->loop == Loop::sustain && s.loop_end > 0;
-        if (exitLoop || s.release_start >= 0) {
-            const double phase = s.position + travel(s, frame - s.begin);
-            split(voice, frame, exitLoop ? wrapLoop(s, phase) : phase);
-            applyBend(voice);
-            if (exitLoop)
-                s.loop_end = 0;
+        const float level = levelAt(s, frame - 1);
+        if (v.region->loop == LoopMode::sustain && s.loop_end > 0) {
+            // Leaves the sustain loop: the source runs on from the loop position.
+            cut(v, frame, insideLoop(s, s.position + sourceTravel(s, frame - s.begin)));
+            bendSpan(v);
+            s.loop_end = 0;
         }
-        beginRelease(s, frame, rate_, level, seconds);
-        for (int shape : voice.flexShapes) {
-            auto& track = plan_.controls[size_t(shape)];
-            track.erase(std::lower_bound(track.begin(), track.end(), frame,
-                                         [](const clay_control_point& p, int f) { return p.frame < f; }),
-                        track.end());
-            track.push_back({frame, 0, 0});
-        }
+        startReleaseAt(s, frame, rate_, level, v.releaseSeconds);
         for (int slot = 0; slot < 2; ++slot) {
-            auto& e = s.extra_envelope[slot];
+            clay_envelope_shape& e = s.extra_envelope[slot];
             if (s.active_envelope[slot] && e.release_start < 0)
-                beginRelease(e, frame, rate_, envelopeLevel(e, frame - 1), e.release_seconds);
+                startReleaseAt(e, frame, rate_, levelAt(e, frame - 1), e.release_seconds);
         }
-        s.end = voiceEnd(s, fadeEnd(s));
+        s.end = spanEnd(s, fadeEnd(s));
     }
 
-    void choke(const Region& incoming, int frame, int key, VoiceKind cause)
+    // Voices whose key is up and that no pedal holds any more.
+    void releaseUnheld(int frame, int cc, float before)
     {
-        std::vector<std::pair<int, int>> released;
-        for (auto& voice : active_) {
-            const auto& r = *voice.region;
-            if (voice.span.end <= frame || voice.kind == VoiceKind::release || voice.choked || !r.chokedByGroup ||
-                r.offBy != incoming.group || (r.group == incoming.group && voice.key == key && controllersMatch(r)))
-                continue;
-            forceOff(voice, frame);
-            if (cause != VoiceKind::controller && voice.kind == VoiceKind::attack)
-                released.emplace_back(voice.key, voice.velocity);
-        }
-        for (const auto& [k, v] : released)
-            releaseRegions(frame, k, v, std::max(0, frame - noteStart_[size_t(k)]));
-    }
-
-    void forceOff(Voice& voice, int frame)
-    {
-        const auto& r = *voice.region;
-        voice.choked = true;
-        const float time = r.offMode == OffMode::normal ? voice.releaseSeconds : r.offMode == OffMode::fast ? .006f : r.offTime;
-        startRelease(voice, frame, time);
-    }
-
-    void limitNoteVoices(const Region& region, int frame, int key, int velocity)
-    {
-        std::vector<Voice*> candidates;
-        int count = 0;
-        for (auto& voice : active_) {
-            if (voice.choked || voice.span.end <= frame || voice.key != key || voice.region->group != region.group)
-                continue;
-            ++count;
-            if (!region.selfMask || voice.velocity <= velocity)
-                candidates.push_back(&voice);
-        }
-        if (count < region.notePolyphony)
-            return;
-        std::stable_sort(candidates.begin(), candidates.end(), [&](const Voice* a, const Voice* b) {
-            return region.selfMask ? a->velocity < b->velocity : a->triggerFrame < b->triggerFrame;
-        });
-        for (auto* candidate : candidates) {
-            if (candidate->choked)
-                continue;
-            for (auto& sister : active_)
-                if (!sister.choked && sister.key == candidate->key && sister.kind == candidate->kind &&
-                    sister.triggerFrame == candidate->triggerFrame)
-                    forceOff(sister, frame);
-            if (--count < region.notePolyphony)
-                break;
-        }
-    }
-
-    void noteOn(const Event& e)
-    {
-        const bool switching = inst_.switchKeys[size_t(e.a)];
-        if (switching)
-            currentSwitch_ = e.a;
-        if (!down_[size_t(e.a)] && !switching)
-            ++keysDown_;
-        if (!switching) {
-            keyDelta_ = lastNote_ < 0 ? 0.f : float(e.a - lastNote_);
-            lastNote_ = e.a;
-            randomValue_ = std::generate_canonical<float, 24>(extendedRandom_);
-            alternate_ = !alternate_;
-        }
-        down_[size_t(e.a)] = true;
-        noteVelocity_[size_t(e.a)] = e.b;
-        noteStart_[size_t(e.a)] = e.frame;
-        const float draw = std::generate_canonical<float, 24>(random_);
-        for (const int index : inst_.byKey[size_t(e.a)]) {
-            const auto& r = inst_.regions[size_t(index)];
-            const bool step = sequence_[size_t(index)]++ % unsigned(r.sequenceLength) == unsigned(r.sequencePosition - 1);
-            if (!step || !matches(r, e.b, draw))
-                continue;
-            if (r.trigger == Trigger::attack || (r.trigger == Trigger::first && keysDown_ == 1) ||
-                (r.trigger == Trigger::legato && keysDown_ > 1))
-                trigger(r, e.frame, e.a, e.b, VoiceKind::attack);
-        }
-        previousNote_ = e.a;
-    }
-
-    bool hasAttackVoice(const Region& r) const
-    {
-        return std::any_of(active_.begin(), active_.end(), [&](const Voice& v) {
-            return v.kind == VoiceKind::attack && v.key >= r.lowKey && v.key <= r.highKey && v.velocity >= r.lowVelocity &&
-                   v.velocity <= r.highVelocity;
-        });
-    }
-
-    void noteOff(const Event& e)
-    {
-        if (!down_[size_t(e.a)])
-            return;
-        down_[size_t(e.a)] = false;
-        if (!inst_.switchKeys[size_t(e.a)])
-            --keysDown_;
-        for (auto& voice : active_) {
-            if (voice.key == e.a && voice.kind == VoiceKind::attack) {
-                voice.noteOff = true;
-                if (!pedal(*voice.region) && !voice.sostenuto)
-                    release(voice, e.frame);
-            }
-        }
-        releaseRegions(e.frame, e.a, noteVelocity_[size_t(e.a)], e.frame - noteStart_[size_t(e.a)]);
-    }
-
-    void releaseRegions(int frame, int key, int velocity, int duration)
-    {
-        const float draw = std::generate_canonical<float, 24>(random_);
-        for (const int index : inst_.byKey[size_t(key)]) {
-            const auto& r = inst_.regions[size_t(index)];
-            const unsigned counter = sequence_[size_t(index)];
-            const bool sequenceMatches = counter == 0 ? !r.explicitSequencePosition :
-                (counter - 1) % unsigned(r.sequenceLength) == unsigned(r.sequencePosition - 1);
-            if ((r.trigger != Trigger::release && r.trigger != Trigger::releaseKey) || !sequenceMatches ||
-                !matches(r, velocity, draw))
-                continue;
-            if (r.trigger == Trigger::release && (pedal(r) || sostenutoKeys_[size_t(key)]))
-                delayed_.push_back({key, velocity, duration, &r, sostenutoKeys_[size_t(key)]});
-            else if (r.trigger == Trigger::releaseKey || r.releaseDead || hasAttackVoice(r))
-                trigger(r, frame, key, velocity, VoiceKind::release, duration);
-        }
-    }
-
-    void updateDynamicEnvelopes(const std::array<float, 128>& before, int frame)
-    {
-        for (auto& voice : active_) {
-            if (voice.choked)
-                continue;
-            for (const auto& m : voice.region->modulation) {
-                if (m.cc >= 128 || before[size_t(m.cc)] == cc_[size_t(m.cc)] || m.depth == 0)
-                    continue;
-                const bool filter = m.target >= ModTarget::filterEnvDelay && m.target <= ModTarget::filterEnvRelease;
-                const bool pitch = m.target >= ModTarget::pitchEnvDelay && m.target <= ModTarget::pitchEnvRelease;
-                if ((filter && voice.region->secondary[0].dynamic) || (pitch && voice.region->secondary[1].dynamic))
-                    throw std::runtime_error("changing a dynamic pitch or filter envelope mid-voice is unsupported");
-                if (!voice.region->envelopeDynamic || m.target < ModTarget::envDelay || m.target > ModTarget::envRelease)
-                    continue;
-                if (m.target != ModTarget::envRelease)
-                    throw std::runtime_error("changing a dynamic amplitude envelope mid-voice is unsupported for CC" +
-                                             std::to_string(m.cc));
-                if (voice.span.release_start >= 0 && frame < voice.span.fade_start) {
-                    const float secs = param(*voice.region, ModTarget::envRelease, voice.region->release, voice.key,
-                                             voice.velocity, voice.kind);
-                    startRelease(voice, frame, secs);
-                    break;
-                }
-            }
-        }
-    }
-
-    void control(const Event& e)
-    {
-        if (e.a == 121) {
-            const auto before = cc_;
-            cc_ = inst_.initialCC;
-            updateDynamicEnvelopes(before, e.frame);
-            if (natural()) {
-                setSostenuto(false);
-                releasePedals(e.frame, 66, before[66]);
-            }
-            for (auto& voice : active_)
-                if (voice.noteOff && !pedal(*voice.region))
-                    release(voice, e.frame);
-            return;
-        }
-        if (!natural() && e.a == 66 && e.b != 0)
-            throw std::runtime_error("unsupported MIDI sostenuto CC66");
-        if (natural() && (e.a == 120 || e.a == 123)) {
-            for (const auto& voice : active_)
-                emit(voice, e.frame);
-            active_.clear();
-            delayed_.clear();
-            down_.fill(false);
-            sostenutoKeys_.fill(false);
-            keysDown_ = 0;
-            return;
-        }
-        if (e.a >= 120)
-            throw std::runtime_error("unsupported MIDI channel mode CC" + std::to_string(e.a));
-        const bool sostenuto = natural() && e.a == 66;
-        if (!inst_.usedCC[size_t(e.a)] && !sostenuto)
-            return;
-        const auto before = cc_;
-        const float previous = before[size_t(e.a)];
-        cc_[size_t(e.a)] = float(e.b);
-        if (previous != cc_[size_t(e.a)])
-            updateDynamicEnvelopes(before, e.frame);
-        if (sostenuto)
-            setSostenuto(e.b == 127);
-        releasePedals(e.frame, e.a, previous);
-        const auto& triggers = inst_.byCC[size_t(e.a)];
-        if (triggers.empty())
-            return;
-        const float draw = std::generate_canonical<float, 24>(random_);
-        for (const int index : triggers) {
-            const auto& r = inst_.regions[size_t(index)];
-            const auto c = std::find_if(r.ccTriggers.begin(), r.ccTriggers.end(), [&](const Condition& x) { return x.cc == e.a; });
-            if (e.b < c->low || e.b > c->high || draw < r.randomLow || draw >= r.randomHigh)
-                continue;
-            const bool step = sequence_[size_t(index)]++ % unsigned(r.sequenceLength) == unsigned(r.sequencePosition - 1);
-            if (step && switchesMatch(r) && previous != float(e.b) && controllersMatch(r))
-                trigger(r, e.frame, r.rootKey, e.b, VoiceKind::controller, 0, e.a);
+        const bool sostenutoLifted = natural_ && cc == 66 && !sostenutoDown_;
+        for (auto& v : voices_) {
+            const Region& r = *v.region;
+            const bool sustainLifted = r.sustainSwitch && r.sustainCC == cc && before >= r.sustainLow;
+            if (v.keyReleased && (sustainLifted || sostenutoLifted) && !pedalDown(r) && !v.sostenuto)
+                release(v, frame);
         }
     }
 
@@ -1079,61 +685,94 @@ private:
         if (sostenutoDown_ == down)
             return;
         sostenutoDown_ = down;
-        sostenutoKeys_ = down ? down_ : std::array<bool, 128>{};
-        for (auto& voice : active_)
-            voice.sostenuto = down && !voice.noteOff;
-        if (!down)
-            for (auto& note : delayed_)
-                note.sostenuto = false;
+        for (auto& v : voices_)
+            v.sostenuto = down && !v.keyReleased;
     }
 
-    void releasePedals(int frame, int cc, float previous)
+    // ---------------------------------------------------- events
+
+    void noteOn(const Event& e)
     {
-        const bool sostenutoUp = natural() && cc == 66 && !sostenutoDown_;
-        for (auto& voice : active_) {
-            const auto& r = *voice.region;
-            const bool sustainChanged = r.sustainEnabled && r.sustainCC == cc && previous >= float(r.sustainLow);
-            if (voice.noteOff && (sustainChanged || sostenutoUp) && !pedal(r) && !voice.sostenuto)
-                release(voice, frame);
+        if (!keyDown_[size_t(e.a)])
+            ++keysHeld_;
+        keyStep_ = lastKey_ < 0 ? 0.f : float(e.a - lastKey_);
+        lastKey_ = e.a;
+        keyDown_[size_t(e.a)] = true;
+        for (int index : inst_.regionsOfKey[size_t(e.a)]) {
+            const Region& r = inst_.regions[size_t(index)];
+            if (selected(r, e.b))
+                startVoice(r, e.frame, e.a, e.b);
         }
-        std::vector<DelayedRelease> due;
-        auto keep = std::remove_if(delayed_.begin(), delayed_.end(), [&](const DelayedRelease& note) {
-            const auto& r = *note.region;
-            if ((r.sustainCC != cc && !sostenutoUp) || pedal(r) || note.sostenuto)
-                return false;
-            if (r.releaseDead || hasAttackVoice(r))
-                due.push_back(note);
-            return true;
-        });
-        delayed_.erase(keep, delayed_.end());
-        for (const auto& note : due)
-            trigger(*note.region, frame, note.key, note.velocity, VoiceKind::release, note.duration);
+    }
+
+    void noteOff(const Event& e)
+    {
+        if (!keyDown_[size_t(e.a)])
+            return;
+        keyDown_[size_t(e.a)] = false;
+        --keysHeld_;
+        for (auto& v : voices_)
+            if (v.key == e.a) {
+                v.keyReleased = true;
+                if (!pedalDown(*v.region) && !v.sostenuto)
+                    release(v, e.frame);
+            }
+    }
+
+    void controller(const Event& e)
+    {
+        if (e.a == 121) {  // reset all controllers
+            const float sostenutoBefore = controllers_[66];
+            controllers_ = inst_.ccDefault;
+            if (natural_) {
+                setSostenuto(false);
+                releaseUnheld(e.frame, 66, sostenutoBefore);
+            }
+            for (auto& v : voices_)
+                if (v.keyReleased && !pedalDown(*v.region))
+                    release(v, e.frame);
+            return;
+        }
+        if (!natural_ && e.a == 66 && e.b != 0)
+            throw std::runtime_error("unsupported MIDI sostenuto CC66");
+        if (natural_ && (e.a == 120 || e.a == 123)) {  // all sound / all notes off: everything stops
+            for (auto& v : voices_)
+                finish(v.span, e.frame);
+            voices_.clear();
+            keyDown_.fill(false);
+            keysHeld_ = 0;
+            return;
+        }
+        if (e.a >= 120)
+            throw std::runtime_error("unsupported MIDI channel mode CC" + std::to_string(e.a));
+        const bool sostenuto = natural_ && e.a == 66;
+        if (!inst_.ccUsed[size_t(e.a)] && !sostenuto)
+            return;
+        const float before = controllers_[size_t(e.a)];
+        controllers_[size_t(e.a)] = float(e.b);
+        if (sostenuto)
+            setSostenuto(e.b == 127);
+        releaseUnheld(e.frame, e.a, before);
     }
 
     const Instrument& inst_;
     AudioBank& bank_;
-    int rate_, limit_;
-    TailPolicy tail_;
-    std::array<float, 128> cc_;
-    std::array<bool, 128> down_{}, sostenutoKeys_{};
-    bool sostenutoDown_ = false;
-    std::array<int, 128> noteStart_{}, noteVelocity_{};
-    std::vector<unsigned> sequence_;
-    std::mt19937 random_, variation_;
-    std::mt19937 extendedRandom_{1};
-    std::vector<Voice> active_;
-    std::vector<DelayedRelease> delayed_;
-    int keysDown_ = 0;
-    int currentSwitch_;
-    Bend bend_;
+    const int rate_, limit_;
+    const bool natural_;
     const Midi* midi_ = nullptr;
-    bool hasBend_ = false, alternate_ = false;
-    int lastNote_ = -1, previousNote_ = -1;
-    float keyDelta_ = 0, randomValue_ = 0;
-    std::vector<LfoTrack> lfos_;
-    std::map<std::tuple<int, int, int>, int> tracks_;
+    bool anyBend_ = false;
+    // Channel.
+    std::array<float, 128> controllers_;
+    std::array<bool, 128> keyDown_{};
+    int keysHeld_ = 0, lastKey_ = -1;
+    float keyStep_ = 0;
+    bool sostenutoDown_ = false;
+    BendChange bend_;
+    std::mt19937 noise_;  // noise generator seeds
+    // Voices and output.
+    std::vector<Voice> voices_;
+    std::map<std::tuple<int, int, int>, int> bindingTracks_;
     std::map<std::tuple<float, float, int>, int> bendTracks_;
-    std::map<std::tuple<int, int, int, int, int, bool>, int> fadeTracks_;
     Plan plan_;
 };
 
@@ -1142,7 +781,7 @@ private:
 Plan planRender(const Instrument& instrument, AudioBank& bank, const Midi& midi, int rate, unsigned seed, int limit,
                 TailPolicy tail)
 {
-    return Planner(instrument, bank, rate, seed, limit, tail).render(midi);
+    return JobPlanner(instrument, bank, rate, seed, limit, tail).run(midi);
 }
 
 void makeView(const Plan& plan, PlanView& out)
@@ -1153,7 +792,7 @@ void makeView(const Plan& plan, PlanView& out)
         out.points.insert(out.points.end(), track.begin(), track.end());
         out.offsets.push_back(int32_t(out.points.size()));
     }
-    auto& v = out.view;
+    clay_plan_view& v = out.view;
     v = {};
     v.spans = plan.spans.data();
     v.span_count = int32_t(plan.spans.size());
