@@ -65,6 +65,7 @@ HIP_SYMBOLS = [
     "clay_hip_batch_destroy", "clay_hip_batch_voice_tails", "clay_mlp_init", "clay_hip_mlp_gradient",
     "clay_hip_mlp_train", "clay_hip_mlp_predict", "clay_hip_mlp_gradient_noalloc",
     "clay_hip_batch_prepare_bounded", "clay_hip_batch_overflows", "clay_hip_batch_scratch_bytes",
+    "clay_hip_set_memory_limit", "clay_hip_batch_memory",
 ]
 
 
@@ -105,6 +106,8 @@ def hip():
         _bind(lib, "clay_hip_batch_overflows", C.c_int, P, P, C.POINTER(C.c_int32), C.c_size_t,
               C.POINTER(C.c_size_t))
         _bind(lib, "clay_hip_batch_scratch_bytes", C.c_int, P, P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64))
+        _bind(lib, "clay_hip_set_memory_limit", C.c_int, P, C.c_uint64)
+        _bind(lib, "clay_hip_batch_memory", C.c_int, P, P, C.POINTER(C.c_int32), C.POINTER(C.c_int32))
         _hip = lib
     return _hip
 
@@ -199,6 +202,10 @@ class Renderer:
             self._check(self.lib.clay_hip_batch_prepare_bounded(
                 self.handle, plans, len(plans), ends.ctypes.data_as(C.POINTER(C.c_int32)), ends.size, C.byref(b)))
         return Batch(self, b, len(plans))
+
+    def set_memory_limit(self, bytes_):
+        """Cap on one batch arena's device bytes (0 = none); arenas over it fail as out of memory."""
+        self._check(self.lib.clay_hip_set_memory_limit(self.handle, int(bytes_)))
 
     def stats(self):
         s = abi.HipStats()
@@ -298,6 +305,12 @@ class Batch:
                                                           out.ctypes.data_as(C.POINTER(C.c_int32)), out.size,
                                                           C.byref(n)))
         return out[:n.value]
+
+    def memory(self):
+        """(render window frames, 0 = whole batch; filtered scratch in host memory) of the placed batch."""
+        w, host = C.c_int32(), C.c_int32()
+        self.r._check(self.r.lib.clay_hip_batch_memory(self.r.handle, self.handle, C.byref(w), C.byref(host)))
+        return w.value, bool(host.value)
 
     def scratch_bytes(self):
         """(filtered-voice scratch bytes, whole device arena bytes) of the batch."""

@@ -21,6 +21,7 @@
 #include <hip/hip_runtime.h>
 
 #include "device_types.h"
+#include "memory_policy.h"
 #include "staging.h"
 
 #include <algorithm>
@@ -237,6 +238,12 @@ struct clay_hip_batch {
     // Optional per-chain scratch bounds (absolute frames, batch chain order),
     // read by flatten; empty = every filtered span backed in full.
     std::vector<int32_t> scratchEnd;
+    // Plain form under memory pressure: frames per render window (0 = the
+    // whole batch at once), sizing the filtered and phase scratch; and that
+    // scratch in mapped host memory (a single job that does not fit otherwise).
+    int window = 0;
+    bool hostScratch = false;
+    void* hostScratchBuffer = nullptr;
     bool assess = false;
     bool allWet = false;  // every span renders through a voice chain (no dry voice in the mix)
     bool simpleMix = false;  // every dry voice is plain with no loop and no bend (the mix's Simple variant)
@@ -297,6 +304,7 @@ struct clay_hip {
     std::vector<Slot> slots;
     size_t residentBytes = 0;
     int tunedSegment = 0;  // persistent form: segment frames chosen by the round tuner, 0 = not yet
+    uint64_t memoryLimit = 0;  // bytes a batch arena may take (clay_hip_set_memory_limit), 0 = no limit
     // Round descriptors: a header and up to unitCapacity unit views on the
     // device, written through a ring of pinned staging buffers.
     static constexpr int kStages = 16;
@@ -824,6 +832,50 @@ void release(clay_hip* h, clay_hip_batch& b)
             (void)hipFree(b.arena);
     }
     b.arena = nullptr;
+    if (b.hostScratchBuffer) {
+        if (h)
+            (void)hipStreamSynchronize(h->stream);  // the kernels that used it have finished
+        (void)hipHostFree(b.hostScratchBuffer);
+    }
+    b.hostScratchBuffer = nullptr;
+}
+
+// Filtered spans of a batch as the memory policy sees them.
+std::vector<clay_memory::ScratchUse> scratchUses(const clay_hip_batch& b)
+{
+    std::vector<clay_memory::ScratchUse> uses;
+    for (const auto& v : b.spans)
+        if (v.filterTrack >= 0)
+            uses.push_back({v.begin, v.end, v.channels, v.variablePitch != 0, v.scratchEnd == INT32_MAX
+                                                                               ? INT32_MAX
+                                                                               : std::max(0, v.scratchEnd - v.begin)});
+    return uses;
+}
+
+// Scratch offsets of the filtered spans for render windows of `window`
+// frames (0: whole spans), in span order.
+void assignScratch(clay_hip_batch& b, int window)
+{
+    uint64_t floats = 0, doubles = 0;
+    for (auto& v : b.spans) {
+        if (v.filterTrack < 0)
+            continue;
+        int64_t frames = v.scratchEnd == INT32_MAX ? v.end - v.begin : std::max(0, std::min(v.end, v.scratchEnd) - v.begin);
+        int64_t phases = v.end - v.begin;
+        if (window > 0) {
+            frames = std::min<int64_t>(frames, window);
+            phases = std::min<int64_t>(phases, window);
+        }
+        v.filtered = floats;
+        floats += uint64_t(frames) * uint64_t(v.channels);
+        if (v.variablePitch) {
+            v.phase = doubles;
+            doubles += uint64_t(phases);
+        }
+    }
+    b.filteredSamples = floats;
+    b.pitchFrames = doubles;
+    b.window = window;
 }
 
 // Where each record array and buffer of a batch sits in its one device arena:
@@ -857,8 +909,8 @@ ArenaLayout layoutOf(const clay_hip* h, const clay_hip_batch& b)
     L.records = a.bytes;
     L.states = a.add(size_t(b.chains) * sizeof(VoiceState));
     L.mix = a.add(size_t(b.frames) * (h->natural ? sizeof(float2) : sizeof(short2)));
-    L.filtered = a.add(size_t(b.filteredSamples) * sizeof(float));
-    L.phase = a.add(size_t(b.pitchFrames) * sizeof(double));
+    L.filtered = a.add(b.hostScratch ? 0 : size_t(b.filteredSamples) * sizeof(float));
+    L.phase = a.add(b.hostScratch ? 0 : size_t(b.pitchFrames) * sizeof(double));
     L.mono = a.add(h->natural ? size_t(b.frames) * sizeof(int16_t) : 0);
     L.stereo = a.add(h->natural ? size_t(b.blocks) * sizeof(StereoStats) : 0);
     L.windows = a.add(size_t(b.windows) * sizeof(MonoStats));
@@ -905,6 +957,8 @@ void packRecords(const clay_hip_batch& b, const ArenaLayout& L, char* image, boo
 // The arena: stream-ordered (persistent form) or plain.
 void allocateArena(clay_hip* h, clay_hip_batch& b, const ArenaLayout& L, bool streamOrdered)
 {
+    if (h->memoryLimit && L.total > h->memoryLimit)
+        throw Failure(CLAY_E_MEMORY, "batch arena: over the renderer's memory limit");
     if (streamOrdered)
         check(hipMallocAsync(&b.arena, L.total, h->stream), "batch arena");
     else
@@ -936,6 +990,16 @@ void placeRecords(clay_hip* h, clay_hip_batch& b, const ArenaLayout& L, const ch
     b.dMix = base + L.mix;
     b.dFiltered = reinterpret_cast<float*>(base + L.filtered);
     b.dPhase = reinterpret_cast<double*>(base + L.phase);
+    if (b.hostScratch) {
+        // Mapped host memory the kernels reach over the bus (slow, but it fits).
+        const size_t floatBytes = (size_t(b.filteredSamples) * sizeof(float) + 255) & ~size_t(255);
+        const size_t bytes = std::max<size_t>(1, floatBytes + size_t(b.pitchFrames) * sizeof(double));
+        check(hipHostMalloc(&b.hostScratchBuffer, bytes, hipHostMallocMapped), "host scratch");
+        void* device = nullptr;
+        check(hipHostGetDevicePointer(&device, b.hostScratchBuffer, 0), "host scratch");
+        b.dFiltered = static_cast<float*>(device);
+        b.dPhase = reinterpret_cast<double*>(static_cast<char*>(device) + floatBytes);
+    }
     b.dMono = reinterpret_cast<int16_t*>(base + L.mono);
     b.dStereo = reinterpret_cast<StereoStats*>(base + L.stereo);
     b.dWindows = reinterpret_cast<MonoStats*>(base + L.windows);
@@ -964,6 +1028,57 @@ void upload(clay_hip* h, clay_hip_batch& b)
     placeRecords(h, b, L, image);
     h->staging.commit();
     check(hipStreamSynchronize(h->stream), "batch upload");
+}
+
+// A plain batch onto the device within the memory it can get
+// (renderer.cpp:864-989): render windows short enough for the filtered and
+// phase scratch to fit the scratch budget; after a failed allocation, halve
+// that scratch while it exceeds 2 MiB, then give up with CLAY_E_MEMORY so
+// that the caller halves the batch, and for a single job move the scratch to
+// mapped host memory before giving up.
+void placeWithinMemory(clay_hip* h, clay_hip_batch& b)
+{
+    const auto uses = scratchUses(b);
+    size_t freeBytes = 0, totalBytes = 0;
+    check(hipMemGetInfo(&freeBytes, &totalBytes), "hipMemGetInfo");
+    if (h->memoryLimit) {
+        freeBytes = std::min<size_t>(freeBytes, h->memoryLimit);
+        totalBytes = std::min<size_t>(totalBytes, h->memoryLimit);
+    }
+    const ArenaLayout whole = layoutOf(h, b);
+    const uint64_t fullScratch = clay_memory::scratchBytes(uses, 0);
+    const uint64_t fixed = whole.total > fullScratch ? whole.total - fullScratch : 0;
+    uint64_t budget = clay_memory::scratchBudget(freeBytes, totalBytes, fixed, UINT64_MAX);
+    int window = clay_memory::windowFor(uses, budget, b.lastFrame);
+    if (window)
+        assignScratch(b, window);
+    for (;;) {
+        try {
+            upload(h, b);
+            return;
+        } catch (const Failure& f) {
+            if (f.code != CLAY_E_MEMORY)
+                throw;
+            release(h, b);
+            const uint64_t scratch = clay_memory::scratchBytes(uses, b.window);
+            auto next = clay_memory::afterOutOfMemory(scratch, b.jobs.size(), b.hostScratch);
+            if (next == clay_memory::Next::halveScratch) {
+                budget = scratch / 2;
+                const int shorter = clay_memory::windowFor(uses, budget, b.lastFrame);
+                if (clay_memory::scratchBytes(uses, shorter) < scratch) {
+                    assignScratch(b, shorter);
+                    continue;
+                }
+                next = b.jobs.size() > 1 ? clay_memory::Next::halveJobs
+                                         : b.hostScratch ? clay_memory::Next::reject : clay_memory::Next::hostScratch;
+            }
+            if (next == clay_memory::Next::hostScratch) {
+                b.hostScratch = true;
+                continue;
+            }
+            throw;  // halve the batch, or reject its one job
+        }
+    }
 }
 
 constexpr int kMaxBlocksPerLaunch = 1 << 20;
@@ -1260,7 +1375,9 @@ void renderPlain(clay_hip* h, clay_hip_batch& b)
 {
     if (b.chains)
         check(hipMemsetAsync(b.dStates, 0, size_t(b.chains) * sizeof(VoiceState), h->stream), "state reset");
-    const int step = h->cfg.segment_frames;
+    int step = h->cfg.segment_frames;
+    if (b.window > 0)  // the scratch holds windows of b.window frames at most
+        step = step > 0 ? std::min(step, b.window) : b.window;
     if (step <= 0 || step >= b.lastFrame) {
         renderRound(h, Window{&b, 0, b.lastFrame, -1, -1});
         return;
@@ -1598,7 +1715,7 @@ int clay_hip_batch_prepare(clay_hip* h, const clay_plan_view* plans, size_t coun
             throw Failure(CLAY_E_INVALID, "empty batch");
         check(hipSetDevice(h->device), "hipSetDevice");
         flatten(h, plans, count, *b);
-        upload(h, *b);
+        placeWithinMemory(h, *b);
     });
     if (status != CLAY_OK) {
         release(h, *b);
@@ -1715,7 +1832,7 @@ int clay_hip_batch_prepare_bounded(clay_hip* h, const clay_plan_view* plans, siz
         check(hipSetDevice(h->device), "hipSetDevice");
         b->scratchEnd.assign(scratch_end, scratch_end + chains);
         flatten(h, plans, count, *b);
-        upload(h, *b);
+        placeWithinMemory(h, *b);
     });
     if (status != CLAY_OK) {
         release(h, *b);
@@ -1751,6 +1868,25 @@ int clay_hip_batch_overflows(clay_hip* h, clay_hip_batch* b, int32_t* jobs, size
         if (jobs && found > capacity)
             throw Failure(CLAY_E_INVALID, "overflow capacity too small");
     });
+}
+
+int clay_hip_set_memory_limit(clay_hip* h, uint64_t bytes)
+{
+    if (!h)
+        return CLAY_E_INVALID;
+    h->memoryLimit = bytes;
+    return CLAY_OK;
+}
+
+int clay_hip_batch_memory(clay_hip* h, clay_hip_batch* b, int32_t* window, int32_t* host_scratch)
+{
+    if (!h || !b)
+        return CLAY_E_INVALID;
+    if (window)
+        *window = b->window;
+    if (host_scratch)
+        *host_scratch = b->hostScratch ? 1 : 0;
+    return CLAY_OK;
 }
 
 int clay_hip_batch_scratch_bytes(clay_hip* h, clay_hip_batch* b, uint64_t* filtered, uint64_t* arena)
@@ -1845,7 +1981,7 @@ int renderWindowPlain(clay_hip* h, const clay_plan_view* plans, size_t count, cl
                 ahead.b = nullptr;
                 status = guarded(h, [&] {
                     check(hipSetDevice(h->device), "hipSetDevice");
-                    upload(h, *b);
+                    placeWithinMemory(h, *b);
                 });
                 if (status != CLAY_OK) {
                     release(h, *b);

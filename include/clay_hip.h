@@ -371,6 +371,16 @@ int clay_hip_batch_overflows(clay_hip* h, clay_hip_batch* b, int32_t* jobs, size
 /* Bytes of the batch's filtered-voice scratch and of its whole device arena. */
 int clay_hip_batch_scratch_bytes(clay_hip* h, clay_hip_batch* b, uint64_t* filtered, uint64_t* arena);
 
+/* The most device memory one batch arena may take (0 = no limit): a cap for
+ * sharing the GPU, and how tests drive each step of the allocation-failure
+ * chain (src/cuda/renderer.cpp:948-989) without exhausting the device. An
+ * arena over the limit fails as an out-of-memory allocation would. */
+int clay_hip_set_memory_limit(clay_hip* h, uint64_t bytes);
+/* How a prepared plain batch was placed: the frames per render window its
+ * scratch is sized for (0 = the whole batch in one window), and whether its
+ * filtered-voice scratch lives in mapped host memory. */
+int clay_hip_batch_memory(clay_hip* h, clay_hip_batch* b, int32_t* window, int32_t* host_scratch);
+
 #ifdef __cplusplus
 }
 #endif
