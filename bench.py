@@ -11,7 +11,7 @@ polyphony 4-64), 16-tap sinc, natural tail policy, 44.1 kHz. A step renders
 all 512 jobs resident on the device (one batch, plain form). The same line
 reports the render windows of the plain and persistent forms at
 --gpu-batch-jobs 128 (host transfers included) and whether their PCM16 is
-identical, the roofline of the resample-mix kernel (clayMixWaves) on the
+identical, the roofline of the resample-mix kernel (clayMixStraight for batches of straight voices) on the
 algorithmic bytes of its source windows, mix writes and span records, and
 the FP64 oracle on the host cores as the CPU baseline. Prints one JSON line.
 """
@@ -61,7 +61,7 @@ def kernel_bytes(plans, taps, natural):
     wet_samples = 0
     spans = 0
     frames = 0
-    span_bytes = 216  # device span record (the hot part the mix reads per voice)
+    span_bytes = 224  # device span record (the hot part the mix reads per voice)
     for p in plans:
         frames += p.frames
         spans += p.span_count
@@ -88,6 +88,8 @@ def kernel_bytes(plans, taps, natural):
         # dry source windows + wet reads + the mix write + span records
         "clayMixTiles": src["dry"] + 4 * wet_samples + mix_bytes + spans * span_bytes,
         "clayMixWaves": src["dry"] + 4 * wet_samples + mix_bytes + spans * span_bytes,
+        # straight batches: dry source windows + the mix write + span records
+        "clayMixStraight": src["dry"] + mix_bytes + spans * span_bytes,
     }, voice_frames, frames
 
 
@@ -525,14 +527,15 @@ def main():
     os.environ["CLAY_KERNEL_TIMING"] = "0"
     bytes_per, voice_frames, frames = kernel_bytes(plans, taps, True)
     kernels = {}
-    for name in ("clayChainAudio", "clayVoiceChains", "clayMixTiles", "clayMixWaves", "clayScanStereo", "clayQuantizeMono", "clayFinalizePcm",
-                 "clayPitchChains", "claySelectNative", "claySelectTrim"):
+    for name in ("clayChainAudio", "clayVoiceChains", "clayMixTiles", "clayMixWaves", "clayMixStraight",
+                 "clayScanStereo", "clayQuantizeMono", "clayFinalizePcm", "clayPitchChains", "claySelectNative",
+                 "claySelectTrim"):
         ms, n = batch.kernel_time(name)
         if n:
             kernels[name] = {"ms": ms, "launches": n}
     # The roofline goes on the resample-mix kernel where it renders the voices
     # (configurations 1 and 3), else on the kernel that takes longest.
-    mix_kernel = "clayMixWaves" if "clayMixWaves" in kernels else None
+    mix_kernel = next((k for k in ("clayMixStraight", "clayMixWaves") if k in kernels), None)
     dominant = mix_kernel or max((k for k in kernels if k in bytes_per), key=lambda k: kernels[k]["ms"])
     avg_s = kernels[dominant]["ms"] / kernels[dominant]["launches"] / 1e3
     achieved = bytes_per[dominant] / kernels[dominant]["launches"] / avg_s / 1e9

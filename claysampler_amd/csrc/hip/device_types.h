@@ -20,6 +20,7 @@ constexpr int kPhases = CLAY_SINC_PHASES;
 constexpr int kTailBlock = 1024;   // natural end / finalize block (src/cuda/render_types.h:92)
 constexpr int kFinalizeBlocks = 8;  // tail blocks per wave of clayFinalizePcm
 constexpr int kTileFrames = 256;   // output frames per mix tile = threads per mix block
+constexpr int kStraightFrames = 1024;  // output frames per tile of a straight batch (clayMixStraight)
 
 // Hot span record: everything the per-frame voice audio needs.
 struct Span {
@@ -47,6 +48,10 @@ struct Span {
     // predicted tail of the chain; INT32_MAX = the whole span). A chain that
     // reaches it without retiring is stopped there and flagged (overflow).
     int32_t scratchEnd;
+    // 1: a plain voice with no loop and no bend at positions in [0, 2^30) that
+    // never fall (straight: the mix's integer FIR at exact positions).
+    int32_t straight;
+    int32_t pad;
 };
 
 // Cold span record: filter slots and the filter/pitch envelopes.
@@ -123,19 +128,24 @@ struct UnitView {
     float* filtered;
     VoiceState* states;
     void* mix;
+    // A straight unit (every voice straight) has 1024-frame tiles and no
+    // 256-frame ones; any other unit the reverse.
+    const Tile* straightTiles;
+    const int* straightVoices;
     int32_t segFirst, segEnd;           // frame window of this round
     int32_t tileFirst, tileCount;       // tiles overlapping the window
+    int32_t straightFirst, straightCount;
     int32_t chainCount, pitchCount;
     // Exclusive prefix sums over the round's units (work item -> unit search).
-    int32_t tileBase, chainBase, pitchBase;
-    int32_t pad;
+    int32_t tileBase, straightBase, chainBase, pitchBase;
 };
 
 // Totals of one round, read by the grid-stride loops of the render kernels.
 struct RoundHeader {
     int32_t units, tiles, chains, pitches;
     float silence;                      // voice-silence threshold (linear), 0 = off
-    int32_t pad[3];
+    int32_t straightTiles;
+    int32_t pad[2];
 };
 
 struct StereoStats {

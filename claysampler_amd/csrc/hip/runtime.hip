@@ -60,12 +60,21 @@ template<int T>
 __global__ void clayRoundChainAudio(const int16_t*, const float*, const UnitView*, const RoundHeader*);
 __global__ void clayRoundVoiceChains(const UnitView*, const RoundHeader*);
 // Mix (mix.hip): one wave per tile, integer FIR for plain voices; plain and persistent forms.
+// The bank and, for the straight voices' aligned window loads, the byte
+// offset of its shifted copy minus 2 (clay_hip::bankOdd).
 template<int T, bool FloatOut, bool Simple>
-__global__ void clayMixWaves(const int16_t*, const uint32_t*, const float*, const Span*, const Job*, const Tile*,
-                             const int*, const clay_control_point*, const Mod*, const float*, const VoiceState*, void*,
-                             int, int, int, int);
+__global__ void clayMixWaves(const int16_t*, uint32_t, const uint32_t*, const float*, const Span*, const Job*,
+                             const Tile*, const int*, const clay_control_point*, const Mod*, const float*,
+                             const VoiceState*, void*, int, int, int, int);
 template<int T, bool FloatOut, bool Simple>
-__global__ void clayRoundMixWaves(const int16_t*, const uint32_t*, const float*, const UnitView*, const RoundHeader*);
+__global__ void clayRoundMixWaves(const int16_t*, uint32_t, const uint32_t*, const float*, const UnitView*,
+                                  const RoundHeader*);
+// Mix of straight batches on kStraightFrames tiles (mix.hip).
+template<int T, bool FloatOut>
+__global__ void clayMixStraight(const int16_t*, uint32_t, const uint32_t*, const Span*, const Job*, const Tile*,
+                                const int*, const clay_control_point*, void*, int, int, int, int);
+template<int T, bool FloatOut>
+__global__ void clayRoundMixStraight(const int16_t*, uint32_t, const uint32_t*, const UnitView*, const RoundHeader*);
 // Finishing kernels (finish.hip).
 extern "C" __global__ void clayScanStereo(const Job*, int, const float2*, StereoStats*, int);
 extern "C" __global__ void claySelectNative(const Job*, int, const float2*, const StereoStats*, clay_tail_result*);
@@ -247,6 +256,9 @@ struct clay_hip_batch {
     bool assess = false;
     bool allWet = false;  // every span renders through a voice chain (no dry voice in the mix)
     bool simpleMix = false;  // every dry voice is plain with no loop and no bend (the mix's Simple variant)
+    // Every voice is straight (dry, plain, no loop, no bend, position and step
+    // >= 0): `tiles` are then kStraightFrames long and mixed by clayMixStraight.
+    bool straight = false;
     // Device arena and views into it.
     void* arena = nullptr;
     size_t arenaBytes = 0;
@@ -285,8 +297,15 @@ struct clay_hip {
     hipDeviceProp_t props{};
     float* sinc = nullptr;       // phase-major taps x (phases+1)
     uint32_t* sincInt = nullptr; // the same weights as packed int16 hi/lo pairs (intSincTable)
+    // The sample bank, and one allocation-stride after it the same samples one
+    // place later (copy[i] = bank[i + 1]): a window that starts on an odd
+    // sample is a dword-aligned run of the copy. bankOdd = stride - 2, the byte
+    // offset from bank sample j to copy sample j - 1 less the 2 bytes the
+    // window offset 2 j already counts.
     int16_t* bank = nullptr;
     uint64_t bankCount = 0, bankCapacity = 0;
+    uint32_t bankOdd = 0;
+    uint64_t bankShifted = 0;  // samples of the bank the shifted copy reflects
     std::string error;
     clay_hip_stats stats{};
     bool natural = true;
@@ -322,7 +341,8 @@ struct clay_hip {
         hipGraphExec_t exec;
         int mixBlocks, chainBlocks;
         bool simpleMix;
-        const void* bank;
+        const void* bank;  // captured launch arguments: the bank and its copy's offset
+        uint32_t bankOdd;
     };
     std::vector<Graph> graphs;
     // Pinned image of a plain batch's records (grown on demand): the copy runs
@@ -491,6 +511,21 @@ void parallelFor(size_t count, F&& f)
         thread.join();
 }
 
+// A span the straight mix renders (clayMixStraight, and the straight path of
+// clayMixWaves): dry, mono, sampled, no variable pitch, stereo matrix or
+// modulation of its audio, no loop and no bend, at positions from 0 to 2^30
+// that never fall with the frame.
+bool straightSpan(const clay_plan_view& p, const clay_voice_span& v)
+{
+    if (v.filter_track >= 0 || v.channels != 1 || v.generator != CLAY_GEN_NONE || v.variable_pitch ||
+        v.stereo_matrix || v.loop_end > 0 || v.pitch_slope != 0)
+        return false;
+    for (int k = 0; k < v.modulation_count; ++k)
+        if (p.modulation[v.first_modulation + k].target <= CLAY_MOD_POSITION)
+            return false;
+    return v.position >= 0 && v.step >= 0 && v.position + double(v.end - v.begin) * v.step < 0x1p30;
+}
+
 // Flatten plans into device records (renderer.cpp:378-462 semantics). Three
 // passes: per plan (parallel) validation and the sizes that depend on its
 // spans; record offsets by prefix sums (serial); per plan (parallel) the
@@ -515,7 +550,8 @@ void flatten(clay_hip* h, const clay_plan_view* plans, size_t count, clay_hip_ba
     };
     // 1. Validation and span-dependent sizes.
     struct Sizes {
-        int voices = 0;
+        int voices = 0, straightVoices = 0;
+        bool straight = true;
         uint64_t filtered = 0, pitch = 0;
         std::string error;
     };
@@ -547,6 +583,8 @@ void flatten(clay_hip* h, const clay_plan_view* plans, size_t count, clay_hip_ba
         for (int s = 0; s < p.span_count; ++s) {
             const auto& v = p.spans[s];
             z.voices += (v.end - 1) / kTileFrames - v.begin / kTileFrames + 1;
+            z.straightVoices += (v.end - 1) / kStraightFrames - v.begin / kStraightFrames + 1;
+            z.straight = z.straight && straightSpan(p, v);
             if (v.filter_track >= 0) {
                 z.filtered += uint64_t(scratchFrames(i, v)) * uint64_t(v.channels);
                 if (v.variable_pitch)
@@ -554,9 +592,15 @@ void flatten(clay_hip* h, const clay_plan_view* plans, size_t count, clay_hip_ba
             }
         }
     });
-    for (const auto& z : sizes)
+    b.straight = false;
+    for (const auto& z : sizes) {
         if (!z.error.empty())
             throw Failure(CLAY_E_INVALID, z.error);
+        b.straight = b.straight || z.voices > 0;
+    }
+    for (const auto& z : sizes)
+        b.straight = b.straight && z.straight;
+    const int tileFrames = b.straight ? kStraightFrames : kTileFrames;
     // 2. Offsets of each plan's records.
     struct Base {
         int point, mod, tile, voice, span, chain, onset, block, window, body;
@@ -569,8 +613,8 @@ void flatten(clay_hip* h, const clay_plan_view* plans, size_t count, clay_hip_ba
         bases[i] = at;
         at.point += p.track_offsets[p.track_count];
         at.mod += p.modulation_count;
-        at.tile += int(ceilDiv(uint64_t(p.frames), kTileFrames));
-        at.voice += sizes[i].voices;
+        at.tile += int(ceilDiv(uint64_t(p.frames), uint64_t(tileFrames)));
+        at.voice += b.straight ? sizes[i].straightVoices : sizes[i].voices;
         at.span += p.span_count;
         at.chain += p.filter_tracks;
         at.onset += assess && h->cfg.audio.enabled ? p.onset_count : 0;
@@ -653,19 +697,19 @@ void flatten(clay_hip* h, const clay_plan_view* plans, size_t count, clay_hip_ba
                     std::copy(p.onsets, p.onsets + p.onset_count, b.onsets.begin() + o.onset);
             }
             // Tiles and their voice lists (difference array over tile indices).
-            const int tileCount = int(ceilDiv(uint64_t(p.frames), kTileFrames));
+            const int tileCount = int(ceilDiv(uint64_t(p.frames), uint64_t(tileFrames)));
             std::vector<int> perTile(size_t(tileCount) + 1, 0);
             for (int s = 0; s < p.span_count; ++s) {
-                ++perTile[size_t(p.spans[s].begin / kTileFrames)];
-                --perTile[size_t((p.spans[s].end - 1) / kTileFrames + 1)];
+                ++perTile[size_t(p.spans[s].begin / tileFrames)];
+                --perTile[size_t((p.spans[s].end - 1) / tileFrames + 1)];
             }
             for (int t = 1; t <= tileCount; ++t)
                 perTile[size_t(t)] += perTile[size_t(t - 1)];
             std::vector<int> cursor(static_cast<size_t>(tileCount), 0);
             int voiceAt = o.voice;
             for (int t = 0; t < tileCount; ++t) {
-                const int frame = t * kTileFrames;
-                b.tiles[size_t(o.tile + t)] = {jobIndex, frame, std::min(kTileFrames, p.frames - frame), voiceAt,
+                const int frame = t * tileFrames;
+                b.tiles[size_t(o.tile + t)] = {jobIndex, frame, std::min(tileFrames, p.frames - frame), voiceAt,
                                                perTile[size_t(t)]};
                 cursor[size_t(t)] = voiceAt;
                 voiceAt += perTile[size_t(t)];
@@ -676,7 +720,7 @@ void flatten(clay_hip* h, const clay_plan_view* plans, size_t count, clay_hip_ba
             for (int s = 0; s < p.span_count; ++s) {
                 const auto& v = p.spans[s];
                 const int index = o.span + s;
-                for (int t = v.begin / kTileFrames; t <= (v.end - 1) / kTileFrames; ++t)
+                for (int t = v.begin / tileFrames; t <= (v.end - 1) / tileFrames; ++t)
                     b.voices[size_t(cursor[size_t(t)]++)] = index;
                 Span d{};
                 d.sample = v.sample;
@@ -729,6 +773,7 @@ void flatten(clay_hip* h, const clay_plan_view* plans, size_t count, clay_hip_ba
                     audioMods = audioMods || p.modulation[v.first_modulation + k].target <= CLAY_MOD_POSITION;
                 d.plain = v.filter_track < 0 && v.channels == 1 && v.generator == CLAY_GEN_NONE && !v.variable_pitch &&
                           !v.stereo_matrix && !audioMods;
+                d.straight = straightSpan(p, v);
                 SpanCold c{};
                 std::memcpy(c.filters, v.filters, sizeof c.filters);
                 std::memcpy(c.extra, v.extra_envelope, sizeof c.extra);
@@ -737,7 +782,7 @@ void flatten(clay_hip* h, const clay_plan_view* plans, size_t count, clay_hip_ba
                 c.sampleRate = v.sample_rate;
                 if (v.filter_track < 0)
                     dry[i] = 1;
-                if (v.filter_track < 0 && (!d.plain || v.loop_end > 0 || v.pitch_slope != 0))
+                if (v.filter_track < 0 && !d.straight)
                     complexDry[i] = 1;
                 if (v.filter_track >= 0) {
                     int& tail = chainTail[size_t(v.filter_track)];
@@ -1013,8 +1058,9 @@ void placeRecords(clay_hip* h, clay_hip_batch& b, const ArenaLayout& L, const ch
     if (b.chains)
         check(hipMemsetAsync(b.dStates, 0, size_t(b.chains) * sizeof(VoiceState), h->stream), "state reset");
     h->residentBytes = L.total;
-    h->stats.peak_bytes = std::max<uint64_t>(h->stats.peak_bytes, L.total + h->bankCapacity * 2 +
-                                                                      size_t(h->cfg.sinc_taps) * (kPhases + 1) * 4);
+    // The bank and its shifted copy, the float and integer sinc tables.
+    h->stats.peak_bytes = std::max<uint64_t>(h->stats.peak_bytes, L.total + 2 * (uint64_t(h->bankOdd) + 2) +
+                                                                      size_t(h->cfg.sinc_taps) * (kPhases + 1) * 8);
 }
 
 // Plain form: the records through the renderer's one pinned image, grown as
@@ -1130,13 +1176,25 @@ UnitView viewOf(const Window& w)
     v.mix = b.dMix;
     v.segFirst = w.first;
     v.segEnd = w.end;
-    // Window bounds are multiples of the tile size (or the batch end).
-    v.tileFirst = int(std::lower_bound(b.tiles.begin(), b.tiles.end(), w.first,
-                                       [](const Tile& t, int f) { return t.frame < f; }) -
-                      b.tiles.begin());
-    v.tileCount = int(std::lower_bound(b.tiles.begin() + v.tileFirst, b.tiles.end(), w.end,
-                                       [](const Tile& t, int f) { return t.frame < f; }) -
-                      b.tiles.begin()) - v.tileFirst;
+    // Tiles overlapping the window: window bounds are multiples of 256 frames
+    // (or the batch end), straight tiles start on multiples of 1024.
+    const int tileFrames = b.straight ? kStraightFrames : kTileFrames;
+    const int from = w.first / tileFrames * tileFrames;
+    const int first = int(std::lower_bound(b.tiles.begin(), b.tiles.end(), from,
+                                           [](const Tile& t, int f) { return t.frame < f; }) -
+                          b.tiles.begin());
+    const int count = int(std::lower_bound(b.tiles.begin() + first, b.tiles.end(), w.end,
+                                           [](const Tile& t, int f) { return t.frame < f; }) -
+                          b.tiles.begin()) - first;
+    if (b.straight) {
+        v.straightTiles = b.dTiles;
+        v.straightVoices = b.dVoices;
+        v.straightFirst = first;
+        v.straightCount = count;
+    } else {
+        v.tileFirst = first;
+        v.tileCount = count;
+    }
     if (w.chainLo < 0) {
         v.chainHeads = b.dChainHeads;
         v.chainCount = int(b.chainHeads.size());
@@ -1158,6 +1216,7 @@ constexpr int kChainWavesPerBlock = 2;   // render.hip: voice chains per 128-thr
 constexpr int kAudioWavesPerBlock = 16;  // render.hip: voice chains per 1024-thread audio block
 constexpr int kSharedTapsMax = 32;       // render.hip: widest sinc table the audio pass keeps in LDS
 constexpr int kMixWaves = 16;            // mix.hip: tiles (one per wave) in flight per 1024-thread block
+constexpr int kStraightWaves = 12;       // mix.hip: straight tiles in flight per 768-thread block
 constexpr int kXcds = 8;                 // mix.hip: XCDs the mix blocks are spread over
 
 // Grid of the wave mix for `tiles` tiles: one 1024-thread block per CU at
@@ -1166,6 +1225,15 @@ int mixGrid(clay_hip* h, int tiles)
 {
     const int cus = std::max(kXcds, h->props.multiProcessorCount / kXcds * kXcds);
     const int wanted = int(std::min<int64_t>(cus, (int64_t(tiles) + kMixWaves - 1) / kMixWaves));
+    return std::max(kXcds, (wanted + kXcds - 1) / kXcds * kXcds);
+}
+
+// Grid of the straight mix: one 768-thread block per CU at most (each holds
+// the 160 KB weight table in LDS), a multiple of the XCD count.
+int straightGrid(clay_hip* h, int tiles)
+{
+    const int cus = std::max(kXcds, h->props.multiProcessorCount / kXcds * kXcds);
+    const int wanted = int(std::min<int64_t>(cus, (int64_t(tiles) + kStraightWaves - 1) / kStraightWaves));
     return std::max(kXcds, (wanted + kXcds - 1) / kXcds * kXcds);
 }
 
@@ -1199,13 +1267,21 @@ void launchRound(clay_hip* h, const Shape& shape, bool timed)
     run("clayRoundMixWaves", [&] {
         const int blocks = mixGrid(h, 1 << 30);
         const auto mix = [&](auto kernel) {
-            hipLaunchKernelGGL(kernel, dim3(blocks), dim3(64 * kMixWaves), 0, s, h->bank, h->sincInt, h->sinc, units,
-                               round);
+            hipLaunchKernelGGL(kernel, dim3(blocks), dim3(64 * kMixWaves), 0, s, h->bank, h->bankOdd, h->sincInt,
+                               h->sinc, units, round);
         };
         if (shape.simpleMix)
             h->natural ? mix(clayRoundMixWaves<T, true, true>) : mix(clayRoundMixWaves<T, false, true>);
         else
             h->natural ? mix(clayRoundMixWaves<T, true, false>) : mix(clayRoundMixWaves<T, false, false>);
+    });
+    run("clayRoundMixStraight", [&] {
+        const int blocks = straightGrid(h, 1 << 30);
+        const auto mix = [&](auto kernel) {
+            hipLaunchKernelGGL(kernel, dim3(blocks), dim3(64 * kStraightWaves), 0, s, h->bank, h->bankOdd,
+                               h->sincInt, units, round);
+        };
+        h->natural ? mix(clayRoundMixStraight<T, true>) : mix(clayRoundMixStraight<T, false>);
     });
 }
 
@@ -1252,11 +1328,24 @@ void launchWindow(clay_hip* h, const Window& w)
         });
         return;
     }
+    if (b.straight) {
+        if (v.straightCount)
+            h->launch("clayMixStraight", [&] {
+                const int blocks = straightGrid(h, v.straightCount);
+                const auto mix = [&](auto kernel) {
+                    hipLaunchKernelGGL(kernel, dim3(blocks), dim3(64 * kStraightWaves), 0, s, h->bank, h->bankOdd,
+                                       h->sincInt, b.dSpans, b.dJobs, b.dTiles, b.dVoices, b.dPoints, b.dMix, w.first, w.end,
+                                       v.straightFirst, v.straightCount);
+                };
+                h->natural ? mix(clayMixStraight<T, true>) : mix(clayMixStraight<T, false>);
+            });
+        return;
+    }
     if (v.tileCount)
         h->launch("clayMixWaves", [&] {
             const int blocks = mixGrid(h, v.tileCount);
             const auto mix = [&](auto kernel) {
-                hipLaunchKernelGGL(kernel, dim3(blocks), dim3(64 * kMixWaves), 0, s, h->bank, h->sincInt, h->sinc,
+                hipLaunchKernelGGL(kernel, dim3(blocks), dim3(64 * kMixWaves), 0, s, h->bank, h->bankOdd, h->sincInt, h->sinc,
                                    b.dSpans, b.dJobs, b.dTiles, b.dVoices, b.dPoints, b.dMods, b.dFiltered, b.dStates,
                                    b.dMix, w.first, w.end, v.tileFirst, v.tileCount);
             };
@@ -1286,14 +1375,16 @@ RoundHeader stageRound(clay_hip* h, const std::vector<Window>& windows)
     h->stageNext = (h->stageNext + 1) % clay_hip::kStages;
     if (stage.used)
         check(hipEventSynchronize(stage.copied), "round staging");
-    RoundHeader header{int(windows.size()), 0, 0, 0, h->silenceLevel, {0, 0, 0}};
+    RoundHeader header{int(windows.size()), 0, 0, 0, h->silenceLevel, 0, {0, 0}};
     auto* views = reinterpret_cast<UnitView*>(static_cast<char*>(stage.host) + 256);
     for (size_t i = 0; i < windows.size(); ++i) {
         UnitView v = viewOf(windows[i]);
         v.tileBase = header.tiles;
+        v.straightBase = header.straightTiles;
         v.chainBase = header.chains;
         v.pitchBase = header.pitches;
         header.tiles += v.tileCount;
+        header.straightTiles += v.straightCount;
         header.chains += v.chainCount;
         header.pitches += v.pitchCount;
         views[i] = v;
@@ -1326,7 +1417,7 @@ const clay_hip::Graph& roundGraph(clay_hip* h, const Shape& shape)
     const clay_hip::Graph* found = nullptr;
     for (const auto& g : h->graphs)
         if (g.mixBlocks == shape.mixBlocks && g.chainBlocks == shape.chainBlocks && g.simpleMix == shape.simpleMix &&
-            g.bank == h->bank)
+            g.bank == h->bank && g.bankOdd == h->bankOdd)
             found = &g;
     if (!found) {
         hipGraph_t graph = nullptr;
@@ -1344,7 +1435,7 @@ const clay_hip::Graph& roundGraph(clay_hip* h, const Shape& shape)
         const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
         (void)hipGraphDestroy(graph);
         check(e, "hipGraphInstantiate");
-        h->graphs.push_back({exec, shape.mixBlocks, shape.chainBlocks, shape.simpleMix, h->bank});
+        h->graphs.push_back({exec, shape.mixBlocks, shape.chainBlocks, shape.simpleMix, h->bank, h->bankOdd});
         found = &h->graphs.back();
     }
     return *found;
@@ -1680,27 +1771,44 @@ int clay_hip_upload_bank(clay_hip* h, const int16_t* pcm, uint64_t count, int fr
             return;
         if (count && !pcm)
             throw Failure(CLAY_E_INVALID, "null sample bank");
+        constexpr uint64_t kPad = 2 * CLAY_MAX_SINC_TAPS;  // samples past the capacity (see below)
         if (count > h->bankCapacity || !h->bank) {
             // Append-only growth with slack (renderer.cpp:553-603).
             const uint64_t slack = std::min<uint64_t>(count / 2, 64ull << 20);
             const uint64_t capacity = std::max<uint64_t>(count + slack, 1);
+            // Padding: a window clamped into a source shorter than the taps reads
+            // up to taps samples from the source's start. The bank and its shifted
+            // copy share one allocation, `stride` bytes apart; the mix addresses
+            // the copy with 32-bit offsets from the bank.
+            const uint64_t stride = ((capacity + kPad) * sizeof(int16_t) + 255) & ~uint64_t(255);
+            if (stride > 0xF0000000ull)
+                throw Failure(CLAY_E_MEMORY, "sample bank over 3.75 GB (the mix addresses it with 32-bit offsets)");
             int16_t* grown = nullptr;
-            // Padding: the FIR's dword-aligned window loads may read one dword past
-            // the last sample, and the mix clamps the window of a source shorter
-            // than the taps to the source's start (up to taps + 2 samples read).
-            check(hipMalloc(&grown, (capacity + 2 * CLAY_MAX_SINC_TAPS) * sizeof(int16_t)), "sample bank");
+            check(hipMalloc(&grown, 2 * stride), "sample bank");
             if (h->bankCount)
                 check(hipMemcpyAsync(grown, h->bank, h->bankCount * sizeof(int16_t), hipMemcpyDeviceToDevice, h->stream),
                       "sample bank growth");
             check(hipStreamSynchronize(h->stream), "sample bank growth");
             if (h->bank)
                 check(hipFree(h->bank), "hipFree");
+            // Captured rounds hold the old bank's address and offset.
+            for (auto& g : h->graphs)
+                (void)hipGraphExecDestroy(g.exec);
+            h->graphs.clear();
             h->bank = grown;
             h->bankCapacity = capacity;
+            h->bankOdd = uint32_t(stride - 2);
+            h->bankShifted = 0;  // the copy holds nothing yet
         }
         if (count > h->bankCount)
             check(hipMemcpy(h->bank + h->bankCount, pcm + h->bankCount, (count - h->bankCount) * sizeof(int16_t),
                             hipMemcpyHostToDevice), "sample upload");
+        // The shifted copy from the first sample it lacks: copy[i] = bank[i + 1].
+        const uint64_t first = h->bankShifted ? h->bankShifted - 1 : 0;
+        char* copy = reinterpret_cast<char*>(h->bank) + (uint64_t(h->bankOdd) + 2);
+        check(hipMemcpy(copy + 2 * first, h->bank + first + 1, (count + kPad - 1 - first) * sizeof(int16_t),
+                        hipMemcpyDeviceToDevice), "shifted sample copy");
+        h->bankShifted = count;
         h->bankCount = count;
     });
 }
