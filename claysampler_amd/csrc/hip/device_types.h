@@ -86,6 +86,15 @@ struct Tile {
     int32_t job, frame, frames, first, count;
 };
 
+// Tile claims of one mix launch (mix.h, TileClaims): a counter per XCD and
+// the waves that have finished, cleared by the launch's last wave.
+constexpr int kXcds = 8;  // MI355X: 8 XCDs, workgroups dispatched round-robin over them
+struct MixQueue {
+    uint32_t next[kXcds];
+    uint32_t done;
+    uint32_t pad[15 - kXcds];
+};
+
 // Filter and pitch state carried across segments (src/cuda/render_types.h:22-28).
 // The recurrences run in FP64: a float32 biquad with a 200 Hz cutoff at
 // 44.1 kHz (poles near z = 1) amplifies its own rounding by ~1/(1-r)^2, and a

@@ -31,7 +31,6 @@ constexpr int kMixWaves = 16;                  // tiles in flight per 1024-threa
 constexpr int kStraightWaves = 12;             // tiles in flight per 768-thread straight block (168 VGPRs)
 constexpr int kLaneFrames = kTileFrames / 64;  // rows of a 256-frame tile
 constexpr int kStraightRows = kStraightFrames / 64;
-constexpr int kXcds = 8;                       // MI355X: 8 XCDs, blocks dispatched round-robin over them
 constexpr int kIntSharedTaps = 16;             // widest integer table held in LDS
 constexpr int kRowQuads = 5;                   // LDS row stride: 4 quads of weights and one pad quad (80 B)
 
@@ -861,22 +860,49 @@ __device__ __forceinline__ void withIntTaps(const uint32_t* __restrict__ table, 
     }
 }
 
-// Tile n of this wave's XCD share: tiles go to XCDs in chunks of `Chunk`
-// consecutive tiles, round-robin, so neighbouring tiles (tiles are sorted by
-// frame: the same frames of consecutive jobs) meet in one L2.
+// Work queue of one mix launch: the waves claim tiles one at a time, so a
+// launch ends when its work does, not when the wave dealt the costliest tiles
+// finishes (tiles differ by their voice counts). Tiles go to XCDs in chunks of
+// `Chunk` consecutive tiles, round-robin (tiles are sorted by frame: the same
+// frames of consecutive jobs), and a block's waves claim from the counter of
+// the XCD the block runs on (blocks are dispatched to the 8 XCDs in turn), so
+// neighbouring tiles meet in one L2; a wave whose XCD has no tiles left takes
+// them from the next XCD's counter. The last wave of the launch to finish
+// clears the counters for the next launch on the stream (the runtime clears
+// them once, at creation). The queue's layout is MixQueue (device_types.h).
 template<int Chunk, int Waves>
-struct XcdWalk {
-    int xcd, wave, waves;
-    __device__ XcdWalk(int waveInBlock)
+struct TileClaims {
+    MixQueue* queue;
+    int xcd, lane, from = 0;
+    __device__ TileClaims(MixQueue* q, int laneIn) : queue(q), xcd(int(blockIdx.x) % kXcds), lane(laneIn) {}
+    __device__ __forceinline__ static int tileOf(int x, unsigned n)
     {
-        const int perXcd = max(1, int(gridDim.x) / kXcds);
-        xcd = int(blockIdx.x) % kXcds;
-        wave = (int(blockIdx.x) / kXcds) * Waves + waveInBlock;
-        waves = perXcd * Waves;
-        if (int(blockIdx.x) >= perXcd * kXcds)
-            wave = -1;  // a block past the last full group of kXcds renders nothing
+        return int(((n / Chunk) * kXcds + unsigned(x)) * Chunk + n % Chunk);
     }
-    __device__ __forceinline__ int tile(int n) const { return ((n / Chunk) * kXcds + xcd) * Chunk + n % Chunk; }
+    // The next tile, or -1 when every XCD's share is claimed (tileOf grows
+    // with n, so an XCD whose claim falls past the end has none left).
+    __device__ __forceinline__ int next(int tiles)
+    {
+        for (; from < kXcds; ++from) {
+            const int x = (xcd + from) % kXcds;
+            unsigned n = 0;
+            if (lane == 0)
+                n = atomicAdd(&queue->next[x], 1u);
+            const int t = tileOf(x, unsigned(uniform(int(n))));
+            if (t < tiles)
+                return t;
+        }
+        return -1;
+    }
+    // Every wave of the launch calls this once, after its last claim.
+    __device__ __forceinline__ void finish()
+    {
+        if (lane == 0 && atomicAdd(&queue->done, 1u) == gridDim.x * Waves - 1) {
+            for (int x = 0; x < kXcds; ++x)
+                queue->next[x] = 0;
+            queue->done = 0;
+        }
+    }
 };
 
 } // namespace
@@ -888,22 +914,18 @@ __global__ void __launch_bounds__(64 * kMixWaves) clayMixWaves(
     const Span* __restrict__ spans, const Job* __restrict__ jobs, const Tile* __restrict__ tiles,
     const int* __restrict__ voices, const clay_control_point* __restrict__ points, const Mod* __restrict__ mods,
     const float* __restrict__ filtered, const VoiceState* __restrict__ states, void* __restrict__ out, int segFirst,
-    int segEnd, int firstTile, int tileCount)
+    int segEnd, int firstTile, int tileCount, MixQueue* __restrict__ queue)
 {
     const int lane = int(threadIdx.x) & 63;
-    const XcdWalk<64, kMixWaves> walk(uniform(int(threadIdx.x) >> 6));
+    TileClaims<64, kMixWaves> claims(queue, lane);
     withIntTaps<T>(intTable, [&](const auto& taps) {
-        if (walk.wave < 0)
-            return;
-        for (int n = walk.wave;; n += walk.waves) {
-            const int t = walk.tile(n);
-            if (t >= tileCount)
-                break;
+        for (int t; (t = claims.next(tileCount)) >= 0;) {
             const Tile tile = tiles[firstTile + t];
             mixTileWave<T, FloatOut, Simple>(bank, odd, taps, sinc, intTable, spans, jobs, voices, points, mods, filtered, states,
                                      out, segFirst, segEnd, tile, lane);
         }
     });
+    claims.finish();
 }
 
 // Persistent form: the 256-frame tiles of every unit of the round.
@@ -911,20 +933,15 @@ template<int T, bool FloatOut, bool Simple>
 __global__ void __launch_bounds__(64 * kMixWaves) clayRoundMixWaves(const int16_t* __restrict__ bank, uint32_t odd,
                                                                     const uint32_t* __restrict__ intTable,
                                                                     const float* __restrict__ sinc, UnitsPtr units,
-                                                                    RoundPtr round)
+                                                                    RoundPtr round, MixQueue* __restrict__ queue)
 {
     const int lane = int(threadIdx.x) & 63;
-    const XcdWalk<64, kMixWaves> walk(uniform(int(threadIdx.x) >> 6));
+    TileClaims<64, kMixWaves> claims(queue, lane);
     const RoundHeader header = *round;
     if (header.tiles == 0)
-        return;  // every unit of the round is straight (clayRoundMixStraight)
+        return;  // every unit of the round is straight (clayRoundMixStraight); nothing claimed
     withIntTaps<T>(intTable, [&](const auto& taps) {
-        if (walk.wave < 0)
-            return;
-        for (int n = walk.wave;; n += walk.waves) {
-            const int item = walk.tile(n);
-            if (item >= header.tiles)
-                break;
+        for (int item; (item = claims.next(header.tiles)) >= 0;) {
             const int unit = unitOf(header, item, [&](int i) { return units[i].tileBase; });
             const auto& u = constant(units)[unit];
             const Tile tile = global(u.tiles)[u.tileFirst + item - u.tileBase];
@@ -933,6 +950,7 @@ __global__ void __launch_bounds__(64 * kMixWaves) clayRoundMixWaves(const int16_
                                      global(u.mix), u.segFirst, u.segEnd, tile, lane);
         }
     });
+    claims.finish();
 }
 
 // Plain form, straight batches: 1024-frame tiles [firstTile, firstTile + tileCount).
@@ -941,41 +959,33 @@ __global__ void __launch_bounds__(64 * kStraightWaves) clayMixStraight(
     const int16_t* __restrict__ bank, uint32_t odd, const uint32_t* __restrict__ intTable, const Span* __restrict__ spans,
     const Job* __restrict__ jobs, const Tile* __restrict__ tiles, const int* __restrict__ voices,
     const clay_control_point* __restrict__ points, void* __restrict__ out, int segFirst, int segEnd, int firstTile,
-    int tileCount)
+    int tileCount, MixQueue* __restrict__ queue)
 {
     const int lane = int(threadIdx.x) & 63;
-    const XcdWalk<16, kStraightWaves> walk(uniform(int(threadIdx.x) >> 6));
+    TileClaims<16, kStraightWaves> claims(queue, lane);
     withIntTaps<T>(intTable, [&](const auto& taps) {
-        if (walk.wave < 0)
-            return;
-        for (int n = walk.wave;; n += walk.waves) {
-            const int t = walk.tile(n);
-            if (t >= tileCount)
-                break;
+        for (int t; (t = claims.next(tileCount)) >= 0;) {
             const Tile tile = tiles[firstTile + t];
             straightTileWave<T, FloatOut>(bank, odd, taps, spans, jobs, voices, points, out, segFirst, segEnd, tile, lane);
         }
     });
+    claims.finish();
 }
 
 // Persistent form: the 1024-frame tiles of every straight unit of the round.
 template<int T, bool FloatOut>
 __global__ void __launch_bounds__(64 * kStraightWaves) clayRoundMixStraight(const int16_t* __restrict__ bank, uint32_t odd,
                                                                        const uint32_t* __restrict__ intTable,
-                                                                       UnitsPtr units, RoundPtr round)
+                                                                       UnitsPtr units, RoundPtr round,
+                                                                       MixQueue* __restrict__ queue)
 {
     const int lane = int(threadIdx.x) & 63;
-    const XcdWalk<16, kStraightWaves> walk(uniform(int(threadIdx.x) >> 6));
+    TileClaims<16, kStraightWaves> claims(queue, lane);
     const RoundHeader header = *round;
     if (header.straightTiles == 0)
-        return;
+        return;  // nothing claimed
     withIntTaps<T>(intTable, [&](const auto& taps) {
-        if (walk.wave < 0)
-            return;
-        for (int n = walk.wave;; n += walk.waves) {
-            const int item = walk.tile(n);
-            if (item >= header.straightTiles)
-                break;
+        for (int item; (item = claims.next(header.straightTiles)) >= 0;) {
             const int unit = unitOf(header, item, [&](int i) { return units[i].straightBase; });
             const auto& u = constant(units)[unit];
             const Tile tile = global(u.straightTiles)[u.straightFirst + item - u.straightBase];
@@ -983,23 +993,25 @@ __global__ void __launch_bounds__(64 * kStraightWaves) clayRoundMixStraight(cons
                                           global(u.points), global(u.mix), u.segFirst, u.segEnd, tile, lane);
         }
     });
+    claims.finish();
 }
 
 #define CLAY_INSTANTIATE_MIX_WAVES(T, FloatOut, Simple)                                                                \
     template __global__ void clayMixWaves<T, FloatOut, Simple>(const int16_t*, uint32_t, const uint32_t*, const float*, \
                                                                const Span*, const Job*, const Tile*, const int*,      \
                                                                const clay_control_point*, const Mod*, const float*,   \
-                                                               const VoiceState*, void*, int, int, int, int);          \
+                                                               const VoiceState*, void*, int, int, int, int,          \
+                                                               MixQueue*);                                            \
     template __global__ void clayRoundMixWaves<T, FloatOut, Simple>(const int16_t*, uint32_t, const uint32_t*,        \
                                                                     const float*,                                      \
-                                                                    UnitsPtr, RoundPtr);
+                                                                    UnitsPtr, RoundPtr, MixQueue*);
 #define CLAY_INSTANTIATE_MIX_STRAIGHT(T, FloatOut)                                                                     \
     template __global__ void clayMixStraight<T, FloatOut>(const int16_t*, uint32_t, const uint32_t*, const Span*,     \
                                                           const Job*,                                                 \
                                                           const Tile*, const int*, const clay_control_point*, void*,  \
-                                                          int, int, int, int);                                        \
+                                                          int, int, int, int, MixQueue*);                             \
     template __global__ void clayRoundMixStraight<T, FloatOut>(const int16_t*, uint32_t, const uint32_t*, UnitsPtr,   \
-                                                               RoundPtr);
+                                                               RoundPtr, MixQueue*);
 #define CLAY_INSTANTIATE_MIX_WAVES_TAPS(T)                                                                             \
     CLAY_INSTANTIATE_MIX_WAVES(T, true, true)                                                                          \
     CLAY_INSTANTIATE_MIX_WAVES(T, false, true)                                                                         \

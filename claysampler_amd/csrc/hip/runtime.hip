@@ -65,16 +65,17 @@ __global__ void clayRoundVoiceChains(const UnitView*, const RoundHeader*);
 template<int T, bool FloatOut, bool Simple>
 __global__ void clayMixWaves(const int16_t*, uint32_t, const uint32_t*, const float*, const Span*, const Job*,
                              const Tile*, const int*, const clay_control_point*, const Mod*, const float*,
-                             const VoiceState*, void*, int, int, int, int);
+                             const VoiceState*, void*, int, int, int, int, MixQueue*);
 template<int T, bool FloatOut, bool Simple>
 __global__ void clayRoundMixWaves(const int16_t*, uint32_t, const uint32_t*, const float*, const UnitView*,
-                                  const RoundHeader*);
+                                  const RoundHeader*, MixQueue*);
 // Mix of straight batches on kStraightFrames tiles (mix.hip).
 template<int T, bool FloatOut>
 __global__ void clayMixStraight(const int16_t*, uint32_t, const uint32_t*, const Span*, const Job*, const Tile*,
-                                const int*, const clay_control_point*, void*, int, int, int, int);
+                                const int*, const clay_control_point*, void*, int, int, int, int, MixQueue*);
 template<int T, bool FloatOut>
-__global__ void clayRoundMixStraight(const int16_t*, uint32_t, const uint32_t*, const UnitView*, const RoundHeader*);
+__global__ void clayRoundMixStraight(const int16_t*, uint32_t, const uint32_t*, const UnitView*, const RoundHeader*,
+                                     MixQueue*);
 // Finishing kernels (finish.hip).
 extern "C" __global__ void clayScanStereo(const Job*, int, const float2*, StereoStats*, int);
 extern "C" __global__ void claySelectNative(const Job*, int, const float2*, const StereoStats*, clay_tail_result*);
@@ -305,6 +306,9 @@ struct clay_hip {
     int16_t* bank = nullptr;
     uint64_t bankCount = 0, bankCapacity = 0;
     uint32_t bankOdd = 0;
+    // The mix launches' tile queues (0: clayMixWaves, 1: clayMixStraight), zero
+    // between launches.
+    MixQueue* queues = nullptr;
     uint64_t bankShifted = 0;  // samples of the bank the shifted copy reflects
     std::string error;
     clay_hip_stats stats{};
@@ -1217,7 +1221,6 @@ constexpr int kAudioWavesPerBlock = 16;  // render.hip: voice chains per 1024-th
 constexpr int kSharedTapsMax = 32;       // render.hip: widest sinc table the audio pass keeps in LDS
 constexpr int kMixWaves = 16;            // mix.hip: tiles (one per wave) in flight per 1024-thread block
 constexpr int kStraightWaves = 12;       // mix.hip: straight tiles in flight per 768-thread block
-constexpr int kXcds = 8;                 // mix.hip: XCDs the mix blocks are spread over
 
 // Grid of the wave mix for `tiles` tiles: one 1024-thread block per CU at
 // most, a multiple of the XCD count (the kernel maps blocks to XCDs).
@@ -1268,7 +1271,7 @@ void launchRound(clay_hip* h, const Shape& shape, bool timed)
         const int blocks = mixGrid(h, 1 << 30);
         const auto mix = [&](auto kernel) {
             hipLaunchKernelGGL(kernel, dim3(blocks), dim3(64 * kMixWaves), 0, s, h->bank, h->bankOdd, h->sincInt,
-                               h->sinc, units, round);
+                               h->sinc, units, round, h->queues + 0);
         };
         if (shape.simpleMix)
             h->natural ? mix(clayRoundMixWaves<T, true, true>) : mix(clayRoundMixWaves<T, false, true>);
@@ -1279,7 +1282,7 @@ void launchRound(clay_hip* h, const Shape& shape, bool timed)
         const int blocks = straightGrid(h, 1 << 30);
         const auto mix = [&](auto kernel) {
             hipLaunchKernelGGL(kernel, dim3(blocks), dim3(64 * kStraightWaves), 0, s, h->bank, h->bankOdd,
-                               h->sincInt, units, round);
+                               h->sincInt, units, round, h->queues + 1);
         };
         h->natural ? mix(clayRoundMixStraight<T, true>) : mix(clayRoundMixStraight<T, false>);
     });
@@ -1335,7 +1338,7 @@ void launchWindow(clay_hip* h, const Window& w)
                 const auto mix = [&](auto kernel) {
                     hipLaunchKernelGGL(kernel, dim3(blocks), dim3(64 * kStraightWaves), 0, s, h->bank, h->bankOdd,
                                        h->sincInt, b.dSpans, b.dJobs, b.dTiles, b.dVoices, b.dPoints, b.dMix, w.first, w.end,
-                                       v.straightFirst, v.straightCount);
+                                       v.straightFirst, v.straightCount, h->queues + 1);
                 };
                 h->natural ? mix(clayMixStraight<T, true>) : mix(clayMixStraight<T, false>);
             });
@@ -1347,7 +1350,7 @@ void launchWindow(clay_hip* h, const Window& w)
             const auto mix = [&](auto kernel) {
                 hipLaunchKernelGGL(kernel, dim3(blocks), dim3(64 * kMixWaves), 0, s, h->bank, h->bankOdd, h->sincInt, h->sinc,
                                    b.dSpans, b.dJobs, b.dTiles, b.dVoices, b.dPoints, b.dMods, b.dFiltered, b.dStates,
-                                   b.dMix, w.first, w.end, v.tileFirst, v.tileCount);
+                                   b.dMix, w.first, w.end, v.tileFirst, v.tileCount, h->queues + 0);
             };
             if (b.simpleMix)
                 h->natural ? mix(clayMixWaves<T, true, true>) : mix(clayMixWaves<T, false, true>);
@@ -1689,6 +1692,8 @@ int clay_hip_create(const clay_hip_config* config, clay_hip** out)
         h->unitCapacity = size_t(std::min<uint64_t>(h->cfg.batch_jobs, 1u << 16)) + 1;
         const size_t roundBytes = 256 + h->unitCapacity * sizeof(UnitView);
         check(hipMalloc(&h->dRound, roundBytes), "round descriptors");
+        check(hipMalloc(&h->queues, 2 * sizeof(MixQueue)), "mix queues");
+        check(hipMemset(h->queues, 0, 2 * sizeof(MixQueue)), "mix queues");
         for (auto& stage : h->stages) {
             check(hipHostMalloc(&stage.host, roundBytes, hipHostMallocDefault), "round staging");
             check(hipEventCreateWithFlags(&stage.copied, hipEventDisableTiming), "hipEventCreate");
@@ -1740,6 +1745,8 @@ int clay_hip_destroy(clay_hip* h)
     }
     if (h->dRound)
         (void)hipFree(h->dRound);
+    if (h->queues)
+        (void)hipFree(h->queues);
     if (h->bank)
         (void)hipFree(h->bank);
     if (h->sinc)
