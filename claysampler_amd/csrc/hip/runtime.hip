@@ -112,8 +112,36 @@ void check(hipError_t e, const char* what)
 
 thread_local std::string g_createError;
 
+template<class T, class A>
+size_t bytesOf(const std::vector<T, A>& v) { return v.size() * sizeof(T); }
+
+// An allocator whose elements start uninitialised: the large record arrays of
+// a batch are sized first and then written element by element (on several
+// threads), so value-initialising them would only add a pass over hundreds of
+// megabytes.
 template<class T>
-size_t bytesOf(const std::vector<T>& v) { return v.size() * sizeof(T); }
+struct Uninitialised : std::allocator<T> {
+    using value_type = T;
+    template<class U>
+    struct rebind {
+        using other = Uninitialised<U>;
+    };
+    Uninitialised() = default;
+    template<class U>
+    Uninitialised(const Uninitialised<U>&) noexcept {}
+    template<class U>
+    void construct(U* p) noexcept
+    {
+        ::new (static_cast<void*>(p)) U;
+    }
+    template<class U, class... A>
+    void construct(U* p, A&&... a)
+    {
+        ::new (static_cast<void*>(p)) U(std::forward<A>(a)...);
+    }
+};
+template<class T>
+using Records = std::vector<T, Uninitialised<T>>;
 
 uint64_t ceilDiv(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
 
@@ -230,13 +258,13 @@ struct Timer {
 
 struct clay_hip_batch {
     // Host records.
-    std::vector<Span> spans;
-    std::vector<SpanCold> cold;
+    Records<Span> spans;
+    Records<SpanCold> cold;  // empty for a batch without voice chains (nothing reads it)
     std::vector<Mod> mods;
     std::vector<clay_control_point> points;
     std::vector<Job> jobs;
-    std::vector<Tile> tiles;
-    std::vector<int> voices;
+    Records<Tile> tiles;
+    Records<int> voices;
     std::vector<int> pitchHeads, chainHeads;  // variable-pitch chains; every voice chain (longest first)
     std::vector<int> chainByStart;            // voice chains by first frame (windowed rounds)
     std::vector<int2> chainExtent;            // [first, end) frames of chainByStart[i]
@@ -636,7 +664,8 @@ void flatten(clay_hip* h, const clay_plan_view* plans, size_t count, clay_hip_ba
     b.tiles.resize(size_t(at.tile));
     b.voices.resize(size_t(at.voice));
     b.spans.resize(size_t(at.span));
-    b.cold.resize(size_t(at.span));
+    const bool needCold = at.chain > 0;  // filter and pitch recurrences read it
+    b.cold.resize(needCold ? size_t(at.span) : 0);
     if (assess)
         b.audioJobs.resize(count);
     b.onsets.resize(size_t(at.onset));
@@ -647,6 +676,30 @@ void flatten(clay_hip* h, const clay_plan_view* plans, size_t count, clay_hip_ba
     b.frames = at.frame;
     b.filteredSamples = at.filtered;
     b.pitchFrames = at.pitch;
+    // Tiles by frame (the tiles of a frame window form one contiguous range),
+    // plans in order within a frame: tile t of plan i goes to slot
+    // first[t] + (plans before i with more than t tiles).
+    Records<int> tileSlot(size_t(at.tile));
+    {
+        int most = 0;
+        for (size_t i = 0; i < count; ++i)
+            most = std::max(most, int(ceilDiv(uint64_t(plans[i].frames), uint64_t(tileFrames))));
+        std::vector<int> next(size_t(most) + 1, 0);
+        for (size_t i = 0; i < count; ++i)
+            ++next[ceilDiv(uint64_t(plans[i].frames), uint64_t(tileFrames))];
+        // next[t]: plans with exactly t tiles -> first slot of frame t.
+        int slot = 0, alive = int(count);
+        for (int t = 0; t < most; ++t) {
+            alive -= next[size_t(t)];
+            next[size_t(t)] = slot;
+            slot += alive;
+        }
+        for (size_t i = 0; i < count; ++i) {
+            const int n = int(ceilDiv(uint64_t(plans[i].frames), uint64_t(tileFrames)));
+            for (int t = 0; t < n; ++t)
+                tileSlot[size_t(bases[i].tile + t)] = next[size_t(t)]++;
+        }
+    }
     // 3. Each plan's records, and each voice chain's head, frame count, first
     // and last frame (in head order), so that sorting them walks no span list.
     struct ChainInfo {
@@ -713,8 +766,8 @@ void flatten(clay_hip* h, const clay_plan_view* plans, size_t count, clay_hip_ba
             int voiceAt = o.voice;
             for (int t = 0; t < tileCount; ++t) {
                 const int frame = t * tileFrames;
-                b.tiles[size_t(o.tile + t)] = {jobIndex, frame, std::min(tileFrames, p.frames - frame), voiceAt,
-                                               perTile[size_t(t)]};
+                b.tiles[size_t(tileSlot[size_t(o.tile + t)])] = {jobIndex, frame, std::min(tileFrames, p.frames - frame),
+                                                                 voiceAt, perTile[size_t(t)]};
                 cursor[size_t(t)] = voiceAt;
                 voiceAt += perTile[size_t(t)];
             }
@@ -816,7 +869,8 @@ void flatten(clay_hip* h, const clay_plan_view* plans, size_t count, clay_hip_ba
                     }
                 }
                 b.spans[size_t(index)] = d;
-                b.cold[size_t(index)] = c;
+                if (needCold)
+                    b.cold[size_t(index)] = c;
             }
         } catch (...) {
             failed[i] = std::current_exception();
@@ -856,8 +910,6 @@ void flatten(clay_hip* h, const clay_plan_view* plans, size_t count, clay_hip_ba
         b.chainByStart.push_back(c.second);
         b.chainExtent.push_back(c.first);
     }
-    // Tiles by frame: the tiles of a frame window form one contiguous range.
-    std::stable_sort(b.tiles.begin(), b.tiles.end(), [](const Tile& x, const Tile& y) { return x.frame < y.frame; });
     for (const auto& j : b.jobs)
         b.lastFrame = std::max(b.lastFrame, j.frames);
     // Volume groups: consecutive jobs sharing a nonzero id.
@@ -2511,7 +2563,10 @@ int renderWindowPersistent(clay_hip* h, const clay_plan_view* plans, size_t coun
                     bool simpleMix = true;
                     for (auto& u : resident) {
                         simpleMix = simpleMix && u->b->simpleMix;
-                        const int first = u->cursor, end = std::min(u->b->lastFrame, first + shape.first);
+                        // A straight cohort carries no state from frame to frame
+                        // (no filter or pitch recurrence): all of it in one round.
+                        const int first = u->cursor,
+                                  end = u->b->straight ? u->b->lastFrame : std::min(u->b->lastFrame, first + shape.first);
                         int hi = 0;
                         windowChains(*u->b, first, end, u->chainLo, hi);
                         windows.push_back({u->b, first, end, u->chainLo, hi});
