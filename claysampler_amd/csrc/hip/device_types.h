@@ -86,13 +86,15 @@ struct Tile {
     int32_t job, frame, frames, first, count;
 };
 
-// Tile claims of one mix launch (mix.h, TileClaims): a counter per XCD and
-// the waves that have finished, cleared by the launch's last wave.
+// Tile claims of one mix launch (mix.h, TileClaims): a counter per XCD, each
+// on a cache line of its own (device-scope atomics on one line would queue
+// behind each other), cleared by clayClearQueues before the launch.
 constexpr int kXcds = 8;  // MI355X: 8 XCDs, workgroups dispatched round-robin over them
 struct MixQueue {
-    uint32_t next[kXcds];
-    uint32_t done;
-    uint32_t pad[15 - kXcds];
+    struct alignas(128) Counter {
+        uint32_t next;
+        uint32_t pad[31];
+    } xcd[kXcds];
 };
 
 // Filter and pitch state carried across segments (src/cuda/render_types.h:22-28).

@@ -867,9 +867,9 @@ __device__ __forceinline__ void withIntTaps(const uint32_t* __restrict__ table, 
 // frames of consecutive jobs), and a block's waves claim from the counter of
 // the XCD the block runs on (blocks are dispatched to the 8 XCDs in turn), so
 // neighbouring tiles meet in one L2; a wave whose XCD has no tiles left takes
-// them from the next XCD's counter. The last wave of the launch to finish
-// clears the counters for the next launch on the stream (the runtime clears
-// them once, at creation). The queue's layout is MixQueue (device_types.h).
+// them from the next XCD's counter. The runtime clears the counters
+// (clayClearQueues) before every launch; the queue's layout is MixQueue
+// (device_types.h).
 template<int Chunk, int Waves>
 struct TileClaims {
     MixQueue* queue;
@@ -887,21 +887,12 @@ struct TileClaims {
             const int x = (xcd + from) % kXcds;
             unsigned n = 0;
             if (lane == 0)
-                n = atomicAdd(&queue->next[x], 1u);
+                n = atomicAdd(&queue->xcd[x].next, 1u);
             const int t = tileOf(x, unsigned(uniform(int(n))));
             if (t < tiles)
                 return t;
         }
         return -1;
-    }
-    // Every wave of the launch calls this once, after its last claim.
-    __device__ __forceinline__ void finish()
-    {
-        if (lane == 0 && atomicAdd(&queue->done, 1u) == gridDim.x * Waves - 1) {
-            for (int x = 0; x < kXcds; ++x)
-                queue->next[x] = 0;
-            queue->done = 0;
-        }
     }
 };
 
@@ -925,7 +916,6 @@ __global__ void __launch_bounds__(64 * kMixWaves) clayMixWaves(
                                      out, segFirst, segEnd, tile, lane);
         }
     });
-    claims.finish();
 }
 
 // Persistent form: the 256-frame tiles of every unit of the round.
@@ -950,7 +940,6 @@ __global__ void __launch_bounds__(64 * kMixWaves) clayRoundMixWaves(const int16_
                                      global(u.mix), u.segFirst, u.segEnd, tile, lane);
         }
     });
-    claims.finish();
 }
 
 // Plain form, straight batches: 1024-frame tiles [firstTile, firstTile + tileCount).
@@ -969,7 +958,6 @@ __global__ void __launch_bounds__(64 * kStraightWaves) clayMixStraight(
             straightTileWave<T, FloatOut>(bank, odd, taps, spans, jobs, voices, points, out, segFirst, segEnd, tile, lane);
         }
     });
-    claims.finish();
 }
 
 // Persistent form: the 1024-frame tiles of every straight unit of the round.
@@ -993,7 +981,6 @@ __global__ void __launch_bounds__(64 * kStraightWaves) clayRoundMixStraight(cons
                                           global(u.points), global(u.mix), u.segFirst, u.segEnd, tile, lane);
         }
     });
-    claims.finish();
 }
 
 #define CLAY_INSTANTIATE_MIX_WAVES(T, FloatOut, Simple)                                                                \

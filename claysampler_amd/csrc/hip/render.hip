@@ -68,6 +68,15 @@ __device__ void pitchChain(const Span* __restrict__ spans, const SpanCold* __res
 }
 
 // One lane per variable-pitch chain of the round, grid-stride.
+// Clears the tile counters of `count` mix queues (mix.h, TileClaims) before a
+// mix launch: a kernel store, so that a captured round holds no memset node.
+extern "C" __global__ void clayClearQueues(MixQueue* queues, int count)
+{
+    const int i = int(threadIdx.x);
+    if (i < count * kXcds)
+        queues[i / kXcds].xcd[i % kXcds].next = 0;
+}
+
 extern "C" __global__ void clayRoundPitchChains(UnitsPtr units, RoundPtr round)
 {
     const RoundHeader header = *round;

@@ -56,6 +56,7 @@ __global__ void clayMixTiles(const int16_t*, const float*, const Span*, const Jo
                              int);
 // Persistent form: the work of every resident unit listed in the round descriptors.
 extern "C" __global__ void clayRoundPitchChains(const UnitView*, const RoundHeader*);
+extern "C" __global__ void clayClearQueues(MixQueue*, int);
 template<int T>
 __global__ void clayRoundChainAudio(const int16_t*, const float*, const UnitView*, const RoundHeader*);
 __global__ void clayRoundVoiceChains(const UnitView*, const RoundHeader*);
@@ -142,6 +143,53 @@ struct Uninitialised : std::allocator<T> {
 };
 template<class T>
 using Records = std::vector<T, Uninitialised<T>>;
+
+// The large record arrays of a batch, kept by the renderer once the batch is
+// gone and handed to the next batch it flattens: their pages are mapped
+// already, so a batch writes its records without first touching hundreds of
+// megabytes of fresh memory. Shared by the renderer and its batches, so a
+// batch may outlive the renderer.
+struct RecordSet {
+    Records<Span> spans;
+    Records<SpanCold> cold;
+    Records<Tile> tiles;
+    Records<int> voices;
+    size_t bytes() const
+    {
+        return spans.capacity() * sizeof(Span) + cold.capacity() * sizeof(SpanCold) +
+               tiles.capacity() * sizeof(Tile) + voices.capacity() * sizeof(int);
+    }
+};
+
+class RecordPool {
+public:
+    static constexpr size_t kMaxBytes = size_t(2) << 30;  // host memory kept at most
+    RecordSet take()
+    {
+        std::lock_guard<std::mutex> lock(mutex_);
+        RecordSet out;
+        if (!spare_.empty()) {
+            out = std::move(spare_.back());
+            spare_.pop_back();
+            held_ -= out.bytes();
+        }
+        return out;
+    }
+    void give(RecordSet&& set)
+    {
+        const size_t bytes = set.bytes();
+        std::lock_guard<std::mutex> lock(mutex_);
+        if (bytes && held_ + bytes <= kMaxBytes) {
+            held_ += bytes;
+            spare_.push_back(std::move(set));
+        }
+    }
+
+private:
+    std::mutex mutex_;
+    std::vector<RecordSet> spare_;
+    size_t held_ = 0;
+};
 
 uint64_t ceilDiv(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
 
@@ -257,6 +305,14 @@ struct Timer {
 } // namespace
 
 struct clay_hip_batch {
+    // The renderer's pool of record arrays: flatten takes from it, and the
+    // arrays go back to it when the batch is destroyed.
+    std::shared_ptr<RecordPool> pool;
+    ~clay_hip_batch()
+    {
+        if (pool)
+            pool->give({std::move(spans), std::move(cold), std::move(tiles), std::move(voices)});
+    }
     // Host records.
     Records<Span> spans;
     Records<SpanCold> cold;  // empty for a batch without voice chains (nothing reads it)
@@ -337,6 +393,7 @@ struct clay_hip {
     // The mix launches' tile queues (0: clayMixWaves, 1: clayMixStraight), zero
     // between launches.
     MixQueue* queues = nullptr;
+    std::shared_ptr<RecordPool> records = std::make_shared<RecordPool>();
     uint64_t bankShifted = 0;  // samples of the bank the shifted copy reflects
     std::string error;
     clay_hip_stats stats{};
@@ -657,6 +714,14 @@ void flatten(clay_hip* h, const clay_plan_view* plans, size_t count, clay_hip_ba
         at.frame += uint64_t(p.frames);
         at.filtered += sizes[i].filtered;
         at.pitch += sizes[i].pitch;
+    }
+    if (!b.pool) {
+        b.pool = h->records;
+        RecordSet kept = b.pool->take();
+        b.spans = std::move(kept.spans);
+        b.cold = std::move(kept.cold);
+        b.tiles = std::move(kept.tiles);
+        b.voices = std::move(kept.voices);
     }
     b.points.resize(size_t(at.point));
     b.mods.resize(size_t(at.mod));
@@ -1319,6 +1384,7 @@ void launchRound(clay_hip* h, const Shape& shape, bool timed)
         hipLaunchKernelGGL(clayRoundVoiceChains, dim3(shape.chainBlocks), dim3(64 * kChainWavesPerBlock), 0, s, units,
                            round);
     });
+    run("clayClearQueues", [&] { hipLaunchKernelGGL(clayClearQueues, dim3(1), dim3(64), 0, s, h->queues, 2); });
     run("clayRoundMixWaves", [&] {
         const int blocks = mixGrid(h, 1 << 30);
         const auto mix = [&](auto kernel) {
@@ -1382,6 +1448,10 @@ void launchWindow(clay_hip* h, const Window& w)
             });
         });
         return;
+    }
+    if (b.straight || v.tileCount) {
+        hipLaunchKernelGGL(clayClearQueues, dim3(1), dim3(64), 0, s, h->queues, 2);
+        check(hipGetLastError(), "clayClearQueues");
     }
     if (b.straight) {
         if (v.straightCount)
@@ -2611,11 +2681,12 @@ int renderWindowPersistent(clay_hip* h, const clay_plan_view* plans, size_t coun
                         resident.erase(resident.begin() + std::ptrdiff_t(i));
                     }
                 }
-                // 3. Completions, oldest first; block only when nothing else can run.
+                // 3. Completions, oldest first; block only when nothing else can
+                // run (no cohort resident, none being flattened or ready to admit).
                 for (size_t i = 0; i < completing.size();) {
                     auto& u = completing[i];
                     hipError_t q = hipEventQuery(u->ready);
-                    if (q == hipErrorNotReady && resident.empty() && i == 0) {
+                    if (q == hipErrorNotReady && resident.empty() && ahead.empty() && i == 0) {
                         check(hipEventSynchronize(u->ready), "completion");
                         q = hipSuccess;
                     }
