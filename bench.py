@@ -476,6 +476,8 @@ def main():
     ap.add_argument("--forms", action="store_true",
                     help="plain vs persistent form through render_window (configuration 3)")
     ap.add_argument("--batch-jobs", type=int, default=128)
+    ap.add_argument("--launch-sizes", action="store_true",
+                    help="configuration 3: mix time of the jobs split into resident batches of 16 to 512 jobs")
     ap.add_argument("--tail-batches", type=int, default=32, help="configuration 4: training batches of 64 jobs")
     ap.add_argument("--epochs", type=int, default=20, help="configuration 4: training epochs")
     args = ap.parse_args()
@@ -506,6 +508,9 @@ def main():
     log(f"planned {jobs} jobs, {sum(p.span_count for p in plans)} spans in {time.perf_counter() - t0:.1f} s")
     if args.forms:
         forms_main(args, jobs, sfz, mids, taps, plans, (bank_ptr, bank_count))
+        return
+    if args.launch_sizes:
+        launch_sizes(plans, (bank_ptr, bank_count), taps)
         return
 
     renderer = ca.Renderer(policy=abi.POLICY_NATURAL, taps=taps)
@@ -597,6 +602,35 @@ def measured_valu(config, kernel):
         if len(parts) == 3 and parts[0] == kernel and not line.startswith("#"):
             out[parts[1]] = float(parts[2])
     return out or None
+
+
+def launch_sizes(plans, bank, taps):
+    """Mix kernel time of the jobs rendered as resident batches of 16 to all
+    jobs (HIP events on the render stream, second run of each batch): what a
+    launch costs beyond its tiles."""
+    r = ca.Renderer(policy=abi.POLICY_NATURAL, taps=taps)
+    r.upload_bank(*bank)
+    os.environ["CLAY_KERNEL_TIMING"] = "1"
+    rows = []
+    n = 16
+    while n <= len(plans):
+        total, launches = 0.0, 0
+        for first in range(0, len(plans), n):
+            part = plans[first:first + n]
+            b = r.prepare((abi.PlanView * len(part))(*part))
+            b.run()
+            b.run()
+            for name in ("clayMixStraight", "clayMixWaves", "clayMixTiles"):
+                ms, k = b.kernel_time(name)
+                total += ms
+                launches += k
+            b.close()
+        rows.append({"batch_jobs": n, "launches": launches, "mix_ms": total})
+        log(f"batches of {n:3d}: mix {total:.1f} ms over {launches} launches")
+        n *= 2
+    os.environ["CLAY_KERNEL_TIMING"] = "0"
+    r.close()
+    print(json.dumps({"launch_sizes": rows}), flush=True)
 
 
 def forms_compare(args, plans, bank, taps, frames):

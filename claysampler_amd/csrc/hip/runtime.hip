@@ -19,6 +19,9 @@
 //    the others keep rendering, and queued jobs are admitted in their place
 //    (the role of renderRolling, renderer.cpp:1351-1746).
 #include <hip/hip_runtime.h>
+#include <sched.h>
+#include <hsa/hsa.h>
+#include <hsa/hsa_ext_amd.h>
 
 #include "device_types.h"
 #include "memory_policy.h"
@@ -28,6 +31,7 @@
 #include <atomic>
 #include <chrono>
 #include <cmath>
+#include <condition_variable>
 #include <cstring>
 #include <deque>
 #include <functional>
@@ -60,7 +64,7 @@ extern "C" __global__ void clayClearQueues(MixQueue*, int);
 template<int T>
 __global__ void clayRoundChainAudio(const int16_t*, const float*, const UnitView*, const RoundHeader*);
 __global__ void clayRoundVoiceChains(const UnitView*, const RoundHeader*);
-// Mix (mix.hip): one wave per tile, integer FIR for plain voices; plain and persistent forms.
+// Mix (mix.h): one wave per tile, integer FIR for plain voices; plain and persistent forms.
 // The bank and, for the straight voices' aligned window loads, the byte
 // offset of its shifted copy minus 2 (clay_hip::bankOdd).
 template<int T, bool FloatOut, bool Simple>
@@ -70,7 +74,7 @@ __global__ void clayMixWaves(const int16_t*, uint32_t, const uint32_t*, const fl
 template<int T, bool FloatOut, bool Simple>
 __global__ void clayRoundMixWaves(const int16_t*, uint32_t, const uint32_t*, const float*, const UnitView*,
                                   const RoundHeader*, MixQueue*);
-// Mix of straight batches on kStraightFrames tiles (mix.hip).
+// Mix of straight batches on kStraightFrames tiles (mix.h).
 template<int T, bool FloatOut>
 __global__ void clayMixStraight(const int16_t*, uint32_t, const uint32_t*, const Span*, const Job*, const Tile*,
                                 const int*, const clay_control_point*, void*, int, int, int, int, MixQueue*);
@@ -112,6 +116,192 @@ void check(hipError_t e, const char* what)
 }
 
 thread_local std::string g_createError;
+
+// Device-to-host copies on the SDMA engines that ROCr names as the fastest for
+// that direction (hsa_amd_memory_get_preferred_copy_engine: two of MI355X's 16
+// engines). Measured into the renderer's pinned PCM slots while the mix
+// kernel runs, each of the two moves about 28 GB/s and engines 4-15 at most
+// 13 GB/s; HIP's asynchronous device-to-host copies on a second stream ran
+// at 13-29 GB/s. Copies go through kLanes lanes, one thread each, every lane
+// with its own share of the engines and its own completion signal, so that a
+// copy that stalls holds up one lane only. A copy has no dependencies: the
+// caller waits for the kernels that wrote its source first.
+struct Downloader {
+    static constexpr int kLanes = 2;
+    bool open = false, usable = false, initialised = false;
+    hsa_agent_t gpu{}, cpu{};
+    std::vector<hsa_amd_sdma_engine_id_t> engines;
+    hsa_signal_t signals[kLanes]{};
+    int signalCount = 0;
+
+    static hsa_status_t collect(hsa_agent_t a, void* data)
+    {
+        static_cast<std::vector<hsa_agent_t>*>(data)->push_back(a);
+        return HSA_STATUS_SUCCESS;
+    }
+
+    // Finds the device's agent by PCI location; leaves usable false (plain
+    // synchronous HIP copies) when any step fails.
+    void start(const hipDeviceProp_t& p)
+    {
+        if (open)
+            return;
+        open = true;
+        if (hsa_init() != HSA_STATUS_SUCCESS)
+            return;
+        initialised = true;
+        std::vector<hsa_agent_t> agents;
+        if (hsa_iterate_agents(collect, &agents) != HSA_STATUS_SUCCESS)
+            return;
+        const uint32_t bdf = (uint32_t(p.pciBusID) << 8) | (uint32_t(p.pciDeviceID) << 3);
+        bool found = false;
+        for (hsa_agent_t a : agents) {
+            hsa_device_type_t type;
+            uint32_t id = 0, domain = 0;
+            if (hsa_agent_get_info(a, HSA_AGENT_INFO_DEVICE, &type) != HSA_STATUS_SUCCESS || type != HSA_DEVICE_TYPE_GPU)
+                continue;
+            if (hsa_agent_get_info(a, hsa_agent_info_t(HSA_AMD_AGENT_INFO_BDFID), &id) != HSA_STATUS_SUCCESS ||
+                hsa_agent_get_info(a, hsa_agent_info_t(HSA_AMD_AGENT_INFO_DOMAIN), &domain) != HSA_STATUS_SUCCESS)
+                continue;
+            if ((id & ~7u) == bdf && domain == uint32_t(p.pciDomainID)) {
+                gpu = a;
+                found = true;
+                break;
+            }
+        }
+        if (!found || hsa_agent_get_info(gpu, hsa_agent_info_t(HSA_AMD_AGENT_INFO_NEAREST_CPU), &cpu) != HSA_STATUS_SUCCESS)
+            return;
+        uint32_t preferred = 0;
+        if (hsa_amd_memory_get_preferred_copy_engine(cpu, gpu, &preferred) != HSA_STATUS_SUCCESS || !preferred)
+            return;
+        for (uint32_t bit = 1; bit; bit <<= 1)
+            if (preferred & bit)
+                engines.push_back(hsa_amd_sdma_engine_id_t(bit));
+        for (; signalCount < kLanes; ++signalCount)
+            if (hsa_signal_create(1, 0, nullptr, &signals[signalCount]) != HSA_STATUS_SUCCESS)
+                return;
+        usable = true;
+    }
+
+    // Called by one thread per lane at a time.
+    void copy(void* host, const void* device, size_t bytes, const char* what, int lane)
+    {
+        if (!bytes)
+            return;
+        if (!usable) {
+            check(hipMemcpy(host, device, bytes, hipMemcpyDeviceToHost), what);
+            return;
+        }
+        // The lane's engines: every kLanes-th preferred engine from `lane`, or
+        // all of them when there are fewer engines than lanes.
+        const bool shared = engines.size() < size_t(kLanes);
+        const size_t mine = shared ? engines.size() : (engines.size() - size_t(lane) + kLanes - 1) / kLanes;
+        const auto engineOf = [&](size_t k) { return shared ? engines[k] : engines[size_t(lane) + k * kLanes]; };
+        hsa_signal_t signal = signals[lane];
+        // One part per engine, 4 KiB aligned; copies under 1 MiB take one engine.
+        const size_t parts = bytes < (size_t(1) << 20) ? 1 : mine;
+        const size_t part = ((bytes + parts - 1) / parts + 4095) & ~size_t(4095);
+        size_t issued = 0;
+        hsa_signal_store_screlease(signal, hsa_signal_value_t(parts));
+        for (size_t k = 0; k < parts; ++k) {
+            const size_t at = std::min(bytes, k * part), n = std::min(bytes, at + part) - at;
+            hsa_status_t st = HSA_STATUS_SUCCESS;
+            if (n)
+                st = hsa_amd_memory_async_copy_on_engine(static_cast<char*>(host) + at, cpu,
+                                                         static_cast<const char*>(device) + at, gpu, n, 0, nullptr,
+                                                         signal, engineOf(k), false);
+            else
+                hsa_signal_subtract_screlease(signal, 1);
+            if (st != HSA_STATUS_SUCCESS) {
+                // Wait for the parts already issued before reporting.
+                hsa_signal_subtract_screlease(signal, hsa_signal_value_t(parts - issued));
+                (void)hsa_signal_wait_scacquire(signal, HSA_SIGNAL_CONDITION_LT, 1, UINT64_MAX, HSA_WAIT_STATE_BLOCKED);
+                throw Failure(CLAY_E_DEVICE, std::string(what) + ": SDMA copy not accepted (status " +
+                                                 std::to_string(int(st)) + ")");
+            }
+            ++issued;
+        }
+        const hsa_signal_value_t v =
+            hsa_signal_wait_scacquire(signal, HSA_SIGNAL_CONDITION_LT, 1, UINT64_MAX, HSA_WAIT_STATE_BLOCKED);
+        if (v < 0)
+            throw Failure(CLAY_E_DEVICE, std::string(what) + ": SDMA copy failed");
+    }
+
+    void stop()
+    {
+        for (int k = 0; k < signalCount; ++k)
+            hsa_signal_destroy(signals[k]);
+        signalCount = 0;
+        if (initialised)
+            hsa_shut_down();
+        open = usable = initialised = false;
+    }
+};
+
+// Device blocks for the persistent form's cohort arenas, allocated with
+// hipMalloc in 64 MiB steps and kept for the next cohorts. Arenas from the
+// stream-ordered pool (hipMallocAsync) slowed the SDMA downloads that run
+// beside the mix kernel to 4-13 GB/s, against 40-44 GB/s over hipMalloc
+// blocks, and hipFree would stall the round loop (it waits for the device).
+// A block is given back only when no queued work uses it any more, or when
+// all later users are ordered after that work on the render stream.
+struct ArenaCache {
+    struct Block {
+        void* base = nullptr;
+        size_t bytes = 0;
+        bool busy = false;
+    };
+    std::vector<Block> blocks;
+
+    void* take(size_t bytes)
+    {
+        Block* best = nullptr;
+        for (auto& b : blocks)
+            if (!b.busy && b.bytes >= bytes && (!best || b.bytes < best->bytes))
+                best = &b;
+        if (best) {
+            best->busy = true;
+            return best->base;
+        }
+        const size_t step = size_t(64) << 20;
+        Block b;
+        b.bytes = (bytes + step - 1) / step * step;
+        hipError_t e = hipMalloc(&b.base, b.bytes);
+        if (e == hipErrorOutOfMemory) {
+            (void)hipGetLastError();
+            trim(0);  // idle blocks back to the device, then once more
+            e = hipMalloc(&b.base, b.bytes);
+        }
+        check(e, "cohort arena");
+        b.busy = true;
+        blocks.push_back(b);
+        return b.base;
+    }
+
+    void give(void* base)
+    {
+        for (auto& b : blocks)
+            if (b.base == base) {
+                b.busy = false;
+                return;
+            }
+    }
+
+    // Frees idle blocks until at most `keep` bytes of idle blocks remain
+    // (hipFree waits for the device).
+    void trim(size_t keep)
+    {
+        size_t idle = 0;
+        for (auto& b : blocks)
+            idle += b.busy ? 0 : b.bytes;
+        for (size_t i = blocks.size(); i-- > 0 && idle > keep;)
+            if (!blocks[i].busy) {
+                (void)hipFree(blocks[i].base);
+                idle -= blocks[i].bytes;
+                blocks.erase(blocks.begin() + std::ptrdiff_t(i));
+            }
+    }
+};
 
 template<class T, class A>
 size_t bytesOf(const std::vector<T, A>& v) { return v.size() * sizeof(T); }
@@ -356,7 +546,7 @@ struct clay_hip_batch {
     int* dVoices = nullptr;
     int *dPitchHeads = nullptr, *dChainHeads = nullptr, *dChainByStart = nullptr;
     int lastFrame = 0;
-    bool streamOrdered = false;  // arena from hipMallocAsync (persistent form)
+    bool cachedArena = false;  // arena from the renderer's ArenaCache (persistent form)
     VoiceState* dStates = nullptr;
     AudioJobDesc* dAudioJobs = nullptr;
     int* dOnsets = nullptr;
@@ -380,6 +570,8 @@ struct clay_hip {
     int device = 0;
     hipStream_t stream = nullptr;
     hipDeviceProp_t props{};
+    Downloader downloader;  // persistent form's download thread
+    ArenaCache arenas;      // persistent form's cohort arenas
     float* sinc = nullptr;       // phase-major taps x (phases+1)
     uint32_t* sincInt = nullptr; // the same weights as packed int16 hi/lo pairs (intSincTable)
     // The sample bank, and one allocation-stride after it the same samples one
@@ -468,16 +660,6 @@ struct clay_hip {
     std::map<void*, size_t> pinnedSizes;
 
     int channels() const { return natural ? 1 : 2; }
-
-    // Second stream for the persistent form's downloads (default priority),
-    // created on first use.
-    hipStream_t copies = nullptr;
-    hipStream_t copyStream()
-    {
-        if (!copies)
-            check(hipStreamCreateWithFlags(&copies, hipStreamNonBlocking), "hipStreamCreate");
-        return copies;
-    }
 
     hipEvent_t event()
     {
@@ -577,12 +759,27 @@ void validatePlan(const clay_hip* h, const clay_plan_view& p, size_t index)
     }
 }
 
-// Runs f(i) for i in [0, count) on up to 16 host threads (the plans of a batch
-// are independent once their record offsets are known).
-template<class F>
-void parallelFor(size_t count, F&& f)
+// CPUs this process may run on (its affinity mask, which a container or
+// scheduler may set well below the machine's count).
+size_t hostThreads()
 {
-    const size_t threads = std::min<size_t>({count, 16, std::max(1u, std::thread::hardware_concurrency())});
+    static const size_t n = [] {
+        cpu_set_t set;
+        CPU_ZERO(&set);
+        if (sched_getaffinity(0, sizeof(set), &set) == 0 && CPU_COUNT(&set) > 0)
+            return size_t(CPU_COUNT(&set));
+        return size_t(std::max(1u, std::thread::hardware_concurrency()));
+    }();
+    return n;
+}
+
+// Runs f(i) for i in [0, count) on up to `limit` host threads, at most 16 and
+// at most the CPUs of the process (the plans of a batch are independent once
+// their record offsets are known).
+template<class F>
+void parallelFor(size_t count, F&& f, size_t limit = 16)
+{
+    const size_t threads = std::min<size_t>({count, limit, 16, hostThreads()});
     if (threads <= 1) {
         for (size_t i = 0; i < count; ++i)
             f(i);
@@ -619,17 +816,11 @@ bool straightSpan(const clay_plan_view& p, const clay_voice_span& v)
 // passes: per plan (parallel) validation and the sizes that depend on its
 // spans; record offsets by prefix sums (serial); per plan (parallel) the
 // records at those offsets. Records land exactly where a serial append would
-// put them. `threaded` false keeps it on the calling thread (callers that
-// flatten several batches in parallel already).
-void flatten(clay_hip* h, const clay_plan_view* plans, size_t count, clay_hip_batch& b, bool threaded = true)
+// put them, on up to `threads` host threads (callers that flatten several
+// batches at once share the CPUs between them).
+void flatten(clay_hip* h, const clay_plan_view* plans, size_t count, clay_hip_batch& b, size_t threads = 16)
 {
-    const auto forEachPlan = [&](auto&& f) {
-        if (threaded)
-            parallelFor(count, f);
-        else
-            for (size_t i = 0; i < count; ++i)
-                f(i);
-    };
+    const auto forEachPlan = [&](auto&& f) { parallelFor(count, f, threads); };
     const bool assess = h->cfg.audio.enabled || h->cfg.group_minimum;
     b.assess = assess;
     const int window = h->tail.window;
@@ -992,8 +1183,8 @@ void flatten(clay_hip* h, const clay_plan_view* plans, size_t count, clay_hip_ba
 void release(clay_hip* h, clay_hip_batch& b)
 {
     if (b.arena) {
-        if (b.streamOrdered && h)
-            (void)hipFreeAsync(b.arena, h->stream);
+        if (b.cachedArena && h)
+            h->arenas.give(b.arena);
         else
             (void)hipFree(b.arena);
     }
@@ -1090,18 +1281,18 @@ ArenaLayout layoutOf(const clay_hip* h, const clay_hip_batch& b)
 }
 
 // The batch's records at their arena offsets in a host image of L.records
-// bytes; large arrays are copied by several threads when `threaded`.
-void packRecords(const clay_hip_batch& b, const ArenaLayout& L, char* image, bool threaded)
+// bytes; large arrays are copied by up to `threads` threads.
+void packRecords(const clay_hip_batch& b, const ArenaLayout& L, char* image, size_t threads)
 {
     const auto put = [&](size_t offset, const auto& v) {
         const size_t bytes = bytesOf(v);
         const auto* src = reinterpret_cast<const char*>(v.data());
         constexpr size_t kPiece = size_t(8) << 20;
-        if (threaded && bytes > 2 * kPiece)
+        if (threads > 1 && bytes > 2 * kPiece)
             parallelFor(ceilDiv(bytes, kPiece), [&](size_t k) {
                 const size_t at = k * kPiece;
                 std::memcpy(image + offset + at, src + at, std::min(kPiece, bytes - at));
-            });
+            }, threads);
         else if (bytes)
             std::memcpy(image + offset, src, bytes);
     };
@@ -1120,16 +1311,16 @@ void packRecords(const clay_hip_batch& b, const ArenaLayout& L, char* image, boo
     put(L.groups, b.groups);
 }
 
-// The arena: stream-ordered (persistent form) or plain.
-void allocateArena(clay_hip* h, clay_hip_batch& b, const ArenaLayout& L, bool streamOrdered)
+// The arena: from the renderer's cache (persistent form) or its own block.
+void allocateArena(clay_hip* h, clay_hip_batch& b, const ArenaLayout& L, bool cached)
 {
     if (h->memoryLimit && L.total > h->memoryLimit)
         throw Failure(CLAY_E_MEMORY, "batch arena: over the renderer's memory limit");
-    if (streamOrdered)
-        check(hipMallocAsync(&b.arena, L.total, h->stream), "batch arena");
+    if (cached)
+        b.arena = h->arenas.take(L.total);
     else
         check(hipMalloc(&b.arena, L.total), "batch arena");
-    b.streamOrdered = streamOrdered;
+    b.cachedArena = cached;
     b.arenaBytes = L.total;
 }
 
@@ -1191,7 +1382,7 @@ void upload(clay_hip* h, clay_hip_batch& b)
     const ArenaLayout L = layoutOf(h, b);
     allocateArena(h, b, L, false);
     char* image = h->staging.take(L.records);
-    packRecords(b, L, image, true);
+    packRecords(b, L, image, 16);
     placeRecords(h, b, L, image);
     h->staging.commit();
     check(hipStreamSynchronize(h->stream), "batch upload");
@@ -1336,8 +1527,8 @@ struct Shape {
 constexpr int kChainWavesPerBlock = 2;   // render.hip: voice chains per 128-thread filter block
 constexpr int kAudioWavesPerBlock = 16;  // render.hip: voice chains per 1024-thread audio block
 constexpr int kSharedTapsMax = 32;       // render.hip: widest sinc table the audio pass keeps in LDS
-constexpr int kMixWaves = 16;            // mix.hip: tiles (one per wave) in flight per 1024-thread block
-constexpr int kStraightWaves = 12;       // mix.hip: straight tiles in flight per 768-thread block
+constexpr int kMixWaves = 16;            // mix.h: tiles (one per wave) in flight per 1024-thread block
+constexpr int kStraightWaves = 12;       // mix.h: straight tiles in flight per 768-thread block
 
 // Grid of the wave mix for `tiles` tiles: one 1024-thread block per CU at
 // most, a multiple of the XCD count (the kernel maps blocks to XCDs).
@@ -1846,8 +2037,9 @@ int clay_hip_destroy(clay_hip* h)
     (void)hipSetDevice(h->device);
     if (h->stream)
         (void)hipStreamSynchronize(h->stream);
-    if (h->copies)
-        (void)hipStreamSynchronize(h->copies);
+    h->downloader.stop();
+    for (auto& b : h->arenas.blocks)
+        (void)hipFree(b.base);
     for (auto& slot : h->slots)
         if (slot.host)
             (void)hipHostFree(slot.host);
@@ -1875,8 +2067,6 @@ int clay_hip_destroy(clay_hip* h)
         (void)hipFree(h->sinc);
     if (h->sincInt)
         (void)hipFree(h->sincInt);
-    if (h->copies)
-        (void)hipStreamDestroy(h->copies);
     if (h->stream)
         (void)hipStreamDestroy(h->stream);
     delete h;
@@ -2388,11 +2578,15 @@ int renderWindowPersistent(clay_hip* h, const clay_plan_view* plans, size_t coun
         clay_hip_batch* b = nullptr;
         size_t first = 0, jobs = 0;
         int cursor = 0, chainLo = 0;
-        int stage = 0;  // 0 rendering, 1 results in flight, 2 PCM in flight
-        hipEvent_t finished = nullptr, ready = nullptr, begin = nullptr;
+        hipEvent_t finished = nullptr, begin = nullptr;
         clay_tail_result* results = nullptr;  // pinned
-        void* slot = nullptr;
+        void* slot = nullptr;                 // pinned, slotBytes long
+        size_t slotBytes = 0;
         uint64_t samples = 0;
+        // Set by the download thread under Downloads::lock.
+        bool downloaded = false;
+        int code = CLAY_OK;
+        std::string message;
         std::vector<int> ends;  // frames of each job, ascending
         // Jobs still rendering: a finished job frees its place for admission
         // while the rest of its cohort renders on.
@@ -2416,6 +2610,88 @@ int renderWindowPersistent(clay_hip* h, const clay_plan_view* plans, size_t coun
                 worker.join();
         }
     };
+    // Finished cohorts come back through one thread that waits for each
+    // cohort's finishing and copies its results, then its PCM, on the fastest
+    // device-to-host SDMA engine (Downloader), so that downloads overlap the
+    // rounds that follow at the full PCIe rate.
+    struct Downloads {
+        std::mutex lock;
+        std::condition_variable changed;
+        std::deque<Cohort*> queue;
+        bool stop = false;
+        std::vector<std::thread> workers;  // one per Downloader lane
+        bool started() const { return !workers.empty(); }
+        void start(clay_hip* h)
+        {
+            h->downloader.start(h->props);
+            for (int lane = 0; lane < Downloader::kLanes; ++lane)
+                workers.emplace_back([this, h, lane] { run(h, lane); });
+        }
+        void run(clay_hip* h, int lane)
+        {
+            (void)hipSetDevice(h->device);
+            for (;;) {
+                Cohort* u = nullptr;
+                {
+                    std::unique_lock<std::mutex> l(lock);
+                    changed.wait(l, [&] { return stop || !queue.empty(); });
+                    if (stop)
+                        return;
+                    u = queue.front();
+                    queue.pop_front();
+                }
+                int code = CLAY_OK;
+                std::string message;
+                uint64_t samples = 0;
+                try {
+                    check(hipEventSynchronize(u->finished), "completion");
+                    h->downloader.copy(u->results, u->b->dResults, u->jobs * sizeof(clay_tail_result),
+                                       "results download", lane);
+                    for (size_t k = 0; k < u->jobs; ++k)
+                        samples = std::max<uint64_t>(
+                            samples, (u->results[k].offset + uint64_t(u->results[k].data_frames)) * uint64_t(h->channels()));
+                    if (samples * sizeof(int16_t) > u->slotBytes)
+                        throw Failure(CLAY_E_INVALID, "cohort PCM larger than the batch's frames");
+                    h->downloader.copy(u->slot, u->b->dOutput, samples * sizeof(int16_t), "pcm download", lane);
+                } catch (const Failure& e) {
+                    code = e.code;
+                    message = e.what();
+                } catch (const std::exception& e) {
+                    code = CLAY_E_INVALID;
+                    message = e.what();
+                }
+                {
+                    std::lock_guard<std::mutex> l(lock);
+                    u->samples = samples;
+                    u->code = code;
+                    u->message = message;
+                    u->downloaded = true;
+                }
+                changed.notify_all();
+            }
+        }
+        void push(Cohort* u)
+        {
+            {
+                std::lock_guard<std::mutex> l(lock);
+                queue.push_back(u);
+            }
+            changed.notify_all();
+        }
+        // Stops after the cohorts being copied, if any; the rest stay queued.
+        void finish()
+        {
+            {
+                std::lock_guard<std::mutex> l(lock);
+                stop = true;
+            }
+            changed.notify_all();
+            for (auto& w : workers)
+                if (w.joinable())
+                    w.join();
+        }
+        ~Downloads() { finish(); }
+    };
     constexpr size_t kAhead = 4;            // cohorts flattened ahead of admission
     constexpr size_t kCohortsPerWindow = 8; // resident cohorts when every one is full
     const size_t cohortJobs = std::max<size_t>(1, size_t(h->cfg.batch_jobs) / kCohortsPerWindow);
@@ -2423,7 +2699,7 @@ int renderWindowPersistent(clay_hip* h, const clay_plan_view* plans, size_t coun
     std::vector<clay_plan_view> queue(plans, plans + count);
     std::vector<std::unique_ptr<Cohort>> resident, completing;
     std::deque<std::unique_ptr<Flattening>> ahead;
-    hipStream_t copies = h->copyStream();
+    Downloads downloads;  // declared after the cohorts: stopped before they are freed
     const auto drop = [&](Cohort& u) {
         if (u.b) {
             release(h, *u.b);
@@ -2432,7 +2708,7 @@ int renderWindowPersistent(clay_hip* h, const clay_plan_view* plans, size_t coun
         }
         h->unpin(u.results);
         u.results = nullptr;
-        for (hipEvent_t* e : {&u.finished, &u.ready, &u.begin})
+        for (hipEvent_t* e : {&u.finished, &u.begin})
             if (*e) {
                 h->eventPool.push_back(*e);
                 *e = nullptr;
@@ -2456,13 +2732,16 @@ int renderWindowPersistent(clay_hip* h, const clay_plan_view* plans, size_t coun
         f->views.assign(queue.begin() + std::ptrdiff_t(first), queue.begin() + std::ptrdiff_t(end));
         f->b = new clay_hip_batch();
         Flattening* raw = f.get();
-        raw->worker = std::thread([h, raw] {
+        // The kAhead cohorts flattening at once share the CPUs, leaving two
+        // for this loop and the download thread.
+        const size_t threads = std::max<size_t>(1, (std::max<size_t>(hostThreads(), 3) - 2) / kAhead);
+        raw->worker = std::thread([h, raw, threads] {
             try {
-                flatten(h, raw->views.data(), raw->views.size(), *raw->b);
+                flatten(h, raw->views.data(), raw->views.size(), *raw->b, threads);
                 raw->layout = layoutOf(h, *raw->b);
                 check(hipSetDevice(h->device), "hipSetDevice");
                 raw->image = h->images.take(raw->layout.records);
-                packRecords(*raw->b, raw->layout, raw->image, true);
+                packRecords(*raw->b, raw->layout, raw->image, threads);
             } catch (const Failure& e) {
                 raw->status = e.code;
                 raw->message = e.what();
@@ -2660,7 +2939,9 @@ int renderWindowPersistent(clay_hip* h, const clay_plan_view* plans, size_t coun
                     h->stats.segment_frames = shape.first;
                     h->stats.mix_blocks = shape.second;
                     // Cohorts whose last window was rendered: finish them behind
-                    // this round; their results come back on the copy stream.
+                    // this round and hand them to the download thread. The PCM
+                    // slot is sized for every frame of the cohort, an upper
+                    // bound of the packed PCM.
                     for (size_t i = 0; i < resident.size();) {
                         auto& u = resident[i];
                         if (u->cursor < u->b->lastFrame) {
@@ -2670,13 +2951,12 @@ int renderWindowPersistent(clay_hip* h, const clay_plan_view* plans, size_t coun
                         finishBatch(h, *u->b);
                         u->finished = h->event();
                         check(hipEventRecord(u->finished, h->stream), "hipEventRecord");
-                        check(hipStreamWaitEvent(copies, u->finished, 0), "hipStreamWaitEvent");
                         u->results = static_cast<clay_tail_result*>(h->pinned(u->jobs * sizeof(clay_tail_result)));
-                        check(hipMemcpyAsync(u->results, u->b->dResults, u->jobs * sizeof(clay_tail_result),
-                                             hipMemcpyDeviceToHost, copies), "results download");
-                        u->ready = h->event();
-                        check(hipEventRecord(u->ready, copies), "hipEventRecord");
-                        u->stage = 1;
+                        u->slotBytes = size_t(u->b->frames) * size_t(h->channels()) * sizeof(int16_t);
+                        u->slot = pinnedSlot(h, u->slotBytes);
+                        if (!downloads.started())
+                            downloads.start(h);
+                        downloads.push(u.get());
                         completing.push_back(std::move(u));
                         resident.erase(resident.begin() + std::ptrdiff_t(i));
                     }
@@ -2685,31 +2965,22 @@ int renderWindowPersistent(clay_hip* h, const clay_plan_view* plans, size_t coun
                 // run (no cohort resident, none being flattened or ready to admit).
                 for (size_t i = 0; i < completing.size();) {
                     auto& u = completing[i];
-                    hipError_t q = hipEventQuery(u->ready);
-                    if (q == hipErrorNotReady && resident.empty() && ahead.empty() && i == 0) {
-                        check(hipEventSynchronize(u->ready), "completion");
-                        q = hipSuccess;
+                    bool downloaded;
+                    {
+                        std::unique_lock<std::mutex> l(downloads.lock);
+                        if (!u->downloaded && resident.empty() && ahead.empty() && i == 0)
+                            downloads.changed.wait(l, [&] { return u->downloaded; });
+                        downloaded = u->downloaded;
                     }
-                    if (q == hipErrorNotReady) {
+                    if (!downloaded) {
                         ++i;
                         continue;
                     }
-                    check(q, "completion");
-                    if (u->stage == 1) {
-                        for (size_t k = 0; k < u->jobs; ++k)
-                            u->samples = std::max<uint64_t>(
-                                u->samples, (u->results[k].offset + uint64_t(u->results[k].data_frames)) * uint64_t(h->channels()));
-                        u->slot = pinnedSlot(h, u->samples * sizeof(int16_t));
-                        if (u->samples)
-                            check(hipMemcpyAsync(u->slot, u->b->dOutput, u->samples * sizeof(int16_t),
-                                                 hipMemcpyDeviceToHost, copies), "pcm download");
-                        h->stats.downloaded_bytes += u->samples * sizeof(int16_t);
-                        check(hipEventRecord(u->ready, copies), "hipEventRecord");
-                        u->stage = 2;
-                        continue;  // its PCM may already have arrived
-                    }
+                    if (u->code != CLAY_OK)
+                        throw Failure(u->code, u->message);
+                    h->stats.downloaded_bytes += u->samples * sizeof(int16_t);
                     float ms = 0;
-                    (void)hipEventElapsedTime(&ms, u->begin, u->ready);
+                    (void)hipEventElapsedTime(&ms, u->begin, u->finished);
                     clay_render_batch batch{};
                     batch.pcm = static_cast<const int16_t*>(u->slot);
                     batch.pcm_handle = u->slot;
@@ -2735,10 +3006,9 @@ int renderWindowPersistent(clay_hip* h, const clay_plan_view* plans, size_t coun
         collect(true);
         tuner.keep(h);
     });
-    if (status != CLAY_OK) {
-        (void)hipStreamSynchronize(copies);
+    downloads.finish();
+    if (status != CLAY_OK)
         (void)hipStreamSynchronize(h->stream);
-    }
     for (auto& f : ahead)
         dropFlattening(*f);
     if (status != CLAY_OK) {
@@ -2750,6 +3020,9 @@ int renderWindowPersistent(clay_hip* h, const clay_plan_view* plans, size_t coun
             drop(*u);
         }
     }
+    // Idle arenas beyond 16 GiB go back to the device; the rest serve the
+    // next window.
+    h->arenas.trim(size_t(16) << 30);
     return status;
 }
 
