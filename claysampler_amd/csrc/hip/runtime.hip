@@ -602,6 +602,7 @@ struct clay_hip {
         bool busy = false;
     };
     std::vector<Slot> slots;
+    std::mutex slotLock;  // slots are taken by download threads and released by callers' threads
     size_t residentBytes = 0;
     int tunedSegment = 0;  // persistent form: segment frames chosen by the round tuner, 0 = not yet
     uint64_t memoryLimit = 0;  // bytes a batch arena may take (clay_hip_set_memory_limit), 0 = no limit
@@ -1880,6 +1881,7 @@ void finishBatch(clay_hip* h, clay_hip_batch& b)
 // more than kSlotBudget.
 void* pinnedSlot(clay_hip* h, size_t bytes)
 {
+    std::lock_guard<std::mutex> guard(h->slotLock);
     constexpr size_t kSlotBudget = size_t(16) << 30;
     clay_hip::Slot* best = nullptr;
     size_t held = 0;
@@ -2580,8 +2582,7 @@ int renderWindowPersistent(clay_hip* h, const clay_plan_view* plans, size_t coun
         int cursor = 0, chainLo = 0;
         hipEvent_t finished = nullptr, begin = nullptr;
         clay_tail_result* results = nullptr;  // pinned
-        void* slot = nullptr;                 // pinned, slotBytes long
-        size_t slotBytes = 0;
+        void* slot = nullptr;                 // pinned, taken by the download thread
         uint64_t samples = 0;
         // Set by the download thread under Downloads::lock.
         bool downloaded = false;
@@ -2650,9 +2651,12 @@ int renderWindowPersistent(clay_hip* h, const clay_plan_view* plans, size_t coun
                     for (size_t k = 0; k < u->jobs; ++k)
                         samples = std::max<uint64_t>(
                             samples, (u->results[k].offset + uint64_t(u->results[k].data_frames)) * uint64_t(h->channels()));
-                    if (samples * sizeof(int16_t) > u->slotBytes)
-                        throw Failure(CLAY_E_INVALID, "cohort PCM larger than the batch's frames");
-                    h->downloader.copy(u->slot, u->b->dOutput, samples * sizeof(int16_t), "pcm download", lane);
+                    void* slot = pinnedSlot(h, samples * sizeof(int16_t));
+                    {
+                        std::lock_guard<std::mutex> l(lock);
+                        u->slot = slot;
+                    }
+                    h->downloader.copy(slot, u->b->dOutput, samples * sizeof(int16_t), "pcm download", lane);
                 } catch (const Failure& e) {
                     code = e.code;
                     message = e.what();
@@ -2939,9 +2943,7 @@ int renderWindowPersistent(clay_hip* h, const clay_plan_view* plans, size_t coun
                     h->stats.segment_frames = shape.first;
                     h->stats.mix_blocks = shape.second;
                     // Cohorts whose last window was rendered: finish them behind
-                    // this round and hand them to the download thread. The PCM
-                    // slot is sized for every frame of the cohort, an upper
-                    // bound of the packed PCM.
+                    // this round and hand them to the download threads.
                     for (size_t i = 0; i < resident.size();) {
                         auto& u = resident[i];
                         if (u->cursor < u->b->lastFrame) {
@@ -2952,8 +2954,6 @@ int renderWindowPersistent(clay_hip* h, const clay_plan_view* plans, size_t coun
                         u->finished = h->event();
                         check(hipEventRecord(u->finished, h->stream), "hipEventRecord");
                         u->results = static_cast<clay_tail_result*>(h->pinned(u->jobs * sizeof(clay_tail_result)));
-                        u->slotBytes = size_t(u->b->frames) * size_t(h->channels()) * sizeof(int16_t);
-                        u->slot = pinnedSlot(h, u->slotBytes);
                         if (!downloads.started())
                             downloads.start(h);
                         downloads.push(u.get());
@@ -3044,11 +3044,14 @@ int clay_hip_release_pcm(clay_hip* h, void* pcm_handle)
 {
     if (!h)
         return CLAY_E_INVALID;
-    for (auto& slot : h->slots)
-        if (slot.host == pcm_handle) {
-            slot.busy = false;
-            return CLAY_OK;
-        }
+    {
+        std::lock_guard<std::mutex> guard(h->slotLock);
+        for (auto& slot : h->slots)
+            if (slot.host == pcm_handle) {
+                slot.busy = false;
+                return CLAY_OK;
+            }
+    }
     h->error = "unknown PCM handle";
     return CLAY_E_INVALID;
 }
