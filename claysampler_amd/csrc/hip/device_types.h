@@ -125,6 +125,12 @@ struct VoiceState {
 // Device pointers and the frame window of one resident unit (a batch of one
 // or more jobs) for one round of render launches. Tiles of a unit are sorted
 // by frame, so the tiles of a window are one contiguous range.
+struct StereoStats {
+    double power;
+    float peak;
+    int32_t pad;
+};
+
 struct UnitView {
     const Span* spans;
     const SpanCold* cold;
@@ -143,6 +149,9 @@ struct UnitView {
     // 256-frame ones; any other unit the reverse.
     const Tile* straightTiles;
     const int* straightVoices;
+    // Power and peak of each 1024-frame block, written by the straight mix
+    // when this round renders the unit whole (else null: clayScanStereo).
+    StereoStats* stereo;
     int32_t segFirst, segEnd;           // frame window of this round
     int32_t tileFirst, tileCount;       // tiles overlapping the window
     int32_t straightFirst, straightCount;
@@ -159,11 +168,6 @@ struct RoundHeader {
     int32_t pad[2];
 };
 
-struct StereoStats {
-    double power;
-    float peak;
-    int32_t pad;
-};
 
 struct MonoStats {
     double power;

@@ -680,13 +680,20 @@ __device__ __forceinline__ void straightRows(const int16_t* __restrict__ bank, u
 
 // Writes the rows of a tile's mix: the frames [lo, hi) of rows starting at
 // `tileFrame`, scaled by the job gain (CC7 x CC11, src/cuda/render.cu:189-193),
-// plus the wet sums.
+// plus the wet sums. With `stats` (a whole 1024-frame tail block in one wave)
+// it also stores the block's power and peak as clayScanStereo would
+// (src/cuda/tail.cu:54-81): the same per-frame terms, summed in another order,
+// which moves the power by a few units in the last place; the natural-end test
+// re-sums serially within 1e-12 of its threshold, so its decisions do not move.
 template<bool FloatOut, int Rows>
 __device__ __forceinline__ void writeRows(const clay_control_point* __restrict__ points, const Job& j, void* out,
                                           int tileFrame, int tileEnd, int lo, int hi, const float (&left)[Rows],
                                           const float (&right)[Rows], const float (&wetLeft)[Rows],
-                                          const float (&wetRight)[Rows], int lane)
+                                          const float (&wetRight)[Rows], int lane, StereoStats* __restrict__ stats = nullptr,
+                                          int statsIndex = 0)
 {
+    double power = 0;
+    float peak = 0;
     // The gain is one value over the tile when both tracks are flat there.
     const int s0 = controlSegment(points, j.controls[0], j.controls[1], tileFrame);
     const int s1 = controlSegment(points, j.controls[1], j.controls[2], tileFrame);
@@ -700,6 +707,14 @@ __device__ __forceinline__ void writeRows(const clay_control_point* __restrict__
             continue;
         const float amp = flat ? amp0 : jobGain(points, j, frame);
         const float2 mixed = make_float2(fmaf(left[k], amp, wetLeft[k]), fmaf(right[k], amp, wetRight[k]));
+        if (stats) {
+            if (!isfinite(mixed.x) || !isfinite(mixed.y)) {
+                peak = __int_as_float(0x7f800000);
+            } else {
+                peak = fmaxf(peak, fmaxf(fabsf(mixed.x), fabsf(mixed.y)));
+                power += double(__fmul_rn(mixed.x, mixed.x)) + double(__fmul_rn(mixed.y, mixed.y));
+            }
+        }
         const uint64_t at = j.mix + uint64_t(frame);
         if (FloatOut) {
             static_cast<float2*>(out)[at] = mixed;
@@ -708,6 +723,12 @@ __device__ __forceinline__ void writeRows(const clay_control_point* __restrict__
             const short r = short(__float2int_rn(fminf(32767.f, fmaxf(-32768.f, mixed.y * 32768.f))));
             static_cast<short2*>(out)[at] = make_short2(l, r);
         }
+    }
+    if (stats) {
+        power = waveReduce(power, Add{});
+        peak = waveReduce(peak, Max{});
+        if (lane == 0)
+            stats[statsIndex] = {power, peak, 0};
     }
 }
 
@@ -811,8 +832,10 @@ __device__ __forceinline__ void straightTileWave(const int16_t* __restrict__ ban
                                                  const Span* __restrict__ spans, const Job* __restrict__ jobs,
                                                  const int* __restrict__ voices,
                                                  const clay_control_point* __restrict__ points, void* __restrict__ out,
-                                                 int segFirst, int segEnd, const Tile& tileIn, int lane)
+                                                 int segFirst, int segEnd, const Tile& tileIn, int lane,
+                                                 StereoStats* __restrict__ stereo)
 {
+    static_assert(kStraightFrames == kTailBlock, "a straight tile is one tail block");
     const Tile tile{uniform(tileIn.job), uniform(tileIn.frame), uniform(tileIn.frames), uniform(tileIn.first),
                     uniform(tileIn.count)};
     const Job& j = jobs[tile.job];
@@ -838,7 +861,10 @@ __device__ __forceinline__ void straightTileWave(const int16_t* __restrict__ ban
             straightRows<T, kStraightRows>(bank, odd, taps, spans[__builtin_amdgcn_readlane(held, i)], tile.frame, lo, hi,
                                            panUniform, pan0, panOf, left, right, panCache, lane);
     }
-    writeRows<FloatOut>(points, j, out, tile.frame, tileEnd, lo, hi, left, right, zero, zero, lane);
+    // The tail block's statistics when the launch renders the whole tile.
+    StereoStats* stats = stereo && lo == tile.frame && hi == tileEnd ? stereo : nullptr;
+    writeRows<FloatOut>(points, j, out, tile.frame, tileEnd, lo, hi, left, right, zero, zero, lane, stats,
+                        j.blockFirst + tile.frame / kTailBlock);
 }
 
 // Runs body(taps) with the integer weights: for T <= kIntSharedTaps rows
@@ -942,20 +968,22 @@ __global__ void __launch_bounds__(64 * kMixWaves) clayRoundMixWaves(const int16_
     });
 }
 
-// Plain form, straight batches: 1024-frame tiles [firstTile, firstTile + tileCount).
+// Plain form, straight batches: 1024-frame tiles [firstTile, firstTile + tileCount);
+// `stereo` (or null) receives the tail-block statistics of whole tiles.
 template<int T, bool FloatOut>
 __global__ void __launch_bounds__(64 * kStraightWaves) clayMixStraight(
     const int16_t* __restrict__ bank, uint32_t odd, const uint32_t* __restrict__ intTable, const Span* __restrict__ spans,
     const Job* __restrict__ jobs, const Tile* __restrict__ tiles, const int* __restrict__ voices,
     const clay_control_point* __restrict__ points, void* __restrict__ out, int segFirst, int segEnd, int firstTile,
-    int tileCount, MixQueue* __restrict__ queue)
+    int tileCount, MixQueue* __restrict__ queue, StereoStats* __restrict__ stereo)
 {
     const int lane = int(threadIdx.x) & 63;
     TileClaims<16, kStraightWaves> claims(queue, lane);
     withIntTaps<T>(intTable, [&](const auto& taps) {
         for (int t; (t = claims.next(tileCount)) >= 0;) {
             const Tile tile = tiles[firstTile + t];
-            straightTileWave<T, FloatOut>(bank, odd, taps, spans, jobs, voices, points, out, segFirst, segEnd, tile, lane);
+            straightTileWave<T, FloatOut>(bank, odd, taps, spans, jobs, voices, points, out, segFirst, segEnd, tile, lane,
+                                          stereo);
         }
     });
 }
@@ -978,7 +1006,8 @@ __global__ void __launch_bounds__(64 * kStraightWaves) clayRoundMixStraight(cons
             const auto& u = constant(units)[unit];
             const Tile tile = global(u.straightTiles)[u.straightFirst + item - u.straightBase];
             straightTileWave<T, FloatOut>(bank, odd, taps, global(u.spans), global(u.jobs), global(u.straightVoices),
-                                          global(u.points), global(u.mix), u.segFirst, u.segEnd, tile, lane);
+                                          global(u.points), global(u.mix), u.segFirst, u.segEnd, tile, lane,
+                                          global(u.stereo));
         }
     });
 }
@@ -996,7 +1025,7 @@ __global__ void __launch_bounds__(64 * kStraightWaves) clayRoundMixStraight(cons
     template __global__ void clayMixStraight<T, FloatOut>(const int16_t*, uint32_t, const uint32_t*, const Span*,     \
                                                           const Job*,                                                 \
                                                           const Tile*, const int*, const clay_control_point*, void*,  \
-                                                          int, int, int, int, MixQueue*);                             \
+                                                          int, int, int, int, MixQueue*, StereoStats*);               \
     template __global__ void clayRoundMixStraight<T, FloatOut>(const int16_t*, uint32_t, const uint32_t*, UnitsPtr,   \
                                                                RoundPtr, MixQueue*);
 #define CLAY_INSTANTIATE_MIX_WAVES_TAPS(T)                                                                             \

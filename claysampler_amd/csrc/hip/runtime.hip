@@ -77,7 +77,8 @@ __global__ void clayRoundMixWaves(const int16_t*, uint32_t, const uint32_t*, con
 // Mix of straight batches on kStraightFrames tiles (mix.h).
 template<int T, bool FloatOut>
 __global__ void clayMixStraight(const int16_t*, uint32_t, const uint32_t*, const Span*, const Job*, const Tile*,
-                                const int*, const clay_control_point*, void*, int, int, int, int, MixQueue*);
+                                const int*, const clay_control_point*, void*, int, int, int, int, MixQueue*,
+                                StereoStats*);
 template<int T, bool FloatOut>
 __global__ void clayRoundMixStraight(const int16_t*, uint32_t, const uint32_t*, const UnitView*, const RoundHeader*,
                                      MixQueue*);
@@ -555,7 +556,8 @@ struct clay_hip_batch {
     float* dFiltered = nullptr;
     double* dPhase = nullptr;
     int16_t* dMono = nullptr;
-    StereoStats* dStereo = nullptr;
+    StereoStats* dStereo = nullptr;  // null unless the natural policy is on
+    bool stereoDone = false;         // the last mix stored the tail-block statistics
     MonoStats* dWindows = nullptr;
     unsigned long long *dBody = nullptr, *dOnsetPower = nullptr;
     PcmJob* dPcmJobs = nullptr;
@@ -1359,7 +1361,7 @@ void placeRecords(clay_hip* h, clay_hip_batch& b, const ArenaLayout& L, const ch
         b.dPhase = reinterpret_cast<double*>(static_cast<char*>(device) + floatBytes);
     }
     b.dMono = reinterpret_cast<int16_t*>(base + L.mono);
-    b.dStereo = reinterpret_cast<StereoStats*>(base + L.stereo);
+    b.dStereo = h->natural ? reinterpret_cast<StereoStats*>(base + L.stereo) : nullptr;
     b.dWindows = reinterpret_cast<MonoStats*>(base + L.windows);
     b.dBody = reinterpret_cast<unsigned long long*>(base + L.body);
     b.dOnsetPower = reinterpret_cast<unsigned long long*>(base + L.onsetPower);
@@ -1504,6 +1506,9 @@ UnitView viewOf(const Window& w)
         v.straightVoices = b.dVoices;
         v.straightFirst = first;
         v.straightCount = count;
+        // One window over the whole batch: the mix stores the tail-block
+        // statistics (clayScanStereo is skipped).
+        v.stereo = w.first == 0 && w.end >= b.lastFrame ? b.dStereo : nullptr;
     } else {
         v.tileFirst = first;
         v.tileCount = count;
@@ -1605,6 +1610,7 @@ void launchWindow(clay_hip* h, const Window& w)
     hipStream_t s = h->stream;
     clay_hip_batch& b = *w.b;
     const UnitView v = viewOf(w);
+    b.stereoDone = v.stereo != nullptr;
     const int pitchCount = v.pitchCount, chainCount = v.chainCount;
     if (pitchCount)
         h->launch("clayPitchChains", [&] {
@@ -1652,7 +1658,7 @@ void launchWindow(clay_hip* h, const Window& w)
                 const auto mix = [&](auto kernel) {
                     hipLaunchKernelGGL(kernel, dim3(blocks), dim3(64 * kStraightWaves), 0, s, h->bank, h->bankOdd,
                                        h->sincInt, b.dSpans, b.dJobs, b.dTiles, b.dVoices, b.dPoints, b.dMix, w.first, w.end,
-                                       v.straightFirst, v.straightCount, h->queues + 1);
+                                       v.straightFirst, v.straightCount, h->queues + 1, v.stereo);
                 };
                 h->natural ? mix(clayMixStraight<T, true>) : mix(clayMixStraight<T, false>);
             });
@@ -1696,6 +1702,7 @@ RoundHeader stageRound(clay_hip* h, const std::vector<Window>& windows)
     auto* views = reinterpret_cast<UnitView*>(static_cast<char*>(stage.host) + 256);
     for (size_t i = 0; i < windows.size(); ++i) {
         UnitView v = viewOf(windows[i]);
+        windows[i].b->stereoDone = v.stereo != nullptr;
         v.tileBase = header.tiles;
         v.straightBase = header.straightTiles;
         v.chainBase = header.chains;
@@ -1807,11 +1814,12 @@ void finishBatch(clay_hip* h, clay_hip_batch& b)
     const auto mix2 = static_cast<const float2*>(b.dMix);
     const int lanes = 64;
     if (h->natural) {
-        chunked(b.blocks, [&](int first, int n) {
-            h->launch("clayScanStereo", [&] {
-                hipLaunchKernelGGL(clayScanStereo, dim3(n), dim3(256), 0, s, b.dJobs, count, mix2, b.dStereo, first);
+        if (!b.stereoDone)
+            chunked(b.blocks, [&](int first, int n) {
+                h->launch("clayScanStereo", [&] {
+                    hipLaunchKernelGGL(clayScanStereo, dim3(n), dim3(256), 0, s, b.dJobs, count, mix2, b.dStereo, first);
+                });
             });
-        });
         h->launch("claySelectNative", [&] {
             hipLaunchKernelGGL(claySelectNative, dim3(count), dim3(64), 0, s, b.dJobs, count, mix2, b.dStereo,
                                b.dResults);
