@@ -518,6 +518,13 @@ def main():
     batch = renderer.prepare(plans)
     for _ in range(args.warmup):
         batch.run()
+    # The launch tuner's trial runs (each candidate launch shape timed twice on
+    # the audio and filter passes), untimed; a batch without voice chains has
+    # nothing to tune and ends them at once.
+    tuning_runs = 0
+    while renderer.stats().launch_tuning and tuning_runs < 16 and any(p.filter_tracks for p in plans):
+        batch.run()
+        tuning_runs += 1
     log("warmup done")
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -546,6 +553,7 @@ def main():
     achieved = bytes_per[dominant] / kernels[dominant]["launches"] / avg_s / 1e9
     results, _ = batch.results()
     statuses = sorted({abi.TAIL_STATUS[r.status] for r in results})
+    tuned = renderer.stats()
 
     audio_seconds = frames / 44100.0
     total_voice_frames = voice_frames["dry"] + voice_frames["wet"]
@@ -572,6 +580,8 @@ def main():
         "audio_seconds": audio_seconds,
         "kernels_ms": {k: round(v["ms"], 3) for k, v in kernels.items()},
         "statuses": statuses,
+        "launch_shapes": {"chain_waves": tuned.chain_waves, "audio_unroll": tuned.audio_unroll,
+                          "tuning_runs": tuning_runs},
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "algorithmic_bytes": bytes_per[dominant],
                      "traffic": measured_traffic(args.config, dominant) if full_size else None},

@@ -124,7 +124,8 @@ class HipStats(C.Structure):
     _fields_ = [("device", C.c_char * 256)] + [(n, C.c_uint64) for n in (
         "peak_bytes", "sample_cache_budget", "peak_waiting_bytes", "spilled_bytes", "downloaded_bytes",
         "rounds", "admissions", "segments", "launches", "graph_launches")] + [
-        ("segment_frames", C.c_int32), ("mix_blocks", C.c_int32)]
+        ("segment_frames", C.c_int32), ("mix_blocks", C.c_int32), ("chain_waves", C.c_int32),
+        ("audio_unroll", C.c_int32), ("launch_tuning", C.c_int32), ("pad", C.c_int32)]
 
 
 class SessionConfig(C.Structure):

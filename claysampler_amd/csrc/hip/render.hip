@@ -213,7 +213,9 @@ struct SilenceTrack {
 // the job gain), one wave per chain and one lane per frame, written to the float
 // scratch that the filter pass then reads and overwrites. Frames carry no state
 // here, so this pass runs at full occupancy apart from the serial filters.
-template<int T, class Taps>
+// `U` frames per lane are rendered per loop step (their window and weight
+// loads issued together); every frame's arithmetic is the same for any U.
+template<int T, int U = 1, class Taps>
 __device__ __forceinline__ void chainAudio(const int16_t* __restrict__ bank, const Taps& taps,
                                            const Span* __restrict__ spans, const Job* __restrict__ jobs,
                                            const clay_control_point* __restrict__ points, const Mod* __restrict__ mods,
@@ -249,20 +251,27 @@ __device__ __forceinline__ void chainAudio(const int16_t* __restrict__ bank, con
         // `Plain`), and for any other.
         const auto frames = [&](auto plainTag) {
             constexpr bool plain = decltype(plainTag)::value;
-            for (int frame = first + lane; frame < stop; frame += 64) {
-                const int local = frame - first;
-                const double ph = !plain && v.variablePitch ? phase[v.phase + uint64_t(local)] : -1.0;
-                // A mono filter input is not panned: only stereo voices read the pan.
-                const float pan =
-                    plain || v.channels == 1 ? 0.f : controlAt(points, j.controls[2], j.controls[3], frame);
-                PanCache panCache;  // unused: mono filter inputs are not panned here
-                const float2 a =
-                    voiceAudio<T, plain, true>(bank, taps, v, points, mods, pan, frame, true, ph, panCache, bend, audioMods);
-                const float g = fixedGain ? pointAt(pointA, frame) * pointAt(pointB, frame) : jobGain(points, j, frame);
-                float* dst = filtered + v.filtered + uint64_t(local) * uint64_t(plain ? 1 : v.channels);
-                dst[0] = a.x * g;
-                if (!plain && v.channels == 2)
-                    dst[1] = a.y * g;
+            for (int step = first + lane; step < stop; step += 64 * U) {
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const int frame = step + 64 * u;
+                    if (frame >= stop)
+                        break;
+                    const int local = frame - first;
+                    const double ph = !plain && v.variablePitch ? phase[v.phase + uint64_t(local)] : -1.0;
+                    // A mono filter input is not panned: only stereo voices read the pan.
+                    const float pan =
+                        plain || v.channels == 1 ? 0.f : controlAt(points, j.controls[2], j.controls[3], frame);
+                    PanCache panCache;  // unused: mono filter inputs are not panned here
+                    const float2 a = voiceAudio<T, plain, true>(bank, taps, v, points, mods, pan, frame, true, ph,
+                                                                panCache, bend, audioMods);
+                    const float g =
+                        fixedGain ? pointAt(pointA, frame) * pointAt(pointB, frame) : jobGain(points, j, frame);
+                    float* dst = filtered + v.filtered + uint64_t(local) * uint64_t(plain ? 1 : v.channels);
+                    dst[0] = a.x * g;
+                    if (!plain && v.channels == 2)
+                        dst[1] = a.y * g;
+                }
             }
         };
         if (v.channels == 1 && v.generator == CLAY_GEN_NONE && !v.variablePitch && !audioMods)
@@ -990,8 +999,8 @@ __global__ void __launch_bounds__(64 * kAudioWaves) clayRoundChainAudio(const in
 
 // Audio pass, plain form: one unit, its pointers as kernel arguments, one wave
 // per chain, grid-stride (the grid is at most one block per CU when the taps
-// sit in LDS).
-template<int T>
+// sit in LDS); `U` frames per lane in flight (chosen by the launch tuner).
+template<int T, int U>
 __global__ void __launch_bounds__(64 * kAudioWaves) clayChainAudio(
     const int16_t* __restrict__ bank, const float* __restrict__ sinc, const Span* __restrict__ spans,
     const Job* __restrict__ jobs, const int* __restrict__ heads, const clay_control_point* __restrict__ points,
@@ -1004,8 +1013,8 @@ __global__ void __launch_bounds__(64 * kAudioWaves) clayChainAudio(
             const int item = snake(row, int(blockIdx.x) * kAudioWaves + wave, int(gridDim.x) * kAudioWaves);
             if (item >= count)
                 break;
-            chainAudio<T>(bank, taps, spans, jobs, points, mods, phase, filtered, states, segFirst, segEnd,
-                          uniform(heads[item]), lane);
+            chainAudio<T, U>(bank, taps, spans, jobs, points, mods, phase, filtered, states, segFirst, segEnd,
+                             uniform(heads[item]), lane);
         }
     });
 }
@@ -1058,16 +1067,18 @@ clayRoundVoiceChains(UnitsPtr units, RoundPtr round)
     }
 }
 
-// Filter pass, plain form: one unit, its pointers as kernel arguments, one wave per chain.
-__global__ void __launch_bounds__(64 * kChainWaves) __attribute__((amdgpu_waves_per_eu(kChainOccupancy)))
+// Filter pass, plain form: one unit, its pointers as kernel arguments, one wave
+// per chain, `W` chains (voices) per workgroup (chosen by the launch tuner).
+template<int W>
+__global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(kChainOccupancy)))
 clayVoiceChains(const Span* __restrict__ spans, const SpanCold* __restrict__ cold, const int* __restrict__ heads,
                 const clay_control_point* __restrict__ points, const Mod* __restrict__ mods,
                 float* __restrict__ filtered, VoiceState* __restrict__ states, int segFirst, int segEnd, int count,
                 float silenceLevel)
 {
-    __shared__ ChainCarry carries[kChainWaves];
+    __shared__ ChainCarry carries[W];
     const int wave = uniform(int(threadIdx.x >> 6)), lane = threadIdx.x & 63;
-    const int item = int(blockIdx.x) * kChainWaves + wave;
+    const int item = int(blockIdx.x) * W + wave;
     if (item >= count)
         return;  // uniform per wave; no block barriers below
     const int head = uniform(heads[item]);
@@ -1244,14 +1255,24 @@ __global__ void __launch_bounds__(kTileFrames) __attribute__((amdgpu_waves_per_e
                                                                 const Tile*, const int*, const clay_control_point*,    \
                                                                 const Mod*, const float*, const VoiceState*, void*,    \
                                                                 int, int, int);
+#define CLAY_INSTANTIATE_AUDIO(T, U)                                                                                   \
+    template __global__ void clayChainAudio<T, U>(const int16_t*, const float*, const Span*, const Job*, const int*,   \
+                                                  const clay_control_point*, const Mod*, const double*, float*,        \
+                                                  const VoiceState*, int, int, int);
 #define CLAY_INSTANTIATE_TAPS(T)                                                                                       \
     template __global__ void clayRoundChainAudio<T>(const int16_t*, const float*, UnitsPtr, RoundPtr);                 \
-    template __global__ void clayChainAudio<T>(const int16_t*, const float*, const Span*, const Job*, const int*,      \
-                                               const clay_control_point*, const Mod*, const double*, float*,           \
-                                               const VoiceState*, int, int, int);                                      \
+    CLAY_INSTANTIATE_AUDIO(T, 1)                                                                                        \
+    CLAY_INSTANTIATE_AUDIO(T, 2)                                                                                        \
     CLAY_INSTANTIATE_MIX(T, true, true)                                                                                 \
     CLAY_INSTANTIATE_MIX(T, false, true)
 CLAY_INSTANTIATE_TAPS(16)
 CLAY_INSTANTIATE_TAPS(32)
 CLAY_INSTANTIATE_TAPS(48)
 CLAY_INSTANTIATE_TAPS(64)
+
+#define CLAY_INSTANTIATE_CHAINS(W)                                                                                     \
+    template __global__ void clayVoiceChains<W>(const Span*, const SpanCold*, const int*, const clay_control_point*,   \
+                                                const Mod*, float*, VoiceState*, int, int, int, float);
+CLAY_INSTANTIATE_CHAINS(1)
+CLAY_INSTANTIATE_CHAINS(2)
+CLAY_INSTANTIATE_CHAINS(4)

@@ -48,10 +48,11 @@ using namespace clay_dev;
 // Render kernels (render.hip). Plain form: one unit, pointers as arguments.
 extern "C" __global__ void clayPitchChains(const Span*, const SpanCold*, const int*, const clay_control_point*,
                                            const Mod*, double*, VoiceState*, int, int, int);
-template<int T>
+template<int T, int U>
 __global__ void clayChainAudio(const int16_t*, const float*, const Span*, const Job*, const int*,
                                const clay_control_point*, const Mod*, const double*, float*, const VoiceState*, int,
                                int, int);
+template<int W>
 __global__ void clayVoiceChains(const Span*, const SpanCold*, const int*, const clay_control_point*, const Mod*, float*,
                                 VoiceState*, int, int, int, float);
 template<int T, bool FloatOut, bool WetOnly>
@@ -567,11 +568,74 @@ struct clay_hip_batch {
     std::map<std::string, std::pair<double, int>> kernelTimes;
 };
 
+// Launch shapes of the plain form's audio and filter passes, chosen by
+// measurement: the filter pass's voice chains (voices) per workgroup, 1, 2 or
+// 4 waves, and the audio pass's frames per lane in flight, its sinc-FIR
+// unroll, 1 or 2. Each candidate is timed on kTrials launches (HIP events on
+// the render stream, read once they have completed) in ms per wet sample, and
+// the cheapest is kept for the renderer's later launches. Every candidate
+// computes the same bits, so tuning never changes the audio.
+struct LaunchTuner {
+    static constexpr int kTrials = 2;
+    struct Dimension {
+        std::vector<int> values;
+        int chosen = 0;  // value used once tuning is over (or off)
+        bool tuning = false;
+        std::vector<double> cost;
+        std::vector<int> trials, issued;
+
+        void init(std::vector<int> v, int fallback, bool tune)
+        {
+            values = std::move(v);
+            chosen = fallback;
+            tuning = tune;
+            cost.assign(values.size(), 0);
+            trials.assign(values.size(), 0);
+            issued.assign(values.size(), 0);
+        }
+        // Candidate to measure on the next launch, or -1 (use `chosen`).
+        int next() const
+        {
+            if (tuning)
+                for (size_t i = 0; i < values.size(); ++i)
+                    if (issued[i] < kTrials)
+                        return int(i);
+            return -1;
+        }
+        int value(int candidate) const { return candidate < 0 ? chosen : values[size_t(candidate)]; }
+        void record(int candidate, double ms, double work)
+        {
+            if (!tuning || candidate < 0 || work <= 0)
+                return;
+            cost[size_t(candidate)] += ms / work;
+            ++trials[size_t(candidate)];
+            for (int t : trials)
+                if (t < kTrials)
+                    return;
+            size_t best = 0;
+            for (size_t i = 1; i < cost.size(); ++i)
+                if (cost[i] < cost[best])
+                    best = i;
+            chosen = values[best];
+            tuning = false;
+        }
+    };
+    Dimension chainWaves, audioUnroll;
+    struct Pending {
+        Dimension* dimension;
+        int candidate;
+        hipEvent_t begin, end;
+        double work;
+    };
+    std::vector<Pending> pending;
+};
+
 struct clay_hip {
     clay_hip_config cfg{};
     int device = 0;
     hipStream_t stream = nullptr;
     hipDeviceProp_t props{};
+    LaunchTuner tuner;      // plain form's audio and filter pass launch shapes
     Downloader downloader;  // persistent form's download thread
     ArenaCache arenas;      // persistent form's cohort arenas
     float* sinc = nullptr;       // phase-major taps x (phases+1)
@@ -693,6 +757,46 @@ struct clay_hip {
             check(hipEventRecord(t.end, stream), "hipEventRecord");
             timers.push_back(t);
         }
+    }
+
+    // Tuner measurements whose launches have completed (all of them with `wait`).
+    void collectTuning(bool wait)
+    {
+        auto& pending = tuner.pending;
+        size_t kept = 0;
+        for (size_t i = 0; i < pending.size(); ++i) {
+            auto& m = pending[i];
+            const hipError_t q = wait ? hipEventSynchronize(m.end) : hipEventQuery(m.end);
+            if (q == hipErrorNotReady) {
+                pending[kept++] = m;
+                continue;
+            }
+            check(q, "tuner timing");
+            float ms = 0;
+            check(hipEventElapsedTime(&ms, m.begin, m.end), "hipEventElapsedTime");
+            m.dimension->record(m.candidate, ms, m.work);
+            eventPool.push_back(m.begin);
+            eventPool.push_back(m.end);
+        }
+        pending.resize(kept);
+    }
+
+    // Runs `launch(value)` with the dimension's next candidate, timing it when
+    // it is a trial.
+    template<class F>
+    void tuned(LaunchTuner::Dimension& d, double work, F&& launch)
+    {
+        const int candidate = d.next();
+        if (candidate < 0) {
+            launch(d.value(candidate));
+            return;
+        }
+        LaunchTuner::Pending m{&d, candidate, event(), event(), work};
+        check(hipEventRecord(m.begin, stream), "hipEventRecord");
+        launch(d.value(candidate));
+        check(hipEventRecord(m.end, stream), "hipEventRecord");
+        ++d.issued[size_t(candidate)];
+        tuner.pending.push_back(m);
     }
 
     void collectTimes(std::map<std::string, std::pair<double, int>>& out)
@@ -1530,7 +1634,7 @@ struct Shape {
     bool simpleMix = false;  // every resident unit's dry voices are plain and straight
 };
 
-constexpr int kChainWavesPerBlock = 2;   // render.hip: voice chains per 128-thread filter block
+constexpr int kChainWavesPerBlock = 2;   // render.hip: voice chains per 128-thread filter block (round kernel)
 constexpr int kAudioWavesPerBlock = 16;  // render.hip: voice chains per 1024-thread audio block
 constexpr int kSharedTapsMax = 32;       // render.hip: widest sinc table the audio pass keeps in LDS
 constexpr int kMixWaves = 16;            // mix.h: tiles (one per wave) in flight per 1024-thread block
@@ -1622,15 +1726,28 @@ void launchWindow(clay_hip* h, const Window& w)
         int audioBlocks = ceilDiv(chainCount, kAudioWavesPerBlock);
         if (T <= kSharedTapsMax)
             audioBlocks = std::min(audioBlocks, std::max(1, h->props.multiProcessorCount));
-        h->launch("clayChainAudio", [&] {
-            hipLaunchKernelGGL(clayChainAudio<T>, dim3(audioBlocks), dim3(64 * kAudioWavesPerBlock), 0, s, h->bank,
-                               h->sinc, b.dSpans, b.dJobs, v.chainHeads, b.dPoints, b.dMods, b.dPhase, b.dFiltered,
-                               b.dStates, w.first, w.end, chainCount);
+        // Wet samples of the window (the batch's, pro rata): the tuner's unit of work.
+        const double work = double(b.filteredSamples) * double(w.end - w.first) / double(std::max(1, b.lastFrame));
+        h->collectTuning(false);
+        h->tuned(h->tuner.audioUnroll, work, [&](int unroll) {
+            h->launch("clayChainAudio", [&] {
+                const auto audio = [&](auto kernel) {
+                    hipLaunchKernelGGL(kernel, dim3(audioBlocks), dim3(64 * kAudioWavesPerBlock), 0, s, h->bank,
+                                       h->sinc, b.dSpans, b.dJobs, v.chainHeads, b.dPoints, b.dMods, b.dPhase,
+                                       b.dFiltered, b.dStates, w.first, w.end, chainCount);
+                };
+                unroll == 2 ? audio(clayChainAudio<T, 2>) : audio(clayChainAudio<T, 1>);
+            });
         });
-        h->launch("clayVoiceChains", [&] {
-            hipLaunchKernelGGL(clayVoiceChains, dim3(ceilDiv(chainCount, kChainWavesPerBlock)),
-                               dim3(64 * kChainWavesPerBlock), 0, s, b.dSpans, b.dCold, v.chainHeads, b.dPoints, b.dMods,
-                               b.dFiltered, b.dStates, w.first, w.end, chainCount, h->silenceLevel);
+        h->tuned(h->tuner.chainWaves, work, [&](int waves) {
+            h->launch("clayVoiceChains", [&] {
+                const auto chains = [&](auto kernel) {
+                    hipLaunchKernelGGL(kernel, dim3(ceilDiv(chainCount, waves)), dim3(64 * waves), 0, s, b.dSpans,
+                                       b.dCold, v.chainHeads, b.dPoints, b.dMods, b.dFiltered, b.dStates, w.first,
+                                       w.end, chainCount, h->silenceLevel);
+                };
+                waves == 1 ? chains(clayVoiceChains<1>) : waves == 4 ? chains(clayVoiceChains<4>) : chains(clayVoiceChains<2>);
+            });
         });
     }
     if (b.allWet) {
@@ -1985,6 +2102,9 @@ int clay_hip_create(const clay_hip_config* config, clay_hip** out)
             throw Failure(CLAY_E_INVALID, "voice_silence_db must be 0 (off) or a negative level in dBFS");
         h->silenceLevel = config->voice_silence_db < 0 ? float(std::pow(10.0, config->voice_silence_db / 20.0)) : 0.f;
         initTail(h.get());
+        const bool tune = !(config->flags & CLAY_FLAG_NO_TUNE);
+        h->tuner.chainWaves.init({1, 2, 4}, 2, tune);
+        h->tuner.audioUnroll.init({1, 2}, 1, tune);
         int devices = 0;
         check(hipGetDeviceCount(&devices), "hipGetDeviceCount");
         if (config->device < 0 || config->device >= devices)
@@ -2048,6 +2168,10 @@ int clay_hip_destroy(clay_hip* h)
     if (h->stream)
         (void)hipStreamSynchronize(h->stream);
     h->downloader.stop();
+    for (auto& m : h->tuner.pending) {
+        h->eventPool.push_back(m.begin);
+        h->eventPool.push_back(m.end);
+    }
     for (auto& b : h->arenas.blocks)
         (void)hipFree(b.base);
     for (auto& slot : h->slots)
@@ -2176,6 +2300,7 @@ int clay_hip_batch_run(clay_hip* h, clay_hip_batch* b, double* kernel_seconds)
         finishBatch(h, *b);
         check(hipEventRecord(end, h->stream), "hipEventRecord");
         check(hipStreamSynchronize(h->stream), "batch run");
+        h->collectTuning(false);
         float ms = 0;
         check(hipEventElapsedTime(&ms, begin, end), "hipEventElapsedTime");
         h->eventPool.push_back(begin);
@@ -3069,6 +3194,9 @@ int clay_hip_get_stats(clay_hip* h, clay_hip_stats* out)
     if (!h || !out)
         return CLAY_E_INVALID;
     *out = h->stats;
+    out->chain_waves = h->tuner.chainWaves.chosen;
+    out->audio_unroll = h->tuner.audioUnroll.chosen;
+    out->launch_tuning = h->tuner.chainWaves.tuning || h->tuner.audioUnroll.tuning;
     return CLAY_OK;
 }
 

@@ -219,7 +219,7 @@ typedef struct clay_hip_config {
 
 /* clay_hip_config.flags */
 #define CLAY_FLAG_NO_GRAPH 1       /* persistent form: launch rounds directly instead of replaying a hipGraph */
-#define CLAY_FLAG_NO_TUNE 2        /* persistent form: fixed launch shape instead of the round tuner */
+#define CLAY_FLAG_NO_TUNE 2        /* fixed launch shapes: no round tuner (persistent form), no launch tuner (plain form) */
 
 /* One published batch (src/cuda/renderer.h:12-18). `pcm` stays valid until
  * clay_hip_release_pcm(h, pcm_handle); results/job_indices only during the call. */
@@ -241,6 +241,10 @@ typedef struct clay_hip_stats {
     uint64_t graph_launches;       /* rounds replayed from a captured hipGraph */
     int32_t segment_frames;        /* persistent form: frames per round chosen (tuned or configured) */
     int32_t mix_blocks;            /* persistent form: grid of the mix kernel chosen */
+    int32_t chain_waves;           /* plain form: voice chains per filter-pass workgroup (launch tuner) */
+    int32_t audio_unroll;          /* plain form: frames per lane in flight in the audio pass (launch tuner) */
+    int32_t launch_tuning;         /* plain form: 1 while the launch tuner still has candidates to time */
+    int32_t pad;
 } clay_hip_stats;
 
 /* publish(user, batch, first): return 0 to continue, nonzero to abort the window. */
