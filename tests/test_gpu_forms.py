@@ -16,7 +16,7 @@ import pytest
 import claysampler_amd as ca
 from claysampler_amd import abi, synth
 
-from helpers import group_case, oracle_render, pedal_case, rich_case, segment_case
+from helpers import assert_parity, group_case, oracle_render, pedal_case, rich_case, segment_case
 
 pytestmark = pytest.mark.gpu
 
@@ -182,3 +182,21 @@ def test_persistent_against_oracle(synth_root):
         assert (t[0], t[1], t[2], t[3]) == (r.status, r.frames, r.data_frames, r.fade)
         want = want_pcm[r.offset:r.offset + r.data_frames]
         assert int(np.abs(pcm.astype(np.int32) - want.astype(np.int32)).max(initial=0)) <= 1
+
+
+def test_straight_tail_statistics_fused_or_scanned(synth_root):
+    """A straight batch (dry plain voices, configuration 3's kind) rendered in
+    one window stores the tail-block statistics from the mix kernel; rendered
+    in 1024- and 4096-frame windows it runs clayScanStereo instead. Statuses,
+    frame counts and PCM16 agree bit for bit, and with the FP64 oracle."""
+    sfz = synth.c1_instrument(f"{synth_root}/c1")
+    s = ca.Session()
+    for j, poly in enumerate((4, 24, 64, 9)):
+        s.add_job(sfz, synth.c3_midi(f"{synth_root}/c1/c3f_{j}.mid", 950 + j, 5.0 + 2 * j, poly))
+    whole, _ = resident_render(s, abi.POLICY_NATURAL, 16)
+    for segment in (1024, 4096):
+        split, _ = resident_render(s, abi.POLICY_NATURAL, 16, segment_frames=segment)
+        assert [full_tuple(r) for r in split[0]] == [full_tuple(r) for r in whole[0]], segment
+        assert np.array_equal(split[1], whole[1]), segment
+    assert_parity(whole, oracle_render(s, abi.POLICY_NATURAL, 16))
+
