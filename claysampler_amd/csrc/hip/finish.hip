@@ -207,16 +207,15 @@ extern "C" __global__ void clayLayoutPcm(const Job* __restrict__ jobs, int count
 // reductions are wave shuffles, with no LDS round or block barrier. `channels`
 // is 1 (natural) or 2. In place (orig/custom) the output offset of a job equals
 // its mix offset, so every frame is read and written at one address.
-extern "C" __global__ void __launch_bounds__(64) clayFinalizePcm(const Job* __restrict__ jobs, int count,
-                                                                 const int16_t* __restrict__ input,
-                                                                 int16_t* __restrict__ output,
-                                                                 clay_tail_result* __restrict__ results, int channels,
-                                                                 int countClipped, int firstBlock, int blocks)
+// The kernel for a compile-time channel count: the job's input and output
+// are addressed from per-job base pointers with 32-bit frame offsets, which
+// keeps the sixteen loads of a lane in flight within the register budget of
+// eight waves per SIMD.
+template<int Ch>
+__device__ __forceinline__ void finalizeRun(const Job* __restrict__ jobs, int count, const int16_t* __restrict__ input,
+                                            int16_t* __restrict__ output, clay_tail_result* __restrict__ results,
+                                            int countClipped, int begin, int stopBlock, int lane)
 {
-    const int begin = firstBlock + int(blockIdx.x) * kFinalizeBlocks;
-    const int stopBlock = min(begin + kFinalizeBlocks, firstBlock + blocks);
-    if (begin >= stopBlock)
-        return;
     int index = ownerOf(count, begin, [&](int i) { return jobs[i].blockFirst; });
     int peak = 0, audibleFirst = 0x7fffffff, audibleLast = -1, clipped = 0;
     // The run's reductions into the job's result, once per job.
@@ -226,10 +225,10 @@ extern "C" __global__ void __launch_bounds__(64) clayFinalizePcm(const Job* __re
         audibleLast = waveReduce(audibleLast, Max{});
         if (countClipped) {
             clipped = waveReduce(clipped, Add{});
-            if (threadIdx.x == 0 && clipped)
+            if (lane == 0 && clipped)
                 atomicAdd(&results[index].audio.clipped, clipped);
         }
-        if (threadIdx.x == 0 && audibleLast >= 0) {
+        if (lane == 0 && audibleLast >= 0) {
             atomicMax(&results[index].peak, peak);
             atomicMin(&results[index].first, audibleFirst);
             atomicMax(&results[index].last, audibleLast);
@@ -246,7 +245,8 @@ extern "C" __global__ void __launch_bounds__(64) clayFinalizePcm(const Job* __re
             while (index + 1 < count && jobs[index + 1].blockFirst <= block);
         }
         const Job j = jobs[index];
-        const uint64_t outAt = results[index].offset;
+        const int16_t* src = input + uint64_t(j.mix) * Ch;
+        int16_t* dst = output + uint64_t(results[index].offset) * Ch;
         const int frames = results[index].frames, fade = results[index].fade;
         const int first = (block - j.blockFirst) * kTailBlock;
         const int end = min(first + kTailBlock, results[index].data_frames);
@@ -257,7 +257,7 @@ extern "C" __global__ void __launch_bounds__(64) clayFinalizePcm(const Job* __re
         // loop, so the FP64 cosine is compiled once rather than per frame slot
         // (unrolled, it made the kernel too large for the instruction cache).
         const auto frame = [&](int f, int c, int v) {
-            output[(outAt + uint64_t(f)) * uint64_t(channels) + uint64_t(c)] = int16_t(v);
+            dst[f * Ch + c] = int16_t(v);
             clipped += abs(v) >= 32767;
             return abs(v);
         };
@@ -269,33 +269,31 @@ extern "C" __global__ void __launch_bounds__(64) clayFinalizePcm(const Job* __re
             }
         };
         if (end <= fadeStart) {
-            int16_t in[kPer][2];
+            int16_t in[kPer][Ch];
 #pragma unroll
             for (int i = 0; i < kPer; ++i) {
-                const int f = first + int(threadIdx.x) + 64 * i;
+                const int f = first + lane + 64 * i;
 #pragma unroll
-                for (int c = 0; c < 2; ++c)
-                    in[i][c] =
-                        f < end && c < channels ? input[(j.mix + uint64_t(f)) * uint64_t(channels) + uint64_t(c)] : 0;
+                for (int c = 0; c < Ch; ++c)
+                    in[i][c] = f < end ? src[f * Ch + c] : int16_t(0);
             }
 #pragma unroll
             for (int i = 0; i < kPer; ++i) {
-                const int f = first + int(threadIdx.x) + 64 * i;
+                const int f = first + lane + 64 * i;
                 if (f >= end)
                     break;
                 int framePeak = 0;
 #pragma unroll
-                for (int c = 0; c < 2; ++c)
-                    if (c < channels)
-                        framePeak = max(framePeak, frame(f, c, in[i][c]));
+                for (int c = 0; c < Ch; ++c)
+                    framePeak = max(framePeak, frame(f, c, in[i][c]));
                 audible(f, framePeak);
             }
         } else {
 #pragma unroll 1
-            for (int f = first + int(threadIdx.x); f < end; f += 64) {
+            for (int f = first + lane; f < end; f += 64) {
                 int framePeak = 0;
-                for (int c = 0; c < channels; ++c) {
-                    int v = input[(j.mix + uint64_t(f)) * uint64_t(channels) + uint64_t(c)];
+                for (int c = 0; c < Ch; ++c) {
+                    int v = src[f * Ch + c];
                     if (f >= fadeStart) {
                         const int k = f - fadeStart;
                         const double gain = fade == 1 ? 0 : .5 * (1 + cos(pi * k / (fade - 1)));
@@ -308,6 +306,29 @@ extern "C" __global__ void __launch_bounds__(64) clayFinalizePcm(const Job* __re
         }
     }
     flush();
+}
+
+// Fade, pack, peak, audible extent and clip count per 1024-frame block
+// (src/cuda/tail.cu:240-297). Each wave of a block takes kFinalizeBlocks
+// consecutive tail blocks (`blocks` in all from `firstBlock`), so the owner
+// search, the job and result loads and the atomics are paid once per run of
+// blocks of one job; the reductions are wave shuffles, with no LDS round or
+// block barrier. `channels` is 1 (natural) or 2. In place (orig/custom) the
+// output offset of a job equals its mix offset, so every frame is read and
+// written at one address.
+extern "C" __global__ void __launch_bounds__(64 * kFinalizeWaves) clayFinalizePcm(
+    const Job* __restrict__ jobs, int count, const int16_t* __restrict__ input, int16_t* __restrict__ output,
+    clay_tail_result* __restrict__ results, int channels, int countClipped, int firstBlock, int blocks)
+{
+    const int lane = int(threadIdx.x) & 63, wave = int(threadIdx.x) >> 6;
+    const int begin = firstBlock + (int(blockIdx.x) * kFinalizeWaves + wave) * kFinalizeBlocks;
+    const int stopBlock = min(begin + kFinalizeBlocks, firstBlock + blocks);
+    if (begin >= stopBlock)
+        return;  // uniform per wave; no block barriers
+    if (channels == 1)
+        finalizeRun<1>(jobs, count, input, output, results, countClipped, begin, stopBlock, lane);
+    else
+        finalizeRun<2>(jobs, count, input, output, results, countClipped, begin, stopBlock, lane);
 }
 
 // ====================================================================== quality

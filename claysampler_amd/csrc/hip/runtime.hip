@@ -1960,7 +1960,8 @@ void finishBatch(clay_hip* h, clay_hip_batch& b)
     int16_t* packed = static_cast<int16_t*>(b.dMix);
     chunked(b.blocks, [&](int first, int n) {
         h->launch("clayFinalizePcm", [&] {
-            hipLaunchKernelGGL(clayFinalizePcm, dim3(ceilDiv(n, kFinalizeBlocks)), dim3(64), 0, s, b.dJobs, count,
+            hipLaunchKernelGGL(clayFinalizePcm, dim3(ceilDiv(ceilDiv(n, kFinalizeBlocks), kFinalizeWaves)),
+                               dim3(64 * kFinalizeWaves), 0, s, b.dJobs, count,
                                finalIn, packed, b.dResults, ch, h->cfg.audio.enabled ? 1 : 0, first, n);
         });
     });
