@@ -127,15 +127,33 @@ extern "C" __global__ void clayQuantizeMono(const Job* __restrict__ jobs, int co
         return;
     double power = 0;
     int peak = 0;
-    for (int f = first + int(threadIdx.x); f < end; f += blockDim.x) {
-        const float2 x = mix[j.mix + uint64_t(f)];
-        const int l = __float2int_rn(fminf(32767.f, fmaxf(-32768.f, x.x * r.scale)));
-        const int rr = __float2int_rn(fminf(32767.f, fmaxf(-32768.f, x.y * r.scale)));
-        const int sum = l + rr;
-        const int v = (sum >> 1) + ((sum & 1) & ((sum >> 1) & 1));
-        mono[j.mix + uint64_t(f)] = int16_t(v);
-        power += double(v) * v;
-        peak = max(peak, abs(v));
+    // kAhead of the thread's frames are loaded before any is converted (one
+    // dependent load per frame left the kernel bound by latency); each thread
+    // still sums its frames in frame order.
+    constexpr int kAhead = 4;
+    const float2* src = mix + j.mix;
+    int16_t* dst = mono + j.mix;
+    const int stride = int(blockDim.x);
+    for (int f0 = first + int(threadIdx.x); f0 < end; f0 += kAhead * stride) {
+        float2 x[kAhead];
+#pragma unroll
+        for (int k = 0; k < kAhead; ++k) {
+            const int f = f0 + k * stride;
+            x[k] = f < end ? src[f] : make_float2(0.f, 0.f);
+        }
+#pragma unroll
+        for (int k = 0; k < kAhead; ++k) {
+            const int f = f0 + k * stride;
+            if (f >= end)
+                break;
+            const int l = __float2int_rn(fminf(32767.f, fmaxf(-32768.f, x[k].x * r.scale)));
+            const int rr = __float2int_rn(fminf(32767.f, fmaxf(-32768.f, x[k].y * r.scale)));
+            const int sum = l + rr;
+            const int v = (sum >> 1) + ((sum & 1) & ((sum >> 1) & 1));
+            dst[f] = int16_t(v);
+            power += double(v) * v;
+            peak = max(peak, abs(v));
+        }
     }
     power = blockReduce(power, 0.0, Add{});
     peak = blockReduce(peak, 0, Max{});
