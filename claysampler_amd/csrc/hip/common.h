@@ -438,6 +438,18 @@ struct Min {
 };
 
 // Index of the last entry whose `first` is <= block (src/cuda/tail.cu:32-44).
+// Mono PCM16 of a stereo frame at `scale` (src/cuda/tail.cu:147-181): each
+// channel clamped and rounded to nearest, then their average rounded half to
+// even. The mix kernel and clayQuantizeMono share it, so both give the same bits.
+__device__ __forceinline__ int monoSample(float2 x, float scale)
+{
+    const int l = __float2int_rn(fminf(32767.f, fmaxf(-32768.f, x.x * scale)));
+    const int r = __float2int_rn(fminf(32767.f, fmaxf(-32768.f, x.y * scale)));
+    const int sum = l + r;
+    return (sum >> 1) + ((sum & 1) & ((sum >> 1) & 1));
+}
+constexpr float kUnitScale = 32768.f;  // the natural policy's scale of a job whose peak stays below 1
+
 template<class F>
 __device__ __forceinline__ int ownerOf(int count, int block, F firstOf)
 {

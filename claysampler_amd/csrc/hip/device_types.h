@@ -153,6 +153,7 @@ struct UnitView {
     // Power and peak of each 1024-frame block, written by the straight mix
     // when this round renders the unit whole (else null: clayScanStereo).
     StereoStats* stereo;
+    int16_t* mono;  // with `stereo`: the unit-scale mono PCM16 (clayQuantizeMono keeps it)
     int32_t segFirst, segEnd;           // frame window of this round
     int32_t tileFirst, tileCount;       // tiles overlapping the window
     int32_t straightFirst, straightCount;

@@ -113,9 +113,12 @@ extern "C" __global__ void __launch_bounds__(64) claySelectNative(const Job* __r
 
 // Mono PCM16 (channel rounding then ties-to-even average) and 50 ms window
 // statistics (src/cuda/tail.cu:147-181).
+// With `monoReady` the mix kernel has already written every frame's mono PCM16
+// at the unit scale: a job at that scale (peak below 1) only reads it back for
+// the window statistics, 2 bytes a frame instead of the 8-byte float mix.
 extern "C" __global__ void clayQuantizeMono(const Job* __restrict__ jobs, int count, const float2* __restrict__ mix,
                                             const clay_tail_result* __restrict__ results, int16_t* __restrict__ mono,
-                                            MonoStats* __restrict__ stats, int window, int firstBlock)
+                                            MonoStats* __restrict__ stats, int window, int firstBlock, int monoReady)
 {
     const int block = firstBlock + blockIdx.x;
     const int index = ownerOf(count, block, [&](int i) { return jobs[i].windowFirst; });
@@ -134,25 +137,40 @@ extern "C" __global__ void clayQuantizeMono(const Job* __restrict__ jobs, int co
     const float2* src = mix + j.mix;
     int16_t* dst = mono + j.mix;
     const int stride = int(blockDim.x);
-    for (int f0 = first + int(threadIdx.x); f0 < end; f0 += kAhead * stride) {
-        float2 x[kAhead];
+    if (monoReady && r.scale == kUnitScale) {
+        for (int f0 = first + int(threadIdx.x); f0 < end; f0 += kAhead * stride) {
+            int x[kAhead];
 #pragma unroll
-        for (int k = 0; k < kAhead; ++k) {
-            const int f = f0 + k * stride;
-            x[k] = f < end ? src[f] : make_float2(0.f, 0.f);
+            for (int k = 0; k < kAhead; ++k) {
+                const int f = f0 + k * stride;
+                x[k] = f < end ? int(dst[f]) : 0;
+            }
+#pragma unroll
+            for (int k = 0; k < kAhead; ++k) {
+                if (f0 + k * stride >= end)
+                    break;
+                power += double(x[k]) * x[k];
+                peak = max(peak, abs(x[k]));
+            }
         }
+    } else {
+        for (int f0 = first + int(threadIdx.x); f0 < end; f0 += kAhead * stride) {
+            float2 x[kAhead];
 #pragma unroll
-        for (int k = 0; k < kAhead; ++k) {
-            const int f = f0 + k * stride;
-            if (f >= end)
-                break;
-            const int l = __float2int_rn(fminf(32767.f, fmaxf(-32768.f, x[k].x * r.scale)));
-            const int rr = __float2int_rn(fminf(32767.f, fmaxf(-32768.f, x[k].y * r.scale)));
-            const int sum = l + rr;
-            const int v = (sum >> 1) + ((sum & 1) & ((sum >> 1) & 1));
-            dst[f] = int16_t(v);
-            power += double(v) * v;
-            peak = max(peak, abs(v));
+            for (int k = 0; k < kAhead; ++k) {
+                const int f = f0 + k * stride;
+                x[k] = f < end ? src[f] : make_float2(0.f, 0.f);
+            }
+#pragma unroll
+            for (int k = 0; k < kAhead; ++k) {
+                const int f = f0 + k * stride;
+                if (f >= end)
+                    break;
+                const int v = monoSample(x[k], r.scale);
+                dst[f] = int16_t(v);
+                power += double(v) * v;
+                peak = max(peak, abs(v));
+            }
         }
     }
     power = blockReduce(power, 0.0, Add{});

@@ -681,7 +681,9 @@ __device__ __forceinline__ void straightRows(const int16_t* __restrict__ bank, u
 // Writes the rows of a tile's mix: the frames [lo, hi) of rows starting at
 // `tileFrame`, scaled by the job gain (CC7 x CC11, src/cuda/render.cu:189-193),
 // plus the wet sums. With `stats` (a whole 1024-frame tail block in one wave)
-// it also stores the block's power and peak as clayScanStereo would
+// it also stores the block's power and peak as clayScanStereo would, and with
+// `mono` the frames' mono PCM16 at the unit scale, which clayQuantizeMono
+// keeps for every job whose scale turns out to be that one
 // (src/cuda/tail.cu:54-81): the same per-frame terms, summed in another order,
 // which moves the power by a few units in the last place; the natural-end test
 // re-sums serially within 1e-12 of its threshold, so its decisions do not move.
@@ -690,7 +692,7 @@ __device__ __forceinline__ void writeRows(const clay_control_point* __restrict__
                                           int tileFrame, int tileEnd, int lo, int hi, const float (&left)[Rows],
                                           const float (&right)[Rows], const float (&wetLeft)[Rows],
                                           const float (&wetRight)[Rows], int lane, StereoStats* __restrict__ stats = nullptr,
-                                          int statsIndex = 0)
+                                          int statsIndex = 0, int16_t* __restrict__ mono = nullptr)
 {
     double power = 0;
     float peak = 0;
@@ -716,6 +718,8 @@ __device__ __forceinline__ void writeRows(const clay_control_point* __restrict__
             }
         }
         const uint64_t at = j.mix + uint64_t(frame);
+        if (mono)
+            mono[at] = int16_t(monoSample(mixed, kUnitScale));
         if (FloatOut) {
             static_cast<float2*>(out)[at] = mixed;
         } else {
@@ -833,7 +837,7 @@ __device__ __forceinline__ void straightTileWave(const int16_t* __restrict__ ban
                                                  const int* __restrict__ voices,
                                                  const clay_control_point* __restrict__ points, void* __restrict__ out,
                                                  int segFirst, int segEnd, const Tile& tileIn, int lane,
-                                                 StereoStats* __restrict__ stereo)
+                                                 StereoStats* __restrict__ stereo, int16_t* __restrict__ mono)
 {
     static_assert(kStraightFrames == kTailBlock, "a straight tile is one tail block");
     const Tile tile{uniform(tileIn.job), uniform(tileIn.frame), uniform(tileIn.frames), uniform(tileIn.first),
@@ -864,7 +868,7 @@ __device__ __forceinline__ void straightTileWave(const int16_t* __restrict__ ban
     // The tail block's statistics when the launch renders the whole tile.
     StereoStats* stats = stereo && lo == tile.frame && hi == tileEnd ? stereo : nullptr;
     writeRows<FloatOut>(points, j, out, tile.frame, tileEnd, lo, hi, left, right, zero, zero, lane, stats,
-                        j.blockFirst + tile.frame / kTailBlock);
+                        j.blockFirst + tile.frame / kTailBlock, stats ? mono : nullptr);
 }
 
 // Runs body(taps) with the integer weights: for T <= kIntSharedTaps rows
@@ -969,13 +973,14 @@ __global__ void __launch_bounds__(64 * kMixWaves) clayRoundMixWaves(const int16_
 }
 
 // Plain form, straight batches: 1024-frame tiles [firstTile, firstTile + tileCount);
-// `stereo` (or null) receives the tail-block statistics of whole tiles.
+// `stereo` and `mono` (or null) receive the tail-block statistics and the
+// unit-scale mono PCM16 of whole tiles.
 template<int T, bool FloatOut>
 __global__ void __launch_bounds__(64 * kStraightWaves) clayMixStraight(
     const int16_t* __restrict__ bank, uint32_t odd, const uint32_t* __restrict__ intTable, const Span* __restrict__ spans,
     const Job* __restrict__ jobs, const Tile* __restrict__ tiles, const int* __restrict__ voices,
     const clay_control_point* __restrict__ points, void* __restrict__ out, int segFirst, int segEnd, int firstTile,
-    int tileCount, MixQueue* __restrict__ queue, StereoStats* __restrict__ stereo)
+    int tileCount, MixQueue* __restrict__ queue, StereoStats* __restrict__ stereo, int16_t* __restrict__ mono)
 {
     const int lane = int(threadIdx.x) & 63;
     TileClaims<16, kStraightWaves> claims(queue, lane);
@@ -983,7 +988,7 @@ __global__ void __launch_bounds__(64 * kStraightWaves) clayMixStraight(
         for (int t; (t = claims.next(tileCount)) >= 0;) {
             const Tile tile = tiles[firstTile + t];
             straightTileWave<T, FloatOut>(bank, odd, taps, spans, jobs, voices, points, out, segFirst, segEnd, tile, lane,
-                                          stereo);
+                                          stereo, mono);
         }
     });
 }
@@ -1007,7 +1012,7 @@ __global__ void __launch_bounds__(64 * kStraightWaves) clayRoundMixStraight(cons
             const Tile tile = global(u.straightTiles)[u.straightFirst + item - u.straightBase];
             straightTileWave<T, FloatOut>(bank, odd, taps, global(u.spans), global(u.jobs), global(u.straightVoices),
                                           global(u.points), global(u.mix), u.segFirst, u.segEnd, tile, lane,
-                                          global(u.stereo));
+                                          global(u.stereo), global(u.mono));
         }
     });
 }
@@ -1025,7 +1030,7 @@ __global__ void __launch_bounds__(64 * kStraightWaves) clayRoundMixStraight(cons
     template __global__ void clayMixStraight<T, FloatOut>(const int16_t*, uint32_t, const uint32_t*, const Span*,     \
                                                           const Job*,                                                 \
                                                           const Tile*, const int*, const clay_control_point*, void*,  \
-                                                          int, int, int, int, MixQueue*, StereoStats*);               \
+                                                          int, int, int, int, MixQueue*, StereoStats*, int16_t*);     \
     template __global__ void clayRoundMixStraight<T, FloatOut>(const int16_t*, uint32_t, const uint32_t*, UnitsPtr,   \
                                                                RoundPtr, MixQueue*);
 #define CLAY_INSTANTIATE_MIX_WAVES_TAPS(T)                                                                             \

@@ -79,7 +79,7 @@ __global__ void clayRoundMixWaves(const int16_t*, uint32_t, const uint32_t*, con
 template<int T, bool FloatOut>
 __global__ void clayMixStraight(const int16_t*, uint32_t, const uint32_t*, const Span*, const Job*, const Tile*,
                                 const int*, const clay_control_point*, void*, int, int, int, int, MixQueue*,
-                                StereoStats*);
+                                StereoStats*, int16_t*);
 template<int T, bool FloatOut>
 __global__ void clayRoundMixStraight(const int16_t*, uint32_t, const uint32_t*, const UnitView*, const RoundHeader*,
                                      MixQueue*);
@@ -87,7 +87,7 @@ __global__ void clayRoundMixStraight(const int16_t*, uint32_t, const uint32_t*, 
 extern "C" __global__ void clayScanStereo(const Job*, int, const float2*, StereoStats*, int);
 extern "C" __global__ void claySelectNative(const Job*, int, const float2*, const StereoStats*, clay_tail_result*);
 extern "C" __global__ void clayQuantizeMono(const Job*, int, const float2*, const clay_tail_result*, int16_t*,
-                                            MonoStats*, int, int);
+                                            MonoStats*, int, int, int);
 extern "C" __global__ void claySelectTrim(const Job*, int, const MonoStats*, clay_tail_result*, TailConfig);
 extern "C" __global__ void clayLayoutPcm(const Job*, int, clay_tail_result*, int);
 extern "C" __global__ void clayFinalizePcm(const Job*, int, const int16_t*, int16_t*, clay_tail_result*, int, int, int,
@@ -1613,6 +1613,7 @@ UnitView viewOf(const Window& w)
         // One window over the whole batch: the mix stores the tail-block
         // statistics (clayScanStereo is skipped).
         v.stereo = w.first == 0 && w.end >= b.lastFrame ? b.dStereo : nullptr;
+        v.mono = v.stereo ? b.dMono : nullptr;
     } else {
         v.tileFirst = first;
         v.tileCount = count;
@@ -1775,7 +1776,7 @@ void launchWindow(clay_hip* h, const Window& w)
                 const auto mix = [&](auto kernel) {
                     hipLaunchKernelGGL(kernel, dim3(blocks), dim3(64 * kStraightWaves), 0, s, h->bank, h->bankOdd,
                                        h->sincInt, b.dSpans, b.dJobs, b.dTiles, b.dVoices, b.dPoints, b.dMix, w.first, w.end,
-                                       v.straightFirst, v.straightCount, h->queues + 1, v.stereo);
+                                       v.straightFirst, v.straightCount, h->queues + 1, v.stereo, v.mono);
                 };
                 h->natural ? mix(clayMixStraight<T, true>) : mix(clayMixStraight<T, false>);
             });
@@ -1944,7 +1945,7 @@ void finishBatch(clay_hip* h, clay_hip_batch& b)
         chunked(b.windows, [&](int first, int n) {
             h->launch("clayQuantizeMono", [&] {
                 hipLaunchKernelGGL(clayQuantizeMono, dim3(n), dim3(256), 0, s, b.dJobs, count, mix2, b.dResults,
-                                   b.dMono, b.dWindows, h->tail.window, first);
+                                   b.dMono, b.dWindows, h->tail.window, first, b.stereoDone ? 1 : 0);
             });
         });
         h->launch("claySelectTrim", [&] {
