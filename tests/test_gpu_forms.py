@@ -184,19 +184,29 @@ def test_persistent_against_oracle(synth_root):
         assert int(np.abs(pcm.astype(np.int32) - want.astype(np.int32)).max(initial=0)) <= 1
 
 
-def test_straight_tail_statistics_fused_or_scanned(synth_root):
+@pytest.mark.parametrize("hot", [False, True])
+def test_straight_tail_statistics_fused_or_scanned(synth_root, tmp_path, hot):
     """A straight batch (dry plain voices, configuration 3's kind) rendered in
-    one window stores the tail-block statistics from the mix kernel; rendered
-    in 1024- and 4096-frame windows it runs clayScanStereo instead. Statuses,
-    frame counts and PCM16 agree bit for bit, and with the FP64 oracle."""
-    sfz = synth.c1_instrument(f"{synth_root}/c1")
+    one window stores the tail-block statistics and the unit-scale mono PCM16
+    from the mix kernel; rendered in 1024- and 4096-frame windows it runs
+    clayScanStereo and quantises every job from the float mix instead.
+    Statuses, frame counts and PCM16 agree bit for bit, and with the FP64
+    oracle. The hot batch (+3 dB) has jobs scaled below the unit scale (peak
+    above full scale), which take the float path in either case."""
+    if hot:
+        sfz = synth.c1_instrument(str(tmp_path / "hot"), volume_db=3)
+        plan = [(970 + j, 4.0, poly) for j, poly in enumerate((2, 4, 8, 16, 32, 64))]
+    else:
+        sfz = synth.c1_instrument(f"{synth_root}/c1")
+        plan = [(950 + j, 5.0 + 2 * j, poly) for j, poly in enumerate((4, 24, 64, 9))]
     s = ca.Session()
-    for j, poly in enumerate((4, 24, 64, 9)):
-        s.add_job(sfz, synth.c3_midi(f"{synth_root}/c1/c3f_{j}.mid", 950 + j, 5.0 + 2 * j, poly))
+    for j, (seed, seconds, poly) in enumerate(plan):
+        s.add_job(sfz, synth.c3_midi(str(tmp_path / f"c3f_{j}.mid"), seed, seconds, poly))
     whole, _ = resident_render(s, abi.POLICY_NATURAL, 16)
+    if hot:
+        assert any(r.status == 0 and r.scale < 32768 for r in whole[0])  # status 0: ok
     for segment in (1024, 4096):
         split, _ = resident_render(s, abi.POLICY_NATURAL, 16, segment_frames=segment)
         assert [full_tuple(r) for r in split[0]] == [full_tuple(r) for r in whole[0]], segment
         assert np.array_equal(split[1], whole[1]), segment
     assert_parity(whole, oracle_render(s, abi.POLICY_NATURAL, 16))
-
