@@ -52,7 +52,7 @@ def workload(name, synth_dir, jobs, seconds):
 
 
 def kernel_bytes(plans, taps, natural):
-    """Algorithmic HBM bytes per launch of each render kernel (see DESIGN.md).
+    """Algorithmic HBM bytes per launch of each render kernel.
 
     Source window bytes of a span = 2 B x channels x (|source travel| + taps);
     travel is step x frames (with the bend ramp factor for bent spans)."""
@@ -88,8 +88,10 @@ def kernel_bytes(plans, taps, natural):
         # dry source windows + wet reads + the mix write + span records
         "clayMixTiles": src["dry"] + 4 * wet_samples + mix_bytes + spans * span_bytes,
         "clayMixWaves": src["dry"] + 4 * wet_samples + mix_bytes + spans * span_bytes,
-        # straight batches: dry source windows + the mix write + span records
-        "clayMixStraight": src["dry"] + mix_bytes + spans * span_bytes,
+        # straight batches: dry source windows + the mix write + span records,
+        # and (natural policy, the batch in one window) the unit-scale mono
+        # PCM16 written beside the float mix
+        "clayMixStraight": src["dry"] + mix_bytes + spans * span_bytes + (2 * frames if natural else 0),
     }, voice_frames, frames
 
 
@@ -482,7 +484,7 @@ def main():
     ap.add_argument("--epochs", type=int, default=20, help="configuration 4: training epochs")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (jobs shard by job; see DESIGN.md)",
+        print("bench.py measures one GPU: the renderer has no multi-GPU path (jobs would shard by job, one process per GPU)",
               file=sys.stderr)
         sys.exit(2)
     if args.config == "c4":

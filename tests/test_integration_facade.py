@@ -1,5 +1,5 @@
-"""The C++ facade in INTEGRATION.md (the reference's `clay::Renderer` over
-this core's C ABI) compiles against the reference's own headers
+"""The documented C++ integration facade (the reference's `clay::Renderer`
+over this core's C ABI, read from the repository's integration notes) compiles against the reference's own headers
 (src/cuda/renderer.h, src/cuda/render_types.h, src/render/plan.h) and converts
 every record field by field.
 
