@@ -116,11 +116,18 @@ extern "C" __global__ void __launch_bounds__(64) claySelectNative(const Job* __r
 // With `monoReady` the mix kernel has already written every frame's mono PCM16
 // at the unit scale: a job at that scale (peak below 1) only reads it back for
 // the window statistics, 2 bytes a frame instead of the 8-byte float mix.
-extern "C" __global__ void clayQuantizeMono(const Job* __restrict__ jobs, int count, const float2* __restrict__ mix,
-                                            const clay_tail_result* __restrict__ results, int16_t* __restrict__ mono,
-                                            MonoStats* __restrict__ stats, int window, int firstBlock, int monoReady)
+// One wave per window, kQuantizeWaves windows per block, wave reductions only
+// (the power is a sum of integers below 2^53, exact in any order).
+extern "C" __global__ void __launch_bounds__(64 * kQuantizeWaves) clayQuantizeMono(
+    const Job* __restrict__ jobs, int count, const float2* __restrict__ mix,
+    const clay_tail_result* __restrict__ results, int16_t* __restrict__ mono, MonoStats* __restrict__ stats,
+    int window, int firstBlock, int blocks, int monoReady)
 {
-    const int block = firstBlock + blockIdx.x;
+    const int lane = int(threadIdx.x) & 63;
+    const int slot = int(blockIdx.x) * kQuantizeWaves + int(threadIdx.x >> 6);
+    if (slot >= blocks)
+        return;  // uniform per wave; no block barriers
+    const int block = firstBlock + slot;
     const int index = ownerOf(count, block, [&](int i) { return jobs[i].windowFirst; });
     const Job j = jobs[index];
     const clay_tail_result r = results[index];
@@ -130,40 +137,35 @@ extern "C" __global__ void clayQuantizeMono(const Job* __restrict__ jobs, int co
         return;
     double power = 0;
     int peak = 0;
-    // kAhead of the thread's frames are loaded before any is converted (one
-    // dependent load per frame left the kernel bound by latency); each thread
-    // still sums its frames in frame order.
+    // kAhead of the lane's frames are loaded before any is used (one
+    // dependent load per frame left the kernel bound by latency).
     constexpr int kAhead = 4;
     const float2* src = mix + j.mix;
     int16_t* dst = mono + j.mix;
-    const int stride = int(blockDim.x);
-    if (monoReady && r.scale == kUnitScale) {
-        for (int f0 = first + int(threadIdx.x); f0 < end; f0 += kAhead * stride) {
+    const bool ready = monoReady && r.scale == kUnitScale;
+    for (int f0 = first + lane; f0 < end; f0 += kAhead * 64) {
+        if (ready) {
             int x[kAhead];
 #pragma unroll
             for (int k = 0; k < kAhead; ++k) {
-                const int f = f0 + k * stride;
+                const int f = f0 + k * 64;
                 x[k] = f < end ? int(dst[f]) : 0;
             }
 #pragma unroll
             for (int k = 0; k < kAhead; ++k) {
-                if (f0 + k * stride >= end)
-                    break;
                 power += double(x[k]) * x[k];
                 peak = max(peak, abs(x[k]));
             }
-        }
-    } else {
-        for (int f0 = first + int(threadIdx.x); f0 < end; f0 += kAhead * stride) {
+        } else {
             float2 x[kAhead];
 #pragma unroll
             for (int k = 0; k < kAhead; ++k) {
-                const int f = f0 + k * stride;
+                const int f = f0 + k * 64;
                 x[k] = f < end ? src[f] : make_float2(0.f, 0.f);
             }
 #pragma unroll
             for (int k = 0; k < kAhead; ++k) {
-                const int f = f0 + k * stride;
+                const int f = f0 + k * 64;
                 if (f >= end)
                     break;
                 const int v = monoSample(x[k], r.scale);
@@ -173,9 +175,9 @@ extern "C" __global__ void clayQuantizeMono(const Job* __restrict__ jobs, int co
             }
         }
     }
-    power = blockReduce(power, 0.0, Add{});
-    peak = blockReduce(peak, 0, Max{});
-    if (threadIdx.x == 0)
+    power = waveReduce(power, Add{});
+    peak = waveReduce(peak, Max{});
+    if (lane == 0)
         stats[block] = {power, peak, 0};
 }
 

@@ -87,7 +87,7 @@ __global__ void clayRoundMixStraight(const int16_t*, uint32_t, const uint32_t*, 
 extern "C" __global__ void clayScanStereo(const Job*, int, const float2*, StereoStats*, int);
 extern "C" __global__ void claySelectNative(const Job*, int, const float2*, const StereoStats*, clay_tail_result*);
 extern "C" __global__ void clayQuantizeMono(const Job*, int, const float2*, const clay_tail_result*, int16_t*,
-                                            MonoStats*, int, int, int);
+                                            MonoStats*, int, int, int, int);
 extern "C" __global__ void claySelectTrim(const Job*, int, const MonoStats*, clay_tail_result*, TailConfig);
 extern "C" __global__ void clayLayoutPcm(const Job*, int, clay_tail_result*, int);
 extern "C" __global__ void clayFinalizePcm(const Job*, int, const int16_t*, int16_t*, clay_tail_result*, int, int, int,
@@ -1944,8 +1944,9 @@ void finishBatch(clay_hip* h, clay_hip_batch& b)
         });
         chunked(b.windows, [&](int first, int n) {
             h->launch("clayQuantizeMono", [&] {
-                hipLaunchKernelGGL(clayQuantizeMono, dim3(n), dim3(256), 0, s, b.dJobs, count, mix2, b.dResults,
-                                   b.dMono, b.dWindows, h->tail.window, first, b.stereoDone ? 1 : 0);
+                hipLaunchKernelGGL(clayQuantizeMono, dim3(ceilDiv(n, kQuantizeWaves)), dim3(64 * kQuantizeWaves), 0, s,
+                                   b.dJobs, count, mix2, b.dResults, b.dMono, b.dWindows, h->tail.window, first, n,
+                                   b.stereoDone ? 1 : 0);
             });
         });
         h->launch("claySelectTrim", [&] {
