@@ -2872,9 +2872,11 @@ int renderWindowPersistent(clay_hip* h, const clay_plan_view* plans, size_t coun
         f->views.assign(queue.begin() + std::ptrdiff_t(first), queue.begin() + std::ptrdiff_t(end));
         f->b = new clay_hip_batch();
         Flattening* raw = f.get();
-        // The kAhead cohorts flattening at once share the CPUs, leaving two
-        // for this loop and the download thread.
-        const size_t threads = std::max<size_t>(1, (std::max<size_t>(hostThreads(), 3) - 2) / kAhead);
+        // The cohorts flattening at once share the CPUs, leaving two for this
+        // loop and the download threads; a cohort started while fewer are in
+        // flight (the first of a window, which rendering waits for) gets a
+        // larger share.
+        const size_t threads = std::max<size_t>(1, (std::max<size_t>(hostThreads(), 3) - 2) / (ahead.size() + 1));
         raw->worker = std::thread([h, raw, threads] {
             try {
                 flatten(h, raw->views.data(), raw->views.size(), *raw->b, threads);
