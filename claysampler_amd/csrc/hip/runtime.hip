@@ -2832,7 +2832,7 @@ int renderWindowPersistent(clay_hip* h, const clay_plan_view* plans, size_t coun
         }
         ~Downloads() { finish(); }
     };
-    constexpr size_t kAhead = 4;            // cohorts flattened ahead of admission
+    constexpr size_t kAhead = 6;            // cohorts flattened ahead of admission
     constexpr size_t kCohortsPerWindow = 8; // resident cohorts when every one is full
     const size_t cohortJobs = std::max<size_t>(1, size_t(h->cfg.batch_jobs) / kCohortsPerWindow);
 
