@@ -18,7 +18,7 @@ namespace clay_dev {
 
 constexpr int kPhases = CLAY_SINC_PHASES;
 constexpr int kTailBlock = 1024;   // natural end / finalize block (src/cuda/render_types.h:92)
-constexpr int kFinalizeBlocks = 8;  // tail blocks per wave of clayFinalizePcm
+constexpr int kFinalizeBlocks = 32;  // tail blocks per wave of clayFinalizePcm
 constexpr int kFinalizeWaves = 4;   // independent waves per block of clayFinalizePcm
 constexpr int kQuantizeWaves = 4;   // windows (one per wave) per block of clayQuantizeMono
 constexpr int kTileFrames = 256;   // output frames per mix tile = threads per mix block
